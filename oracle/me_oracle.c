@@ -1,0 +1,351 @@
+/*
+ * CPU restatement of the reference's H.264 16x16 block-matching motion
+ * estimator.  TEST INFRASTRUCTURE ONLY: this file is the parity checker for
+ * the HIP path.  Only tests/, the smoke entry point and the CPU-baseline leg
+ * of the benchmark may load it; no product code links or calls it.
+ *
+ * It follows, step for step:
+ *   src/CodecUtils/MotionEstimatorH264ImplFull.cpp:330-611   Estimate()
+ *   src/CodecUtils/MotionEstimatorH264ImplFull.cpp:1029-1081 GetMotionRange (9-arg)
+ *   src/CodecUtils/MotionEstimatorH264ImplFull.cpp:1095-1255 half/quarter window
+ *   include/CodecUtils/MotionEstimatorH264ImplFull.h:140-168 GetBestVector
+ *   include/GeneralUtils/Fifo.h:54-106                       candidate fifo
+ *   src/ImageUtils/OverlayMem2Dv2.cpp:704-1052               QuarterRead
+ *   src/ImageUtils/OverlayMem2Dv2.cpp:1623-1983,2655-2983    Tsd/Tad 16x16
+ *   src/ImageUtils/OverlayExtMem2Dv2.cpp:86-217              padded reference
+ *   src/CodecUtils/MacroBlockH264.cpp:757-816,923-936        median predictor
+ *
+ * The padded reference (border 19, edge replication) is restated as a
+ * coordinate clamp; every access the estimator makes lies inside the padded
+ * area ([-19, W+19) x [-19, H+19)), which the accessor asserts.
+ *
+ * Switches beyond the as-shipped class (all default to the shipped value):
+ *   metric   0 = SSD (shipped, Full.cpp:74 #undef MEH264IF_ABS_DIFF), 1 = SAD
+ *   window   0 = reference rule +-(R-1) (Full.cpp:399 -> :1029), 1 = +-R
+ *   filter   1 = running-best candidate filter (Full.cpp:410-411), 0 = off
+ *   subpel   2 = half+quarter (shipped), 1 = half only, 0 = integer only
+ *   pred     1 = H.264 median of A/B/C(D) (H264MotionVectorPredictorImpl1),
+ *            0 = zero predictor
+ *   bestvec  1 = GetBestVector 3-way choice (shipped), 0 = keep refined best
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <stdio.h>
+
+#include "oracle.h"
+
+#define PAD 19 /* 16 + MEH264IF_PADDING (Full.cpp:244) */
+
+typedef struct {
+  const int16_t *p;
+  int w, h;
+} plane_t;
+
+static inline int pix(const plane_t *pl, int x, int y) {
+  if (x < -PAD || x >= pl->w + PAD || y < -PAD || y >= pl->h + PAD) {
+    fprintf(stderr, "oracle: access (%d,%d) outside padded reference\n", x, y);
+    abort();
+  }
+  if (x < 0) x = 0;
+  if (x > pl->w - 1) x = pl->w - 1;
+  if (y < 0) y = 0;
+  if (y > pl->h - 1) y = pl->h - 1;
+  return pl->p[(long)y * pl->w + x];
+}
+
+static inline int clip255(int x) { return x < 0 ? 0 : (x > 255 ? 255 : x); }
+
+/* Block distortion between the 16x16 input block at (n,m) and a 16x16 block
+ * of samples blk[256] (row-major).  Tsd16x16 / Tad16x16 semantics; the
+ * LessThan early exit only ever returns a partial sum > min, which is never
+ * accepted, so the exact sum is equivalent for every decision. */
+static int blk_dist(const int16_t *cur, int W, int n, int m, const int *blk,
+                    int metric) {
+  int acc = 0;
+  for (int r = 0; r < 16; r++)
+    for (int c = 0; c < 16; c++) {
+      int d = cur[(long)(m + r) * W + n + c] - blk[r * 16 + c];
+      acc += metric ? (d < 0 ? -d : d) : d * d;
+    }
+  return acc;
+}
+
+static void read_full(const plane_t *ref, int x, int y, int *blk) {
+  for (int r = 0; r < 16; r++)
+    for (int c = 0; c < 16; c++) blk[r * 16 + c] = pix(ref, x + c, y + r);
+}
+
+/* H.264 6-tap (1,-5,20,20,-5,1) helpers (OverlayMem2Dv2.cpp:92-98). */
+#define TAP6(a, b, c, d, e, f) ((a) - 5 * (b) + 20 * (c) + 20 * (d) - 5 * (e) + (f))
+static inline int vtap(const plane_t *p, int x, int y) {
+  return TAP6(pix(p, x, y - 2), pix(p, x, y - 1), pix(p, x, y), pix(p, x, y + 1),
+              pix(p, x, y + 2), pix(p, x, y + 3));
+}
+static inline int htap(const plane_t *p, int x, int y) {
+  return TAP6(pix(p, x - 2, y), pix(p, x - 1, y), pix(p, x, y), pix(p, x + 1, y),
+              pix(p, x + 2, y), pix(p, x + 3, y));
+}
+static inline int get_b(const plane_t *p, int x, int y) { return clip255((htap(p, x, y) + 16) >> 5); }
+static inline int get_h(const plane_t *p, int x, int y) { return clip255((vtap(p, x, y) + 16) >> 5); }
+static inline int get_j(const plane_t *p, int x, int y) {
+  int s = TAP6(vtap(p, x - 2, y), vtap(p, x - 1, y), vtap(p, x, y), vtap(p, x + 1, y),
+               vtap(p, x + 2, y), vtap(p, x + 3, y));
+  return clip255((s + 512) >> 10);
+}
+
+/* One quarter-pel sample at full-pel (x,y) with fractions fx,fy in 0..3
+ * (OverlayMem2Dv2::QuarterRead switch, :729-1010; H.264 8.4.2.2.1). */
+int orc_qpel_sample(const int16_t *ref, int W, int H, int x, int y, int fx, int fy) {
+  plane_t p = {ref, W, H};
+  int G = pix(&p, x, y);
+  switch (fx | (fy << 2)) {
+    case 0: return G;
+    case 2: return get_b(&p, x, y);
+    case 8: return get_h(&p, x, y);
+    case 10: return get_j(&p, x, y);
+    case 1: return (G + get_b(&p, x, y) + 1) >> 1;                       /* a */
+    case 3: return (pix(&p, x + 1, y) + get_b(&p, x, y) + 1) >> 1;       /* c */
+    case 4: return (G + get_h(&p, x, y) + 1) >> 1;                       /* d */
+    case 12: return (pix(&p, x, y + 1) + get_h(&p, x, y) + 1) >> 1;      /* n */
+    case 6: return (get_j(&p, x, y) + get_b(&p, x, y) + 1) >> 1;         /* f */
+    case 14: return (get_j(&p, x, y) + get_b(&p, x, y + 1) + 1) >> 1;    /* q */
+    case 9: return (get_j(&p, x, y) + get_h(&p, x, y) + 1) >> 1;         /* i */
+    case 11: return (get_j(&p, x, y) + get_h(&p, x + 1, y) + 1) >> 1;    /* k */
+    case 5: return (get_b(&p, x, y) + get_h(&p, x, y) + 1) >> 1;         /* e */
+    case 7: return (get_b(&p, x, y) + get_h(&p, x + 1, y) + 1) >> 1;     /* g */
+    case 13: return (get_h(&p, x, y) + get_b(&p, x, y + 1) + 1) >> 1;    /* p */
+    default: return (get_h(&p, x + 1, y) + get_b(&p, x, y + 1) + 1) >> 1; /* r (15) */
+  }
+}
+
+/* QuarterRead of a 16x16 block whose origin is full-pel (x,y) with signed
+ * quarter offsets qx,qy in -3..3: negative fractions are reflected to the
+ * full-pel position to the left/above (OverlayMem2Dv2.cpp:709-724). */
+void orc_quarter_read(const int16_t *ref, int W, int H, int x, int y, int qx, int qy,
+                      int *blk) {
+  int fox = 0, fx = qx, foy = 0, fy = qy;
+  if (qx < 0) { fox = -1; fx = 4 + qx; }
+  if (qy < 0) { foy = -1; fy = 4 + qy; }
+  for (int r = 0; r < 16; r++)
+    for (int c = 0; c < 16; c++)
+      blk[r * 16 + c] = orc_qpel_sample(ref, W, H, x + c + fox, y + r + foy, fx, fy);
+}
+
+/* GetMotionRange 9-arg with zero offset (Full.cpp:1029-1081), or the
+ * exhaustive +-R variant that keeps the same picture-border clamp. */
+static void motion_range(int n, int m, int W, int H, int R, int window, int *xl,
+                         int *xr, int *yu, int *yd) {
+  int L = window ? R : R - 1;
+  const int boundary = 16; /* _extBoundary - MEH264IF_PADDING */
+  *xl = (n - L >= -boundary) ? -L : -(n + boundary);
+  *xr = (n + L < W) ? L : W - n;
+  *yu = (m - L >= -boundary) ? -L : -(m + boundary);
+  *yd = (m + L < H) ? L : H - m;
+}
+
+/* Median of three (MacroBlockH264.cpp:923-936). */
+static int median3(int x, int y, int z) {
+  int mn, mx;
+  if (y - z < 0) { mn = y; mx = z; } else { mn = z; mx = y; }
+  if (x < mn) mn = x;
+  if (x > mx) mx = x;
+  return x + y + z - mn - mx;
+}
+
+/* 16x16 median prediction for MB (col,row) from MBs already decided in this
+ * Estimate call (MacroBlockH264.cpp:757-816 with neighbours set up by
+ * MacroBlockH264::Initialise, :346-373, one slice). */
+void orc_median_pred(const int16_t *mv, int mbw, int col, int row, int *px, int *py) {
+  int Ax = 0, Ay = 0, Bx = 0, By = 0, Cx = 0, Cy = 0;
+  int hasB = row > 0, hasC = row > 0 && col + 1 < mbw;
+  if (col > 0) { const int16_t *a = mv + 2 * (row * mbw + col - 1); Ax = a[0]; Ay = a[1]; }
+  if (hasB) { const int16_t *b = mv + 2 * ((row - 1) * mbw + col); Bx = b[0]; By = b[1]; }
+  if (hasC) {
+    const int16_t *c = mv + 2 * ((row - 1) * mbw + col + 1); Cx = c[0]; Cy = c[1];
+  } else if (col > 0 && row > 0) {
+    const int16_t *d = mv + 2 * ((row - 1) * mbw + col - 1); Cx = d[0]; Cy = d[1];
+  }
+  if (!hasB && !hasC) { Bx = Ax; By = Ay; Cx = Ax; Cy = Ay; }
+  *px = median3(Ax, Bx, Cx);
+  *py = median3(Ay, By, Cy);
+}
+
+/* GetBestVector (Full.h:140-168) over the fifo [best, pred, zero, zero, zero]
+ * (Fifo::AddFirstIn on an empty fifo fills every slot, Fifo.h:72-92).
+ * Returns the fifo position chosen. */
+static int best_vector(const int *xs, const int *ys, const int *ds, int xref, int yref,
+                       int *mx, int *my) {
+  int pos = 0;
+  double cm = (double)((xs[0] - xref) * (xs[0] - xref) + (ys[0] - yref) * (ys[0] - yref));
+  double cc = ((double)ds[0] / 256.0) + (1.05 * sqrt(cm));
+  *mx = xs[0];
+  *my = ys[0];
+  for (int i = 1; i < 5; i++) {
+    double nm = (double)((xs[i] - xref) * (xs[i] - xref) + (ys[i] - yref) * (ys[i] - yref));
+    double nc = ((double)ds[i] / 256.0) + (1.05 * sqrt(nm));
+    if (nc < cc) { *mx = xs[i]; *my = ys[i]; cc = nc; pos = i; }
+  }
+  return pos;
+}
+
+/* Half-pel window values (LoadHalfQuartPelWindow, Full.cpp:1095-1182) around
+ * full-pel origin (ox,oy), on the estimator's 88x88 quarter-pel grid with
+ * full pel (3,3) at index (12,12).  Only the even/even positions used by the
+ * stages below are produced; they equal the H.264 G/b/h/j samples. */
+typedef struct { int16_t w[88][88]; } qwin_t;
+
+static void load_half_window(qwin_t *W8, const plane_t *ref, int ox, int oy) {
+  /* positions 8..76 even x even, in 1/4-pel units, relative to full pel (0,0)
+     of the window which is ref (ox-3, oy-3) */
+  for (int qr = 8; qr <= 76; qr += 2)
+    for (int qc = 8; qc <= 76; qc += 2) {
+      int fr = qr >> 2, fc = qc >> 2;
+      int x = ox - 3 + fc, y = oy - 3 + fr;
+      int hr = (qr & 2) != 0, hc = (qc & 2) != 0;
+      int v;
+      if (!hr && !hc) v = pix(ref, x, y);
+      else if (!hr && hc) v = get_b(ref, x, y);
+      else if (hr && !hc) v = get_h(ref, x, y);
+      else v = get_j(ref, x, y);
+      W8->w[qr][qc] = (int16_t)v;
+    }
+}
+
+/* LoadQuartPelWindow (Full.cpp:1198-1222): the 8 quarter positions around
+ * the winning half-pel position, each the rounded mean of two window
+ * samples exactly as the reference pairs them. */
+static void load_quarter_window(qwin_t *W8, int hx, int hy) {
+  for (int r = 0; r < 16; r++)
+    for (int c = 0; c < 16; c++) {
+      int qr = 12 + hy + 4 * r, qc = 12 + hx + 4 * c;
+      int16_t(*w)[88] = W8->w;
+      w[qr - 1][qc - 1] = (int16_t)((w[qr - 2][qc - 2] + w[qr][qc] + 1) >> 1);
+      w[qr - 1][qc] = (int16_t)((w[qr - 2][qc] + w[qr][qc] + 1) >> 1);
+      w[qr - 1][qc + 1] = (int16_t)((w[qr - 2][qc] + w[qr][qc + 2] + 1) >> 1);
+      w[qr][qc - 1] = (int16_t)((w[qr][qc - 2] + w[qr][qc] + 1) >> 1);
+      w[qr][qc + 1] = (int16_t)((w[qr][qc + 2] + w[qr][qc] + 1) >> 1);
+      w[qr + 1][qc - 1] = (int16_t)((w[qr][qc - 2] + w[qr + 2][qc] + 1) >> 1);
+      w[qr + 1][qc] = (int16_t)((w[qr + 2][qc] + w[qr][qc] + 1) >> 1);
+      w[qr + 1][qc + 1] = (int16_t)((w[qr + 2][qc] + w[qr][qc + 2] + 1) >> 1);
+    }
+}
+
+static void window_read(const qwin_t *W8, int qx, int qy, int *blk) {
+  for (int r = 0; r < 16; r++)
+    for (int c = 0; c < 16; c++) blk[r * 16 + c] = W8->w[12 + qy + 4 * r][12 + qx + 4 * c];
+}
+
+static const int SUBPOS[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0},
+                                 {1, 0},   {-1, 1}, {0, 1},  {1, 1}}; /* Full.cpp:77-81 */
+
+int orc_me_estimate(const orc_me_cfg *cfg, const int16_t *cur, const int16_t *refp,
+                    const uint8_t *include, int16_t *mv_out, int32_t *dist_out,
+                    int32_t *int_out, long *avg_out) {
+  const int W = cfg->width, H = cfg->height, R = cfg->range;
+  if (W <= 0 || H <= 0 || (W & 15) || (H & 15) || R < 1) return -1;
+  const int mbw = W / 16, mbh = H / 16;
+  plane_t ref = {refp, W, H};
+  qwin_t *win = (qwin_t *)malloc(sizeof(qwin_t));
+  if (!win) return -2;
+  memset(win, 0, sizeof(*win));
+  int blk[256];
+  long total = 0;
+  int included = 0;
+
+  for (int row = 0; row < mbh; row++)
+    for (int col = 0; col < mbw; col++) {
+      const int n = col * 16, m = row * 16, vec = row * mbw + col;
+      int predX = 0, predY = 0;
+      if (cfg->pred) orc_median_pred(mv_out, mbw, col, row, &predX, &predY);
+      int predXQ = predX % 4, predYQ = predY % 4;
+      int predX0 = predX / 4, predY0 = predY / 4;
+      int pXr = predX < 0 ? (predX - 2) / 4 : (predX + 2) / 4;
+      int pYr = predY < 0 ? (predY - 2) / 4 : (predY + 2) / 4;
+
+      read_full(&ref, n, m, blk);
+      const int zeroD = blk_dist(cur, W, n, m, blk, cfg->metric);
+      int minD = zeroD, mx = 0, my = 0;
+
+      int xl, xr, yu, yd;
+      motion_range(n, m, W, H, R, cfg->window, &xl, &xr, &yu, &yd);
+      for (int i = yu; i <= yd; i++)
+        for (int j = xl; j <= xr; j++) {
+          if (!(i || j)) continue;
+          if (cfg->filter &&
+              ((i + my) < yu || (i + my) > yd || (j + mx) < xl || (j + mx) > xr))
+            continue;
+          read_full(&ref, n + j, m + i, blk);
+          int d = blk_dist(cur, W, n, m, blk, cfg->metric);
+          if (d <= minD) {
+            if (d == minD) {
+              int cx = mx - pXr, cy = my - pYr, nx = j - pXr, ny = i - pYr;
+              if ((cy * cy + cx * cx) - (ny * ny + nx * nx) < 0) continue;
+            }
+            minD = d; mx = j; my = i;
+          }
+        }
+      if (int_out) {
+        int_out[3 * vec + 0] = mx;
+        int_out[3 * vec + 1] = my;
+        int_out[3 * vec + 2] = minD;
+      }
+
+      int mvx = mx * 4, mvy = my * 4, hmx = 0, hmy = 0, qmx = 0, qmy = 0;
+      if (cfg->subpel >= 1) {
+        load_half_window(win, &ref, n + mx, m + my);
+        for (int k = 0; k < 8; k++) {
+          int ox = 2 * SUBPOS[k][0], oy = 2 * SUBPOS[k][1];
+          window_read(win, ox, oy, blk);
+          int d = blk_dist(cur, W, n, m, blk, cfg->metric);
+          if (d < minD) { minD = d; hmx = ox; hmy = oy; }
+        }
+        qmx = hmx; qmy = hmy;
+        if (cfg->subpel >= 2) {
+          load_quarter_window(win, hmx, hmy);
+          for (int k = 0; k < 8; k++) {
+            int ox = hmx + SUBPOS[k][0], oy = hmy + SUBPOS[k][1];
+            window_read(win, ox, oy, blk);
+            int d = blk_dist(cur, W, n, m, blk, cfg->metric);
+            if (d < minD) { minD = d; qmx = ox; qmy = oy; }
+          }
+        }
+      }
+      mvx += qmx;
+      mvy += qmy;
+
+      int outx = mvx, outy = mvy, outd = minD;
+      if (cfg->bestvec) {
+        if (predX0 + n > W) predX0 = 16;
+        if (predX0 + n < -16) predX0 = -16;
+        if (predY0 + m > H) predY0 = 16;
+        if (predY0 + m < -16) predY0 = -16;
+        predX = predX0 * 4 + predXQ;
+        predY = predY0 * 4 + predYQ;
+        if (predXQ || predYQ)
+          orc_quarter_read(refp, W, H, predX0 + n, predY0 + m, predXQ, predYQ, blk);
+        else
+          read_full(&ref, predX0 + n, predY0 + m, blk);
+        int predD = blk_dist(cur, W, n, m, blk, cfg->metric);
+        int xs[5] = {mvx, predX, 0, 0, 0}, ys[5] = {mvy, predY, 0, 0, 0};
+        int ds[5] = {minD, predD, zeroD, zeroD, zeroD};
+        int pos = best_vector(xs, ys, ds, predX, predY, &outx, &outy);
+        outd = ds[pos];
+      }
+      if (include && include[vec]) { included++; total += outd; }
+      mv_out[2 * vec] = (int16_t)outx;
+      mv_out[2 * vec + 1] = (int16_t)outy;
+      if (dist_out) dist_out[vec] = outd;
+    }
+  free(win);
+  if (avg_out) *avg_out = included ? total / included : 0;
+  return 0;
+}
+
+/* Tsd16x16 (metric 0) / Tad16x16 (metric 1) between two 16x16 blocks. */
+int orc_block_dist(const int16_t *a256, const int16_t *b256, int metric) {
+  int blk[256];
+  for (int i = 0; i < 256; i++) blk[i] = b256[i];
+  return blk_dist(a256, 16, 0, 0, blk, metric);
+}

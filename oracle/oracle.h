@@ -1,0 +1,59 @@
+/*
+ * CPU oracle for the motion-estimation and 4x4 transform hot path.
+ * TEST INFRASTRUCTURE ONLY (see me_oracle.c / it_oracle.c headers).
+ */
+#ifndef ICU_ORACLE_H
+#define ICU_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int width, height; /* luma, multiples of 16 */
+  int range;         /* full-pel search range R (reference: motionRange/4) */
+  int metric;        /* 0 SSD (as shipped), 1 SAD */
+  int window;        /* 0 reference rule +-(R-1), 1 exhaustive +-R */
+  int filter;        /* 1 running-best candidate filter (as shipped), 0 off */
+  int subpel;        /* 0 integer, 1 half, 2 half+quarter (as shipped) */
+  int pred;          /* 0 zero predictor, 1 H.264 median (as shipped) */
+  int bestvec;       /* 1 GetBestVector (as shipped), 0 off */
+} orc_me_cfg;
+
+/* mv_out: 2*nMB int16 quarter-pel (x,y) raster order.  dist_out: nMB
+ * distortion of the chosen vector (nullable).  int_out: 3*nMB integer-stage
+ * (mx, my, minDiff) (nullable).  include: nMB flags (nullable). */
+int orc_me_estimate(const orc_me_cfg *cfg, const int16_t *cur, const int16_t *ref,
+                    const uint8_t *include, int16_t *mv_out, int32_t *dist_out,
+                    int32_t *int_out, long *avg_out);
+int orc_qpel_sample(const int16_t *ref, int W, int H, int x, int y, int fx, int fy);
+int orc_block_dist(const int16_t *a256, const int16_t *b256, int metric);
+void orc_quarter_read(const int16_t *ref, int W, int H, int x, int y, int qx, int qy,
+                      int *blk);
+void orc_median_pred(const int16_t *mv, int mbw, int col, int row, int *px, int *py);
+
+/* 4x4 transforms.  mode: 0 IT+Q, 1 IT only, 2 Q only. */
+void orc_fwd4x4(int16_t *blk, int qp, int intra, int mode);
+void orc_inv4x4(int16_t *blk, int qp, int mode, const int *weight16);
+void orc_inv4x4_colsfirst(int16_t *blk, int qp, int mode, const int *weight16);
+/* Batched over n contiguous 4x4 blocks (16 int16 each). */
+void orc_fwd4x4_n(int16_t *blks, long n, int qp, int intra, int mode);
+void orc_inv4x4_n(int16_t *blks, long n, int qp, int mode, int cols_first);
+/* Whole 16x16 raster macroblocks (FastForward4x4On16x16ITImpl1 layout). */
+void orc_fwd_mb_n(int16_t *mbs, long n, int qp, int intra, int mode);
+void orc_inv_mb_n(int16_t *mbs, long n, int qp, int mode, int cols_first);
+
+/* Luma motion compensation of a whole frame from a quarter-pel MV list
+ * (MotionCompensatorH264ImplStd::Compensate, luma part). pred_out W*H. */
+void orc_mc_luma(const int16_t *ref, int W, int H, const int16_t *mv, int16_t *pred_out);
+
+/* Residual -> fwd IT+Q -> levels; levels -> dequant+inverse -> recon with
+ * clip, per 4x4 of each 16x16 MB.  levels_out W*H in MB raster layout. */
+void orc_code_residual(const int16_t *cur, const int16_t *pred, int W, int H, int qp,
+                       int intra, int cols_first, int16_t *levels_out, int16_t *recon_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

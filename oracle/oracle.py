@@ -1,0 +1,169 @@
+"""ctypes front of the C oracle (liboracle.so) and, where it has been built,
+the reference shim (_ref/libvppref.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, the smoke entry point and the
+CPU-baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libvppref.so"
+
+_i16p = np.ctypeslib.ndpointer(np.int16, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+
+
+class MeCfg(C.Structure):
+    _fields_ = [(n, C.c_int) for n in
+                ("width", "height", "range", "metric", "window", "filter", "subpel", "pred", "bestvec")]
+
+
+AS_SHIPPED = dict(metric=0, window=0, filter=1, subpel=2, pred=1, bestvec=1)
+
+
+def build(force: bool = False) -> None:
+    if force or not LIB.exists():
+        subprocess.run(["make", "-s", "-C", str(HERE), "all"], check=True)
+
+
+def build_ref() -> bool:
+    """Compile the reference sources into _ref/ (only where they are present)."""
+    if not Path(os.environ.get("ICU_REFERENCE", "/root/reference")).is_dir():
+        return False
+    subprocess.run(["make", "-s", "-C", str(HERE), "ref",
+                    "REF=" + os.environ.get("ICU_REFERENCE", "/root/reference")], check=True)
+    return REF_LIB.exists()
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(str(LIB))
+        L.orc_me_estimate.argtypes = [C.POINTER(MeCfg), _i16p, _i16p, C.c_void_p, _i16p,
+                                      C.c_void_p, C.c_void_p, C.POINTER(C.c_long)]
+        L.orc_me_estimate.restype = C.c_int
+        L.orc_quarter_read.argtypes = [_i16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.c_int, _i32p]
+        L.orc_fwd4x4_n.argtypes = [_i16p, C.c_long, C.c_int, C.c_int, C.c_int]
+        L.orc_inv4x4_n.argtypes = [_i16p, C.c_long, C.c_int, C.c_int, C.c_int]
+        L.orc_fwd_mb_n.argtypes = [_i16p, C.c_long, C.c_int, C.c_int, C.c_int]
+        L.orc_inv_mb_n.argtypes = [_i16p, C.c_long, C.c_int, C.c_int, C.c_int]
+        L.orc_inv4x4.argtypes = [_i16p, C.c_int, C.c_int, C.c_void_p]
+        L.orc_mc_luma.argtypes = [_i16p, C.c_int, C.c_int, _i16p, _i16p]
+        L.orc_block_dist.argtypes = [_i16p, _i16p, C.c_int]
+        L.orc_code_residual.argtypes = [_i16p, _i16p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, _i16p, _i16p]
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The compiled reference shim, or None where it was not built."""
+    global _ref
+    if _ref is None and REF_LIB.exists():
+        L = C.CDLL(str(REF_LIB))
+        L.ref_me_estimate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i16p, _i16p,
+                                      C.c_void_p, _i16p, C.POINTER(C.c_long)]
+        L.ref_me_time.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i16p, _i16p, C.c_int]
+        L.ref_me_time.restype = C.c_double
+        L.ref_fwd.argtypes = [C.c_int, _i16p, C.c_long, C.c_int, C.c_int, C.c_int]
+        L.ref_inv.argtypes = [C.c_int, _i16p, C.c_long, C.c_int, C.c_int, C.c_void_p]
+        L.ref_quarter_read.argtypes = [_i16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.c_int, _i16p]
+        L.ref_block_dist.argtypes = [_i16p, _i16p, C.c_int]
+        L.ref_block_dist_lt.argtypes = [_i16p, _i16p, C.c_int, C.c_int]
+        L.ref_mc_luma.argtypes = [_i16p, C.c_int, C.c_int, C.c_int, _i16p, _i16p]
+        _ref = L
+    return _ref
+
+
+def me_estimate(cur, ref, R, include=None, **modes):
+    """Run the oracle estimator. Returns (mv (nMB,2) int16, dist (nMB,) int32,
+    integer stage (nMB,3) int32, avg)."""
+    cur = np.ascontiguousarray(cur, dtype=np.int16)
+    ref = np.ascontiguousarray(ref, dtype=np.int16)
+    H, W = cur.shape
+    m = dict(AS_SHIPPED)
+    m.update(modes)
+    cfg = MeCfg(W, H, R, m["metric"], m["window"], m["filter"], m["subpel"], m["pred"], m["bestvec"])
+    n = (W // 16) * (H // 16)
+    mv = np.zeros(2 * n, np.int16)
+    dist = np.zeros(n, np.int32)
+    integ = np.zeros(3 * n, np.int32)
+    avg = C.c_long(0)
+    inc = None
+    if include is not None:
+        include = np.ascontiguousarray(include, dtype=np.uint8)
+        inc = include.ctypes.data
+    rc = lib().orc_me_estimate(C.byref(cfg), cur, ref, inc, mv, dist.ctypes.data,
+                               integ.ctypes.data, C.byref(avg))
+    if rc != 0:
+        raise ValueError(f"oracle me_estimate failed ({rc})")
+    return mv.reshape(n, 2), dist, integ.reshape(n, 3), avg.value
+
+
+def fwd4x4(blocks, qp, intra, mode):
+    b = np.ascontiguousarray(blocks, dtype=np.int16).copy()
+    lib().orc_fwd4x4_n(b, b.size // 16, qp, intra, mode)
+    return b
+
+
+def inv4x4(blocks, qp, mode, cols_first=0):
+    b = np.ascontiguousarray(blocks, dtype=np.int16).copy()
+    lib().orc_inv4x4_n(b, b.size // 16, qp, mode, cols_first)
+    return b
+
+
+def fwd_mb(mbs, qp, intra, mode):
+    b = np.ascontiguousarray(mbs, dtype=np.int16).copy()
+    lib().orc_fwd_mb_n(b, b.size // 256, qp, intra, mode)
+    return b
+
+
+def inv_mb(mbs, qp, mode, cols_first=1):
+    b = np.ascontiguousarray(mbs, dtype=np.int16).copy()
+    lib().orc_inv_mb_n(b, b.size // 256, qp, mode, cols_first)
+    return b
+
+
+def quarter_read(ref, x, y, qx, qy):
+    ref = np.ascontiguousarray(ref, dtype=np.int16)
+    out = np.zeros(256, np.int32)
+    lib().orc_quarter_read(ref, ref.shape[1], ref.shape[0], x, y, qx, qy, out)
+    return out.reshape(16, 16)
+
+
+def mc_luma(ref, mv):
+    ref = np.ascontiguousarray(ref, dtype=np.int16)
+    mv = np.ascontiguousarray(mv, dtype=np.int16).reshape(-1)
+    out = np.zeros_like(ref)
+    lib().orc_mc_luma(ref, ref.shape[1], ref.shape[0], mv, out)
+    return out
+
+
+def code_residual(cur, pred, qp, intra=0, cols_first=0):
+    cur = np.ascontiguousarray(cur, dtype=np.int16)
+    pred = np.ascontiguousarray(pred, dtype=np.int16)
+    lv = np.zeros_like(cur)
+    rc = np.zeros_like(cur)
+    lib().orc_code_residual(cur, pred, cur.shape[1], cur.shape[0], qp, intra, cols_first, lv, rc)
+    return lv, rc
+
+
+def block_dist(a, b, metric):
+    return lib().orc_block_dist(np.ascontiguousarray(a, dtype=np.int16).reshape(-1),
+                                np.ascontiguousarray(b, dtype=np.int16).reshape(-1), metric)

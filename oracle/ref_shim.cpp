@@ -1,0 +1,199 @@
+// C entry points over the compiled reference classes, used only to write the
+// golden fixtures and to time the reference CPU path.  TEST INFRASTRUCTURE:
+// built by oracle/Makefile into oracle/_ref/ from the reference sources where
+// they lie; nothing in the product links it.
+//
+// Classes driven (reference file:line):
+//   MotionEstimatorH264ImplFull          src/CodecUtils/MotionEstimatorH264ImplFull.cpp:111-611
+//   H264MotionVectorPredictorImpl1       include/CodecUtils/H264MotionVectorPredictorImpl1.h:54-132
+//   MacroBlockH264::Initialise           src/CodecUtils/MacroBlockH264.cpp:318-400
+//   FastForward4x4ITImpl1/ITImpl2/On16x16ITImpl1, FastInverse4x4ITImpl1/On16x16ITImpl1
+//   OverlayMem2Dv2 / OverlayExtMem2Dv2   src/ImageUtils/OverlayMem2Dv2.cpp:704,1623,2655
+//   MotionCompensatorH264ImplStd         src/CodecUtils/MotionCompensatorH264ImplStd.cpp:323-473
+#include <math.h>  // the estimator header calls sqrt without including it
+
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "FastForward4x4ITImpl1.h"
+#include "FastForward4x4ITImpl2.h"
+#include "FastForward4x4On16x16ITImpl1.h"
+#include "FastInverse4x4ITImpl1.h"
+#include "FastInverse4x4On16x16ITImpl1.h"
+#include "H264MotionVectorPredictorImpl1.h"
+#include "MacroBlockH264.h"
+#include "MotionCompensatorH264ImplStd.h"
+#include "MotionEstimatorH264ImplFull.h"
+#include "OverlayExtMem2Dv2.h"
+#include "OverlayMem2Dv2.h"
+#include "VectorStructList.h"
+
+namespace {
+
+// A predictor that always predicts the zero vector (the GPU bench mode).
+class ZeroPredictor : public IMotionVectorPredictor {
+ public:
+  int Create(void) override { return 1; }
+  int Get16x16Prediction(void*, int, int* px, int* py) override {
+    *px = 0;
+    *py = 0;
+    return 1;
+  }
+};
+
+struct MeRun {
+  int W, H, nmb;
+  std::vector<short> cur, ref;
+  std::vector<MacroBlockH264> mbs;
+  std::vector<MacroBlockH264*> rows;
+  IMotionVectorPredictor* pred = nullptr;
+  MotionEstimatorH264ImplFull* me = nullptr;
+  ~MeRun() {
+    delete me;
+    delete pred;
+  }
+};
+
+MeRun* me_setup(int W, int H, int motionRange, int pred_mode, const short* cur,
+                const short* ref, const unsigned char* include) {
+  MeRun* r = new MeRun();
+  r->W = W;
+  r->H = H;
+  r->nmb = (W / 16) * (H / 16);
+  r->cur.assign(cur, cur + (long)W * H);
+  r->ref.assign(ref, ref + (long)W * H);
+  if (pred_mode) {
+    int mbw = W / 16, mbh = H / 16;
+    r->mbs.resize(r->nmb);
+    r->rows.resize(mbh);
+    for (int i = 0; i < mbh; i++) r->rows[i] = &r->mbs[i * mbw];
+    MacroBlockH264::Initialise(mbh, mbw, 0, r->nmb - 1, 0, r->rows.data());
+    r->pred = new H264MotionVectorPredictorImpl1(r->mbs.data());
+  } else {
+    r->pred = new ZeroPredictor();
+  }
+  if (include)
+    r->me = new MotionEstimatorH264ImplFull(r->cur.data(), r->ref.data(), W, H, motionRange,
+                                            r->pred, (void*)include);
+  else
+    r->me = new MotionEstimatorH264ImplFull(r->cur.data(), r->ref.data(), W, H, motionRange,
+                                            r->pred);
+  if (!r->me->Create()) {
+    delete r;
+    return nullptr;
+  }
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+// One Estimate() of the as-shipped class. include: bool per MB (nullable).
+int ref_me_estimate(int W, int H, int motionRange, int pred_mode, const short* cur,
+                    const short* ref, const unsigned char* include, short* mv_out,
+                    long* avg_out) {
+  static_assert(sizeof(bool) == 1, "bool include map");
+  MeRun* r = me_setup(W, H, motionRange, pred_mode, cur, ref, include);
+  if (!r) return -1;
+  long avg = 0;
+  VectorStructList* l = (VectorStructList*)r->me->Estimate(&avg);
+  for (int i = 0; i < r->nmb; i++) {
+    mv_out[2 * i] = (short)l->GetSimpleElement(i, 0);
+    mv_out[2 * i + 1] = (short)l->GetSimpleElement(i, 1);
+  }
+  if (avg_out) *avg_out = avg;
+  delete r;
+  return 0;
+}
+
+// Times `reps` Estimate() calls on one constructed estimator; returns seconds.
+double ref_me_time(int W, int H, int motionRange, int pred_mode, const short* cur,
+                   const short* ref, int reps) {
+  MeRun* r = me_setup(W, H, motionRange, pred_mode, cur, ref, nullptr);
+  if (!r) return -1.0;
+  long avg = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < reps; i++) r->me->Estimate(&avg);
+  auto t1 = std::chrono::steady_clock::now();
+  delete r;
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// variant 0 = FastForward4x4ITImpl1, 1 = FastForward4x4ITImpl2 (n 4x4 blocks,
+// 16 shorts each); 2 = FastForward4x4On16x16ITImpl1 (n 16x16 raster MBs).
+void ref_fwd(int variant, short* data, long n, int qp, int intra, int mode) {
+  IForwardTransform* t = variant == 0 ? (IForwardTransform*)new FastForward4x4ITImpl1()
+                         : variant == 1 ? (IForwardTransform*)new FastForward4x4ITImpl2()
+                                        : (IForwardTransform*)new FastForward4x4On16x16ITImpl1();
+  t->SetParameter(IForwardTransform::INTRA_FLAG_ID, intra);
+  t->SetParameter(IForwardTransform::QUANT_ID, qp);
+  t->SetMode(mode);
+  const long step = variant == 2 ? 256 : 16;
+  for (long i = 0; i < n; i++) t->Transform(data + step * i);
+  delete t;
+}
+
+// variant 0 = FastInverse4x4ITImpl1 (4x4 blocks), 1 = FastInverse4x4On16x16ITImpl1
+// (16x16 raster MBs). weight16: optional SetScale array.
+void ref_inv(int variant, short* data, long n, int qp, int mode, const int* weight16) {
+  IInverseTransform* t = variant == 0 ? (IInverseTransform*)new FastInverse4x4ITImpl1()
+                                      : (IInverseTransform*)new FastInverse4x4On16x16ITImpl1();
+  if (weight16) t->SetScale((void*)weight16);
+  t->SetParameter(IInverseTransform::QUANT_ID, qp);
+  t->SetMode(mode);
+  const long step = variant == 1 ? 256 : 16;
+  for (long i = 0; i < n; i++) t->InverseTransform(data + step * i);
+  delete t;
+}
+
+// OverlayMem2Dv2::QuarterRead of a 16x16 block at full-pel (x,y) of a
+// reference padded by 19 (OverlayExtMem2Dv2), quarter offsets qx,qy in -3..3.
+void ref_quarter_read(const short* ref, int W, int H, int x, int y, int qx, int qy,
+                      short* out256) {
+  short* ext = nullptr;
+  OverlayExtMem2Dv2::ExtendBoundary((void*)ref, W, H, 19, 19, (void**)&ext);
+  OverlayExtMem2Dv2 ov(ext, W + 38, H + 38, 16, 16, 19, 19);
+  ov.SetOrigin(x, y);
+  short blk[256];
+  OverlayMem2Dv2 dst(blk, 16, 16, 16, 16);
+  ov.QuarterRead(dst, qx, qy);
+  std::memcpy(out256, blk, sizeof(blk));
+  delete[] ext;
+}
+
+// Tsd16x16 (metric 0) or Tad16x16 (metric 1) between two 16x16 blocks.
+int ref_block_dist(const short* a256, const short* b256, int metric) {
+  OverlayMem2Dv2 a((void*)a256, 16, 16, 16, 16), b((void*)b256, 16, 16, 16, 16);
+  return metric ? a.Tad16x16(b) : a.Tsd16x16(b);
+}
+
+// Tsd16x16LessThan / Tad16x16LessThan with an explicit bound.
+int ref_block_dist_lt(const short* a256, const short* b256, int metric, int min) {
+  OverlayMem2Dv2 a((void*)a256, 16, 16, 16, 16), b((void*)b256, 16, 16, 16, 16);
+  return metric ? a.Tad16x16LessThan(b, min) : a.Tsd16x16LessThan(b, min);
+}
+
+// Luma part of MotionCompensatorH264ImplStd::Compensate(list): the
+// prediction is written into (a copy of) the reference luma plane.
+int ref_mc_luma(const short* ref, int W, int H, int range, const short* mv, short* pred_out) {
+  long lum = (long)W * H, chr = lum / 4;
+  std::vector<short> yuv(lum + 2 * chr, 128);
+  std::memcpy(yuv.data(), ref, lum * sizeof(short));
+  MotionCompensatorH264ImplStd mc(range);
+  if (!mc.Create(yuv.data(), W, H, 16, 16)) return -1;
+  int nmb = (W / 16) * (H / 16);
+  VectorStructList l(VectorStructList::SIMPLE2D);
+  l.SetLength(nmb);
+  for (int i = 0; i < nmb; i++) {
+    l.SetSimpleElement(i, 0, mv[2 * i]);
+    l.SetSimpleElement(i, 1, mv[2 * i + 1]);
+  }
+  mc.Compensate(&l);
+  std::memcpy(pred_out, yuv.data(), lum * sizeof(short));
+  return 0;
+}
+
+}  // extern "C"
