@@ -1,0 +1,310 @@
+"""ctypes binding of libicu_hip.so (the C ABI in include/icu_hip.h).
+
+There is no CPU fallback: importing this module loads the in-tree HIP
+library and raises if it is missing.  Device buffers are managed through the
+library's own small runtime (icu_malloc / icu_memcpy_*), so no framework is
+needed to drive it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libicu_hip.so"
+HEADER = Path(__file__).resolve().parents[1] / "include" / "icu_hip.h"
+
+ICU_OK, ICU_E_ARG, ICU_E_RANGE, ICU_E_DEVICE, ICU_E_STATE = 0, -1, -2, -3, -4
+
+
+class IcuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"icu error {code}: {msg}")
+        self.code = code
+
+
+class MeConfig(C.Structure):
+    _fields_ = [(n, C.c_int) for n in (
+        "width", "height", "range", "metric", "window", "filter", "subpel",
+        "predictor", "best_vector", "tile_rows", "mbs_per_task")]
+
+
+_vp = C.c_void_p
+_SIGS = {
+    "icu_me_config_reference": (C.c_int, [C.POINTER(MeConfig), C.c_int, C.c_int, C.c_int]),
+    "icu_me_create": (C.c_int, [C.POINTER(MeConfig), C.POINTER(_vp)]),
+    "icu_me_destroy": (C.c_int, [_vp]),
+    "icu_me_estimate": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_long)]),
+    "icu_me_estimate_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "icu_me_estimate_timed": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_float)]),
+    "icu_me_get_tiling": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "icu_me_autotune": (C.c_int, [_vp, _vp, _vp, C.c_int]),
+    "icu_code_frame_device": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "icu_fwd4x4": (C.c_int, [_vp, _vp, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "icu_inv4x4": (C.c_int, [_vp, _vp, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
+    "icu_fwd4x4_device": (C.c_int, [_vp, _vp, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
+    "icu_inv4x4_device": (C.c_int, [_vp, _vp, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp]),
+    "icu_me_batch_create": (C.c_int, [C.POINTER(MeConfig), C.c_int, C.POINTER(_vp)]),
+    "icu_me_batch_destroy": (C.c_int, [_vp]),
+    "icu_me_batch_num_mbs": (C.c_int, [_vp]),
+    "icu_me_batch_mb_base": (C.c_int, [_vp, C.c_int]),
+    "icu_me_batch_estimate_device": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), _vp, _vp, _vp]),
+    "icu_me_batch_capture": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), _vp, _vp]),
+    "icu_me_batch_graph_launch": (C.c_int, [_vp, _vp]),
+    "icu_last_error": (C.c_char_p, []),
+    "icu_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "icu_malloc": (_vp, [C.c_size_t]),
+    "icu_free": (C.c_int, [_vp]),
+    "icu_memcpy_htod": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    "icu_memcpy_dtoh": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    "icu_memset": (C.c_int, [_vp, C.c_int, C.c_size_t, _vp]),
+    "icu_stream_create": (_vp, []),
+    "icu_stream_destroy": (C.c_int, [_vp]),
+    "icu_stream_sync": (C.c_int, [_vp]),
+    "icu_event_create": (_vp, []),
+    "icu_event_destroy": (C.c_int, [_vp]),
+    "icu_event_record": (C.c_int, [_vp, _vp]),
+    "icu_event_elapsed_ms": (C.c_int, [_vp, _vp, C.POINTER(C.c_float)]),
+    "icu_sqrt_check_device": (C.c_int, [_vp, _vp, C.c_int, _vp]),
+}
+
+
+def header_symbols() -> list[str]:
+    """Every function the public header declares."""
+    import re
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"\b(icu_[a-z0-9_]+)\s*\(", text)))
+
+
+def _load():
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built: run python -m imagecodingutils_amd.build")
+    L = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in _SIGS.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+def _check(rc: int) -> None:
+    if rc != ICU_OK:
+        raise IcuError(rc, lib.icu_last_error().decode())
+
+
+# ------------------------------------------------------------ device memory
+class DeviceBuffer:
+    """A device allocation owned by the library's runtime."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = lib.icu_malloc(max(1, self.nbytes))
+        if not self.ptr:
+            raise IcuError(ICU_E_DEVICE, lib.icu_last_error().decode())
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray, stream=None) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(lib.icu_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes, stream))
+
+    def download(self, dtype, shape, stream=None) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib.icu_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes, stream))
+        return out
+
+    def zero(self) -> None:
+        _check(lib.icu_memset(self.ptr, 0, self.nbytes, None))
+        self.sync()
+
+    @staticmethod
+    def sync(stream=None) -> None:
+        _check(lib.icu_stream_sync(stream))
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib.icu_free(self.ptr)
+            self.ptr = None
+
+
+class Stream:
+    def __init__(self):
+        self.handle = lib.icu_stream_create()
+        if not self.handle:
+            raise IcuError(ICU_E_DEVICE, lib.icu_last_error().decode())
+
+    def sync(self) -> None:
+        _check(lib.icu_stream_sync(self.handle))
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.icu_stream_destroy(self.handle)
+            self.handle = None
+
+
+class Event:
+    def __init__(self):
+        self.handle = lib.icu_event_create()
+
+    def record(self, stream: Stream | None) -> None:
+        _check(lib.icu_event_record(self.handle, stream.handle if stream else None))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float(0)
+        _check(lib.icu_event_elapsed_ms(self.handle, end.handle, C.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib.icu_event_destroy(self.handle)
+            self.handle = None
+
+
+def device_info() -> dict:
+    name = C.create_string_buffer(256)
+    cus, clk = C.c_int(0), C.c_int(0)
+    _check(lib.icu_device_info(name, 256, C.byref(cus), C.byref(clk)))
+    return {"name": name.value.decode(), "compute_units": cus.value, "clock_khz": clk.value}
+
+
+# --------------------------------------------------------------- estimator
+def reference_config(width: int, height: int, motion_range_qpel: int) -> MeConfig:
+    cfg = MeConfig()
+    _check(lib.icu_me_config_reference(C.byref(cfg), width, height, motion_range_qpel))
+    return cfg
+
+
+def make_config(width, height, R, *, metric=0, window=0, filt=1, subpel=2, predictor=1,
+                best_vector=1, tile_rows=0, mbs_per_task=0) -> MeConfig:
+    return MeConfig(width, height, R, metric, window, filt, subpel, predictor, best_vector,
+                    tile_rows, mbs_per_task)
+
+
+class MotionEstimator:
+    """One estimator handle (icu_me)."""
+
+    def __init__(self, cfg: MeConfig):
+        self.cfg = cfg
+        h = _vp()
+        _check(lib.icu_me_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.nmb = (cfg.width // 16) * (cfg.height // 16)
+
+    def estimate(self, cur: np.ndarray, ref: np.ndarray, include=None):
+        """Host int16 planes -> (mv (nMB,2) int16, dist (nMB,) int32, avg)."""
+        cur = np.ascontiguousarray(cur, dtype=np.int16)
+        ref = np.ascontiguousarray(ref, dtype=np.int16)
+        mv = np.zeros((self.nmb, 2), np.int16)
+        dist = np.zeros(self.nmb, np.int32)
+        avg = C.c_long(0)
+        inc = None
+        if include is not None:
+            include = np.ascontiguousarray(include, dtype=np.uint8)
+            inc = include.ctypes.data
+        _check(lib.icu_me_estimate(self.h, cur.ctypes.data, ref.ctypes.data, inc, mv.ctypes.data,
+                                   dist.ctypes.data, C.byref(avg)))
+        return mv, dist, avg.value
+
+    def estimate_device(self, d_cur, d_ref, d_mv, d_dist, d_int=None, stream: Stream | None = None):
+        _check(lib.icu_me_estimate_device(self.h, _p(d_cur), _p(d_ref), _p(d_mv), _p(d_dist),
+                                          _p(d_int), stream.handle if stream else None))
+
+    def estimate_timed(self, d_cur, d_ref, d_mv, d_dist, stream: Stream | None = None):
+        """Returns per-stage device times in ms: (pad, search, decide)."""
+        ms = (C.c_float * 3)()
+        _check(lib.icu_me_estimate_timed(self.h, _p(d_cur), _p(d_ref), _p(d_mv), _p(d_dist),
+                                         stream.handle if stream else None, ms))
+        return tuple(float(x) for x in ms)
+
+    def code_frame_device(self, d_cur, d_mv, qp, intra, cols_first, d_levels, d_recon, d_pred=None,
+                          stream: Stream | None = None):
+        _check(lib.icu_code_frame_device(self.h, _p(d_cur), _p(d_mv), qp, intra, cols_first,
+                                         _p(d_levels), _p(d_recon), _p(d_pred),
+                                         stream.handle if stream else None))
+
+    def tiling(self):
+        v, g = C.c_int(0), C.c_int(0)
+        _check(lib.icu_me_get_tiling(self.h, C.byref(v), C.byref(g)))
+        return v.value, g.value
+
+    def autotune(self, d_cur, d_ref, reps=3):
+        _check(lib.icu_me_autotune(self.h, _p(d_cur), _p(d_ref), reps))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.icu_me_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _p(x):
+    if x is None:
+        return None
+    return x.ptr if isinstance(x, DeviceBuffer) else x
+
+
+class MotionEstimatorBatch:
+    """Many streams of mixed size / range estimated by one launch sequence."""
+
+    def __init__(self, cfgs: list[MeConfig]):
+        arr = (MeConfig * len(cfgs))(*cfgs)
+        h = _vp()
+        _check(lib.icu_me_batch_create(arr, len(cfgs), C.byref(h)))
+        self.h = h
+        self.n = len(cfgs)
+        self.total_mbs = lib.icu_me_batch_num_mbs(h)
+        self.bases = [lib.icu_me_batch_mb_base(h, i) for i in range(self.n)]
+
+    def _ptrs(self, bufs):
+        return (_vp * self.n)(*[_p(b) for b in bufs])
+
+    def estimate_device(self, curs, refs, d_mv, d_dist, stream: Stream | None = None):
+        _check(lib.icu_me_batch_estimate_device(self.h, self._ptrs(curs), self._ptrs(refs), _p(d_mv),
+                                                _p(d_dist), stream.handle if stream else None))
+
+    def capture(self, curs, refs, d_mv, d_dist):
+        _check(lib.icu_me_batch_capture(self.h, self._ptrs(curs), self._ptrs(refs), _p(d_mv), _p(d_dist)))
+
+    def graph_launch(self, stream: Stream | None = None):
+        _check(lib.icu_me_batch_graph_launch(self.h, stream.handle if stream else None))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.icu_me_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+# --------------------------------------------------------------- transforms
+def fwd4x4(blocks: np.ndarray, qp: int, intra: int, mode: int, layout: int = 0) -> np.ndarray:
+    b = np.ascontiguousarray(blocks, dtype=np.int16)
+    out = np.empty_like(b)
+    _check(lib.icu_fwd4x4(b.ctypes.data, out.ctypes.data, b.size // 16, qp, intra, mode, layout))
+    return out
+
+
+def inv4x4(blocks: np.ndarray, qp: int, mode: int, layout: int = 0, cols_first: int = 0,
+           weights=None) -> np.ndarray:
+    b = np.ascontiguousarray(blocks, dtype=np.int16)
+    out = np.empty_like(b)
+    w = None
+    if weights is not None:
+        weights = np.ascontiguousarray(weights, dtype=np.int32)
+        w = weights.ctypes.data
+    _check(lib.icu_inv4x4(b.ctypes.data, out.ctypes.data, b.size // 16, qp, mode, layout, cols_first, w))
+    return out

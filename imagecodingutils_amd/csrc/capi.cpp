@@ -1,0 +1,808 @@
+// C ABI over the HIP kernels: estimator handles, batched estimation with
+// graph capture, the tile tuner, transforms and a minimal device runtime.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/icu_hip.h"
+#include "icu_device.h"
+#include "icu_internal.h"
+
+using namespace icu;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail(ICU_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_));        \
+  } while (0)
+
+// ------------------------------------------------------------------ tiling
+struct Tiling {
+  int V, B, Q, G;
+};
+
+int quads_for(int L) { return (L - tile_dx0(L)) / 4 + 1; }
+
+// Lane tiles: rows V per lane so that B*V covers the 2L+1 window rows with
+// the least waste; MBs per task G so that G*Q*B lanes fill whole 256-lane
+// passes and the staged window fits in LDS.
+Tiling plan_tiling(int mbw, int L, int metric, int V_req, int G_req) {
+  Tiling t{};
+  const int N = 2 * L + 1;
+  t.Q = quads_for(L);
+  static const int kV[1] = {8};
+  int bestV = 8, bestWaste = 1 << 30;
+  for (int v : kV) {
+    if (metric == ICU_METRIC_SSD && v != 8) continue;
+    if (V_req && v != V_req) continue;
+    const int waste = ((N + v - 1) / v) * v - N;
+    if (waste < bestWaste || (waste == bestWaste && v > bestV)) {
+      bestWaste = waste;
+      bestV = v;
+    }
+  }
+  t.V = bestV;
+  t.B = (N + t.V - 1) / t.V;
+  const int U = t.Q * t.B;
+  const int gmax = std::min(search_max_group(), mbw);
+  int bestG = 1;
+  double bestEff = -1;
+  for (int g = 1; g <= gmax; g++) {
+    const int rows = t.B * t.V + 15, row_dw = 4 * (g - 1) + t.Q + 4;
+    if (rows * row_dw > search_max_window_dwords()) break;
+    if (G_req && g != G_req) continue;
+    const int lanes = g * U;
+    const double eff = (double)lanes / (256.0 * ((lanes + 255) / 256));
+    // prefer full passes; among equals prefer at most 4 passes per task
+    const double score = eff - (lanes > 1024 ? 0.01 * (lanes / 1024) : 0.0);
+    if (score > bestEff + 1e-9) {
+      bestEff = score;
+      bestG = g;
+    }
+  }
+  t.G = bestG;
+  return t;
+}
+
+bool valid_cfg(const icu_me_config& c, std::string* why) {
+  if (c.width <= 0 || c.height <= 0 || (c.width & 15) || (c.height & 15)) {
+    *why = "width and height must be positive multiples of 16";
+    return false;
+  }
+  if (c.range < 1 || c.range > kMaxRange) {
+    *why = "range must be 1..64";
+    return false;
+  }
+  if ((c.metric != 0 && c.metric != 1) || (c.window != 0 && c.window != 1) ||
+      (c.filter != 0 && c.filter != 1) || c.subpel < 0 || c.subpel > 2 ||
+      (c.predictor != 0 && c.predictor != 1) || (c.best_vector != 0 && c.best_vector != 1)) {
+    *why = "mode switch out of range";
+    return false;
+  }
+  if (c.tile_rows && c.tile_rows != 8) {
+    *why = "tile_rows must be 0 or 8";
+    return false;
+  }
+  if (c.metric == ICU_METRIC_SSD && c.tile_rows && c.tile_rows != 8) {
+    *why = "SSD tiles are 8 rows";
+    return false;
+  }
+  if (c.mbs_per_task < 0 || c.mbs_per_task > search_max_group()) {
+    *why = "mbs_per_task must be 0..16";
+    return false;
+  }
+  if (2 * c.range + 1 > 129) {
+    *why = "range too large";
+    return false;
+  }
+  return true;
+}
+
+// ------------------------------------------------------------------ engine
+struct StreamPlan {
+  icu_me_config cfg;
+  Tiling t;
+  int L, mbw, mbh, nmb, mb_base;
+  bool table;
+  long long table_base;
+  uint8_t* refpad_base = nullptr;
+  uint8_t* refpad = nullptr;
+  int ref_stride = 0;
+};
+
+struct Engine {
+  int device = 0;
+  hipStream_t own = nullptr;
+  std::vector<StreamPlan> sp;
+  int total_mb = 0;
+  long long table_total = 0;
+  std::vector<StreamDesc> h_streams;
+  std::vector<DecideDesc> h_decide;
+  std::vector<SearchTask> h_tasks;
+  std::vector<DecideItem> h_items;
+  // Descriptor sets: [0] for eager calls, [1] owned by a captured graph, so
+  // eager calls with other buffers never redirect a graph's replays.
+  struct DescSet {
+    StreamDesc* d_streams = nullptr;
+    DecideDesc* d_decide = nullptr;
+    std::vector<const uint8_t*> cur;
+    int16_t* mv = nullptr;
+    int32_t* dist = nullptr;
+    int32_t* integ = nullptr;
+    bool valid = false;
+  } ds[2];
+  SearchTask* d_tasks = nullptr;
+  DecideItem* d_items = nullptr;
+  IntResult* d_ires = nullptr;
+  uint32_t* d_table = nullptr;
+  int* d_wave = nullptr;  // per stream: mbh progress counters + 2 (ticket, error)
+  std::vector<int> wave_off;
+  int search_grid = 0;    // 0: one workgroup per task
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+
+  ~Engine() { release(); }
+
+  void release() {
+    if (gexec) hipGraphExecDestroy(gexec);
+    if (graph) hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    for (auto& s : sp)
+      if (s.refpad_base) hipFree(s.refpad_base);
+    sp.clear();
+    for (auto& d : ds) {
+      hipFree(d.d_streams);
+      hipFree(d.d_decide);
+      d.d_streams = nullptr;
+      d.d_decide = nullptr;
+    }
+    hipFree(d_tasks);
+    hipFree(d_items);
+    hipFree(d_ires);
+    hipFree(d_table);
+    hipFree(d_wave);
+    d_tasks = nullptr; d_items = nullptr;
+    d_ires = nullptr; d_table = nullptr; d_wave = nullptr;
+    if (own) hipStreamDestroy(own);
+    own = nullptr;
+  }
+
+  void build_tasks() {
+    h_tasks.clear();
+    for (size_t si = 0; si < sp.size(); si++) {
+      const StreamPlan& s = sp[si];
+      for (int row = 0; row < s.mbh; row++)
+        for (int c = 0; c < s.mbw; c += s.t.G)
+          h_tasks.push_back(SearchTask{(int)si, row * s.mbw + c, std::min(s.t.G, s.mbw - c)});
+    }
+    // Longest tasks first so a persistent grid ends evenly.
+    std::stable_sort(h_tasks.begin(), h_tasks.end(), [&](const SearchTask& a, const SearchTask& b) {
+      const StreamPlan &sa = sp[a.stream], &sb = sp[b.stream];
+      const long ca = (long)a.count * sa.t.Q * sa.t.B * (sa.t.V + 15);
+      const long cb = (long)b.count * sb.t.Q * sb.t.B * (sb.t.V + 15);
+      return ca > cb;
+    });
+  }
+
+  int init(const icu_me_config* cfgs, int n) {
+    HIP_TRY(hipGetDevice(&device));
+    HIP_TRY(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+    sp.resize(n);
+    total_mb = 0;
+    table_total = 0;
+    int wave_total = 0;
+    for (int i = 0; i < n; i++) {
+      StreamPlan& s = sp[i];
+      s.cfg = cfgs[i];
+      s.L = s.cfg.window == ICU_WINDOW_FULL ? s.cfg.range : s.cfg.range - 1;
+      s.mbw = s.cfg.width / 16;
+      s.mbh = s.cfg.height / 16;
+      s.nmb = s.mbw * s.mbh;
+      s.mb_base = total_mb;
+      total_mb += s.nmb;
+      s.t = plan_tiling(s.mbw, s.L, s.cfg.metric, s.cfg.tile_rows, s.cfg.mbs_per_task);
+      s.table = s.cfg.filter || s.cfg.predictor;
+      s.table_base = table_total;
+      if (s.table) table_total += (long long)s.nmb * (2 * s.L + 1) * (2 * s.L + 1);
+      s.ref_stride = ((s.cfg.width + 2 * kPad) + 255) & ~255;
+      const size_t bytes = (size_t)s.ref_stride * (s.cfg.height + 2 * kPad);
+      HIP_TRY(hipMalloc(&s.refpad_base, bytes));
+      s.refpad = s.refpad_base + (size_t)kPad * s.ref_stride + kPad;
+      wave_off.push_back(wave_total);
+      wave_total += s.mbh + 2;
+    }
+    build_tasks();
+    for (auto& d : ds) {
+      HIP_TRY(hipMalloc(&d.d_streams, sizeof(StreamDesc) * n));
+      HIP_TRY(hipMalloc(&d.d_decide, sizeof(DecideDesc) * n));
+      d.cur.assign(n, nullptr);
+    }
+    HIP_TRY(hipMalloc(&d_tasks, sizeof(SearchTask) * h_tasks.size()));
+    HIP_TRY(hipMalloc(&d_ires, sizeof(IntResult) * total_mb));
+    HIP_TRY(hipMalloc(&d_wave, sizeof(int) * wave_total));
+    HIP_TRY(hipMemset(d_wave, 0, sizeof(int) * wave_total));
+    if (table_total) HIP_TRY(hipMalloc(&d_table, sizeof(uint32_t) * table_total));
+    HIP_TRY(hipMemcpy(d_tasks, h_tasks.data(), sizeof(SearchTask) * h_tasks.size(), hipMemcpyHostToDevice));
+    h_items.clear();
+    for (int i = 0; i < n; i++)
+      if (!sp[i].cfg.predictor)
+        for (int mb = 0; mb < sp[i].nmb; mb++) h_items.push_back(DecideItem{i, mb});
+    if (!h_items.empty()) {
+      HIP_TRY(hipMalloc(&d_items, sizeof(DecideItem) * h_items.size()));
+      HIP_TRY(hipMemcpy(d_items, h_items.data(), sizeof(DecideItem) * h_items.size(), hipMemcpyHostToDevice));
+    }
+    return ICU_OK;
+  }
+
+  // (Re)write the descriptors when the caller's buffers change.
+  int bind(int set, const uint8_t* const* curs, int16_t* mv, int32_t* dist, int32_t* integ) {
+    DescSet& D = ds[set];
+    bool same = D.valid && mv == D.mv && dist == D.dist && integ == D.integ;
+    for (size_t i = 0; same && i < sp.size(); i++) same = D.cur[i] == curs[i];
+    if (same) return ICU_OK;
+    // kernels of an earlier call may still read the old descriptors
+    if (D.valid) HIP_TRY(hipDeviceSynchronize());
+    h_streams.resize(sp.size());
+    h_decide.resize(sp.size());
+    for (size_t i = 0; i < sp.size(); i++) {
+      const StreamPlan& s = sp[i];
+      StreamDesc& d = h_streams[i];
+      d.cur = curs[i];
+      d.ref = s.refpad;
+      d.cur_stride = s.cfg.width;
+      d.ref_stride = s.ref_stride;
+      d.W = s.cfg.width;
+      d.H = s.cfg.height;
+      d.mbw = s.mbw;
+      d.mbh = s.mbh;
+      d.R = s.cfg.range;
+      d.L = s.L;
+      d.V = s.t.V;
+      d.B = s.t.B;
+      d.Q = s.t.Q;
+      d.G = s.t.G;
+      d.metric = s.cfg.metric;
+      d.mb_base = s.mb_base;
+      d.table_base_valid = s.table;
+      d.table_base = s.table_base;
+      d.table_dim = 2 * s.L + 1;
+      DecideDesc& e = h_decide[i];
+      e.cur = curs[i];
+      e.ref = s.refpad;
+      e.cur_stride = s.cfg.width;
+      e.ref_stride = s.ref_stride;
+      e.W = s.cfg.width;
+      e.H = s.cfg.height;
+      e.mbw = s.mbw;
+      e.mbh = s.mbh;
+      e.R = s.cfg.range;
+      e.L = s.L;
+      e.metric = s.cfg.metric;
+      e.filter = s.cfg.filter;
+      e.subpel = s.cfg.subpel;
+      e.pred = s.cfg.predictor;
+      e.bestvec = s.cfg.best_vector;
+      e.mb_base = s.mb_base;
+      e.table = s.table ? d_table : nullptr;
+      e.table_base = s.table_base;
+      e.table_dim = 2 * s.L + 1;
+      e.ires = d_ires;
+      e.mv_out = mv;
+      e.dist_out = dist;
+      e.int_out = integ;
+      e.progress = d_wave + wave_off[i];
+      e.ticket = d_wave + wave_off[i] + s.mbh;
+    }
+    HIP_TRY(hipMemcpy(D.d_streams, h_streams.data(), sizeof(StreamDesc) * sp.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(D.d_decide, h_decide.data(), sizeof(DecideDesc) * sp.size(), hipMemcpyHostToDevice));
+    for (size_t i = 0; i < sp.size(); i++) D.cur[i] = curs[i];
+    D.mv = mv;
+    D.dist = dist;
+    D.integ = integ;
+    D.valid = true;
+    return ICU_OK;
+  }
+
+  // Enqueue the whole estimate on `st` (buffers already bound).
+  // ev (nullable): 4 events recorded before padding, search, decision, end.
+  int enqueue(int set, const uint8_t* const* refs, hipStream_t st, hipEvent_t* ev = nullptr) {
+    StreamDesc* d_streams = ds[set].d_streams;
+    DecideDesc* d_decide = ds[set].d_decide;
+    if (ev) HIP_TRY(hipEventRecord(ev[0], st));
+    for (size_t i = 0; i < sp.size(); i++) {
+      const StreamPlan& s = sp[i];
+      launch_pad_plane(refs[i], s.cfg.width, s.cfg.height, s.cfg.width, s.refpad, s.ref_stride, st);
+    }
+    if (ev) HIP_TRY(hipEventRecord(ev[1], st));
+    launch_search(d_streams, d_tasks, (int)h_tasks.size(), search_grid, sp[0].cfg.metric, d_ires,
+                  d_table, st);
+    if (ev) HIP_TRY(hipEventRecord(ev[2], st));
+    if (!h_items.empty()) launch_decide_parallel(d_decide, d_items, (int)h_items.size(), st);
+    for (size_t i = 0; i < sp.size(); i++) {
+      if (!sp[i].cfg.predictor) continue;
+      launch_fill_i32(d_wave + wave_off[i], 0, sp[i].mbh + 2, st);
+      launch_decide_wavefront(d_decide + i, nullptr, sp[i].mbh, st);
+    }
+    if (ev) HIP_TRY(hipEventRecord(ev[3], st));
+    HIP_TRY(hipGetLastError());
+    return ICU_OK;
+  }
+
+  int check_wave_errors() {
+    for (size_t i = 0; i < sp.size(); i++) {
+      if (!sp[i].cfg.predictor) continue;
+      int flags[2];
+      HIP_TRY(hipMemcpy(flags, d_wave + wave_off[i] + sp[i].mbh, sizeof(flags), hipMemcpyDeviceToHost));
+      if (flags[1]) return fail(ICU_E_STATE, "median-predictor wavefront timed out");
+    }
+    return ICU_OK;
+  }
+
+  void retile(int si, Tiling t) {
+    sp[si].t = t;
+    build_tasks();
+    hipMemcpy(d_tasks, h_tasks.data(), sizeof(SearchTask) * h_tasks.size(), hipMemcpyHostToDevice);
+    ds[0].valid = false;
+    ds[1].valid = false;
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------- handles
+struct icu_me {
+  Engine eng;
+  // staging for the host entry points
+  uint8_t* d_cur = nullptr;
+  uint8_t* d_ref = nullptr;
+  int16_t* d_mv = nullptr;
+  int32_t* d_dist = nullptr;
+  std::vector<uint8_t> h8;
+  ~icu_me() {
+    hipFree(d_cur);
+    hipFree(d_ref);
+    hipFree(d_mv);
+    hipFree(d_dist);
+  }
+};
+
+struct icu_me_batch {
+  Engine eng;
+};
+
+static hipStream_t pick(void* stream, Engine& e) { return stream ? (hipStream_t)stream : e.own; }
+
+extern "C" {
+
+const char* icu_last_error(void) { return g_err.c_str(); }
+
+int icu_me_config_reference(icu_me_config* cfg, int width, int height, int motion_range_qpel) {
+  if (!cfg) return fail(ICU_E_ARG, "null config");
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->width = width;
+  cfg->height = height;
+  cfg->range = motion_range_qpel / 4;
+  cfg->metric = ICU_METRIC_SSD;
+  cfg->window = ICU_WINDOW_REFERENCE;
+  cfg->filter = 1;
+  cfg->subpel = 2;
+  cfg->predictor = 1;
+  cfg->best_vector = 1;
+  return ICU_OK;
+}
+
+int icu_me_create(const icu_me_config* cfg, icu_me** out) {
+  if (!cfg || !out) return fail(ICU_E_ARG, "null argument");
+  std::string why;
+  if (!valid_cfg(*cfg, &why)) return fail(ICU_E_ARG, why);
+  icu_me* h = new icu_me();
+  int rc = h->eng.init(cfg, 1);
+  const size_t px = (size_t)cfg->width * cfg->height, nmb = px / 256;
+  if (rc == ICU_OK) {
+    if (hipMalloc(&h->d_cur, px) != hipSuccess || hipMalloc(&h->d_ref, px) != hipSuccess ||
+        hipMalloc(&h->d_mv, nmb * 4) != hipSuccess || hipMalloc(&h->d_dist, nmb * 4) != hipSuccess)
+      rc = fail(ICU_E_DEVICE, "device allocation failed");
+  }
+  if (rc != ICU_OK) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return ICU_OK;
+}
+
+int icu_me_destroy(icu_me* h) {
+  delete h;
+  return ICU_OK;
+}
+
+static int to_u8(const int16_t* src, size_t n, std::vector<uint8_t>& dst, size_t off) {
+  int bad = 0;
+  for (size_t i = 0; i < n; i++) {
+    const int v = src[i];
+    bad |= (v < 0) | (v > 255);
+    dst[off + i] = (uint8_t)v;
+  }
+  return bad ? fail(ICU_E_RANGE, "input sample outside 0..255") : ICU_OK;
+}
+
+int icu_me_estimate_device(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int16_t* d_mv,
+                           int32_t* d_dist, int32_t* d_int, void* stream) {
+  if (!h || !d_cur || !d_ref || !d_mv || !d_dist) return fail(ICU_E_ARG, "null argument");
+  Engine& e = h->eng;
+  const uint8_t* curs[1] = {d_cur};
+  const uint8_t* refs[1] = {d_ref};
+  int rc = e.bind(0, curs, d_mv, d_dist, d_int);
+  if (rc) return rc;
+  return e.enqueue(0, refs, pick(stream, e));
+}
+
+int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int16_t* d_mv,
+                          int32_t* d_dist, void* stream, float* stage_ms) {
+  if (!h || !d_cur || !d_ref || !d_mv || !d_dist || !stage_ms) return fail(ICU_E_ARG, "null argument");
+  Engine& e = h->eng;
+  const uint8_t* curs[1] = {d_cur};
+  const uint8_t* refs[1] = {d_ref};
+  int rc = e.bind(0, curs, d_mv, d_dist, nullptr);
+  if (rc) return rc;
+  hipEvent_t ev[4];
+  for (auto& x : ev) HIP_TRY(hipEventCreate(&x));
+  const hipStream_t st = pick(stream, e);
+  rc = e.enqueue(0, refs, st, ev);
+  if (!rc) {
+    HIP_TRY(hipEventSynchronize(ev[3]));
+    for (int i = 0; i < 3; i++) HIP_TRY(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
+  }
+  for (auto& x : ev) hipEventDestroy(x);
+  return rc;
+}
+
+int icu_me_estimate(icu_me* h, const int16_t* cur, const int16_t* ref, const uint8_t* include,
+                    int16_t* mv_out, int32_t* dist_out, long* avg_out) {
+  if (!h || !cur || !ref || !mv_out) return fail(ICU_E_ARG, "null argument");
+  Engine& e = h->eng;
+  const icu_me_config& c = e.sp[0].cfg;
+  const size_t px = (size_t)c.width * c.height, nmb = px / 256;
+  h->h8.resize(2 * px);
+  int rc = to_u8(cur, px, h->h8, 0);
+  if (!rc) rc = to_u8(ref, px, h->h8, px);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(h->d_cur, h->h8.data(), px, hipMemcpyHostToDevice, e.own));
+  HIP_TRY(hipMemcpyAsync(h->d_ref, h->h8.data() + px, px, hipMemcpyHostToDevice, e.own));
+  rc = icu_me_estimate_device(h, h->d_cur, h->d_ref, h->d_mv, h->d_dist, nullptr, e.own);
+  if (rc) return rc;
+  std::vector<int32_t> dist(nmb);
+  HIP_TRY(hipMemcpyAsync(mv_out, h->d_mv, nmb * 4, hipMemcpyDeviceToHost, e.own));
+  HIP_TRY(hipMemcpyAsync(dist.data(), h->d_dist, nmb * 4, hipMemcpyDeviceToHost, e.own));
+  HIP_TRY(hipStreamSynchronize(e.own));
+  rc = e.check_wave_errors();
+  if (rc) return rc;
+  if (dist_out) std::memcpy(dist_out, dist.data(), nmb * 4);
+  if (avg_out) {
+    long total = 0;
+    int inc = 0;
+    if (include)
+      for (size_t i = 0; i < nmb; i++)
+        if (include[i]) {
+          total += dist[i];
+          inc++;
+        }
+    *avg_out = inc ? total / inc : 0;
+  }
+  return ICU_OK;
+}
+
+int icu_me_get_tiling(icu_me* h, int* tile_rows, int* mbs_per_task) {
+  if (!h) return fail(ICU_E_ARG, "null handle");
+  if (tile_rows) *tile_rows = h->eng.sp[0].t.V;
+  if (mbs_per_task) *mbs_per_task = h->eng.sp[0].t.G;
+  return ICU_OK;
+}
+
+int icu_me_autotune(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int reps) {
+  if (!h || !d_cur || !d_ref) return fail(ICU_E_ARG, "null argument");
+  Engine& e = h->eng;
+  StreamPlan& s = e.sp[0];
+  if (reps < 1) reps = 1;
+  hipEvent_t a, b;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  Tiling best = s.t;
+  float best_ms = 1e30f;
+  static const int kV[1] = {8};
+  for (int v : kV) {
+    if (s.cfg.metric == ICU_METRIC_SSD && v != 8) continue;
+    if ((2 * s.L + 1 + v - 1) / v * v - (2 * s.L + 1) > v / 2 + 2) continue;  // too much row waste
+    for (int g = 1; g <= std::min(search_max_group(), s.mbw); g *= 2) {
+      Tiling t = plan_tiling(s.mbw, s.L, s.cfg.metric, v, g);
+      if (t.V != v || t.G != g) continue;
+      e.retile(0, t);
+      int rc = icu_me_estimate_device(h, d_cur, d_ref, h->d_mv, h->d_dist, nullptr, e.own);
+      if (rc) return rc;
+      HIP_TRY(hipEventRecord(a, e.own));
+      for (int r = 0; r < reps; r++) {
+        rc = icu_me_estimate_device(h, d_cur, d_ref, h->d_mv, h->d_dist, nullptr, e.own);
+        if (rc) return rc;
+      }
+      HIP_TRY(hipEventRecord(b, e.own));
+      HIP_TRY(hipEventSynchronize(b));
+      float ms = 0;
+      HIP_TRY(hipEventElapsedTime(&ms, a, b));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+  }
+  e.retile(0, best);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return ICU_OK;
+}
+
+int icu_code_frame_device(icu_me* h, const uint8_t* d_cur, const int16_t* d_mv, int qp, int intra,
+                          int cols_first, int16_t* d_levels, uint8_t* d_recon, int16_t* d_pred,
+                          void* stream) {
+  if (!h || !d_cur || !d_mv || !d_levels || !d_recon) return fail(ICU_E_ARG, "null argument");
+  if (qp < 0 || qp > 51) return fail(ICU_E_ARG, "qp must be 0..51");
+  Engine& e = h->eng;
+  const StreamPlan& s = e.sp[0];
+  launch_code_frame(d_cur, s.cfg.width, s.refpad, s.ref_stride, s.cfg.width, s.cfg.height, d_mv, qp,
+                    intra ? 1 : 0, cols_first ? 1 : 0, d_levels, d_recon, d_pred, pick(stream, e));
+  HIP_TRY(hipGetLastError());
+  return ICU_OK;
+}
+
+// ------------------------------------------------------------ transforms
+static int check_tx(long n, int qp, int mode, int layout) {
+  if (n < 0) return fail(ICU_E_ARG, "negative block count");
+  if (qp < 0 || qp > 51) return fail(ICU_E_ARG, "qp must be 0..51");
+  if (mode < 0 || mode > 2) return fail(ICU_E_ARG, "mode must be 0, 1 or 2");
+  if (layout != 0 && layout != 1) return fail(ICU_E_ARG, "layout must be 0 or 1");
+  if (layout == 1 && (n % 16)) return fail(ICU_E_ARG, "layout 1 needs whole macroblocks");
+  return ICU_OK;
+}
+
+int icu_fwd4x4_device(const int16_t* d_in, int16_t* d_out, long n, int qp, int intra, int mode,
+                      int layout, void* stream) {
+  int rc = check_tx(n, qp, mode, layout);
+  if (rc) return rc;
+  launch_fwd4x4(d_in, d_out, n, qp, intra ? 1 : 0, mode, layout, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return ICU_OK;
+}
+
+int icu_inv4x4_device(const int16_t* d_in, int16_t* d_out, long n, int qp, int mode, int layout,
+                      int cols_first, const int* d_w, void* stream) {
+  int rc = check_tx(n, qp, mode, layout);
+  if (rc) return rc;
+  launch_inv4x4(d_in, d_out, n, qp, mode, layout, cols_first ? 1 : 0, d_w, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return ICU_OK;
+}
+
+static int tx_host(const int16_t* in, int16_t* out, long n, int qp, int intra, int mode, int layout,
+                   int inverse, int cols_first, const int* w) {
+  if (!in || !out) return fail(ICU_E_ARG, "null argument");
+  int rc = check_tx(n, qp, mode, layout);
+  if (rc || n == 0) return rc;
+  const size_t bytes = (size_t)n * 16 * sizeof(int16_t);
+  int16_t *di = nullptr, *dout = nullptr;
+  int* dw = nullptr;
+  HIP_TRY(hipMalloc(&di, bytes));
+  HIP_TRY(hipMalloc(&dout, bytes));
+  if (w) HIP_TRY(hipMalloc(&dw, 16 * sizeof(int)));
+  HIP_TRY(hipMemcpy(di, in, bytes, hipMemcpyHostToDevice));
+  if (w) HIP_TRY(hipMemcpy(dw, w, 16 * sizeof(int), hipMemcpyHostToDevice));
+  if (inverse) launch_inv4x4(di, dout, n, qp, mode, layout, cols_first ? 1 : 0, dw, nullptr);
+  else launch_fwd4x4(di, dout, n, qp, intra ? 1 : 0, mode, layout, nullptr);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost));
+  hipFree(di);
+  hipFree(dout);
+  hipFree(dw);
+  return ICU_OK;
+}
+
+int icu_fwd4x4(const int16_t* in, int16_t* out, long n, int qp, int intra, int mode, int layout) {
+  return tx_host(in, out, n, qp, intra, mode, layout, 0, 0, nullptr);
+}
+
+int icu_inv4x4(const int16_t* in, int16_t* out, long n, int qp, int mode, int layout, int cols_first,
+               const int* w) {
+  return tx_host(in, out, n, qp, 0, mode, layout, 1, cols_first, w);
+}
+
+// ---------------------------------------------------------------- batches
+int icu_me_batch_create(const icu_me_config* cfgs, int n, icu_me_batch** out) {
+  if (!cfgs || !out || n <= 0) return fail(ICU_E_ARG, "bad batch arguments");
+  for (int i = 0; i < n; i++) {
+    std::string why;
+    if (!valid_cfg(cfgs[i], &why)) return fail(ICU_E_ARG, "stream " + std::to_string(i) + ": " + why);
+    if (cfgs[i].predictor) return fail(ICU_E_ARG, "batched streams use the zero predictor");
+    if (cfgs[i].metric != cfgs[0].metric) return fail(ICU_E_ARG, "batched streams share one metric");
+  }
+  icu_me_batch* b = new icu_me_batch();
+  int rc = b->eng.init(cfgs, n);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  // persistent search grid: a few workgroups per CU walk the task queue
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, b->eng.device) == hipSuccess)
+    b->eng.search_grid = std::min((int)b->eng.h_tasks.size(), prop.multiProcessorCount * 4);
+  *out = b;
+  return ICU_OK;
+}
+
+int icu_me_batch_destroy(icu_me_batch* b) {
+  delete b;
+  return ICU_OK;
+}
+
+int icu_me_batch_num_mbs(icu_me_batch* b) { return b ? b->eng.total_mb : fail(ICU_E_ARG, "null batch"); }
+
+int icu_me_batch_mb_base(icu_me_batch* b, int i) {
+  if (!b || i < 0 || i >= (int)b->eng.sp.size()) return fail(ICU_E_ARG, "bad stream index");
+  return b->eng.sp[i].mb_base;
+}
+
+int icu_me_batch_estimate_device(icu_me_batch* b, const uint8_t* const* curs, const uint8_t* const* refs,
+                                 int16_t* d_mv, int32_t* d_dist, void* stream) {
+  if (!b || !curs || !refs || !d_mv || !d_dist) return fail(ICU_E_ARG, "null argument");
+  int rc = b->eng.bind(0, curs, d_mv, d_dist, nullptr);
+  if (rc) return rc;
+  return b->eng.enqueue(0, refs, pick(stream, b->eng));
+}
+
+int icu_me_batch_capture(icu_me_batch* b, const uint8_t* const* curs, const uint8_t* const* refs,
+                         int16_t* d_mv, int32_t* d_dist) {
+  if (!b || !curs || !refs || !d_mv || !d_dist) return fail(ICU_E_ARG, "null argument");
+  Engine& e = b->eng;
+  int rc = e.bind(1, curs, d_mv, d_dist, nullptr);
+  if (rc) return rc;
+  if (e.gexec) hipGraphExecDestroy(e.gexec);
+  if (e.graph) hipGraphDestroy(e.graph);
+  e.gexec = nullptr;
+  e.graph = nullptr;
+  HIP_TRY(hipStreamBeginCapture(e.own, hipStreamCaptureModeThreadLocal));
+  rc = e.enqueue(1, refs, e.own);
+  hipGraph_t g = nullptr;
+  hipError_t ce = hipStreamEndCapture(e.own, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (ce != hipSuccess) return fail(ICU_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
+  e.graph = g;
+  HIP_TRY(hipGraphInstantiate(&e.gexec, g, nullptr, nullptr, 0));
+  return ICU_OK;
+}
+
+int icu_me_batch_graph_launch(icu_me_batch* b, void* stream) {
+  if (!b || !b->eng.gexec) return fail(ICU_E_STATE, "no captured graph");
+  HIP_TRY(hipGraphLaunch(b->eng.gexec, pick(stream, b->eng)));
+  return ICU_OK;
+}
+
+// ---------------------------------------------------------------- runtime
+int icu_device_info(char* name, int name_len, int* cus, int* clock_khz) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  HIP_TRY(hipGetDeviceProperties(&p, dev));
+  if (name && name_len > 0) {
+    std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  if (cus) *cus = p.multiProcessorCount;
+  if (clock_khz) *clock_khz = p.clockRate;
+  return ICU_OK;
+}
+
+void* icu_malloc(size_t bytes) {
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    g_err = "hipMalloc failed";
+    return nullptr;
+  }
+  return p;
+}
+
+int icu_free(void* d) {
+  HIP_TRY(hipFree(d));
+  return ICU_OK;
+}
+
+int icu_memcpy_htod(void* d, const void* h, size_t bytes, void* stream) {
+  if (stream) {
+    HIP_TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  } else {
+    HIP_TRY(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+  }
+  return ICU_OK;
+}
+
+int icu_memcpy_dtoh(void* h, const void* d, size_t bytes, void* stream) {
+  if (stream) {
+    HIP_TRY(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+  } else {
+    HIP_TRY(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
+  }
+  return ICU_OK;
+}
+
+int icu_memset(void* d, int value, size_t bytes, void* stream) {
+  HIP_TRY(hipMemsetAsync(d, value, bytes, (hipStream_t)stream));
+  return ICU_OK;
+}
+
+void* icu_stream_create(void) {
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+    g_err = "hipStreamCreate failed";
+    return nullptr;
+  }
+  return (void*)s;
+}
+
+int icu_stream_destroy(void* s) {
+  HIP_TRY(hipStreamDestroy((hipStream_t)s));
+  return ICU_OK;
+}
+
+int icu_stream_sync(void* s) {
+  HIP_TRY(hipStreamSynchronize((hipStream_t)s));
+  return ICU_OK;
+}
+
+void* icu_event_create(void) {
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) {
+    g_err = "hipEventCreate failed";
+    return nullptr;
+  }
+  return (void*)e;
+}
+
+int icu_event_destroy(void* e) {
+  HIP_TRY(hipEventDestroy((hipEvent_t)e));
+  return ICU_OK;
+}
+
+int icu_event_record(void* e, void* s) {
+  HIP_TRY(hipEventRecord((hipEvent_t)e, (hipStream_t)s));
+  return ICU_OK;
+}
+
+int icu_event_elapsed_ms(void* a, void* b, float* ms) {
+  HIP_TRY(hipEventSynchronize((hipEvent_t)b));
+  HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+  return ICU_OK;
+}
+
+int icu_sqrt_check_device(const int* d_m, double* d_out, int n, void* stream) {
+  if (!d_m || !d_out || n < 0) return fail(ICU_E_ARG, "bad arguments");
+  launch_sqrt_check(d_m, d_out, n, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError());
+  return ICU_OK;
+}
+
+}  // extern "C"

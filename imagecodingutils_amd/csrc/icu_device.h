@@ -1,0 +1,95 @@
+// Shared device-side types for the motion-estimation and transform kernels.
+// Host code fills these descriptors; kernels read them from device memory.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace icu {
+
+// Padding of the device reference plane.  Block origins searched lie in
+// [-16, W] x [-16, H] (the reference's 16-pel border rule); the search tiles
+// round the window up to whole lane tiles, and the sub-pel stages read up to
+// 3 pels beyond a block, so 80 pels on every side keep every read in bounds
+// for search ranges up to 64.
+constexpr int kPad = 80;
+constexpr int kMaxRange = 64;
+
+// Window half-width rule.
+enum { kWindowRef = 0, kWindowFull = 1 };
+enum { kMetricSSD = 0, kMetricSAD = 1 };
+
+// One luma frame pair being estimated.  All pointers are device pointers.
+struct StreamDesc {
+  const uint8_t* cur;   // W x H, row stride cur_stride
+  const uint8_t* ref;   // padded plane; points at pixel (0,0), stride ref_stride
+  int cur_stride, ref_stride;
+  int W, H, mbw, mbh;
+  int R;                // search range (full pel)
+  int L;                // window half-width: R - 1 (reference rule) or R
+  int V;                // candidate rows per lane (tile height), chosen per stream
+  int B;                // ceil((2L+1) / V) row tiles
+  int Q;                // ceil((2L+1) / 4) column quads
+  int G;                // MBs per workgroup task
+  int metric;
+  int mb_base;          // first global MB slot of this stream in the outputs
+  int table_base_valid; // 1 when a cost table is written for this stream
+  long long table_base; // offset of this stream's first table entry
+  int table_dim;        // 2L+1 (table rows and columns)
+};
+
+// A workgroup task: `count` consecutive MBs of one MB row of one stream.
+struct SearchTask {
+  int stream;
+  int mb;     // first MB (raster index within the stream)
+  int count;  // MBs in this task (1..G)
+};
+
+// Integer-stage result per MB, packed for a 64-bit atomic min:
+//   bits 63..32  cost (SAD <= 65280, SSD <= 16,646,400)
+//   bits 31..17  squared distance to the rounded predictor (zero here)
+//   bits 16..0   reverse raster rank (later candidates rank lower; the zero
+//                vector, the initial incumbent, ranks highest)
+// The minimum key is the reference's decision with the running-best filter
+// off: lower cost wins, then the candidate closer to the predictor, then the
+// later candidate in raster order.
+__host__ __device__ inline uint64_t pack_key(uint32_t cost, uint32_t dist2, uint32_t rrank) {
+  return ((uint64_t)cost << 32) | ((uint64_t)(dist2 & 0x7FFF) << 17) | (rrank & 0x1FFFF);
+}
+__host__ __device__ inline uint32_t raster_rrank(int i, int j) {
+  // i, j in [-64, 64]; rank increases in raster order (row i, then column j).
+  return 0xFFFFu - (uint32_t)((i + 64) * 256 + (j + 64));
+}
+constexpr uint32_t kZeroRRank = 0x1FFFF;
+__host__ __device__ inline uint32_t key_cost(uint64_t k) { return (uint32_t)(k >> 32); }
+// Vector of a key: the zero vector for kZeroRRank, else the inverse of raster_rrank.
+__host__ __device__ inline void key_vector(uint64_t k, int* dx, int* dy) {
+  uint32_t rr = (uint32_t)(k & 0x1FFFF);
+  if (rr == kZeroRRank) { *dx = 0; *dy = 0; return; }
+  uint32_t rank = 0xFFFFu - rr;
+  *dy = (int)(rank >> 8) - 64;
+  *dx = (int)(rank & 255) - 64;
+}
+
+// The clamped search window of the MB at pel (n, m): the reference's
+// GetMotionRange with offset 0 (half-width L, blocks may reach 16 pels
+// outside the picture, never further).
+__host__ __device__ inline void motion_window(int n, int m, int W, int H, int L, int* xl,
+                                              int* xr, int* yu, int* yd) {
+  *xl = (n - L >= -16) ? -L : -(n + 16);
+  *xr = (n + L < W) ? L : W - n;
+  *yu = (m - L >= -16) ? -L : -(m + 16);
+  *yd = (m + L < H) ? L : H - m;
+}
+
+// First candidate column covered by the lane tiles: -L rounded down to a
+// multiple of 4 so every lane's 4-candidate quad starts on a dword.
+__host__ __device__ inline int tile_dx0(int L) { return -4 * ((L + 3) / 4); }
+
+// Per-MB integer-stage output of the search kernel.
+struct IntResult {
+  int32_t mx, my;   // full-pel vector
+  int32_t cost;     // its cost
+  int32_t zero;     // cost of the zero vector
+};
+
+}  // namespace icu

@@ -1,0 +1,60 @@
+// Host-side launchers shared between the translation units of the library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "icu_device.h"
+
+namespace icu {
+
+// Per-stream parameters of the decision / sub-pel stage.
+struct DecideDesc {
+  const uint8_t* cur;
+  const uint8_t* ref;  // padded plane, pointer at pixel (0,0)
+  int cur_stride, ref_stride;
+  int W, H, mbw, mbh;
+  int R, L;
+  int metric, filter, subpel, pred, bestvec;
+  int mb_base;                 // first MB slot of this stream in ires / outputs
+  const uint32_t* table;       // cost table (nullptr: use ires)
+  long long table_base;
+  int table_dim;
+  const IntResult* ires;
+  int16_t* mv_out;             // 2 per MB slot
+  int32_t* dist_out;           // 1 per MB slot
+  int32_t* int_out;            // 3 per MB slot (integer stage), may be null
+  int* progress;               // mbh counters (median wavefront)
+  int* ticket;                 // [0] row ticket, [1] error flag
+};
+
+// Work item of the parallel decision kernel.
+struct DecideItem {
+  int stream;
+  int mb;
+};
+
+void launch_pad_plane(const uint8_t* src, int W, int H, int src_stride, uint8_t* dst_origin,
+                      int dst_stride, hipStream_t st);
+void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int grid,
+                   int metric, IntResult* d_out, uint32_t* d_table, hipStream_t st);
+int search_max_window_dwords();
+int search_max_group();
+
+void launch_decide_parallel(const DecideDesc* d_descs, const DecideItem* d_items, int nitems,
+                            hipStream_t st);
+void launch_decide_wavefront(const DecideDesc* d_desc, int* d_progress_and_ticket, int mbh,
+                             hipStream_t st);
+void launch_fill_i32(int* p, int v, int n, hipStream_t st);
+
+// Transform / coding launchers (it_kernels.hip).
+void launch_fwd4x4(const int16_t* in, int16_t* out, long n, int qp, int intra, int mode,
+                   int layout, hipStream_t st);
+void launch_inv4x4(const int16_t* in, int16_t* out, long n, int qp, int mode, int layout,
+                   int cols_first, const int* d_weights, hipStream_t st);
+void launch_code_frame(const uint8_t* cur, int cur_stride, const uint8_t* ref_padded,
+                       int ref_stride, int W, int H, const int16_t* mv, int qp, int intra,
+                       int cols_first, int16_t* levels, uint8_t* recon, int16_t* pred_out,
+                       hipStream_t st);
+void launch_sqrt_check(const int* m, double* out, int n, hipStream_t st);
+
+}  // namespace icu

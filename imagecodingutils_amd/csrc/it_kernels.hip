@@ -1,0 +1,273 @@
+// H.264 4x4 integer transform + quantisation, dequantisation + inverse
+// transform, and the fused per-MB residual coder (luma motion compensation ->
+// residual -> forward IT+Q -> levels -> dequant + inverse IT -> clipped
+// reconstruction).  Each 4x4 block lives in registers of one lane.
+//
+// Arithmetic restated from:
+//   src/CodecUtils/FastForward4x4ITImpl1.cpp:57-212,269-278 (forward; the
+//     unrolled Impl2 and the MB-wide On16x16ITImpl1 give identical output)
+//   src/CodecUtils/FastInverse4x4ITImpl1.cpp:60-236 (inverse, rows first)
+//   src/CodecUtils/FastInverse4x4On16x16ITImpl1.cpp:120-360 (cols first)
+//   src/ImageUtils/OverlayMem2Dv2.cpp:1195-1220,1343-1368 (Sub16x16,
+//     Add16x16WithClip255)
+//   src/CodecUtils/MotionCompensatorH264ImplStd.cpp:410-437 (luma compensate)
+// Every (short) store of the reference is an int16 truncation here at the
+// same point.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "icu_device.h"
+#include "icu_internal.h"
+#include "icu_qpel.h"
+
+namespace icu {
+
+namespace {
+
+__constant__ int c_mf[6][3] = {{13107, 5243, 8066}, {11916, 4660, 7490}, {10082, 4194, 6554},
+                               {9362, 3647, 5825},  {8192, 3355, 5243},  {7282, 2893, 4559}};
+__constant__ int c_v[6][3] = {{10, 16, 13}, {11, 18, 14}, {13, 20, 16},
+                              {14, 23, 18}, {16, 25, 20}, {18, 29, 23}};
+
+__device__ __forceinline__ int pos_class(int p) {
+  // 0 = (even,even), 1 = (odd,odd), 2 = mixed
+  const int r = p >> 2, c = p & 3;
+  return ((r | c) & 1) == 0 ? 0 : (((r & c) & 1) ? 1 : 2);
+}
+
+__device__ __forceinline__ int16_t quant(int x, int p, int qm, int f, int scale) {
+  const int mf = c_mf[qm][pos_class(p)];
+  if (x < 0) return (int16_t)(-((((-x) * mf) + f) >> scale));
+  return (int16_t)(((x * mf) + f) >> scale);
+}
+
+__device__ __forceinline__ void fwd4x4(int16_t (&b)[16], int qp, int intra, int mode) {
+  const int qm = qp % 6, qe = qp / 6, scale = 15 + qe;
+  const int f = intra ? (1 << (15 + qe)) / 3 : (1 << (15 + qe)) / 6;
+  if (mode == 2) {
+#pragma unroll
+    for (int p = 0; p < 16; p++) b[p] = quant(b[p], p, qm, f, scale);
+    return;
+  }
+#pragma unroll
+  for (int j = 0; j < 16; j += 4) {
+    const int s0 = b[j] + b[j + 3], s3 = b[j] - b[j + 3];
+    const int s1 = b[j + 1] + b[j + 2], s2 = b[j + 1] - b[j + 2];
+    b[j] = (int16_t)(s0 + s1);
+    b[j + 2] = (int16_t)(s0 - s1);
+    b[j + 1] = (int16_t)(s2 + s3 * 2);
+    b[j + 3] = (int16_t)(s3 - s2 * 2);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int s0 = b[j] + b[j + 12], s3 = b[j] - b[j + 12];
+    const int s1 = b[j + 4] + b[j + 8], s2 = b[j + 4] - b[j + 8];
+    const int x0 = s0 + s1, x2 = s0 - s1, x1 = s2 + s3 * 2, x3 = s3 - s2 * 2;
+    if (mode == 0) {
+      b[j] = quant(x0, j, qm, f, scale);
+      b[j + 8] = quant(x2, j + 8, qm, f, scale);
+      b[j + 4] = quant(x1, j + 4, qm, f, scale);
+      b[j + 12] = quant(x3, j + 12, qm, f, scale);
+    } else {
+      b[j] = (int16_t)x0;
+      b[j + 8] = (int16_t)x2;
+      b[j + 4] = (int16_t)x1;
+      b[j + 12] = (int16_t)x3;
+    }
+  }
+}
+
+struct Dq {
+  int qp, f, rs, lsh, qm;
+};
+__device__ __forceinline__ Dq dq_setup(int qp) {
+  Dq d;
+  d.qp = qp;
+  d.qm = qp % 6;
+  const int qe = qp / 6;
+  d.f = qe <= 3 ? 1 << (3 - qe) : 0;
+  d.rs = 4 - qe;
+  d.lsh = qe - 4;
+  return d;
+}
+__device__ __forceinline__ int dq(const Dq& d, int lvl, int p, const int* w) {
+  const int x = lvl * (c_v[d.qm][pos_class(p)] * (w ? w[p] : 16));
+  return d.qp < 24 ? (x + d.f) >> d.rs : x * (1 << d.lsh);
+}
+
+#define ICU_INV_BFLY(a0, a1, a2, a3, y0, y1, y2, y3) \
+  do {                                               \
+    const int s0_ = (a0) + (a2), s1_ = (a0) - (a2);  \
+    const int s2_ = ((a1) >> 1) - (a3), s3_ = (a1) + ((a3) >> 1); \
+    y0 = s0_ + s3_; y1 = s1_ + s2_; y2 = s1_ - s2_; y3 = s0_ - s3_; \
+  } while (0)
+
+__device__ __forceinline__ void inv4x4(int16_t (&b)[16], int qp, int mode, int cols_first,
+                                       const int* w) {
+  const Dq d = dq_setup(qp);
+  if (mode == 2) {
+#pragma unroll
+    for (int p = 0; p < 16; p++) b[p] = (int16_t)dq(d, b[p], p, w);
+    return;
+  }
+  const bool deq = mode == 0;
+  if (!cols_first) {
+#pragma unroll
+    for (int j = 0; j < 16; j += 4) {
+      int x0 = b[j], x1 = b[j + 1], x2 = b[j + 2], x3 = b[j + 3];
+      if (deq) { x0 = dq(d, x0, j, w); x1 = dq(d, x1, j + 1, w); x2 = dq(d, x2, j + 2, w); x3 = dq(d, x3, j + 3, w); }
+      int y0, y1, y2, y3;
+      ICU_INV_BFLY(x0, x1, x2, x3, y0, y1, y2, y3);
+      b[j] = (int16_t)y0; b[j + 1] = (int16_t)y1; b[j + 2] = (int16_t)y2; b[j + 3] = (int16_t)y3;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      int y0, y1, y2, y3;
+      ICU_INV_BFLY((int)b[j], (int)b[j + 4], (int)b[j + 8], (int)b[j + 12], y0, y1, y2, y3);
+      b[j] = (int16_t)((y0 + 32) >> 6); b[j + 4] = (int16_t)((y1 + 32) >> 6);
+      b[j + 8] = (int16_t)((y2 + 32) >> 6); b[j + 12] = (int16_t)((y3 + 32) >> 6);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      int x0 = b[j], x1 = b[j + 4], x2 = b[j + 8], x3 = b[j + 12];
+      if (deq) { x0 = dq(d, x0, j, w); x1 = dq(d, x1, j + 4, w); x2 = dq(d, x2, j + 8, w); x3 = dq(d, x3, j + 12, w); }
+      int y0, y1, y2, y3;
+      ICU_INV_BFLY(x0, x1, x2, x3, y0, y1, y2, y3);
+      b[j] = (int16_t)y0; b[j + 4] = (int16_t)y1; b[j + 8] = (int16_t)y2; b[j + 12] = (int16_t)y3;
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j += 4) {
+      int y0, y1, y2, y3;
+      ICU_INV_BFLY((int)b[j], (int)b[j + 1], (int)b[j + 2], (int)b[j + 3], y0, y1, y2, y3);
+      b[j] = (int16_t)((y0 + 32) >> 6); b[j + 1] = (int16_t)((y1 + 32) >> 6);
+      b[j + 2] = (int16_t)((y2 + 32) >> 6); b[j + 3] = (int16_t)((y3 + 32) >> 6);
+    }
+  }
+}
+
+// Element offset of coefficient p of 4x4 block `blk` in the given layout.
+__device__ __forceinline__ long elem_off(long blk, int p, int layout) {
+  if (layout == 0) return blk * 16 + p;
+  const long mb = blk >> 4;
+  const int bi = (int)(blk >> 2) & 3, bj = (int)blk & 3;
+  return mb * 256 + (4 * bi + (p >> 2)) * 16 + 4 * bj + (p & 3);
+}
+
+__global__ __launch_bounds__(256) void fwd_kernel(const int16_t* __restrict__ in, int16_t* __restrict__ out,
+                                                  long n, int qp, int intra, int mode, int layout) {
+  const long blk = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blk >= n) return;
+  int16_t b[16];
+#pragma unroll
+  for (int p = 0; p < 16; p++) b[p] = in[elem_off(blk, p, layout)];
+  fwd4x4(b, qp, intra, mode);
+#pragma unroll
+  for (int p = 0; p < 16; p++) out[elem_off(blk, p, layout)] = b[p];
+}
+
+__global__ __launch_bounds__(256) void inv_kernel(const int16_t* __restrict__ in, int16_t* __restrict__ out,
+                                                  long n, int qp, int mode, int layout, int cols_first,
+                                                  const int* __restrict__ w) {
+  const long blk = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blk >= n) return;
+  int16_t b[16];
+#pragma unroll
+  for (int p = 0; p < 16; p++) b[p] = in[elem_off(blk, p, layout)];
+  int wl[16];
+  if (w) {
+#pragma unroll
+    for (int p = 0; p < 16; p++) wl[p] = w[p];
+  }
+  inv4x4(b, qp, mode, cols_first, w ? wl : nullptr);
+#pragma unroll
+  for (int p = 0; p < 16; p++) out[elem_off(blk, p, layout)] = b[p];
+}
+
+// Fused coder: one workgroup of 256 lanes codes 16 MBs, one lane per 4x4.
+constexpr int kMbPerWg = 16;
+
+__global__ __launch_bounds__(256) void code_frame_kernel(const uint8_t* __restrict__ cur, int cur_stride,
+                                                         const uint8_t* __restrict__ ref, int ref_stride,
+                                                         int W, int H, const int16_t* __restrict__ mv,
+                                                         int qp, int intra, int cols_first,
+                                                         int16_t* __restrict__ levels,
+                                                         uint8_t* __restrict__ recon,
+                                                         int16_t* __restrict__ pred_out) {
+  __shared__ uint8_t patch[kMbPerWg][kPatch * kPatch];
+  __shared__ int fr[kMbPerWg][2];
+  const int mbw = W / 16, nmb = mbw * (H / 16);
+  const int lmb = threadIdx.x >> 4, blk = threadIdx.x & 15;
+  const int mb = blockIdx.x * kMbPerWg + lmb;
+  const bool live = mb < nmb;
+  const int n = (mb % mbw) * 16, m = (mb / mbw) * 16;
+  int qx = 0, qy = 0, ox = n, oy = m;
+  if (live) {
+    const int mvx = mv[2 * mb], mvy = mv[2 * mb + 1];
+    // full / quarter split with C truncation, negative fractions reflected
+    // to the full pel to the left / above
+    const int fx = mvx / 4, fy = mvy / 4;
+    qx = mvx % 4;
+    qy = mvy % 4;
+    int fox = 0, foy = 0;
+    if (qx < 0) { fox = -1; qx += 4; }
+    if (qy < 0) { foy = -1; qy += 4; }
+    ox = min(max(n + fx + fox, -kPad + 3), W + kPad - 19);
+    oy = min(max(m + fy + foy, -kPad + 3), H + kPad - 19);
+    for (int i = blk; i < kPatch * kPatch; i += 16) {
+      const int r = i / kPatch, c = i - r * kPatch;
+      patch[lmb][i] = ref[(long)(oy - 3 + r) * ref_stride + ox - 3 + c];
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  const int bi = blk >> 2, bj = blk & 3;
+  int16_t b[16], pr[16];
+#pragma unroll
+  for (int p = 0; p < 16; p++) {
+    const int r = 4 * bi + (p >> 2), c = 4 * bj + (p & 3);
+    pr[p] = (int16_t)qpel_p(patch[lmb], 3 + c, 3 + r, qx, qy);
+    b[p] = (int16_t)(cur[(long)(m + r) * cur_stride + n + c] - pr[p]);
+  }
+  fwd4x4(b, qp, intra, 0);
+#pragma unroll
+  for (int p = 0; p < 16; p++) {
+    const int r = 4 * bi + (p >> 2), c = 4 * bj + (p & 3);
+    levels[(long)(m + r) * W + n + c] = b[p];
+  }
+  inv4x4(b, qp, 0, cols_first, nullptr);
+#pragma unroll
+  for (int p = 0; p < 16; p++) {
+    const int r = 4 * bi + (p >> 2), c = 4 * bj + (p & 3);
+    const long o = (long)(m + r) * W + n + c;
+    recon[o] = (uint8_t)clip255(pr[p] + b[p]);
+    if (pred_out) pred_out[o] = pr[p];
+  }
+}
+
+}  // namespace
+
+void launch_fwd4x4(const int16_t* in, int16_t* out, long n, int qp, int intra, int mode, int layout,
+                   hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fwd_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, out, n, qp,
+                     intra, mode, layout);
+}
+
+void launch_inv4x4(const int16_t* in, int16_t* out, long n, int qp, int mode, int layout,
+                   int cols_first, const int* d_weights, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(inv_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, out, n, qp,
+                     mode, layout, cols_first, d_weights);
+}
+
+void launch_code_frame(const uint8_t* cur, int cur_stride, const uint8_t* ref_padded, int ref_stride,
+                       int W, int H, const int16_t* mv, int qp, int intra, int cols_first,
+                       int16_t* levels, uint8_t* recon, int16_t* pred_out, hipStream_t st) {
+  const int nmb = (W / 16) * (H / 16);
+  hipLaunchKernelGGL(code_frame_kernel, dim3((nmb + kMbPerWg - 1) / kMbPerWg), dim3(256), 0, st, cur,
+                     cur_stride, ref_padded, ref_stride, W, H, mv, qp, intra, cols_first, levels, recon,
+                     pred_out);
+}
+
+}  // namespace icu

@@ -1,0 +1,457 @@
+// Per-macroblock decision, sub-pel refinement and final vector choice.
+//
+// One wavefront (64 lanes) decides one MB.  It restates, in order, the rest
+// of MotionEstimatorH264ImplFull::Estimate
+// (src/CodecUtils/MotionEstimatorH264ImplFull.cpp:350-601):
+//   * the integer decision, either from the search kernel's packed argmin
+//     (zero predictor, filter off) or by replaying the reference's scan over
+//     the cost table: with the running-best filter (:410-411) the scan is
+//     path dependent, so lanes test 64 consecutive candidates at once against
+//     the current best, a ballot finds the first one the reference would
+//     accept, and the scan restarts after it;
+//   * the half-pel stage (LoadHalfQuartPelWindow :1095-1182, loop :452-483)
+//     from a 35x35 grid of G/b/h/j samples built in LDS;
+//   * the quarter-pel stage (LoadQuartPelWindow :1198-1222, loop :485-517)
+//     with the estimator's own pairwise averages;
+//   * the predictor distortion (:529-570) and GetBestVector
+//     (MotionEstimatorH264ImplFull.h:140-168) in correctly rounded doubles.
+// With the median predictor (H264MotionVectorPredictorImpl1 over
+// MacroBlockH264::GetMbMotionMedianPred, MacroBlockH264.cpp:757-816) an MB
+// depends on its left, above-left, above and above-right neighbours, so MB
+// rows run as a wavefront: each row is owned by one wave that waits on the
+// row above through a progress counter.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "icu_device.h"
+#include "icu_internal.h"
+#include "icu_qpel.h"
+
+namespace icu {
+
+namespace {
+
+constexpr int kHalf = 35;       // half-pel grid side
+constexpr int kSpinLimit = 1 << 26;
+
+__constant__ int c_subpos[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0},
+                                   {1, 0},   {-1, 1}, {0, 1},  {1, 1}};
+
+struct WaveLds {
+  uint8_t P[kPatch * kPatch];       // reference patch
+  int16_t hv[17 * kPatch];          // raw vertical 6-tap sums
+  int16_t Hg[kHalf * kHalf];        // G/b/h/j grid on the half-pel lattice
+  uint8_t cur[256];                 // the MB's input block
+};
+
+// The MB's cost-table window [yu..yd] x [xl..xr], read straight from the
+// table (each entry is visited about once, so it is not staged).
+struct TabView {
+  const uint32_t* t;
+  int dim, L, xl, yu, w;
+  __device__ __forceinline__ uint32_t operator[](int idx) const {
+    const int i = idx / w, j = idx - i * w;
+    return t[(i + yu + L) * dim + (j + xl + L)];
+  }
+};
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long w = __shfl_xor(v, o, 64);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int pix_dist(int a, int b, int metric) {
+  const int d = a - b;
+  return metric == kMetricSAD ? (d < 0 ? -d : d) : d * d;
+}
+
+// Stage the 22x22 patch whose (3,3) is reference pel (x0, y0).
+__device__ void load_patch(const DecideDesc& d, int x0, int y0, uint8_t* P, int lane) {
+  for (int i = lane; i < kPatch * kPatch; i += 64) {
+    const int r = i / kPatch, c = i - r * kPatch;
+    P[i] = d.ref[(long)(y0 - 3 + r) * d.ref_stride + (x0 - 3 + c)];
+  }
+}
+
+// Integer decision by argmin over the table with the predictor-distance key
+// (filter off): lower cost, then closer to (pXr, pYr), then later in raster.
+__device__ void table_argmin(const TabView& tab, int xl, int xr, int yu, int yd, int pXr, int pYr,
+                             int lane, int* mx, int* my, int* minD) {
+  const int w = xr - xl + 1, n = w * (yd - yu + 1);
+  unsigned long long best = ~0ull;
+  for (int idx = lane; idx < n; idx += 64) {
+    const int i = yu + idx / w, j = xl + idx % w;
+    const uint32_t c = tab[idx];
+    unsigned long long rr, d2;
+    if (i == 0 && j == 0) {
+      rr = 0x1FFFFull;
+    } else {
+      rr = 0xFFFFull - (unsigned long long)((i + 64) * 256 + (j + 64));
+    }
+    const int ex = j - pXr, ey = i - pYr;
+    d2 = (unsigned long long)(ex * ex + ey * ey);
+    const unsigned long long key = ((unsigned long long)c << 40) | (d2 << 17) | rr;
+    best = key < best ? key : best;
+  }
+  best = wave_min_u64(best);
+  const uint32_t rr = (uint32_t)(best & 0x1FFFF);
+  if (rr == 0x1FFFF) {
+    *mx = 0;
+    *my = 0;
+  } else {
+    const uint32_t rank = 0xFFFFu - rr;
+    *my = (int)(rank >> 8) - 64;
+    *mx = (int)(rank & 255) - 64;
+  }
+  *minD = (int)(best >> 40);
+}
+
+// The reference's raster scan with its running-best filter (Full.cpp:401-446).
+__device__ void table_replay(const TabView& tab, int xl, int xr, int yu, int yd, int pXr, int pYr,
+                             int lane, int* pmx, int* pmy, int* pminD) {
+  const int w = xr - xl + 1, n = w * (yd - yu + 1);
+  int mx = 0, my = 0;
+  int minD = (int)tab[(0 - yu) * w + (0 - xl)];
+  int sidx = 0;
+  while (sidx < n) {
+    const int idx = sidx + lane;
+    bool acc = false;
+    int i = 0, j = 0, c = 0;
+    if (idx < n) {
+      i = yu + idx / w;
+      j = xl + idx % w;
+      if ((i | j) != 0 && !((i + my) < yu || (i + my) > yd || (j + mx) < xl || (j + mx) > xr)) {
+        c = (int)tab[idx];
+        if (c < minD) {
+          acc = true;
+        } else if (c == minD) {
+          const int cx = mx - pXr, cy = my - pYr, nx = j - pXr, ny = i - pYr;
+          acc = (cy * cy + cx * cx) - (ny * ny + nx * nx) >= 0;
+        }
+      }
+    }
+    const unsigned long long mask = __ballot(acc);
+    if (mask == 0) {
+      sidx += 64;
+      continue;
+    }
+    const int f = __ffsll((long long)mask) - 1;
+    minD = __shfl(c, f, 64);
+    mx = __shfl(j, f, 64);
+    my = __shfl(i, f, 64);
+    sidx += f + 1;
+  }
+  *pmx = mx;
+  *pmy = my;
+  *pminD = minD;
+}
+
+// Build the half-pel lattice around full-pel origin (ox, oy) (patch staged).
+__device__ void build_half_grid(WaveLds& s, int lane) {
+  for (int i = lane; i < 17 * kPatch; i += 64) {
+    const int a = i / kPatch, x = i - a * kPatch;
+    s.hv[i] = (int16_t)vtap_p(s.P, x, a + 2);
+  }
+  __syncthreads();
+  for (int i = lane; i < kHalf * kHalf; i += 64) {
+    const int hy = i / kHalf, hx = i - hy * kHalf;
+    int v;
+    if (!(hy & 1)) {
+      const int pr = hy / 2 + 2;
+      if (!(hx & 1)) v = s.P[pr * kPatch + hx / 2 + 2];
+      else v = b_p(s.P, (hx + 3) / 2, pr);
+    } else {
+      const int a = (hy - 1) / 2;
+      if (!(hx & 1)) {
+        v = clip255((s.hv[a * kPatch + hx / 2 + 2] + 16) >> 5);
+      } else {
+        const int xc = (hx + 3) / 2;
+        const int16_t* r = s.hv + a * kPatch;
+        const int t = ICU_TAP6((int)r[xc - 2], (int)r[xc - 1], (int)r[xc], (int)r[xc + 1],
+                               (int)r[xc + 2], (int)r[xc + 3]);
+        v = clip255((t + 512) >> 10);
+      }
+    }
+    s.Hg[i] = (int16_t)v;
+  }
+  __syncthreads();
+}
+
+struct Decision {
+  int mvx, mvy, dist;
+  int imx, imy, iminD;
+};
+
+__device__ Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, int predX, int predY,
+                              int lane) {
+  const int col = mb % d.mbw, row = mb / d.mbw;
+  const int n = col * 16, m = row * 16;
+  const int slot = d.mb_base + mb;
+  // input block
+  for (int i = lane; i < 256; i += 64) s.cur[i] = d.cur[(long)(m + (i >> 4)) * d.cur_stride + n + (i & 15)];
+
+  const int predXQ = predX % 4, predYQ = predY % 4;
+  int predX0 = predX / 4, predY0 = predY / 4;
+  const int pXr = predX < 0 ? (predX - 2) / 4 : (predX + 2) / 4;
+  const int pYr = predY < 0 ? (predY - 2) / 4 : (predY + 2) / 4;
+
+  int mx, my, minD, zeroD;
+  int xl, xr, yu, yd;
+  motion_window(n, m, d.W, d.H, d.L, &xl, &xr, &yu, &yd);
+  if (d.table) {
+    TabView tab;
+    tab.t = d.table + d.table_base + (long long)mb * d.table_dim * d.table_dim;
+    tab.dim = d.table_dim;
+    tab.L = d.L;
+    tab.xl = xl;
+    tab.yu = yu;
+    tab.w = xr - xl + 1;
+    zeroD = (int)tab[(0 - yu) * tab.w + (0 - xl)];
+    if (d.filter) table_replay(tab, xl, xr, yu, yd, pXr, pYr, lane, &mx, &my, &minD);
+    else table_argmin(tab, xl, xr, yu, yd, pXr, pYr, lane, &mx, &my, &minD);
+  } else {
+    const IntResult r = d.ires[slot];
+    mx = r.mx;
+    my = r.my;
+    minD = r.cost;
+    zeroD = r.zero;
+  }
+  Decision out;
+  out.imx = mx;
+  out.imy = my;
+  out.iminD = minD;
+
+  int mvx = mx * 4, mvy = my * 4;
+  if (d.subpel >= 1) {
+    load_patch(d, n + mx, m + my, s.P, lane);
+    __syncthreads();
+    build_half_grid(s, lane);
+    // half-pel candidates (strictly better only)
+    int hb = 0, hxh = 0, hyh = 0;
+    for (int k = 0; k < 8; k++) {
+      const int sx = c_subpos[k][0], sy = c_subpos[k][1];
+      int acc = 0;
+      for (int p = lane; p < 256; p += 64) {
+        const int r = p >> 4, c = p & 15;
+        acc += pix_dist(s.cur[p], s.Hg[(2 + 2 * r + sy) * kHalf + 2 + 2 * c + sx], d.metric);
+      }
+      acc = wave_sum(acc);
+      if (acc < minD) {
+        minD = acc;
+        hxh = sx;
+        hyh = sy;
+        hb = 1;
+      }
+    }
+    (void)hb;
+    int qmx = 2 * hxh, qmy = 2 * hyh;
+    if (d.subpel >= 2) {
+      const int hx = 2 * hxh, hy = 2 * hyh;
+      for (int k = 0; k < 8; k++) {
+        const int dx = c_subpos[k][0], dy = c_subpos[k][1];
+        int acc = 0;
+        for (int p = lane; p < 256; p += 64) {
+          const int r = p >> 4, c = p & 15;
+          const int cr = 2 + 2 * r + hyh, cc = 2 + 2 * c + hxh;
+          int a, b;
+          const int16_t* Hg = s.Hg;
+          if (dy == -1) {
+            if (dx == -1) { a = Hg[(cr - 1) * kHalf + cc - 1]; b = Hg[cr * kHalf + cc]; }
+            else if (dx == 0) { a = Hg[(cr - 1) * kHalf + cc]; b = Hg[cr * kHalf + cc]; }
+            else { a = Hg[(cr - 1) * kHalf + cc]; b = Hg[cr * kHalf + cc + 1]; }
+          } else if (dy == 0) {
+            if (dx == -1) { a = Hg[cr * kHalf + cc - 1]; b = Hg[cr * kHalf + cc]; }
+            else { a = Hg[cr * kHalf + cc + 1]; b = Hg[cr * kHalf + cc]; }
+          } else {
+            if (dx == -1) { a = Hg[cr * kHalf + cc - 1]; b = Hg[(cr + 1) * kHalf + cc]; }
+            else if (dx == 0) { a = Hg[(cr + 1) * kHalf + cc]; b = Hg[cr * kHalf + cc]; }
+            else { a = Hg[(cr + 1) * kHalf + cc]; b = Hg[cr * kHalf + cc + 1]; }
+          }
+          acc += pix_dist(s.cur[p], (a + b + 1) >> 1, d.metric);
+        }
+        acc = wave_sum(acc);
+        if (acc < minD) {
+          minD = acc;
+          qmx = hx + dx;
+          qmy = hy + dy;
+        }
+      }
+    }
+    mvx += qmx;
+    mvy += qmy;
+    __syncthreads();
+  }
+
+  int outx = mvx, outy = mvy, outd = minD;
+  if (d.bestvec) {
+    if (predX0 + n > d.W) predX0 = 16;
+    if (predX0 + n < -16) predX0 = -16;
+    if (predY0 + m > d.H) predY0 = 16;
+    if (predY0 + m < -16) predY0 = -16;
+    const int pX = predX0 * 4 + predXQ, pY = predY0 * 4 + predYQ;
+    int predD;
+    if (predXQ == 0 && predYQ == 0 && predX0 == 0 && predY0 == 0) {
+      predD = zeroD;
+    } else {
+      // QuarterRead at the predictor: negative fractions reflect to the
+      // full pel to the left / above (OverlayMem2Dv2.cpp:709-724).
+      int fox = 0, fx = predXQ, foy = 0, fy = predYQ;
+      if (predXQ < 0) { fox = -1; fx = 4 + predXQ; }
+      if (predYQ < 0) { foy = -1; fy = 4 + predYQ; }
+      load_patch(d, predX0 + n + fox, predY0 + m + foy, s.P, lane);
+      __syncthreads();
+      int acc = 0;
+      for (int p = lane; p < 256; p += 64) {
+        const int r = p >> 4, c = p & 15;
+        acc += pix_dist(s.cur[p], qpel_p(s.P, 3 + c, 3 + r, fx, fy), d.metric);
+      }
+      predD = wave_sum(acc);
+      __syncthreads();
+    }
+    // fifo [best, pred, zero, zero, zero]; strictly smaller cost wins
+    const int xs[3] = {mvx, pX, 0}, ys[3] = {mvy, pY, 0}, ds[3] = {minD, predD, zeroD};
+    double cc = best_vector_cost(ds[0], (xs[0] - pX) * (xs[0] - pX) + (ys[0] - pY) * (ys[0] - pY));
+    int pos = 0;
+    for (int i = 1; i < 3; i++) {
+      const double nc = best_vector_cost(ds[i], (xs[i] - pX) * (xs[i] - pX) + (ys[i] - pY) * (ys[i] - pY));
+      if (nc < cc) {
+        cc = nc;
+        pos = i;
+      }
+    }
+    outx = xs[pos];
+    outy = ys[pos];
+    outd = ds[pos];
+  }
+  out.mvx = outx;
+  out.mvy = outy;
+  out.dist = outd;
+  if (lane == 0) {
+    const uint32_t w = (uint32_t)(uint16_t)(int16_t)outx | ((uint32_t)(uint16_t)(int16_t)outy << 16);
+    __hip_atomic_store((uint32_t*)(d.mv_out + 2 * slot), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    d.dist_out[slot] = outd;
+    if (d.int_out) {
+      d.int_out[3 * slot] = out.imx;
+      d.int_out[3 * slot + 1] = out.imy;
+      d.int_out[3 * slot + 2] = out.iminD;
+    }
+  }
+  return out;
+}
+
+__device__ __forceinline__ int median3(int x, int y, int z) {
+  int mn, mx;
+  if (y - z < 0) { mn = y; mx = z; } else { mn = z; mx = y; }
+  if (x < mn) mn = x;
+  if (x > mx) mx = x;
+  return x + y + z - mn - mx;
+}
+
+__global__ __launch_bounds__(64) void decide_parallel_kernel(const DecideDesc* __restrict__ descs,
+                                                             const DecideItem* __restrict__ items,
+                                                             int nitems) {
+  __shared__ WaveLds s;
+  const int lane = threadIdx.x;
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const DecideItem item = items[it];
+    decide_mb(descs[item.stream], s, item.mb, 0, 0, lane);
+    __syncthreads();
+  }
+}
+
+// One wave per MB row; rows are handed out by ticket in start order, so the
+// row a wave waits on always belongs to a wave that is already running.
+__global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* __restrict__ desc) {
+  __shared__ WaveLds s;
+  __shared__ int row_s;
+  const DecideDesc& d = *desc;
+  const int lane = threadIdx.x;
+  int* progress = d.progress;
+  int* ticket = d.ticket;
+  if (lane == 0) row_s = atomicAdd(&ticket[0], 1);
+  __syncthreads();
+  const int row = row_s;
+  if (row >= d.mbh) return;
+  for (int col = 0; col < d.mbw; col++) {
+    if (row > 0) {
+      // above-right (or above for the last column) must be final
+      const int need = min(col + 2, d.mbw);
+      if (lane == 0) {
+        int spins = 0;
+        while (__hip_atomic_load(&progress[row - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          if (++spins > kSpinLimit) {
+            atomicExch(&ticket[1], 1);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      __syncthreads();
+    }
+    // neighbour vectors (MacroBlockH264.cpp:757-816)
+    const int mb = row * d.mbw + col;
+    // each (x, y) pair is one aligned 32-bit word; read it coherently
+    const uint32_t* mvw = (const uint32_t*)(d.mv_out + 2 * d.mb_base);
+    auto ld = [&](int k, int* x, int* y) {
+      const uint32_t w = __hip_atomic_load(&mvw[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      *x = (int)(int16_t)(w & 0xFFFF);
+      *y = (int)(int16_t)(w >> 16);
+    };
+    int Ax = 0, Ay = 0, Bx = 0, By = 0, Cx = 0, Cy = 0;
+    const bool hasB = row > 0, hasC = row > 0 && col + 1 < d.mbw;
+    if (col > 0) ld(mb - 1, &Ax, &Ay);
+    if (hasB) ld(mb - d.mbw, &Bx, &By);
+    if (hasC) ld(mb - d.mbw + 1, &Cx, &Cy);
+    else if (col > 0 && row > 0) ld(mb - d.mbw - 1, &Cx, &Cy);
+    if (!hasB && !hasC) { Bx = Ax; By = Ay; Cx = Ax; Cy = Ay; }
+    const int px = median3(Ax, Bx, Cx), py = median3(Ay, By, Cy);
+    decide_mb(d, s, mb, px, py, lane);
+    __syncthreads();
+    if (lane == 0) {
+      __threadfence();
+      __hip_atomic_store(&progress[row], col + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+__global__ void fill_i32_kernel(int* p, int v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void sqrt_raw_kernel(const int* m, double* out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = sqrt_int_rn(m[i]);
+}
+
+}  // namespace
+
+void launch_decide_parallel(const DecideDesc* d_descs, const DecideItem* d_items, int nitems,
+                            hipStream_t st) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(decide_parallel_kernel, dim3(nitems), dim3(64), 0, st, d_descs, d_items, nitems);
+}
+
+void launch_decide_wavefront(const DecideDesc* d_desc, int* /*unused*/, int mbh, hipStream_t st) {
+  hipLaunchKernelGGL(decide_wavefront_kernel, dim3(mbh), dim3(64), 0, st, d_desc);
+}
+
+void launch_fill_i32(int* p, int v, int n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
+}
+
+void launch_sqrt_check(const int* m, double* out, int n, hipStream_t st) {
+  hipLaunchKernelGGL(sqrt_raw_kernel, dim3((n + 255) / 256), dim3(256), 0, st, m, out, n);
+}
+
+}  // namespace icu

@@ -1,0 +1,172 @@
+/*
+ * icu_hip.h - C ABI of the MI355X (gfx950) motion-estimation and H.264 4x4
+ * transform/quant library.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns 0 on success or a negative ICU_E* code; icu_last_error() then holds
+ * a message for the calling thread.  Host-buffer entry points are
+ * synchronous.  *_device entry points take device pointers and a HIP stream
+ * (hipStream_t passed as void*, NULL = the handle's own stream) and are
+ * asynchronous on that stream.
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef ICU_HIP_H
+#define ICU_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ICU_OK 0
+#define ICU_E_ARG -1      /* invalid argument or configuration */
+#define ICU_E_RANGE -2    /* input sample outside 0..255 */
+#define ICU_E_DEVICE -3   /* HIP runtime / allocation failure */
+#define ICU_E_STATE -4    /* handle not ready, or a kernel reported failure */
+
+/* Metric, window and mode switches of the estimator.  The first value of
+ * each is what the reference class ships with. */
+#define ICU_METRIC_SSD 0 /* MotionEstimatorH264ImplFull.cpp:74 (#undef MEH264IF_ABS_DIFF) */
+#define ICU_METRIC_SAD 1 /* the MEH264IF_ABS_DIFF build */
+#define ICU_WINDOW_REFERENCE 0 /* +-(R-1): GetMotionRange, Full.cpp:399 -> :1029-1081 */
+#define ICU_WINDOW_FULL 1      /* +-R, same 16-pel picture-border clamp */
+
+typedef struct icu_me_config {
+  int width, height; /* luma size, multiples of 16 */
+  int range;         /* full-pel search range R (the reference's motionRange / 4), 1..64 */
+  int metric;        /* ICU_METRIC_SSD | ICU_METRIC_SAD */
+  int window;        /* ICU_WINDOW_REFERENCE | ICU_WINDOW_FULL */
+  int filter;        /* 1: reference running-best candidate filter (Full.cpp:410-411); 0: off */
+  int subpel;        /* 0 integer, 1 half, 2 half then quarter (reference) */
+  int predictor;     /* 0 zero vector, 1 H.264 median of A/B/C(D) (H264MotionVectorPredictorImpl1) */
+  int best_vector;   /* 1: GetBestVector over {best, predictor, zero} (reference); 0: keep best */
+  int tile_rows;     /* 0: chosen by the tuner; else candidate rows per lane: 8, 11, 13 or 16 */
+  int mbs_per_task;  /* 0: chosen by the tuner; else MBs staged per workgroup task (1..16) */
+} icu_me_config;
+
+typedef struct icu_me icu_me;
+
+/* Fill cfg with the reference class's behaviour for a W x H picture and a
+ * quarter-pel motionRange, as MotionEstimatorH264ImplFull(pSrc, pRef, W, H,
+ * motionRange, pMVPred) with an H264MotionVectorPredictorImpl1 predictor
+ * (include/CodecUtils/MotionEstimatorH264ImplFull.h:77-90). */
+int icu_me_config_reference(icu_me_config* cfg, int width, int height, int motion_range_qpel);
+
+/* Create / destroy an estimator: replaces the constructor + Create() and the
+ * destructor of MotionEstimatorH264ImplFull
+ * (src/CodecUtils/MotionEstimatorH264ImplFull.cpp:111-153,216-320). */
+int icu_me_create(const icu_me_config* cfg, icu_me** out);
+int icu_me_destroy(icu_me* h);
+
+/* Estimate one frame pair from host int16 planes (W*H samples each, 0..255):
+ * replaces IMotionEstimator::Estimate(long* avgDistortion)
+ * (include/CodecUtils/IMotionEstimator.h:86-96, Full.cpp:330-611).
+ * mv_out: 2 int16 per MB (x, y) in quarter pels, raster order - the
+ * VectorStructList SIMPLE2D layout (src/CodecUtils/VectorStructList.cpp:120-196).
+ * dist_out (nullable): distortion of each chosen vector.  include (nullable):
+ * one flag per MB; avg_out receives sum(dist of included) / count, or 0
+ * without a map, as the reference's avgDistortion. */
+int icu_me_estimate(icu_me* h, const int16_t* cur, const int16_t* ref, const uint8_t* include,
+                    int16_t* mv_out, int32_t* dist_out, long* avg_out);
+
+/* Same on device-resident 8-bit planes (stride = width).  d_mv / d_dist as
+ * above, device memory; d_int (nullable) receives the integer-stage
+ * (mx, my, cost) per MB. */
+int icu_me_estimate_device(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int16_t* d_mv,
+                           int32_t* d_dist, int32_t* d_int, void* stream);
+
+/* icu_me_estimate_device with HIP events between its stages, recorded on
+ * the stream the kernels run on; waits for completion and writes the
+ * duration of each stage in ms: [0] reference padding, [1] integer search,
+ * [2] decision + sub-pel refinement.  No reference counterpart (the
+ * reference has no timers); used by the benchmark. */
+int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int16_t* d_mv,
+                          int32_t* d_dist, void* stream, float* stage_ms);
+
+/* Tile choice in effect for this handle (after tuning). */
+int icu_me_get_tiling(icu_me* h, int* tile_rows, int* mbs_per_task);
+
+/* Time each candidate tiling on the given device frames and keep the
+ * fastest (the tuner).  reps: timed repetitions per candidate. */
+int icu_me_autotune(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int reps);
+
+/* Luma motion compensation + residual coding of one frame on the device:
+ * prediction from the handle's padded reference (the last estimated frame)
+ * at the vectors d_mv (MotionCompensatorH264ImplStd::Compensate,
+ * src/CodecUtils/MotionCompensatorH264ImplStd.cpp:323-437), residual
+ * (OverlayMem2Dv2::Sub16x16), forward IT+Q (FastForward4x4ITImpl1), levels
+ * out, dequant + inverse IT (rows first: FastInverse4x4ITImpl1; cols first:
+ * FastInverse4x4On16x16ITImpl1) and clipped reconstruction
+ * (OverlayMem2Dv2::Add16x16WithClip255).  Levels and pred are int16 W*H in
+ * picture layout, recon 8-bit W*H.  d_pred is nullable. */
+int icu_code_frame_device(icu_me* h, const uint8_t* d_cur, const int16_t* d_mv, int qp, int intra,
+                          int cols_first, int16_t* d_levels, uint8_t* d_recon, int16_t* d_pred,
+                          void* stream);
+
+/* Batched 4x4 transforms on host buffers.  layout 0: contiguous 4x4 blocks
+ * (16 int16 each, raster within the block); layout 1: 16x16 raster MBs of
+ * 16 blocks (the FastForward4x4On16x16ITImpl1 layout).  mode 0 IT+Q, 1 IT
+ * only, 2 Q only (IForwardTransform::SetMode).
+ * icu_fwd4x4 replaces IForwardTransform::Transform + SetParameter(QUANT_ID,
+ *   INTRA_FLAG_ID) (include/CodecUtils/IForwardTransform.h:56-126,
+ *   src/CodecUtils/FastForward4x4ITImpl1.cpp:100-278).
+ * icu_inv4x4 replaces IInverseTransform::InverseTransform + SetParameter +
+ *   SetScale (include/CodecUtils/IInverseTransform.h:56-115,
+ *   src/CodecUtils/FastInverse4x4ITImpl1.cpp:119-302); cols_first selects the
+ *   FastInverse4x4On16x16ITImpl1 order (:120-360).  weights16 nullable (flat 16). */
+int icu_fwd4x4(const int16_t* in, int16_t* out, long nblocks, int qp, int intra, int mode, int layout);
+int icu_inv4x4(const int16_t* in, int16_t* out, long nblocks, int qp, int mode, int layout,
+               int cols_first, const int* weights16);
+int icu_fwd4x4_device(const int16_t* d_in, int16_t* d_out, long nblocks, int qp, int intra, int mode,
+                      int layout, void* stream);
+int icu_inv4x4_device(const int16_t* d_in, int16_t* d_out, long nblocks, int qp, int mode, int layout,
+                      int cols_first, const int* d_weights16, void* stream);
+
+/* Batched estimation of many streams of mixed size and range in one launch
+ * sequence (no reference counterpart: the reference estimates one frame per
+ * call).  cfgs[i] configures stream i; predictor must be 0 (zero). */
+typedef struct icu_me_batch icu_me_batch;
+int icu_me_batch_create(const icu_me_config* cfgs, int nstreams, icu_me_batch** out);
+int icu_me_batch_destroy(icu_me_batch* b);
+/* Total MB slots; stream i's MBs start at slot icu_me_batch_mb_base(b, i). */
+int icu_me_batch_num_mbs(icu_me_batch* b);
+int icu_me_batch_mb_base(icu_me_batch* b, int stream);
+/* d_curs/d_refs: host arrays of nstreams device pointers (8-bit, stride W).
+ * d_mv: 2 int16 per MB slot, d_dist: 1 int32 per slot (device). */
+int icu_me_batch_estimate_device(icu_me_batch* b, const uint8_t* const* d_curs,
+                                 const uint8_t* const* d_refs, int16_t* d_mv, int32_t* d_dist,
+                                 void* stream);
+/* Capture the whole batch (padding, one persistent search launch over the
+ * MB work queue, decisions) into a HIP graph bound to these buffers, then
+ * replay it with icu_me_batch_graph_launch. */
+int icu_me_batch_capture(icu_me_batch* b, const uint8_t* const* d_curs, const uint8_t* const* d_refs,
+                         int16_t* d_mv, int32_t* d_dist);
+int icu_me_batch_graph_launch(icu_me_batch* b, void* stream);
+
+/* Errors and a minimal device runtime for callers without their own. */
+const char* icu_last_error(void);
+int icu_device_info(char* name, int name_len, int* compute_units, int* clock_khz);
+void* icu_malloc(size_t bytes);
+int icu_free(void* d);
+int icu_memcpy_htod(void* d, const void* h, size_t bytes, void* stream);
+int icu_memcpy_dtoh(void* h, const void* d, size_t bytes, void* stream);
+int icu_memset(void* d, int value, size_t bytes, void* stream);
+void* icu_stream_create(void);
+int icu_stream_destroy(void* stream);
+int icu_stream_sync(void* stream);
+void* icu_event_create(void);
+int icu_event_destroy(void* ev);
+int icu_event_record(void* ev, void* stream);
+int icu_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* Correctly rounded sqrt of integers used by the final vector choice,
+ * evaluated on the device for n values (test hook). */
+int icu_sqrt_check_device(const int* d_m, double* d_out, int n, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

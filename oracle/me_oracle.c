@@ -243,6 +243,16 @@ static const int SUBPOS[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0},
 int orc_me_estimate(const orc_me_cfg *cfg, const int16_t *cur, const int16_t *refp,
                     const uint8_t *include, int16_t *mv_out, int32_t *dist_out,
                     int32_t *int_out, long *avg_out) {
+  return orc_me_estimate_subset(cfg, cur, refp, include, 1, mv_out, dist_out, int_out, avg_out);
+}
+
+/* As orc_me_estimate, but only every `step`-th MB in raster order is decided
+ * (the others are left untouched).  Only meaningful with the zero predictor,
+ * where MBs do not depend on each other. */
+int orc_me_estimate_subset(const orc_me_cfg *cfg, const int16_t *cur, const int16_t *refp,
+                           const uint8_t *include, int step, int16_t *mv_out, int32_t *dist_out,
+                           int32_t *int_out, long *avg_out) {
+  if (step < 1 || (step > 1 && cfg->pred)) return -3;
   const int W = cfg->width, H = cfg->height, R = cfg->range;
   if (W <= 0 || H <= 0 || (W & 15) || (H & 15) || R < 1) return -1;
   const int mbw = W / 16, mbh = H / 16;
@@ -257,6 +267,7 @@ int orc_me_estimate(const orc_me_cfg *cfg, const int16_t *cur, const int16_t *re
   for (int row = 0; row < mbh; row++)
     for (int col = 0; col < mbw; col++) {
       const int n = col * 16, m = row * 16, vec = row * mbw + col;
+      if (vec % step) continue;
       int predX = 0, predY = 0;
       if (cfg->pred) orc_median_pred(mv_out, mbw, col, row, &predX, &predY);
       int predXQ = predX % 4, predYQ = predY % 4;

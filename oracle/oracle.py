@@ -56,6 +56,8 @@ def lib():
         L.orc_me_estimate.argtypes = [C.POINTER(MeCfg), _i16p, _i16p, C.c_void_p, _i16p,
                                       C.c_void_p, C.c_void_p, C.POINTER(C.c_long)]
         L.orc_me_estimate.restype = C.c_int
+        L.orc_me_estimate_subset.argtypes = [C.POINTER(MeCfg), _i16p, _i16p, C.c_void_p, C.c_int,
+                                             _i16p, C.c_void_p, C.c_void_p, C.POINTER(C.c_long)]
         L.orc_quarter_read.argtypes = [_i16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_int, _i32p]
         L.orc_fwd4x4_n.argtypes = [_i16p, C.c_long, C.c_int, C.c_int, C.c_int]
@@ -91,9 +93,10 @@ def ref_lib():
     return _ref
 
 
-def me_estimate(cur, ref, R, include=None, **modes):
+def me_estimate(cur, ref, R, include=None, step=1, **modes):
     """Run the oracle estimator. Returns (mv (nMB,2) int16, dist (nMB,) int32,
-    integer stage (nMB,3) int32, avg)."""
+    integer stage (nMB,3) int32, avg).  step > 1 decides only every step-th
+    MB (zero predictor only); the rest stay zero."""
     cur = np.ascontiguousarray(cur, dtype=np.int16)
     ref = np.ascontiguousarray(ref, dtype=np.int16)
     H, W = cur.shape
@@ -109,8 +112,8 @@ def me_estimate(cur, ref, R, include=None, **modes):
     if include is not None:
         include = np.ascontiguousarray(include, dtype=np.uint8)
         inc = include.ctypes.data
-    rc = lib().orc_me_estimate(C.byref(cfg), cur, ref, inc, mv, dist.ctypes.data,
-                               integ.ctypes.data, C.byref(avg))
+    rc = lib().orc_me_estimate_subset(C.byref(cfg), cur, ref, inc, step, mv, dist.ctypes.data,
+                                      integ.ctypes.data, C.byref(avg))
     if rc != 0:
         raise ValueError(f"oracle me_estimate failed ({rc})")
     return mv.reshape(n, 2), dist, integ.reshape(n, 3), avg.value

@@ -1,0 +1,64 @@
+"""CPU-side checks of the C ABI: the library loads, exports every declared
+symbol, and rejects bad arguments before touching a device."""
+import ctypes as C
+import subprocess
+
+import numpy as np
+import pytest
+
+from imagecodingutils_amd import capi
+
+
+def test_library_exports_every_header_symbol():
+    syms = capi.header_symbols()
+    assert len(syms) >= 30
+    missing = [s for s in syms if not hasattr(capi.lib, s)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", str(capi.LIB_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert set(syms) <= exported
+
+
+def test_binding_covers_header():
+    assert set(capi.header_symbols()) == set(capi._SIGS)
+
+
+@pytest.mark.parametrize("bad", [
+    dict(width=30), dict(height=0), dict(R=0), dict(R=65), dict(metric=2), dict(subpel=3),
+    dict(predictor=2), dict(tile_rows=5), dict(mbs_per_task=17)])
+def test_create_rejects_bad_config(bad):
+    kw = dict(width=64, height=48, R=8)
+    kw.update(bad)
+    cfg = capi.make_config(kw.pop("width"), kw.pop("height"), kw.pop("R"), **kw)
+    h = C.c_void_p()
+    rc = capi.lib.icu_me_create(C.byref(cfg), C.byref(h))
+    assert rc == capi.ICU_E_ARG
+    assert capi.lib.icu_last_error()
+
+
+def test_reference_config_matches_shipped_class():
+    cfg = capi.reference_config(352, 288, 64)
+    assert (cfg.range, cfg.metric, cfg.window, cfg.filter, cfg.subpel, cfg.predictor,
+            cfg.best_vector) == (16, 0, 0, 1, 2, 1, 1)
+
+
+def test_transform_argument_errors():
+    x = np.zeros((16, 16), np.int16)
+    with pytest.raises(capi.IcuError) as e:
+        capi.fwd4x4(x, 52, 0, 0)
+    assert e.value.code == capi.ICU_E_ARG
+    with pytest.raises(capi.IcuError):
+        capi.fwd4x4(x, 28, 0, 3)
+    with pytest.raises(capi.IcuError):
+        capi.inv4x4(x[:3], 28, 0, layout=1)  # 3 blocks are not whole MBs
+
+
+def test_batch_rejects_predictor_and_mixed_metrics():
+    a = capi.make_config(64, 48, 8, metric=1, filt=0, predictor=1)
+    with pytest.raises(capi.IcuError):
+        capi.MotionEstimatorBatch([a])
+    b = capi.make_config(64, 48, 8, metric=1, filt=0, predictor=0)
+    c = capi.make_config(64, 48, 8, metric=0, filt=0, predictor=0)
+    with pytest.raises(capi.IcuError):
+        capi.MotionEstimatorBatch([b, c])

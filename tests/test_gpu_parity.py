@@ -1,0 +1,246 @@
+"""GPU parity: the HIP path through the C ABI against the oracle and the
+reference's golden fixtures.  Integer work, so every comparison is exact."""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+capi = pytest.importorskip("imagecodingutils_amd.capi")
+from imagecodingutils_amd import synth  # noqa: E402
+
+
+def _cfg(w, h, R, **m):
+    base = dict(metric=0, window=0, filt=1, subpel=2, predictor=1, best_vector=1)
+    base.update(m)
+    return capi.make_config(w, h, R, **base)
+
+
+def _oracle_modes(m):
+    return dict(metric=m.get("metric", 0), window=m.get("window", 0), filter=m.get("filt", 1),
+                subpel=m.get("subpel", 2), pred=m.get("predictor", 1), bestvec=m.get("best_vector", 1))
+
+
+# ------------------------------------------------------------- transforms
+def test_forward_transform_matches_reference_fixtures():
+    fx = np.load(GOLDEN / "transforms.npz")
+    for qp in range(52):
+        for intra in (0, 1):
+            for mode in (0, 1, 2):
+                got = capi.fwd4x4(fx["fwd_in"], qp, intra, mode, layout=0)
+                np.testing.assert_array_equal(got, fx["fwd"][qp, intra, mode], err_msg=f"qp{qp} i{intra} m{mode}")
+                got = capi.fwd4x4(fx["fwd_mb_in"], qp, intra, mode, layout=1)
+                np.testing.assert_array_equal(got, fx["fwd_mb"][qp, intra, mode])
+
+
+def test_inverse_transform_matches_reference_fixtures():
+    fx = np.load(GOLDEN / "transforms.npz")
+    for qp in range(52):
+        x = fx["inv_in"][qp]
+        for mode in (0, 1, 2):
+            np.testing.assert_array_equal(capi.inv4x4(x, qp, mode, 0, 0), fx["inv"][qp, mode])
+            np.testing.assert_array_equal(capi.inv4x4(x.reshape(4, 256), qp, mode, 1, 1), fx["inv_mb"][qp, mode])
+        np.testing.assert_array_equal(capi.inv4x4(x, qp, 0, 0, 0, weights=fx["weights"]),
+                                      fx["inv_weighted"][qp])
+
+
+def test_transforms_match_oracle_on_large_batch():
+    rng = np.random.default_rng(11)
+    x = rng.integers(-255, 256, (20000, 16)).astype(np.int16)
+    for qp in (0, 17, 22, 28, 34, 40, 51):
+        lv = capi.fwd4x4(x, qp, 0, 0)
+        np.testing.assert_array_equal(lv, orc.fwd4x4(x, qp, 0, 0))
+        for cf in (0, 1):
+            np.testing.assert_array_equal(capi.inv4x4(lv, qp, 0, 0, cf), orc.inv4x4(lv, qp, 0, cf))
+
+
+# ------------------------------------------------------- final-choice sqrt
+def test_sqrt_is_correctly_rounded_over_the_vector_domain():
+    n = 1 << 21  # |v - pred|^2 stays below 2 * 640^2 < 2^20
+    m = np.arange(n, dtype=np.int32)
+    dm = capi.DeviceBuffer.from_array(m)
+    dout = capi.DeviceBuffer(8 * n)
+    capi._check(capi.lib.icu_sqrt_check_device(dm.ptr, dout.ptr, n, None))
+    capi.DeviceBuffer.sync()
+    got = dout.download(np.float64, n)
+    np.testing.assert_array_equal(got, np.sqrt(m.astype(np.float64)))
+
+
+# ----------------------------------------------------- estimator, shipped
+ME_FIXTURES = sorted(GOLDEN.glob("me_*.npz"))
+
+
+@pytest.mark.parametrize("path", ME_FIXTURES, ids=lambda p: p.stem)
+def test_as_shipped_estimator_matches_reference_fixture(path):
+    fx = np.load(path)
+    H, W = fx["cur"].shape
+    R = int(fx["R"])
+    est = capi.MotionEstimator(capi.reference_config(W, H, 4 * R) if int(fx["pred"]) else
+                               _cfg(W, H, R, predictor=0))
+    inc = fx["include"] if "include" in fx.files else None
+    mv, dist, avg = est.estimate(fx["cur"].astype(np.int16), fx["ref"].astype(np.int16), include=inc)
+    np.testing.assert_array_equal(mv, fx["mv"])
+    assert avg == int(fx["avg"])
+
+
+# ----------------------------------------------------------- mode matrix
+MODES = [
+    dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=1),   # bench mode
+    dict(metric=1, window=0, filt=0, subpel=0, predictor=0, best_vector=0),
+    dict(metric=1, window=1, filt=0, subpel=2, predictor=0, best_vector=0),
+    dict(metric=0, window=1, filt=0, subpel=1, predictor=0, best_vector=1),
+    dict(metric=1, window=0, filt=1, subpel=2, predictor=0, best_vector=1),
+    dict(metric=1, window=1, filt=1, subpel=2, predictor=1, best_vector=1),
+    dict(metric=0, window=0, filt=0, subpel=2, predictor=1, best_vector=1),
+    dict(metric=1, window=0, filt=0, subpel=1, predictor=1, best_vector=0),
+]
+
+
+@pytest.mark.parametrize("modes", MODES, ids=lambda m: "-".join(f"{k}{v}" for k, v in m.items()))
+@pytest.mark.parametrize("size,R,seed", [((176, 144), 8, 1), ((352, 288), 16, 2), ((64, 48), 32, 3)])
+def test_estimator_modes_match_oracle(modes, size, R, seed):
+    W, H = size
+    fp = synth.rect_motion_pair(W, H, seed, n_rects=6, max_mv=min(R, 12))
+    est = capi.MotionEstimator(_cfg(W, H, R, **modes))
+    d_cur = capi.DeviceBuffer.from_array(fp.cur)
+    d_ref = capi.DeviceBuffer.from_array(fp.ref)
+    n = est.nmb
+    d_mv, d_dist, d_int = capi.DeviceBuffer(4 * n), capi.DeviceBuffer(4 * n), capi.DeviceBuffer(12 * n)
+    est.estimate_device(d_cur, d_ref, d_mv, d_dist, d_int)
+    capi.DeviceBuffer.sync()
+    capi.lib.icu_stream_sync(None)
+    mv = d_mv.download(np.int16, (n, 2))
+    dist = d_dist.download(np.int32, n)
+    integ = d_int.download(np.int32, (n, 3))
+    omv, odist, ointeg, _ = orc.me_estimate(fp.cur, fp.ref, R, **_oracle_modes(modes))
+    np.testing.assert_array_equal(integ, ointeg)
+    np.testing.assert_array_equal(mv, omv)
+    np.testing.assert_array_equal(dist, odist)
+
+
+def test_bench_workload_1080p_subset_matches_oracle():
+    """1080p, +-32 exhaustive SAD, half-pel: every 16th MB checked against the
+    oracle, and the whole frame recovers the painted motion."""
+    W, H, R = 1920, 1088, 32
+    fp = synth.rect_motion_pair(W, H, 1080, n_rects=24, max_mv=32, sigma_cur=3.0)
+    modes = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=1)
+    est = capi.MotionEstimator(_cfg(W, H, R, **modes))
+    mv, dist, _ = est.estimate(fp.cur.astype(np.int16), fp.ref.astype(np.int16))
+    omv, odist, _, _ = orc.me_estimate(fp.cur, fp.ref, R, step=16, **_oracle_modes(modes))
+    sel = np.arange(0, mv.shape[0], 16)
+    np.testing.assert_array_equal(mv[sel], omv[sel])
+    np.testing.assert_array_equal(dist[sel], odist[sel])
+    truth = (4 * fp.true_mv.reshape(-1, 2))
+    assert (np.abs(mv - truth) <= 2).all(axis=1).mean() > 0.8
+
+
+def test_estimate_is_deterministic_and_rerunnable():
+    fp = synth.rect_motion_pair(352, 288, 5, n_rects=8, max_mv=16)
+    est = capi.MotionEstimator(_cfg(352, 288, 16))
+    a = est.estimate(fp.cur.astype(np.int16), fp.ref.astype(np.int16))
+    b = est.estimate(fp.cur.astype(np.int16), fp.ref.astype(np.int16))
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_out_of_range_input_is_rejected():
+    est = capi.MotionEstimator(_cfg(64, 48, 8))
+    cur = np.full((48, 64), 300, np.int16)
+    with pytest.raises(capi.IcuError) as e:
+        est.estimate(cur, cur)
+    assert e.value.code == capi.ICU_E_RANGE
+
+
+# ----------------------------------------------------- MC + residual coding
+def test_motion_compensation_matches_reference_fixture():
+    fx = np.load(GOLDEN / "mc_cif.npz")
+    ref = fx["ref"]
+    H, W = ref.shape
+    est = capi.MotionEstimator(_cfg(W, H, 16, predictor=0))
+    d_ref = capi.DeviceBuffer.from_array(ref)
+    n = est.nmb
+    d_mv, d_dist = capi.DeviceBuffer(4 * n), capi.DeviceBuffer(4 * n)
+    est.estimate_device(d_ref, d_ref, d_mv, d_dist)  # pads the reference
+    d_mv.upload(fx["mv"].astype(np.int16))
+    d_lv, d_rc, d_pr = capi.DeviceBuffer(2 * W * H), capi.DeviceBuffer(W * H), capi.DeviceBuffer(2 * W * H)
+    est.code_frame_device(d_ref, d_mv, 28, 0, 0, d_lv, d_rc, d_pr)
+    capi.lib.icu_stream_sync(None)
+    capi.DeviceBuffer.sync(None)
+    import ctypes
+    pred = d_pr.download(np.int16, (H, W))
+    np.testing.assert_array_equal(pred, fx["pred"].astype(np.int16))
+
+
+@pytest.mark.parametrize("qp", [22, 28, 34])
+@pytest.mark.parametrize("cols_first", [0, 1])
+def test_code_frame_matches_oracle(qp, cols_first):
+    W, H, R = 352, 288, 16
+    fp = synth.rect_motion_pair(W, H, 40 + qp, n_rects=8, max_mv=12)
+    est = capi.MotionEstimator(_cfg(W, H, R, metric=1, window=1, filt=0, subpel=2, predictor=0))
+    d_cur, d_ref = capi.DeviceBuffer.from_array(fp.cur), capi.DeviceBuffer.from_array(fp.ref)
+    n = est.nmb
+    d_mv, d_dist = capi.DeviceBuffer(4 * n), capi.DeviceBuffer(4 * n)
+    d_lv, d_rc, d_pr = capi.DeviceBuffer(2 * W * H), capi.DeviceBuffer(W * H), capi.DeviceBuffer(2 * W * H)
+    est.estimate_device(d_cur, d_ref, d_mv, d_dist)
+    est.code_frame_device(d_cur, d_mv, qp, 0, cols_first, d_lv, d_rc, d_pr)
+    capi.lib.icu_stream_sync(None)
+    mv = d_mv.download(np.int16, (n, 2))
+    pred = orc.mc_luma(fp.ref, mv)
+    np.testing.assert_array_equal(d_pr.download(np.int16, (H, W)), pred)
+    lv, rc = orc.code_residual(fp.cur, pred, qp, 0, cols_first)
+    np.testing.assert_array_equal(d_lv.download(np.int16, (H, W)), lv)
+    np.testing.assert_array_equal(d_rc.download(np.uint8, (H, W)).astype(np.int16), rc)
+
+
+# ------------------------------------------------------------- batching
+def _mixed_streams(seed=0):
+    rng = np.random.default_rng(seed)
+    specs = [((176, 144), 8), ((176, 144), 16), ((352, 288), 16), ((352, 288), 32),
+             ((1280, 720), 32), ((176, 144), 64)]
+    out = []
+    for i, ((w, h), R) in enumerate(specs):
+        fp = synth.rect_motion_pair(w, h, 500 + i, n_rects=int(rng.integers(8, 25)), max_mv=min(R, 32))
+        out.append((w, h, R, fp))
+    return out
+
+
+def test_batch_alone_equals_in_mix_and_graph_replays():
+    streams = _mixed_streams()
+    modes = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=1)
+    cfgs = [_cfg(w, h, R, **modes) for (w, h, R, _) in streams]
+    batch = capi.MotionEstimatorBatch(cfgs)
+    curs = [capi.DeviceBuffer.from_array(fp.cur) for (*_, fp) in streams]
+    refs = [capi.DeviceBuffer.from_array(fp.ref) for (*_, fp) in streams]
+    T = batch.total_mbs
+    d_mv, d_dist = capi.DeviceBuffer(4 * T), capi.DeviceBuffer(4 * T)
+    batch.estimate_device(curs, refs, d_mv, d_dist)
+    capi.lib.icu_stream_sync(None)
+    capi.DeviceBuffer.sync()
+    s = capi.Stream()
+    # eager result through the batch's own stream
+    batch.estimate_device(curs, refs, d_mv, d_dist, stream=s)
+    s.sync()
+    mv_b = d_mv.download(np.int16, (T, 2))
+    dist_b = d_dist.download(np.int32, T)
+    for i, (w, h, R, fp) in enumerate(streams):
+        est = capi.MotionEstimator(cfgs[i])
+        mv, dist, _ = est.estimate(fp.cur.astype(np.int16), fp.ref.astype(np.int16))
+        b0 = batch.bases[i]
+        np.testing.assert_array_equal(mv_b[b0:b0 + est.nmb], mv, err_msg=f"stream {i}")
+        np.testing.assert_array_equal(dist_b[b0:b0 + est.nmb], dist)
+        if w <= 352:
+            omv, _, _, _ = orc.me_estimate(fp.cur, fp.ref, R, **_oracle_modes(modes))
+            np.testing.assert_array_equal(mv, omv)
+    # graph: capture once, replay several times with eager work in between
+    g_mv, g_dist = capi.DeviceBuffer(4 * T), capi.DeviceBuffer(4 * T)
+    batch.capture(curs, refs, g_mv, g_dist)
+    for rep in range(3):
+        g_mv.zero()
+        batch.graph_launch(s)
+        s.sync()
+        np.testing.assert_array_equal(g_mv.download(np.int16, (T, 2)), mv_b, err_msg=f"replay {rep}")
+        np.testing.assert_array_equal(g_dist.download(np.int32, T), dist_b)
+        batch.estimate_device(curs, refs, d_mv, d_dist, stream=s)
+        s.sync()
