@@ -3,7 +3,7 @@ IT/quant, plain per-frame launch vs batched, with p50/p99 step latency.
 
 A step is one pass of the hot path over one frame pair already resident in
 HBM: reference padding, integer full search (SAD, +-32 exhaustive window,
-zero predictor), decision + half-pel refinement + final vector choice, then
+zero predictor), decision + half-pel refinement, then
 luma motion compensation and residual coding (forward IT+Q, dequant +
 inverse IT, clipped reconstruction) at QP 22, 28 and 34.
 
@@ -26,7 +26,7 @@ sys.path.insert(0, str(ROOT))
 
 W, H, R = 1920, 1088, 32
 QPS = (22, 28, 34)
-MODES = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=1)
+MODES = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=0)
 
 
 def sad_ops(W, H, L):
@@ -210,7 +210,7 @@ def main():
         "dtype": "u8",
         "data": "synthetic (seeded sinusoid texture, 24 moving rectangles, sigma 3)",
         "config": {"workload": "1080p 1920x1088 single stream, SAD full search +-32 (exhaustive), "
-                               "half-pel, GetBestVector, MC + 4x4 IT/quant/dequant/inverse at QP 22/28/34",
+                               "half-pel, MC + 4x4 IT/quant/dequant/inverse at QP 22/28/34",
                    "mbs_per_step": plain["nmb"], "tiling": list(plain["tiling"])},
         "p50_ms": round(pctl(plain["per_ms"], 50), 4),
         "p99_ms": round(pctl(plain["per_ms"], 99), 4),

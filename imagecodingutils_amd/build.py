@@ -66,3 +66,17 @@ def build(verbose: bool = True) -> Path:
 
 if __name__ == "__main__":
     build()
+
+
+def build_dropin_test() -> Path:
+    """Compile tests/cpp/dropin_test.cpp (the C++ drop-in classes driven like
+    the reference classes) against the built library."""
+    src = PKG.parent / "tests" / "cpp" / "dropin_test.cpp"
+    exe = src.with_suffix("")
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", str(PKG.parent / "include"), str(src), "-o", str(exe),
+           "-L", str(PKG), "-licu_hip", "-Wl,-rpath,$ORIGIN/../../imagecodingutils_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"drop-in test build failed:\n{r.stderr}")
+    return exe

@@ -44,7 +44,7 @@ Tiling plan_tiling(int mbw, int L, int metric, int V_req, int G_req) {
   Tiling t{};
   const int N = 2 * L + 1;
   t.Q = quads_for(L);
-  static const int kV[1] = {8};
+  static const int kV[4] = {8, 11, 13, 16};
   int bestV = 8, bestWaste = 1 << 30;
   for (int v : kV) {
     if (metric == ICU_METRIC_SSD && v != 8) continue;
@@ -93,8 +93,8 @@ bool valid_cfg(const icu_me_config& c, std::string* why) {
     *why = "mode switch out of range";
     return false;
   }
-  if (c.tile_rows && c.tile_rows != 8) {
-    *why = "tile_rows must be 0 or 8";
+  if (c.tile_rows && c.tile_rows != 8 && c.tile_rows != 11 && c.tile_rows != 13 && c.tile_rows != 16) {
+    *why = "tile_rows must be 0, 8, 11, 13 or 16";
     return false;
   }
   if (c.metric == ICU_METRIC_SSD && c.tile_rows && c.tile_rows != 8) {
@@ -201,7 +201,9 @@ struct Engine {
 
   int init(const icu_me_config* cfgs, int n) {
     HIP_TRY(hipGetDevice(&device));
-    HIP_TRY(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+    // a blocking stream: work on it is ordered with the null stream, so a
+    // caller's plain hipMemcpy after an asynchronous call sees the results
+    HIP_TRY(hipStreamCreate(&own));
     sp.resize(n);
     total_mb = 0;
     table_total = 0;
@@ -524,7 +526,7 @@ int icu_me_autotune(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int r
   HIP_TRY(hipEventCreate(&b));
   Tiling best = s.t;
   float best_ms = 1e30f;
-  static const int kV[1] = {8};
+  static const int kV[4] = {8, 11, 13, 16};
   for (int v : kV) {
     if (s.cfg.metric == ICU_METRIC_SSD && v != 8) continue;
     if ((2 * s.L + 1 + v - 1) / v * v - (2 * s.L + 1) > v / 2 + 2) continue;  // too much row waste

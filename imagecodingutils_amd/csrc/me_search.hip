@@ -56,22 +56,31 @@ __global__ __launch_bounds__(256) void pad_plane_kernel(const uint8_t* __restric
 }
 
 // ------------------------------------------------------------- tile bodies
+// An opaque zero that depends on `dep`: adding it to an address keeps the
+// load from being issued before `dep` is computed.
+__device__ __forceinline__ uint32_t opaque_zero(uint32_t dep) {
+  uint32_t z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z) : "v"(dep));
+  return z;
+}
+
 // Rows are software-pipelined one ahead: the LDS loads of row rho+1 are
-// issued before the qsads of row rho, and a scheduling barrier per row keeps
-// the compiler from hoisting every row's loads (which would exhaust VGPRs).
-template <int V>
+// issued before the qsads of row rho.  Their address carries an opaque
+// dependency on row rho-1's sums, so the scheduler cannot hoist every row's
+// loads to the top of the tile (which would exhaust the VGPRs).
+template <int V, typename Done>
 __device__ __forceinline__ void sad_tile(const uint32_t* wp, int row_dw,
-                                         const uint32_t (&c)[64], uint32_t (&lo)[V],
-                                         uint32_t (&hi)[V]) {
+                                         const uint32_t (&c)[64], Done&& done) {
   uint64_t acc[V];
 #pragma unroll
   for (int v = 0; v < V; v++) acc[v] = 0;
   uint32_t n0 = wp[0], n1 = wp[1], n2 = wp[2], n3 = wp[3], n4 = wp[4];
+  uint32_t dep = 0;
 #pragma unroll
   for (int rho = 0; rho < V + 15; rho++) {
     const uint32_t d0 = n0, d1 = n1, d2 = n2, d3 = n3, d4 = n4;
     if (rho + 1 < V + 15) {
-      const uint32_t* rp = wp + (rho + 1) * row_dw;
+      const uint32_t* rp = wp + (rho + 1) * row_dw + opaque_zero(dep);
       n0 = rp[0]; n1 = rp[1]; n2 = rp[2]; n3 = rp[3]; n4 = rp[4];
     }
     const uint64_t p0 = ((uint64_t)d1 << 32) | d0;
@@ -88,20 +97,19 @@ __device__ __forceinline__ void sad_tile(const uint32_t* wp, int row_dw,
         acc[v] = __builtin_amdgcn_qsad_pk_u16_u8(p3, c[4 * r + 3], acc[v]);
       }
     }
-    asm volatile("" ::: "memory");  // keep each row's LDS loads in place
-  }
-#pragma unroll
-  for (int v = 0; v < V; v++) {
-    lo[v] = (uint32_t)acc[v];
-    hi[v] = (uint32_t)(acc[v] >> 32);
+    dep = (uint32_t)acc[rho < V ? rho : V - 1];
+    // accumulator rho-15 has seen all 16 block rows: retire it now so its
+    // registers free up while the remaining rows run
+    if (rho >= 15) done(rho - 15, (uint32_t)acc[rho - 15], (uint32_t)(acc[rho - 15] >> 32));
   }
 }
 
 // SSD tile: acc[v][k] = sum over rows of (sum r^2 - 2 sum c*r) for column
 // offset k, in wrapping 32-bit arithmetic; the caller adds sum c^2.
-template <int V>
+template <int V, typename Done>
 __device__ __forceinline__ void ssd_tile(const uint32_t* wp, int row_dw,
-                                         const uint32_t (&c)[64], uint32_t (&acc)[V][4]) {
+                                         const uint32_t (&c)[64], Done&& done) {
+  uint32_t acc[V][4];
 #pragma unroll
   for (int v = 0; v < V; v++)
 #pragma unroll
@@ -109,13 +117,14 @@ __device__ __forceinline__ void ssd_tile(const uint32_t* wp, int row_dw,
   uint32_t nx[5];
 #pragma unroll
   for (int i = 0; i < 5; i++) nx[i] = wp[i];
+  uint32_t dep = 0;
 #pragma unroll
   for (int rho = 0; rho < V + 15; rho++) {
     uint32_t d[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) d[i] = nx[i];
     if (rho + 1 < V + 15) {
-      const uint32_t* rp = wp + (rho + 1) * row_dw;
+      const uint32_t* rp = wp + (rho + 1) * row_dw + opaque_zero(dep);
 #pragma unroll
       for (int i = 0; i < 5; i++) nx[i] = rp[i];
     }
@@ -149,7 +158,8 @@ __device__ __forceinline__ void ssd_tile(const uint32_t* wp, int row_dw,
         }
       }
     }
-    asm volatile("" ::: "memory");  // keep each row's LDS loads in place
+    dep = acc[rho < V ? rho : V - 1][0];
+    if (rho >= 15) done(rho - 15, acc[rho - 15]);
   }
 }
 
@@ -203,27 +213,19 @@ __device__ __forceinline__ void run_tile(const StreamDesc& s, const SearchTask& 
   const long long tdim = s.table_dim;
   uint32_t* trow = table ? table + s.table_base + (long long)mb * tdim * tdim : nullptr;
   if (METRIC == kMetricSAD) {
-    uint32_t lo[V], hi[V];
-    sad_tile<V>(wp, row_dw, c, lo, hi);
-#pragma unroll
-    for (int v = 0; v < V; v++) {
-      visit(dx0 + 0, dy0 + v, lo[v] & 0xFFFF, xl, xr, yu, yd, s.L, trow, tdim, &zero[g], kbest);
-      visit(dx0 + 1, dy0 + v, lo[v] >> 16, xl, xr, yu, yd, s.L, trow, tdim, &zero[g], kbest);
-      visit(dx0 + 2, dy0 + v, hi[v] & 0xFFFF, xl, xr, yu, yd, s.L, trow, tdim, &zero[g], kbest);
-      visit(dx0 + 3, dy0 + v, hi[v] >> 16, xl, xr, yu, yd, s.L, trow, tdim, &zero[g], kbest);
-      asm volatile("" ::: "memory");
-    }
+    sad_tile<V>(wp, row_dw, c, [&](int v, uint32_t lo, uint32_t hi) {
+      visit(dx0 + 0, dy0 + v, lo & 0xFFFF, xl, xr, yu, yd, s.L, trow, tdim, &zero[g], kbest);
+      visit(dx0 + 1, dy0 + v, lo >> 16, xl, xr, yu, yd, s.L, trow, tdim, &zero[g], kbest);
+      visit(dx0 + 2, dy0 + v, hi & 0xFFFF, xl, xr, yu, yd, s.L, trow, tdim, &zero[g], kbest);
+      visit(dx0 + 3, dy0 + v, hi >> 16, xl, xr, yu, yd, s.L, trow, tdim, &zero[g], kbest);
+    });
   } else {
-    uint32_t acc[V][4];
-    ssd_tile<V>(wp, row_dw, c, acc);
     const uint32_t c2 = csq[g];
-#pragma unroll
-    for (int v = 0; v < V; v++) {
+    ssd_tile<V>(wp, row_dw, c, [&](int v, const uint32_t (&a)[4]) {
 #pragma unroll
       for (int k = 0; k < 4; k++)
-        visit(dx0 + k, dy0 + v, acc[v][k] + c2, xl, xr, yu, yd, s.L, trow, tdim, &zero[g], kbest);
-      asm volatile("" ::: "memory");
-    }
+        visit(dx0 + k, dy0 + v, a[k] + c2, xl, xr, yu, yd, s.L, trow, tdim, &zero[g], kbest);
+    });
   }
   atomicMin((unsigned long long*)&best[g], (unsigned long long)kbest);
 }
@@ -269,9 +271,19 @@ __global__ __launch_bounds__(kThreads) void me_search_kernel(const StreamDesc* _
     const int units = t.count * s.Q * s.B;
     for (int u = tid; u < units; u += kThreads) {
       if (METRIC == kMetricSSD) {
-        run_tile<METRIC, 8>(s, t, u, win, row_dw, curs, csq, best, zero, table);
+        switch (s.V) {
+          case 8: run_tile<METRIC, 8>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
+          case 11: run_tile<METRIC, 11>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
+          case 13: run_tile<METRIC, 13>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
+          default: run_tile<METRIC, 16>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
+        }
       } else {
-        run_tile<METRIC, 8>(s, t, u, win, row_dw, curs, csq, best, zero, table);
+        switch (s.V) {
+          case 8: run_tile<METRIC, 8>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
+          case 11: run_tile<METRIC, 11>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
+          case 13: run_tile<METRIC, 13>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
+          default: run_tile<METRIC, 16>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
+        }
       }
     }
     __syncthreads();

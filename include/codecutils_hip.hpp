@@ -1,0 +1,314 @@
+// codecutils_hip.hpp - C++ drop-in classes over the icu_hip C ABI.
+//
+// The classes keep the reference's class names, constructor arguments,
+// two-stage Create(), method names and in-place buffer semantics, so codec
+// code written against CodecUtils reads the same:
+//   IMotionEstimator / MotionEstimatorH264ImplFull
+//       include/CodecUtils/IMotionEstimator.h:51-96,
+//       include/CodecUtils/MotionEstimatorH264ImplFull.h:72-221
+//   IMotionVectorPredictor / H264MotionVectorPredictorImpl1
+//       include/CodecUtils/IMotionVectorPredictor.h:51-83,
+//       include/CodecUtils/H264MotionVectorPredictorImpl1.h:54-132
+//   VectorStructList (SIMPLE2D)   include/CodecUtils/VectorStructList.h:56-110
+//   IForwardTransform / FastForward4x4ITImpl1 / FastForward4x4On16x16ITImpl1
+//       include/CodecUtils/IForwardTransform.h:56-126
+//   IInverseTransform / FastInverse4x4ITImpl1 / FastInverse4x4On16x16ITImpl1
+//       include/CodecUtils/IInverseTransform.h:56-115
+// They live in namespace icu_codecutils; INTEGRATION.md shows the adapter a
+// codec adds to bind them to its own CodecUtils interface types instead.
+//
+// Header only; link with libicu_hip.so.
+#pragma once
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "icu_hip.h"
+
+namespace icu_codecutils {
+
+// ------------------------------------------------------------ MV container
+class VectorStructList {
+ public:
+  static const int SIMPLE2D = 0;
+  explicit VectorStructList(int type = SIMPLE2D) : _type(type) {}
+  int SetLength(int length) {
+    _v.assign(2 * (size_t)length, 0);
+    return 1;
+  }
+  int GetLength(void) const { return (int)(_v.size() / 2); }
+  int GetType(void) const { return _type; }
+  int GetSimpleElement(int pos, int element) const { return _v[2 * (size_t)pos + element]; }
+  void SetSimpleElement(int pos, int element, int value) { _v[2 * (size_t)pos + element] = (int16_t)value; }
+  int16_t* Data(void) { return _v.data(); }
+
+ private:
+  int _type;
+  std::vector<int16_t> _v;
+};
+
+// ------------------------------------------------------------- predictors
+class IMotionVectorPredictor {
+ public:
+  virtual ~IMotionVectorPredictor() {}
+  virtual int Create(void) = 0;
+  virtual int Get16x16Prediction(void* pList, int blk, int* predX, int* predY) = 0;
+  virtual void Set16x16MotionVector(int blk, int mvX, int mvY) {}
+  // 1 when the predictor is the H.264 16x16 median of A/B/C(D), which the
+  // device evaluates itself as a wavefront; 0 for the zero vector.
+  virtual int IsH264Median(void) const = 0;
+};
+
+// H.264 median predictor over the MBs of one picture (one slice).  The
+// estimator decides the vectors on the device; afterwards it writes them
+// back here with Set16x16MotionVector, as the reference estimator does, so
+// Get16x16Prediction answers for later users.
+class H264MotionVectorPredictorImpl1 : public IMotionVectorPredictor {
+ public:
+  H264MotionVectorPredictorImpl1(int mbCols, int mbRows) : _w(mbCols), _h(mbRows), _mv(2 * (size_t)mbCols * mbRows, 0) {}
+  int Create(void) override { return 1; }
+  int IsH264Median(void) const override { return 1; }
+  void Set16x16MotionVector(int blk, int mvX, int mvY) override {
+    _mv[2 * (size_t)blk] = mvX;
+    _mv[2 * (size_t)blk + 1] = mvY;
+  }
+  int Get16x16Prediction(void*, int blk, int* predX, int* predY) override {
+    const int col = blk % _w, row = blk / _w;
+    int A[2] = {0, 0}, B[2] = {0, 0}, Cc[2] = {0, 0};
+    const bool hasB = row > 0, hasC = row > 0 && col + 1 < _w;
+    if (col > 0) get(blk - 1, A);
+    if (hasB) get(blk - _w, B);
+    if (hasC) get(blk - _w + 1, Cc);
+    else if (col > 0 && row > 0) get(blk - _w - 1, Cc);
+    if (!hasB && !hasC) {
+      B[0] = Cc[0] = A[0];
+      B[1] = Cc[1] = A[1];
+    }
+    *predX = median(A[0], B[0], Cc[0]);
+    *predY = median(A[1], B[1], Cc[1]);
+    return 1;
+  }
+
+ private:
+  void get(int blk, int* v) const {
+    v[0] = _mv[2 * (size_t)blk];
+    v[1] = _mv[2 * (size_t)blk + 1];
+  }
+  static int median(int x, int y, int z) {
+    int mn = y < z ? y : z, mx = y < z ? z : y;
+    if (x < mn) mn = x;
+    if (x > mx) mx = x;
+    return x + y + z - mn - mx;
+  }
+  int _w, _h;
+  std::vector<int> _mv;
+};
+
+// Always predicts the zero vector.
+class ZeroMotionVectorPredictor : public IMotionVectorPredictor {
+ public:
+  int Create(void) override { return 1; }
+  int IsH264Median(void) const override { return 0; }
+  int Get16x16Prediction(void*, int, int* predX, int* predY) override {
+    *predX = 0;
+    *predY = 0;
+    return 1;
+  }
+};
+
+// ---------------------------------------------------------------- estimator
+class IMotionEstimator {
+ public:
+  virtual ~IMotionEstimator() {}
+  virtual int Create(void) = 0;
+  virtual void Reset(void) = 0;
+  virtual int Ready(void) = 0;
+  virtual void SetMode(int mode) = 0;
+  virtual int GetMode(void) = 0;
+  virtual void* Estimate(const void* pSrc, const void* pRef, long* avgDistortion) = 0;
+  virtual void* Estimate(long* avgDistortion) = 0;
+};
+
+// The full-search estimator.  pSrc / pRef are int16 planes of 8-bit
+// samples, borrowed for the object's lifetime and re-read by every
+// Estimate().  The returned list holds quarter-pel (x, y) per MB in raster
+// order and stays valid until the next Estimate() or destruction.
+// Beyond the reference's constructor, an optional icu_me_config selects the
+// metric / window / filter / sub-pel depth (defaults: the shipped class).
+class MotionEstimatorH264ImplFull : public IMotionEstimator {
+ public:
+  MotionEstimatorH264ImplFull(const void* pSrc, const void* pRef, int imgWidth, int imgHeight,
+                              int motionRange, IMotionVectorPredictor* pMVPred,
+                              void* pDistortionIncluded = nullptr, const icu_me_config* modes = nullptr)
+      : _src((const int16_t*)pSrc), _ref((const int16_t*)pRef), _w(imgWidth), _h(imgHeight),
+        _range(motionRange), _pred(pMVPred), _include((const bool*)pDistortionIncluded) {
+    if (modes) {
+      _cfg = *modes;
+      _have_modes = true;
+    }
+  }
+  ~MotionEstimatorH264ImplFull() override { Destroy(); }
+
+  int Create(void) override {
+    Destroy();
+    if (!_have_modes) icu_me_config_reference(&_cfg, _w, _h, _range);
+    _cfg.width = _w;
+    _cfg.height = _h;
+    _cfg.range = _range / 4;
+    _cfg.predictor = (_pred && _pred->IsH264Median()) ? 1 : 0;
+    if (icu_me_create(&_cfg, &_h_me) != ICU_OK) return 0;
+    _list.SetLength((_w / 16) * (_h / 16));
+    _ready = 1;
+    return 1;
+  }
+  void Reset(void) override {}
+  int Ready(void) override { return _ready; }
+  void SetMode(int mode) override { _mode = mode; }
+  int GetMode(void) override { return _mode; }
+  // As the reference, the pointer arguments are ignored: the planes given
+  // at construction are estimated.
+  void* Estimate(const void*, const void*, long* avgDistortion) override { return Estimate(avgDistortion); }
+  void* Estimate(long* avgDistortion) override {
+    if (!_ready) return nullptr;
+    const int n = _list.GetLength();
+    std::vector<uint8_t> inc;
+    if (_include) {
+      inc.resize(n);
+      for (int i = 0; i < n; i++) inc[i] = _include[i] ? 1 : 0;
+    }
+    long avg = 0;
+    if (icu_me_estimate(_h_me, _src, _ref, _include ? inc.data() : nullptr, _list.Data(), nullptr,
+                        &avg) != ICU_OK)
+      return nullptr;
+    if (_pred)
+      for (int i = 0; i < n; i++)
+        _pred->Set16x16MotionVector(i, _list.GetSimpleElement(i, 0), _list.GetSimpleElement(i, 1));
+    if (avgDistortion) *avgDistortion = avg;
+    return (void*)&_list;
+  }
+  const char* LastError(void) const { return icu_last_error(); }
+
+ private:
+  void Destroy(void) {
+    if (_h_me) icu_me_destroy(_h_me);
+    _h_me = nullptr;
+    _ready = 0;
+  }
+  const int16_t* _src;
+  const int16_t* _ref;
+  int _w, _h, _range;
+  IMotionVectorPredictor* _pred;
+  const bool* _include;
+  icu_me_config _cfg{};
+  bool _have_modes = false;
+  icu_me* _h_me = nullptr;
+  VectorStructList _list;
+  int _ready = 0, _mode = 1;
+};
+
+// --------------------------------------------------------------- transforms
+class IForwardTransform {
+ public:
+  static const int QUANT_ID = 0;
+  static const int INTRA_FLAG_ID = 1;
+  static const int TransformAndQuant = 0;
+  static const int TransformOnly = 1;
+  static const int QuantOnly = 2;
+  virtual ~IForwardTransform() {}
+  virtual void Transform(void* ptr) = 0;
+  virtual void Transform(void* pIn, void* pCoeff) = 0;
+  virtual void SetScale(void* pScale) = 0;
+  virtual void* GetScale(void) = 0;
+  virtual void SetMode(int mode) = 0;
+  virtual int GetMode(void) = 0;
+  virtual void SetParameter(int paramID, int paramVal) = 0;
+  virtual int GetParameter(int paramID) = 0;
+  virtual int QuantiseValue(short val, int pos, int qp) = 0;
+};
+
+class IInverseTransform {
+ public:
+  static const int QUANT_ID = 0;
+  static const int TransformAndQuant = 0;
+  static const int TransformOnly = 1;
+  static const int QuantOnly = 2;
+  virtual ~IInverseTransform() {}
+  virtual void InverseTransform(void* ptr) = 0;
+  virtual void InverseTransform(void* pCoeff, void* pOut) = 0;
+  virtual void SetScale(void* pScale) = 0;
+  virtual void* GetScale(void) = 0;
+  virtual void SetMode(int mode) = 0;
+  virtual int GetMode(void) = 0;
+  virtual void SetParameter(int paramID, int paramVal) = 0;
+  virtual int GetParameter(int paramID) = 0;
+};
+
+// Forward 4x4 IT + quant.  Layout 0 (one 4x4 block, 16 shorts per call) is
+// FastForward4x4ITImpl1; layout 1 (a 16x16 raster MB) is
+// FastForward4x4On16x16ITImpl1.  TransformBlocks(ptr, n) is the batched
+// form: n calls' worth in one launch.
+template <int LAYOUT>
+class FastForward4x4ITHip : public IForwardTransform {
+ public:
+  void Transform(void* ptr) override { TransformBlocks(ptr, 1); }
+  void Transform(void* pIn, void* pCoeff) override {
+    std::memcpy(pCoeff, pIn, sizeof(short) * (LAYOUT ? 256 : 16));
+    Transform(pCoeff);
+  }
+  void TransformBlocks(void* ptr, long calls) {
+    const long nb = calls * (LAYOUT ? 16 : 1);
+    icu_fwd4x4((const int16_t*)ptr, (int16_t*)ptr, nb, _q, _intra, _mode, LAYOUT);
+  }
+  void SetScale(void*) override {}
+  void* GetScale(void) override { return nullptr; }
+  void SetMode(int mode) override { _mode = mode; }
+  int GetMode(void) override { return _mode; }
+  void SetParameter(int id, int v) override {
+    if (id == QUANT_ID) _q = v;
+    else if (id == INTRA_FLAG_ID) _intra = v;
+  }
+  int GetParameter(int id) override { return id == QUANT_ID ? _q : _intra; }
+  // Not implemented by the reference Impl1 either (FastForward4x4ITImpl1.h:118).
+  int QuantiseValue(short, int, int) override { return 0; }
+
+ private:
+  int _mode = TransformAndQuant, _intra = 1, _q = 1;
+};
+typedef FastForward4x4ITHip<0> FastForward4x4ITImpl1;
+typedef FastForward4x4ITHip<1> FastForward4x4On16x16ITImpl1;
+
+// Dequant + inverse 4x4 IT.  Layout 0 / rows first is FastInverse4x4ITImpl1;
+// layout 1 / columns first is FastInverse4x4On16x16ITImpl1 (not bit-equal).
+template <int LAYOUT>
+class FastInverse4x4ITHip : public IInverseTransform {
+ public:
+  FastInverse4x4ITHip() {
+    for (int i = 0; i < 16; i++) _w[i] = 16;
+  }
+  void InverseTransform(void* ptr) override { InverseTransformBlocks(ptr, 1); }
+  void InverseTransform(void* pCoeff, void* pOut) override {
+    std::memcpy(pOut, pCoeff, sizeof(short) * (LAYOUT ? 256 : 16));
+    InverseTransform(pOut);
+  }
+  void InverseTransformBlocks(void* ptr, long calls) {
+    const long nb = calls * (LAYOUT ? 16 : 1);
+    icu_inv4x4((const int16_t*)ptr, (int16_t*)ptr, nb, _q, _mode, LAYOUT, LAYOUT, _w);
+  }
+  void SetScale(void* pScale) override { std::memcpy(_w, pScale, sizeof(_w)); }
+  void* GetScale(void) override { return (void*)_w; }
+  void SetMode(int mode) override { _mode = mode; }
+  int GetMode(void) override { return _mode; }
+  void SetParameter(int id, int v) override {
+    if (id == QUANT_ID) _q = v;
+  }
+  int GetParameter(int) override { return _q; }
+
+ private:
+  int _mode = TransformAndQuant, _q = 1;
+  int _w[16];
+};
+typedef FastInverse4x4ITHip<0> FastInverse4x4ITImpl1;
+typedef FastInverse4x4ITHip<1> FastInverse4x4On16x16ITImpl1;
+
+}  // namespace icu_codecutils

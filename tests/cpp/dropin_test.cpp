@@ -1,0 +1,112 @@
+// Drives the C++ drop-in classes the way codec code drives the reference
+// classes (construct, Create, Estimate, read the VectorStructList; transform
+// objects with SetParameter / SetMode / Transform in place).  Inputs and
+// outputs are raw files; tests/test_gpu_dropin.py compares them.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "codecutils_hip.hpp"
+
+using namespace icu_codecutils;
+
+template <typename T>
+static std::vector<T> load(const char* path, size_t n) {
+  std::vector<T> v(n);
+  FILE* f = std::fopen(path, "rb");
+  if (!f || std::fread(v.data(), sizeof(T), n, f) != n) {
+    std::fprintf(stderr, "cannot read %s\n", path);
+    std::exit(2);
+  }
+  std::fclose(f);
+  return v;
+}
+
+template <typename T>
+static void save(const char* path, const T* p, size_t n) {
+  FILE* f = std::fopen(path, "wb");
+  if (!f || std::fwrite(p, sizeof(T), n, f) != n) std::exit(3);
+  std::fclose(f);
+}
+
+int main(int argc, char** argv) {
+  if (argc < 2) return 1;
+  const std::string cmd = argv[1];
+  if (cmd == "me" && argc >= 9) {
+    const int W = std::atoi(argv[2]), H = std::atoi(argv[3]), range = std::atoi(argv[4]);
+    const int median = std::atoi(argv[5]);
+    auto cur = load<short>(argv[6], (size_t)W * H);
+    auto ref = load<short>(argv[7], (size_t)W * H);
+    const int nmb = (W / 16) * (H / 16);
+    std::vector<unsigned char> inc;
+    if (argc >= 10) inc = load<unsigned char>(argv[9], nmb);
+    H264MotionVectorPredictorImpl1 med(W / 16, H / 16);
+    ZeroMotionVectorPredictor zero;
+    IMotionVectorPredictor* pred = median ? (IMotionVectorPredictor*)&med : (IMotionVectorPredictor*)&zero;
+    MotionEstimatorH264ImplFull me(cur.data(), ref.data(), W, H, range, pred,
+                                   inc.empty() ? nullptr : (void*)inc.data());
+    if (!me.Create()) {
+      std::fprintf(stderr, "Create failed: %s\n", me.LastError());
+      return 4;
+    }
+    long avg = -1;
+    VectorStructList* l = (VectorStructList*)me.Estimate(&avg);
+    if (!l) {
+      std::fprintf(stderr, "Estimate failed: %s\n", me.LastError());
+      return 5;
+    }
+    std::vector<short> mv(2 * nmb);
+    for (int i = 0; i < nmb; i++) {
+      mv[2 * i] = (short)l->GetSimpleElement(i, 0);
+      mv[2 * i + 1] = (short)l->GetSimpleElement(i, 1);
+    }
+    save(argv[8], mv.data(), mv.size());
+    // the predictor now holds the decided vectors, as after the reference's Estimate
+    int px = 0, py = 0;
+    pred->Get16x16Prediction(nullptr, nmb - 1, &px, &py);
+    std::printf("avg %ld pred_last %d %d\n", avg, px, py);
+    return 0;
+  }
+  if ((cmd == "fwd" || cmd == "inv") && argc >= 9) {
+    const int layout = std::atoi(argv[2]), qp = std::atoi(argv[3]), a = std::atoi(argv[4]);
+    const int mode = std::atoi(argv[5]);
+    const long calls = std::atol(argv[8]);
+    const size_t n = (size_t)calls * (layout ? 256 : 16);
+    auto x = load<short>(argv[6], n);
+    if (cmd == "fwd") {
+      if (layout) {
+        FastForward4x4On16x16ITImpl1 t;
+        t.SetParameter(IForwardTransform::INTRA_FLAG_ID, a);
+        t.SetParameter(IForwardTransform::QUANT_ID, qp);
+        t.SetMode(mode);
+        t.TransformBlocks(x.data(), calls);
+      } else {
+        FastForward4x4ITImpl1 t;
+        t.SetParameter(IForwardTransform::INTRA_FLAG_ID, a);
+        t.SetParameter(IForwardTransform::QUANT_ID, qp);
+        t.SetMode(mode);
+        // one Transform call per block, as codec code calls the reference
+        for (long i = 0; i < (calls < 4 ? calls : 4); i++) t.Transform(x.data() + 16 * i);
+        if (calls > 4) t.TransformBlocks(x.data() + 64, calls - 4);
+      }
+    } else {
+      if (layout) {
+        FastInverse4x4On16x16ITImpl1 t;
+        t.SetParameter(IInverseTransform::QUANT_ID, qp);
+        t.SetMode(mode);
+        t.InverseTransformBlocks(x.data(), calls);
+      } else {
+        FastInverse4x4ITImpl1 t;
+        t.SetParameter(IInverseTransform::QUANT_ID, qp);
+        t.SetMode(mode);
+        t.InverseTransformBlocks(x.data(), calls);
+      }
+    }
+    save(argv[7], x.data(), n);
+    return 0;
+  }
+  std::fprintf(stderr, "usage: dropin_test me|fwd|inv ...\n");
+  return 1;
+}

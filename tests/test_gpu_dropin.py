@@ -1,0 +1,60 @@
+"""The C++ drop-in classes (include/codecutils_hip.hpp), driven from a C++
+program the way codec code drives the reference classes, against the
+reference's golden fixtures."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+EXE = Path(__file__).resolve().parent / "cpp" / "dropin_test"
+
+
+def _run(*args):
+    if not EXE.exists():
+        pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
+    r = subprocess.run([str(EXE), *map(str, args)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+@pytest.mark.parametrize("name", ["cif_r16_median_global", "qcif_r16_zero_rects", "tiny_64x48_r32_median_rects"])
+def test_dropin_estimator_matches_reference(tmp_path, name):
+    fx = np.load(GOLDEN / f"me_{name}.npz")
+    H, W = fx["cur"].shape
+    fx["cur"].astype(np.int16).tofile(tmp_path / "cur.raw")
+    fx["ref"].astype(np.int16).tofile(tmp_path / "ref.raw")
+    args = ["me", W, H, 4 * int(fx["R"]), int(fx["pred"]), tmp_path / "cur.raw", tmp_path / "ref.raw",
+            tmp_path / "mv.raw"]
+    if "include" in fx.files:
+        fx["include"].astype(np.uint8).tofile(tmp_path / "inc.raw")
+        args.append(tmp_path / "inc.raw")
+    out = _run(*args)
+    mv = np.fromfile(tmp_path / "mv.raw", np.int16).reshape(-1, 2)
+    np.testing.assert_array_equal(mv, fx["mv"])
+    assert int(out.split()[1]) == int(fx["avg"])
+
+
+@pytest.mark.parametrize("qp", [0, 13, 28, 51])
+def test_dropin_transforms_match_reference(tmp_path, qp):
+    fx = np.load(GOLDEN / "transforms.npz")
+    for intra in (0, 1):
+        for mode in (0, 1, 2):
+            fx["fwd_in"].tofile(tmp_path / "in.raw")
+            _run("fwd", 0, qp, intra, mode, tmp_path / "in.raw", tmp_path / "out.raw", fx["fwd_in"].shape[0])
+            np.testing.assert_array_equal(np.fromfile(tmp_path / "out.raw", np.int16).reshape(-1, 16),
+                                          fx["fwd"][qp, intra, mode])
+            fx["fwd_mb_in"].tofile(tmp_path / "in.raw")
+            _run("fwd", 1, qp, intra, mode, tmp_path / "in.raw", tmp_path / "out.raw", fx["fwd_mb_in"].shape[0])
+            np.testing.assert_array_equal(np.fromfile(tmp_path / "out.raw", np.int16).reshape(-1, 256),
+                                          fx["fwd_mb"][qp, intra, mode])
+    for mode in (0, 1, 2):
+        fx["inv_in"][qp].tofile(tmp_path / "in.raw")
+        _run("inv", 0, qp, 0, mode, tmp_path / "in.raw", tmp_path / "out.raw", 64)
+        np.testing.assert_array_equal(np.fromfile(tmp_path / "out.raw", np.int16).reshape(-1, 16), fx["inv"][qp, mode])
+        _run("inv", 1, qp, 0, mode, tmp_path / "in.raw", tmp_path / "out.raw", 4)
+        np.testing.assert_array_equal(np.fromfile(tmp_path / "out.raw", np.int16).reshape(-1, 256),
+                                      fx["inv_mb"][qp, mode])

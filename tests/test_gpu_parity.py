@@ -125,7 +125,7 @@ def test_bench_workload_1080p_subset_matches_oracle():
     oracle, and the whole frame recovers the painted motion."""
     W, H, R = 1920, 1088, 32
     fp = synth.rect_motion_pair(W, H, 1080, n_rects=24, max_mv=32, sigma_cur=3.0)
-    modes = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=1)
+    modes = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=0)
     est = capi.MotionEstimator(_cfg(W, H, R, **modes))
     mv, dist, _ = est.estimate(fp.cur.astype(np.int16), fp.ref.astype(np.int16))
     omv, odist, _, _ = orc.me_estimate(fp.cur, fp.ref, R, step=16, **_oracle_modes(modes))
