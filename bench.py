@@ -46,11 +46,13 @@ def pctl(xs, p):
     return float(np.percentile(np.asarray(xs), p))
 
 
-def run_plain(capi, synth, steps, warmup):
+def run_plain(capi, synth, steps, warmup, tune=True):
     fp = synth.rect_motion_pair(W, H, 1080, n_rects=24, max_mv=32, sigma_cur=3.0)
     est = capi.MotionEstimator(capi.make_config(W, H, R, **MODES))
     s = capi.Stream()
     d_cur, d_ref = capi.DeviceBuffer.from_array(fp.cur), capi.DeviceBuffer.from_array(fp.ref)
+    if tune:
+        est.autotune(d_cur, d_ref, reps=3)  # the tile tuner: rows/lane x MBs/task x grid mode
     n = est.nmb
     d_mv, d_dist = capi.DeviceBuffer(4 * n), capi.DeviceBuffer(4 * n)
     d_lv = [capi.DeviceBuffer(2 * W * H) for _ in QPS]
@@ -58,8 +60,8 @@ def run_plain(capi, synth, steps, warmup):
 
     def step():
         est.estimate_device(d_cur, d_ref, d_mv, d_dist, stream=s)
-        for i, qp in enumerate(QPS):
-            est.code_frame_device(d_cur, d_mv, qp, 0, 0, d_lv[i], d_rc[i], stream=s)
+        # one prediction per frame, coded and reconstructed at each QP
+        est.code_frame_qps_device(d_cur, d_mv, QPS, 0, 0, d_lv, d_rc, stream=s)
 
     for _ in range(warmup):
         step()
@@ -179,6 +181,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the planner's tiling")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by "
@@ -187,7 +190,7 @@ def main():
     from imagecodingutils_amd import capi, synth
 
     info = capi.device_info()
-    plain = run_plain(capi, synth, args.steps, args.warmup)
+    plain = run_plain(capi, synth, args.steps, args.warmup, tune=not args.no_tune)
     ms_per_step = plain["total_ms"] / args.steps
     value = plain["nmb"] / (ms_per_step / 1e3)
     ops = sad_ops(W, H, R)

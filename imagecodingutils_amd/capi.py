@@ -38,9 +38,11 @@ _SIGS = {
     "icu_me_estimate": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_long)]),
     "icu_me_estimate_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "icu_me_estimate_timed": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_float)]),
-    "icu_me_get_tiling": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "icu_me_get_tiling": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_me_autotune": (C.c_int, [_vp, _vp, _vp, C.c_int]),
     "icu_code_frame_device": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "icu_code_frame_qps_device": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int,
+                                            C.POINTER(_vp), C.POINTER(_vp), _vp, _vp]),
     "icu_fwd4x4": (C.c_int, [_vp, _vp, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int]),
     "icu_inv4x4": (C.c_int, [_vp, _vp, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
     "icu_fwd4x4_device": (C.c_int, [_vp, _vp, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
@@ -233,10 +235,20 @@ class MotionEstimator:
                                          _p(d_levels), _p(d_recon), _p(d_pred),
                                          stream.handle if stream else None))
 
+    def code_frame_qps_device(self, d_cur, d_mv, qps, intra, cols_first, d_levels, d_recons, d_pred=None,
+                              stream: Stream | None = None):
+        """One prediction, coded at every QP in qps (d_levels / d_recons: one buffer per QP)."""
+        n = len(qps)
+        _check(lib.icu_code_frame_qps_device(
+            self.h, _p(d_cur), _p(d_mv), n, (C.c_int * n)(*qps), intra, cols_first,
+            (_vp * n)(*[_p(x) for x in d_levels]), (_vp * n)(*[_p(x) for x in d_recons]), _p(d_pred),
+            stream.handle if stream else None))
+
     def tiling(self):
-        v, g = C.c_int(0), C.c_int(0)
-        _check(lib.icu_me_get_tiling(self.h, C.byref(v), C.byref(g)))
-        return v.value, g.value
+        """(candidate rows per lane, MBs per task, persistent grid)."""
+        v, g, p = C.c_int(0), C.c_int(0), C.c_int(0)
+        _check(lib.icu_me_get_tiling(self.h, C.byref(v), C.byref(g), C.byref(p)))
+        return v.value, g.value, p.value
 
     def autotune(self, d_cur, d_ref, reps=3):
         _check(lib.icu_me_autotune(self.h, _p(d_cur), _p(d_ref), reps))

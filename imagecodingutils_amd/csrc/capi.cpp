@@ -39,15 +39,22 @@ int quads_for(int L) { return (L - tile_dx0(L)) / 4 + 1; }
 
 // Lane tiles: rows V per lane so that B*V covers the 2L+1 window rows with
 // the least waste; MBs per task G so that G*Q*B lanes fill whole 256-lane
-// passes and the staged window fits in LDS.
-Tiling plan_tiling(int mbw, int L, int metric, int V_req, int G_req) {
+// passes and the staged window fits in LDS.  SAD argmin tiles (no cost
+// table) also need lane_keys_fit; V = 8 always fits.  Returns V = 0 when a
+// requested V is not allowed.
+Tiling plan_tiling(int mbw, int L, int metric, bool argmin, int V_req, int G_req) {
   Tiling t{};
   const int N = 2 * L + 1;
   t.Q = quads_for(L);
   static const int kV[4] = {8, 11, 13, 16};
+  auto allowed = [&](int v) {
+    if (metric == ICU_METRIC_SSD) return v == 8;
+    return !(argmin && !lane_keys_fit(L, v));
+  };
+  if (V_req && !allowed(V_req)) return t;
   int bestV = 8, bestWaste = 1 << 30;
   for (int v : kV) {
-    if (metric == ICU_METRIC_SSD && v != 8) continue;
+    if (!allowed(v)) continue;
     if (V_req && v != V_req) continue;
     const int waste = ((N + v - 1) / v) * v - N;
     if (waste < bestWaste || (waste == bestWaste && v > bestV)) {
@@ -62,8 +69,7 @@ Tiling plan_tiling(int mbw, int L, int metric, int V_req, int G_req) {
   int bestG = 1;
   double bestEff = -1;
   for (int g = 1; g <= gmax; g++) {
-    const int rows = t.B * t.V + 15, row_dw = 4 * (g - 1) + t.Q + 4;
-    if (rows * row_dw > search_max_window_dwords()) break;
+    if (search_stage_dwords(t.B, t.V, t.Q, g) > search_max_window_dwords()) break;
     if (G_req && g != G_req) continue;
     const int lanes = g * U;
     const double eff = (double)lanes / (256.0 * ((lanes + 255) / 256));
@@ -151,7 +157,13 @@ struct Engine {
   uint32_t* d_table = nullptr;
   int* d_wave = nullptr;  // per stream: mbh progress counters + 2 (ticket, error)
   std::vector<int> wave_off;
-  int search_grid = 0;    // 0: one workgroup per task
+  int stage_dwords = 0;   // one LDS staging buffer of the search kernel
+  int cus = 256;          // compute units of the device
+  // search grid mode (the tuner may switch it): persistent workgroups, as
+  // many as stay resident, claim tasks until none are left; otherwise one
+  // task per workgroup
+  bool persistent = true;
+  int ticket_off = 0;     // search task tickets (2 ints) inside d_wave
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
 
@@ -197,6 +209,11 @@ struct Engine {
       const long cb = (long)b.count * sb.t.Q * sb.t.B * (sb.t.V + 15);
       return ca > cb;
     });
+    // one staging buffer fits the largest task; the kernel holds two
+    stage_dwords = 0;
+    for (const auto& s : sp)
+      stage_dwords = std::max(stage_dwords, search_stage_dwords(s.t.B, s.t.V, s.t.Q, s.t.G));
+    stage_dwords = (stage_dwords + 63) & ~63;
   }
 
   int init(const icu_me_config* cfgs, int n) {
@@ -204,6 +221,9 @@ struct Engine {
     // a blocking stream: work on it is ordered with the null stream, so a
     // caller's plain hipMemcpy after an asynchronous call sees the results
     HIP_TRY(hipStreamCreate(&own));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    cus = prop.multiProcessorCount;
     sp.resize(n);
     total_mb = 0;
     table_total = 0;
@@ -217,8 +237,12 @@ struct Engine {
       s.nmb = s.mbw * s.mbh;
       s.mb_base = total_mb;
       total_mb += s.nmb;
-      s.t = plan_tiling(s.mbw, s.L, s.cfg.metric, s.cfg.tile_rows, s.cfg.mbs_per_task);
       s.table = s.cfg.filter || s.cfg.predictor;
+      s.t = plan_tiling(s.mbw, s.L, s.cfg.metric, !s.table, s.cfg.tile_rows, s.cfg.mbs_per_task);
+      if (s.t.V == 0)
+        return fail(ICU_E_ARG, "tile_rows " + std::to_string(s.cfg.tile_rows) +
+                                   " is too tall for exact SAD argmin keys at range " +
+                                   std::to_string(s.cfg.range));
       s.table_base = table_total;
       if (s.table) table_total += (long long)s.nmb * (2 * s.L + 1) * (2 * s.L + 1);
       s.ref_stride = ((s.cfg.width + 2 * kPad) + 255) & ~255;
@@ -228,13 +252,16 @@ struct Engine {
       wave_off.push_back(wave_total);
       wave_total += s.mbh + 2;
     }
+    ticket_off = wave_total;
+    wave_total += 2;
     build_tasks();
     for (auto& d : ds) {
       HIP_TRY(hipMalloc(&d.d_streams, sizeof(StreamDesc) * n));
       HIP_TRY(hipMalloc(&d.d_decide, sizeof(DecideDesc) * n));
       d.cur.assign(n, nullptr);
     }
-    HIP_TRY(hipMalloc(&d_tasks, sizeof(SearchTask) * h_tasks.size()));
+    // every task holds at least one MB: total_mb entries fit any retiling
+    HIP_TRY(hipMalloc(&d_tasks, sizeof(SearchTask) * total_mb));
     HIP_TRY(hipMalloc(&d_ires, sizeof(IntResult) * total_mb));
     HIP_TRY(hipMalloc(&d_wave, sizeof(int) * wave_total));
     HIP_TRY(hipMemset(d_wave, 0, sizeof(int) * wave_total));
@@ -254,6 +281,9 @@ struct Engine {
   // (Re)write the descriptors when the caller's buffers change.
   int bind(int set, const uint8_t* const* curs, int16_t* mv, int32_t* dist, int32_t* integ) {
     DescSet& D = ds[set];
+    for (size_t i = 0; i < sp.size(); i++)
+      if (((uintptr_t)curs[i] & 15) != 0)
+        return fail(ICU_E_ARG, "device current planes must be 16-byte aligned");
     bool same = D.valid && mv == D.mv && dist == D.dist && integ == D.integ;
     for (size_t i = 0; same && i < sp.size(); i++) same = D.cur[i] == curs[i];
     if (same) return ICU_OK;
@@ -331,8 +361,14 @@ struct Engine {
       launch_pad_plane(refs[i], s.cfg.width, s.cfg.height, s.cfg.width, s.refpad, s.ref_stride, st);
     }
     if (ev) HIP_TRY(hipEventRecord(ev[1], st));
-    launch_search(d_streams, d_tasks, (int)h_tasks.size(), search_grid, sp[0].cfg.metric, d_ires,
-                  d_table, st);
+    int vfix = sp[0].t.V;
+    bool any_table = false;
+    for (const auto& x : sp) {
+      if (x.t.V != vfix) vfix = 0;
+      any_table |= x.table;
+    }
+    launch_search(d_streams, d_tasks, (int)h_tasks.size(), persistent, sp[0].cfg.metric, vfix,
+                  any_table, stage_dwords, d_ires, d_table, d_wave + ticket_off, st);
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
     if (!h_items.empty()) launch_decide_parallel(d_decide, d_items, (int)h_items.size(), st);
     for (size_t i = 0; i < sp.size(); i++) {
@@ -355,12 +391,16 @@ struct Engine {
     return ICU_OK;
   }
 
-  void retile(int si, Tiling t) {
+  int retile(int si, Tiling t) {
     sp[si].t = t;
     build_tasks();
-    hipMemcpy(d_tasks, h_tasks.data(), sizeof(SearchTask) * h_tasks.size(), hipMemcpyHostToDevice);
+    if ((int)h_tasks.size() > total_mb) return fail(ICU_E_STATE, "task list exceeds its allocation");
+    // kernels of an earlier call may still read the task list
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(d_tasks, h_tasks.data(), sizeof(SearchTask) * h_tasks.size(), hipMemcpyHostToDevice));
     ds[0].valid = false;
     ds[1].valid = false;
+    return ICU_OK;
   }
 };
 
@@ -509,10 +549,11 @@ int icu_me_estimate(icu_me* h, const int16_t* cur, const int16_t* ref, const uin
   return ICU_OK;
 }
 
-int icu_me_get_tiling(icu_me* h, int* tile_rows, int* mbs_per_task) {
+int icu_me_get_tiling(icu_me* h, int* tile_rows, int* mbs_per_task, int* persistent) {
   if (!h) return fail(ICU_E_ARG, "null handle");
   if (tile_rows) *tile_rows = h->eng.sp[0].t.V;
   if (mbs_per_task) *mbs_per_task = h->eng.sp[0].t.G;
+  if (persistent) *persistent = h->eng.persistent ? 1 : 0;
   return ICU_OK;
 }
 
@@ -525,49 +566,78 @@ int icu_me_autotune(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int r
   HIP_TRY(hipEventCreate(&a));
   HIP_TRY(hipEventCreate(&b));
   Tiling best = s.t;
+  bool best_persistent = e.persistent;
   float best_ms = 1e30f;
   static const int kV[4] = {8, 11, 13, 16};
-  for (int v : kV) {
-    if (s.cfg.metric == ICU_METRIC_SSD && v != 8) continue;
-    if ((2 * s.L + 1 + v - 1) / v * v - (2 * s.L + 1) > v / 2 + 2) continue;  // too much row waste
-    for (int g = 1; g <= std::min(search_max_group(), s.mbw); g *= 2) {
-      Tiling t = plan_tiling(s.mbw, s.L, s.cfg.metric, v, g);
-      if (t.V != v || t.G != g) continue;
-      e.retile(0, t);
-      int rc = icu_me_estimate_device(h, d_cur, d_ref, h->d_mv, h->d_dist, nullptr, e.own);
-      if (rc) return rc;
-      HIP_TRY(hipEventRecord(a, e.own));
-      for (int r = 0; r < reps; r++) {
+  for (int pers = 0; pers < 2; pers++) {
+    for (int v : kV) {
+      if (s.cfg.metric == ICU_METRIC_SSD && v != 8) continue;
+      if ((2 * s.L + 1 + v - 1) / v * v - (2 * s.L + 1) > v / 2 + 2) continue;  // too much row waste
+      for (int g = 1; g <= std::min(search_max_group(), s.mbw); g++) {
+        Tiling t = plan_tiling(s.mbw, s.L, s.cfg.metric, !s.table, v, g);
+        if (t.V != v || t.G != g) continue;
+        const int lanes = g * t.Q * t.B;
+        if (lanes < 192 && g < s.mbw) continue;  // leaves most of a workgroup idle
+        e.persistent = pers != 0;
+        int rc = e.retile(0, t);
+        if (rc) return rc;
         rc = icu_me_estimate_device(h, d_cur, d_ref, h->d_mv, h->d_dist, nullptr, e.own);
         if (rc) return rc;
-      }
-      HIP_TRY(hipEventRecord(b, e.own));
-      HIP_TRY(hipEventSynchronize(b));
-      float ms = 0;
-      HIP_TRY(hipEventElapsedTime(&ms, a, b));
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = t;
+        HIP_TRY(hipEventRecord(a, e.own));
+        for (int r = 0; r < reps; r++) {
+          rc = icu_me_estimate_device(h, d_cur, d_ref, h->d_mv, h->d_dist, nullptr, e.own);
+          if (rc) return rc;
+        }
+        HIP_TRY(hipEventRecord(b, e.own));
+        HIP_TRY(hipEventSynchronize(b));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, a, b));
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = t;
+          best_persistent = e.persistent;
+        }
       }
     }
   }
-  e.retile(0, best);
+  e.persistent = best_persistent;
   hipEventDestroy(a);
   hipEventDestroy(b);
+  return e.retile(0, best);
+}
+
+int icu_code_frame_qps_device(icu_me* h, const uint8_t* d_cur, const int16_t* d_mv, int nqp,
+                              const int* qps, int intra, int cols_first, int16_t* const* d_levels,
+                              uint8_t* const* d_recon, int16_t* d_pred, void* stream) {
+  if (!h || !d_cur || !d_mv || !qps || !d_levels || !d_recon) return fail(ICU_E_ARG, "null argument");
+  if (nqp < 1 || nqp > kMaxQpSet) return fail(ICU_E_ARG, "nqp must be 1..8");
+  QpSet qs{};
+  qs.n = nqp;
+  // the coder reads and writes whole rows of 4 samples
+  if (((uintptr_t)d_cur & 3) || ((uintptr_t)d_pred & 7))
+    return fail(ICU_E_ARG, "d_cur must be 4-byte, d_pred 8-byte aligned");
+  for (int i = 0; i < nqp; i++) {
+    if (qps[i] < 0 || qps[i] > 51) return fail(ICU_E_ARG, "qp must be 0..51");
+    if (!d_levels[i] || !d_recon[i]) return fail(ICU_E_ARG, "null output");
+    if (((uintptr_t)d_recon[i] & 3) || ((uintptr_t)d_levels[i] & 7))
+      return fail(ICU_E_ARG, "d_recon must be 4-byte, d_levels 8-byte aligned");
+    qs.qp[i] = qps[i];
+    qs.levels[i] = d_levels[i];
+    qs.recon[i] = d_recon[i];
+  }
+  Engine& e = h->eng;
+  const StreamPlan& s = e.sp[0];
+  launch_code_frame(d_cur, s.cfg.width, s.refpad, s.ref_stride, s.cfg.width, s.cfg.height, d_mv, qs,
+                    intra ? 1 : 0, cols_first ? 1 : 0, d_pred, pick(stream, e));
+  HIP_TRY(hipGetLastError());
   return ICU_OK;
 }
 
 int icu_code_frame_device(icu_me* h, const uint8_t* d_cur, const int16_t* d_mv, int qp, int intra,
                           int cols_first, int16_t* d_levels, uint8_t* d_recon, int16_t* d_pred,
                           void* stream) {
-  if (!h || !d_cur || !d_mv || !d_levels || !d_recon) return fail(ICU_E_ARG, "null argument");
-  if (qp < 0 || qp > 51) return fail(ICU_E_ARG, "qp must be 0..51");
-  Engine& e = h->eng;
-  const StreamPlan& s = e.sp[0];
-  launch_code_frame(d_cur, s.cfg.width, s.refpad, s.ref_stride, s.cfg.width, s.cfg.height, d_mv, qp,
-                    intra ? 1 : 0, cols_first ? 1 : 0, d_levels, d_recon, d_pred, pick(stream, e));
-  HIP_TRY(hipGetLastError());
-  return ICU_OK;
+  return icu_code_frame_qps_device(h, d_cur, d_mv, 1, &qp, intra, cols_first, &d_levels, &d_recon,
+                                   d_pred, stream);
 }
 
 // ------------------------------------------------------------ transforms
@@ -645,10 +715,6 @@ int icu_me_batch_create(const icu_me_config* cfgs, int n, icu_me_batch** out) {
     delete b;
     return rc;
   }
-  // persistent search grid: a few workgroups per CU walk the task queue
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, b->eng.device) == hipSuccess)
-    b->eng.search_grid = std::min((int)b->eng.h_tasks.size(), prop.multiProcessorCount * 4);
   *out = b;
   return ICU_OK;
 }

@@ -85,6 +85,34 @@ __host__ __device__ inline void motion_window(int n, int m, int W, int H, int L,
 // multiple of 4 so every lane's 4-candidate quad starts on a dword.
 __host__ __device__ inline int tile_dx0(int L) { return -4 * ((L + 3) / 4); }
 
+// Bits of a lane tile's raster index (4 V candidates) in the search kernel's
+// lane-local argmin keys.
+__host__ __device__ constexpr int lane_key_bits(int V) { return 4 * V <= 32 ? 5 : 6; }
+
+// True when every lane tile of the window (half-width L, tile rows V) keeps
+// its argmin codes ((dist2 - least dist2 of the tile) << bits | raster part)
+// below 2^16, so that (cost << 16) | code orders the tile exactly.
+inline bool lane_keys_fit(int L, int V) {
+  const int ib = lane_key_bits(V);
+  const int N = 2 * L + 1, B = (N + V - 1) / V, Q = (L - tile_dx0(L)) / 4 + 1;
+  auto span = [](int a, int len, int* lo2, int* hi2) {  // min / max of x^2 on [a, a+len)
+    const int b = a + len - 1;
+    const int m = a > 0 ? a : (b < 0 ? -b : 0);
+    const int M = (a < 0 ? -a : a) > (b < 0 ? -b : b) ? (a < 0 ? -a : a) : (b < 0 ? -b : b);
+    *lo2 = m * m;
+    *hi2 = M * M;
+  };
+  for (int b = 0; b < B; b++)
+    for (int q = 0; q < Q; q++) {
+      int xl2, xh2, yl2, yh2;
+      span(tile_dx0(L) + 4 * q, 4, &xl2, &xh2);
+      span(-L + b * V, V, &yl2, &yh2);
+      const long code = ((long)(xh2 + yh2 - xl2 - yl2) << ib) + (1 << ib) - 1;
+      if (code > 0xFFFF) return false;
+    }
+  return true;
+}
+
 // Per-MB integer-stage output of the search kernel.
 struct IntResult {
   int32_t mx, my;   // full-pel vector

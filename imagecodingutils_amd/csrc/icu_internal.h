@@ -35,8 +35,11 @@ struct DecideItem {
 
 void launch_pad_plane(const uint8_t* src, int W, int H, int src_stride, uint8_t* dst_origin,
                       int dst_stride, hipStream_t st);
-void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int grid,
-                   int metric, IntResult* d_out, uint32_t* d_table, hipStream_t st);
+void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, bool persistent,
+                   int metric, int vfix, bool table, int buf_dwords, IntResult* d_out,
+                   uint32_t* d_table, int* d_tickets, hipStream_t st);
+// Dwords of one staging buffer (window rows padded to 16 bytes + blocks).
+int search_stage_dwords(int B, int V, int Q, int G);
 int search_max_window_dwords();
 int search_max_group();
 
@@ -51,10 +54,17 @@ void launch_fwd4x4(const int16_t* in, int16_t* out, long n, int qp, int intra, i
                    int layout, hipStream_t st);
 void launch_inv4x4(const int16_t* in, int16_t* out, long n, int qp, int mode, int layout,
                    int cols_first, const int* d_weights, hipStream_t st);
+// QPs a frame's residual is coded at in one pass, with an output pair each.
+constexpr int kMaxQpSet = 8;
+struct QpSet {
+  int n;
+  int qp[kMaxQpSet];
+  int16_t* levels[kMaxQpSet];
+  uint8_t* recon[kMaxQpSet];
+};
 void launch_code_frame(const uint8_t* cur, int cur_stride, const uint8_t* ref_padded,
-                       int ref_stride, int W, int H, const int16_t* mv, int qp, int intra,
-                       int cols_first, int16_t* levels, uint8_t* recon, int16_t* pred_out,
-                       hipStream_t st);
+                       int ref_stride, int W, int H, const int16_t* mv, const QpSet& qs, int intra,
+                       int cols_first, int16_t* pred_out, hipStream_t st);
 void launch_sqrt_check(const int* m, double* out, int n, hipStream_t st);
 
 }  // namespace icu

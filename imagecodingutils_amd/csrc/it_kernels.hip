@@ -190,18 +190,18 @@ constexpr int kMbPerWg = 16;
 __global__ __launch_bounds__(256) void code_frame_kernel(const uint8_t* __restrict__ cur, int cur_stride,
                                                          const uint8_t* __restrict__ ref, int ref_stride,
                                                          int W, int H, const int16_t* __restrict__ mv,
-                                                         int qp, int intra, int cols_first,
-                                                         int16_t* __restrict__ levels,
-                                                         uint8_t* __restrict__ recon,
+                                                         const QpSet qs, int intra, int cols_first,
                                                          int16_t* __restrict__ pred_out) {
   __shared__ uint8_t patch[kMbPerWg][kPatch * kPatch];
-  __shared__ int fr[kMbPerWg][2];
   const int mbw = W / 16, nmb = mbw * (H / 16);
   const int lmb = threadIdx.x >> 4, blk = threadIdx.x & 15;
   const int mb = blockIdx.x * kMbPerWg + lmb;
   const bool live = mb < nmb;
   const int n = (mb % mbw) * 16, m = (mb / mbw) * 16;
+  const int bi = blk >> 2, bj = blk & 3;
   int qx = 0, qy = 0, ox = n, oy = m;
+  uint32_t w[10], cw[4];
+  int sh = 0;
   if (live) {
     const int mvx = mv[2 * mb], mvy = mv[2 * mb + 1];
     // full / quarter split with C truncation, negative fractions reflected
@@ -212,36 +212,87 @@ __global__ __launch_bounds__(256) void code_frame_kernel(const uint8_t* __restri
     int fox = 0, foy = 0;
     if (qx < 0) { fox = -1; qx += 4; }
     if (qy < 0) { foy = -1; qy += 4; }
-    ox = min(max(n + fx + fox, -kPad + 3), W + kPad - 19);
+    // Clamped into the padding, where rows and columns past the picture
+    // repeat its edge, so a clamped patch predicts what the far vector
+    // would; the dword reads below span [ox-6, ox+24] x [oy-3, oy+18].
+    ox = min(max(n + fx + fox, -kPad + 3), W + kPad - 25);
     oy = min(max(m + fy + foy, -kPad + 3), H + kPad - 19);
-    for (int i = blk; i < kPatch * kPatch; i += 16) {
-      const int r = i / kPatch, c = i - r * kPatch;
-      patch[lmb][i] = ref[(long)(oy - 3 + r) * ref_stride + ox - 3 + c];
+    // 22 rows x 7 dwords from the dword boundary at or left of ox-3 (the
+    // padded plane's origin and stride are multiples of 16): ten loads per
+    // lane, plus the four rows of this lane's 4x4 input block, all in flight
+    // before the first use
+    const int xa = ox - 3;
+    sh = xa & 3;
+    const uint32_t* base = (const uint32_t*)(ref + (long)(oy - 3) * ref_stride + (xa - sh));
+    const int sdw = ref_stride >> 2;
+#pragma unroll
+    for (int j = 0; j < 10; j++) {
+      const int i = blk + 16 * j, r = i / 7, k = i - 7 * r;
+      w[j] = i < 7 * kPatch ? base[r * sdw + k] : 0u;
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++)
+      cw[rr] = *(const uint32_t*)(cur + (long)(m + 4 * bi + rr) * cur_stride + n + 4 * bj);
+#pragma unroll
+    for (int j = 0; j < 10; j++) {
+      const int i = blk + 16 * j, r = i / 7, k = i - 7 * r;
+      if (i < 7 * kPatch) {
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+          const int c = 4 * k + b - sh;
+          if (c >= 0 && c < kPatch) patch[lmb][r * kPatch + c] = (uint8_t)(w[j] >> (8 * b));
+        }
+      }
     }
   }
   __syncthreads();
   if (!live) return;
-  const int bi = blk >> 2, bj = blk & 3;
   int16_t b[16], pr[16];
 #pragma unroll
   for (int p = 0; p < 16; p++) {
     const int r = 4 * bi + (p >> 2), c = 4 * bj + (p & 3);
     pr[p] = (int16_t)qpel_p(patch[lmb], 3 + c, 3 + r, qx, qy);
-    b[p] = (int16_t)(cur[(long)(m + r) * cur_stride + n + c] - pr[p]);
+    b[p] = (int16_t)((int)((cw[p >> 2] >> (8 * (p & 3))) & 0xFF) - pr[p]);
   }
-  fwd4x4(b, qp, intra, 0);
+  // 4 samples per row: 8-byte level / prediction stores, 4-byte recon stores
+  // (W and n are multiples of 16)
+  if (pred_out) {
 #pragma unroll
-  for (int p = 0; p < 16; p++) {
-    const int r = 4 * bi + (p >> 2), c = 4 * bj + (p & 3);
-    levels[(long)(m + r) * W + n + c] = b[p];
+    for (int rr = 0; rr < 4; rr++) {
+      const long o = (long)(m + 4 * bi + rr) * W + n + 4 * bj;
+      uint2 pv;
+      pv.x = (uint32_t)(uint16_t)pr[4 * rr] | ((uint32_t)(uint16_t)pr[4 * rr + 1] << 16);
+      pv.y = (uint32_t)(uint16_t)pr[4 * rr + 2] | ((uint32_t)(uint16_t)pr[4 * rr + 3] << 16);
+      *(uint2*)(pred_out + o) = pv;
+    }
   }
-  inv4x4(b, qp, 0, cols_first, nullptr);
+  // the residual is coded once per QP of the set
+#pragma unroll 1
+  for (int iq = 0; iq < qs.n; iq++) {
+    const int qp = qs.qp[iq];
+    int16_t t[16];
 #pragma unroll
-  for (int p = 0; p < 16; p++) {
-    const int r = 4 * bi + (p >> 2), c = 4 * bj + (p & 3);
-    const long o = (long)(m + r) * W + n + c;
-    recon[o] = (uint8_t)clip255(pr[p] + b[p]);
-    if (pred_out) pred_out[o] = pr[p];
+    for (int p = 0; p < 16; p++) t[p] = b[p];
+    fwd4x4(t, qp, intra, 0);
+    int16_t* levels = qs.levels[iq];
+    uint8_t* recon = qs.recon[iq];
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      const long o = (long)(m + 4 * bi + rr) * W + n + 4 * bj;
+      uint2 lv;
+      lv.x = (uint32_t)(uint16_t)t[4 * rr] | ((uint32_t)(uint16_t)t[4 * rr + 1] << 16);
+      lv.y = (uint32_t)(uint16_t)t[4 * rr + 2] | ((uint32_t)(uint16_t)t[4 * rr + 3] << 16);
+      *(uint2*)(levels + o) = lv;
+    }
+    inv4x4(t, qp, 0, cols_first, nullptr);
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      const long o = (long)(m + 4 * bi + rr) * W + n + 4 * bj;
+      uint32_t rc = 0;
+#pragma unroll
+      for (int cc = 0; cc < 4; cc++) rc |= (uint32_t)clip255(pr[4 * rr + cc] + t[4 * rr + cc]) << (8 * cc);
+      *(uint32_t*)(recon + o) = rc;
+    }
   }
 }
 
@@ -262,12 +313,11 @@ void launch_inv4x4(const int16_t* in, int16_t* out, long n, int qp, int mode, in
 }
 
 void launch_code_frame(const uint8_t* cur, int cur_stride, const uint8_t* ref_padded, int ref_stride,
-                       int W, int H, const int16_t* mv, int qp, int intra, int cols_first,
-                       int16_t* levels, uint8_t* recon, int16_t* pred_out, hipStream_t st) {
+                       int W, int H, const int16_t* mv, const QpSet& qs, int intra, int cols_first,
+                       int16_t* pred_out, hipStream_t st) {
   const int nmb = (W / 16) * (H / 16);
   hipLaunchKernelGGL(code_frame_kernel, dim3((nmb + kMbPerWg - 1) / kMbPerWg), dim3(256), 0, st, cur,
-                     cur_stride, ref_padded, ref_stride, W, H, mv, qp, intra, cols_first, levels, recon,
-                     pred_out);
+                     cur_stride, ref_padded, ref_stride, W, H, mv, qs, intra, cols_first, pred_out);
 }
 
 }  // namespace icu

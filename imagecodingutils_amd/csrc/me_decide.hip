@@ -34,15 +34,18 @@ namespace {
 constexpr int kHalf = 35;       // half-pel grid side
 constexpr int kSpinLimit = 1 << 26;
 
-__constant__ int c_subpos[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0},
-                                   {1, 0},   {-1, 1}, {0, 1},  {1, 1}};
 
 struct WaveLds {
+  uint32_t cur4[64];                // the MB's input block, 16 rows of 4 dwords
   uint8_t P[kPatch * kPatch];       // reference patch
   int16_t hv[17 * kPatch];          // raw vertical 6-tap sums
   int16_t Hg[kHalf * kHalf];        // G/b/h/j grid on the half-pel lattice
-  uint8_t cur[256];                 // the MB's input block
 };
+
+// Half-pel neighbour offsets in the reference's candidate order
+// (Full.cpp:452-483): row above left to right, left, right, row below.
+constexpr int kSubX[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+constexpr int kSubY[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
 
 // The MB's cost-table window [yu..yd] x [xl..xr], read straight from the
 // table (each entry is visited about once, so it is not staged).
@@ -60,6 +63,14 @@ __device__ __forceinline__ int wave_sum(int v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
+// Eight wave sums at once: the shuffles of one step are independent, so
+// their latencies overlap.
+__device__ __forceinline__ void wave_sum8(int (&a)[8]) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int k = 0; k < 8; k++) a[k] += __shfl_xor(a[k], o, 64);
+}
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -74,11 +85,30 @@ __device__ __forceinline__ int pix_dist(int a, int b, int metric) {
   return metric == kMetricSAD ? (d < 0 ? -d : d) : d * d;
 }
 
-// Stage the 22x22 patch whose (3,3) is reference pel (x0, y0).
+// Stage the 22x22 patch whose (3,3) is reference pel (x0, y0): 22 rows of 7
+// dwords read from the dword boundary at or left of x0-3 (the padded plane's
+// origin and stride are multiples of 16), every load issued before the first
+// LDS store.
 __device__ void load_patch(const DecideDesc& d, int x0, int y0, uint8_t* P, int lane) {
-  for (int i = lane; i < kPatch * kPatch; i += 64) {
-    const int r = i / kPatch, c = i - r * kPatch;
-    P[i] = d.ref[(long)(y0 - 3 + r) * d.ref_stride + (x0 - 3 + c)];
+  const int xa = x0 - 3, sh = xa & 3;  // 0..3 also for negative xa
+  const uint32_t* base = (const uint32_t*)(d.ref + (long)(y0 - 3) * d.ref_stride + (xa - sh));
+  const int sdw = d.ref_stride >> 2;
+  uint32_t w[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const int i = lane + 64 * j, r = i / 7, k = i - 7 * r;
+    w[j] = i < 7 * kPatch ? base[r * sdw + k] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const int i = lane + 64 * j, r = i / 7, k = i - 7 * r;
+    if (i < 7 * kPatch) {
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const int c = 4 * k + b - sh;
+        if (c >= 0 && c < kPatch) P[r * kPatch + c] = (uint8_t)(w[j] >> (8 * b));
+      }
+    }
   }
 }
 
@@ -156,32 +186,41 @@ __device__ void table_replay(const TabView& tab, int xl, int xr, int yu, int yd,
 }
 
 // Build the half-pel lattice around full-pel origin (ox, oy) (patch staged).
+// Hg[hy][hx]: even/even G, even/odd b, odd/even h, odd/odd j.  Each lane
+// fills whole 2x2 cells (G, b, h, j of one full-pel position), so no lane
+// branches on the parity; fixed trip counts let the LDS reads of all
+// iterations overlap.
 __device__ void build_half_grid(WaveLds& s, int lane) {
-  for (int i = lane; i < 17 * kPatch; i += 64) {
+  constexpr int kHv = 17 * kPatch, kCells = 18 * 18;
+#pragma unroll
+  for (int j = 0; j < (kHv + 63) / 64; j++) {
+    const int i = min(lane + 64 * j, kHv - 1);
     const int a = i / kPatch, x = i - a * kPatch;
     s.hv[i] = (int16_t)vtap_p(s.P, x, a + 2);
   }
   __syncthreads();
-  for (int i = lane; i < kHalf * kHalf; i += 64) {
-    const int hy = i / kHalf, hx = i - hy * kHalf;
-    int v;
-    if (!(hy & 1)) {
-      const int pr = hy / 2 + 2;
-      if (!(hx & 1)) v = s.P[pr * kPatch + hx / 2 + 2];
-      else v = b_p(s.P, (hx + 3) / 2, pr);
-    } else {
-      const int a = (hy - 1) / 2;
-      if (!(hx & 1)) {
-        v = clip255((s.hv[a * kPatch + hx / 2 + 2] + 16) >> 5);
-      } else {
-        const int xc = (hx + 3) / 2;
-        const int16_t* r = s.hv + a * kPatch;
-        const int t = ICU_TAP6((int)r[xc - 2], (int)r[xc - 1], (int)r[xc], (int)r[xc + 1],
-                               (int)r[xc + 2], (int)r[xc + 3]);
-        v = clip255((t + 512) >> 10);
-      }
+#pragma unroll
+  for (int j = 0; j < (kCells + 63) / 64; j++) {
+    const int i = min(lane + 64 * j, kCells - 1);
+    const int cy = i / 18, cx = i - cy * 18;
+    // (b, h and j of the last row / column are not stored; their reads are
+    // clamped into the arrays)
+    const int cxi = min(cx, 16), cyi = min(cy, 16);
+    const int G = s.P[(cy + 2) * kPatch + cx + 2];
+    const int b = b_p(s.P, cxi + 2, cy + 2);
+    const int16_t* r = s.hv + cyi * kPatch;
+    const int h = clip255((r[cx + 2] + 16) >> 5);
+    const int t = ICU_TAP6((int)r[cxi], (int)r[cxi + 1], (int)r[cxi + 2], (int)r[cxi + 3],
+                           (int)r[cxi + 4], (int)r[cxi + 5]);
+    const int jv = clip255((t + 512) >> 10);
+    int16_t* o = s.Hg + (2 * cy) * kHalf + 2 * cx;
+    // cells on the last row / column have only their even samples inside
+    o[0] = (int16_t)G;
+    if (cx < 17) o[1] = (int16_t)b;
+    if (cy < 17) {
+      o[kHalf] = (int16_t)h;
+      if (cx < 17) o[kHalf + 1] = (int16_t)jv;
     }
-    s.Hg[i] = (int16_t)v;
   }
   __syncthreads();
 }
@@ -196,8 +235,21 @@ __device__ Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, int predX
   const int col = mb % d.mbw, row = mb / d.mbw;
   const int n = col * 16, m = row * 16;
   const int slot = d.mb_base + mb;
-  // input block
-  for (int i = lane; i < 256; i += 64) s.cur[i] = d.cur[(long)(m + (i >> 4)) * d.cur_stride + n + (i & 15)];
+  // input block: one aligned dword per lane (current planes are 16-byte
+  // aligned with a stride that is a multiple of 16)
+  const uint32_t cw = *(const uint32_t*)(d.cur + (long)(m + (lane >> 2)) * d.cur_stride + n + 4 * (lane & 3));
+  s.cur4[lane] = cw;
+  const uint8_t* cur8 = (const uint8_t*)s.cur4;
+  // the zero vector's cost, which the argmin search does not report; only
+  // GetBestVector reads it (the padded plane's rows are 16-byte aligned)
+  int zero_direct = 0;
+  if (!d.table && d.bestvec) {
+    const uint32_t rw = *(const uint32_t*)(d.ref + (long)(m + (lane >> 2)) * d.ref_stride + n + 4 * (lane & 3));
+    int z = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) z += pix_dist((int)((cw >> (8 * i)) & 0xFF), (int)((rw >> (8 * i)) & 0xFF), d.metric);
+    zero_direct = wave_sum(z);
+  }
 
   const int predXQ = predX % 4, predYQ = predY % 4;
   int predX0 = predX / 4, predY0 = predY / 4;
@@ -223,7 +275,7 @@ __device__ Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, int predX
     mx = r.mx;
     my = r.my;
     minD = r.cost;
-    zeroD = r.zero;
+    zeroD = zero_direct;
   }
   Decision out;
   out.imx = mx;
@@ -235,56 +287,57 @@ __device__ Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, int predX
     load_patch(d, n + mx, m + my, s.P, lane);
     __syncthreads();
     build_half_grid(s, lane);
-    // half-pel candidates (strictly better only)
-    int hb = 0, hxh = 0, hyh = 0;
-    for (int k = 0; k < 8; k++) {
-      const int sx = c_subpos[k][0], sy = c_subpos[k][1];
-      int acc = 0;
+    // half-pel candidates, all eight summed at once, then taken in the
+    // reference's order (strictly better only)
+    int hxh = 0, hyh = 0;
+    {
+      int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
       for (int p = lane; p < 256; p += 64) {
         const int r = p >> 4, c = p & 15;
-        acc += pix_dist(s.cur[p], s.Hg[(2 + 2 * r + sy) * kHalf + 2 + 2 * c + sx], d.metric);
+        const int cv = cur8[p];
+        const int16_t* h = s.Hg + (2 + 2 * r) * kHalf + 2 + 2 * c;
+#pragma unroll
+        for (int k = 0; k < 8; k++) acc[k] += pix_dist(cv, h[kSubY[k] * kHalf + kSubX[k]], d.metric);
       }
-      acc = wave_sum(acc);
-      if (acc < minD) {
-        minD = acc;
-        hxh = sx;
-        hyh = sy;
-        hb = 1;
-      }
+      wave_sum8(acc);
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (acc[k] < minD) {
+          minD = acc[k];
+          hxh = kSubX[k];
+          hyh = kSubY[k];
+        }
     }
-    (void)hb;
     int qmx = 2 * hxh, qmy = 2 * hyh;
     if (d.subpel >= 2) {
+      // quarter candidates: the reference's pairwise averages of half-grid
+      // samples around the half-pel winner H = (cr, cc) (LoadQuartPelWindow)
       const int hx = 2 * hxh, hy = 2 * hyh;
-      for (int k = 0; k < 8; k++) {
-        const int dx = c_subpos[k][0], dy = c_subpos[k][1];
-        int acc = 0;
-        for (int p = lane; p < 256; p += 64) {
-          const int r = p >> 4, c = p & 15;
-          const int cr = 2 + 2 * r + hyh, cc = 2 + 2 * c + hxh;
-          int a, b;
-          const int16_t* Hg = s.Hg;
-          if (dy == -1) {
-            if (dx == -1) { a = Hg[(cr - 1) * kHalf + cc - 1]; b = Hg[cr * kHalf + cc]; }
-            else if (dx == 0) { a = Hg[(cr - 1) * kHalf + cc]; b = Hg[cr * kHalf + cc]; }
-            else { a = Hg[(cr - 1) * kHalf + cc]; b = Hg[cr * kHalf + cc + 1]; }
-          } else if (dy == 0) {
-            if (dx == -1) { a = Hg[cr * kHalf + cc - 1]; b = Hg[cr * kHalf + cc]; }
-            else { a = Hg[cr * kHalf + cc + 1]; b = Hg[cr * kHalf + cc]; }
-          } else {
-            if (dx == -1) { a = Hg[cr * kHalf + cc - 1]; b = Hg[(cr + 1) * kHalf + cc]; }
-            else if (dx == 0) { a = Hg[(cr + 1) * kHalf + cc]; b = Hg[cr * kHalf + cc]; }
-            else { a = Hg[(cr + 1) * kHalf + cc]; b = Hg[cr * kHalf + cc + 1]; }
-          }
-          acc += pix_dist(s.cur[p], (a + b + 1) >> 1, d.metric);
-        }
-        acc = wave_sum(acc);
-        if (acc < minD) {
-          minD = acc;
-          qmx = hx + dx;
-          qmy = hy + dy;
-        }
+      int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int p = lane; p < 256; p += 64) {
+        const int r = p >> 4, c = p & 15;
+        const int cv = cur8[p];
+        const int16_t* h = s.Hg + (2 + 2 * r + hyh) * kHalf + 2 + 2 * c + hxh;
+        const int uu = h[-kHalf - 1], u0 = h[-kHalf], l0 = h[-1], c0 = h[0], r0 = h[1], d0 = h[kHalf];
+        acc[0] += pix_dist(cv, (uu + c0 + 1) >> 1, d.metric);  // (-1,-1)
+        acc[1] += pix_dist(cv, (u0 + c0 + 1) >> 1, d.metric);  // ( 0,-1)
+        acc[2] += pix_dist(cv, (u0 + r0 + 1) >> 1, d.metric);  // ( 1,-1)
+        acc[3] += pix_dist(cv, (l0 + c0 + 1) >> 1, d.metric);  // (-1, 0)
+        acc[4] += pix_dist(cv, (r0 + c0 + 1) >> 1, d.metric);  // ( 1, 0)
+        acc[5] += pix_dist(cv, (l0 + d0 + 1) >> 1, d.metric);  // (-1, 1)
+        acc[6] += pix_dist(cv, (d0 + c0 + 1) >> 1, d.metric);  // ( 0, 1)
+        acc[7] += pix_dist(cv, (d0 + r0 + 1) >> 1, d.metric);  // ( 1, 1)
       }
+      wave_sum8(acc);
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (acc[k] < minD) {
+          minD = acc[k];
+          qmx = hx + kSubX[k];
+          qmy = hy + kSubY[k];
+        }
     }
     mvx += qmx;
     mvy += qmy;
@@ -312,25 +365,27 @@ __device__ Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, int predX
       int acc = 0;
       for (int p = lane; p < 256; p += 64) {
         const int r = p >> 4, c = p & 15;
-        acc += pix_dist(s.cur[p], qpel_p(s.P, 3 + c, 3 + r, fx, fy), d.metric);
+        acc += pix_dist(cur8[p], qpel_p(s.P, 3 + c, 3 + r, fx, fy), d.metric);
       }
       predD = wave_sum(acc);
       __syncthreads();
     }
     // fifo [best, pred, zero, zero, zero]; strictly smaller cost wins
-    const int xs[3] = {mvx, pX, 0}, ys[3] = {mvy, pY, 0}, ds[3] = {minD, predD, zeroD};
-    double cc = best_vector_cost(ds[0], (xs[0] - pX) * (xs[0] - pX) + (ys[0] - pY) * (ys[0] - pY));
-    int pos = 0;
-    for (int i = 1; i < 3; i++) {
-      const double nc = best_vector_cost(ds[i], (xs[i] - pX) * (xs[i] - pX) + (ys[i] - pY) * (ys[i] - pY));
-      if (nc < cc) {
-        cc = nc;
-        pos = i;
-      }
+    // (selects rather than an indexed array, which would live in scratch)
+    double cc = best_vector_cost(minD, (mvx - pX) * (mvx - pX) + (mvy - pY) * (mvy - pY));
+    const double cp = best_vector_cost(predD, 0);
+    if (cp < cc) {
+      cc = cp;
+      outx = pX;
+      outy = pY;
+      outd = predD;
     }
-    outx = xs[pos];
-    outy = ys[pos];
-    outd = ds[pos];
+    const double cz = best_vector_cost(zeroD, pX * pX + pY * pY);
+    if (cz < cc) {
+      outx = 0;
+      outy = 0;
+      outd = zeroD;
+    }
   }
   out.mvx = outx;
   out.mvy = outy;

@@ -23,36 +23,54 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "icu_device.h"
 #include "icu_internal.h"
 
 namespace icu {
 
 constexpr int kThreads = 256;
-constexpr int kWinDwords = 8192;  // 32 KB staged reference window
+// One staging buffer (window + blocks); the kernel holds two, and the pair
+// plus the static arrays stays under the 64 KB dynamic-LDS launch limit.
+constexpr int kWinDwords = 7680;
 constexpr int kMaxG = 16;         // MBs per task
 
 // ---------------------------------------------------------------- padding
 // Edge-replicated copy of an 8-bit plane into the padded plane
 // (OverlayExtMem2Dv2::ExtendBoundary/FillBoundary,
 // src/ImageUtils/OverlayExtMem2Dv2.cpp:120-217, as a coordinate clamp).
+// 16 bytes per thread: chunks inside the picture copy one 16-byte load
+// (kPad, W and the source stride are multiples of 16), border chunks gather
+// clamped bytes.
 __global__ __launch_bounds__(256) void pad_plane_kernel(const uint8_t* __restrict__ src, int W,
                                                         int H, int src_stride,
                                                         uint8_t* __restrict__ dst_origin,
                                                         int dst_stride) {
-  const int pw = W + 2 * kPad;
-  const int x4 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;  // 4 bytes per thread
-  const int y = blockIdx.y;
-  if (x4 >= pw) return;
+  const int cpr = (W + 2 * kPad) / 16;  // 16-byte chunks per padded row
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= cpr * (H + 2 * kPad)) return;
+  const int y = idx / cpr, x16 = (idx - y * cpr) * 16;
   const int yy = min(max(y - kPad, 0), H - 1);
   const uint8_t* row = src + (long)yy * src_stride;
-  uint32_t w = 0;
+  const int x = x16 - kPad;
+  uint4 v;
+  if (x >= 0 && x + 16 <= W && (((uintptr_t)row & 15) == 0)) {
+    v = *(const uint4*)(row + x);
+  } else {
+    uint32_t w[4];
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    int xx = min(max(x4 + i - kPad, 0), W - 1);
-    w |= (uint32_t)row[xx] << (8 * i);
+    for (int q = 0; q < 4; q++) {
+      w[q] = 0;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int xx = min(max(x + 4 * q + i, 0), W - 1);
+        w[q] |= (uint32_t)row[xx] << (8 * i);
+      }
+    }
+    v = make_uint4(w[0], w[1], w[2], w[3]);
   }
-  *(uint32_t*)(dst_origin - (long)kPad * dst_stride - kPad + (long)y * dst_stride + x4) = w;
+  *(uint4*)(dst_origin - (long)kPad * dst_stride - kPad + (long)y * dst_stride + x16) = v;
 }
 
 // ------------------------------------------------------------- tile bodies
@@ -164,6 +182,14 @@ __device__ __forceinline__ void ssd_tile(const uint32_t* wp, int row_dw,
 }
 
 // ------------------------------------------------------------- the kernel
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// Lane-wise min of two pairs of packed u16 (v_pk_min_u16).
+__device__ __forceinline__ uint32_t min_pk_u16(uint32_t a, uint32_t b) {
+  const u16x2 r = __builtin_elementwise_min(__builtin_bit_cast(u16x2, a), __builtin_bit_cast(u16x2, b));
+  return __builtin_bit_cast(uint32_t, r);
+}
+
 // Fold one candidate into the MB's running key / the cost table.
 __device__ __forceinline__ void visit(int dx, int dy, uint32_t cc, int xl, int xr, int yu, int yd,
                                       int L, uint32_t* trow, long long tdim, int* zero_g,
@@ -183,7 +209,7 @@ __device__ __forceinline__ void visit(int dx, int dy, uint32_t cc, int xl, int x
     trow[(dy + L) * tdim + (dx + L)] = inwin ? cc : 0xFFFFFFFFu;
 }
 
-template <int METRIC, int V>
+template <int METRIC, int V, bool TABLE>
 __device__ __forceinline__ void run_tile(const StreamDesc& s, const SearchTask& t, int u,
                                          const uint32_t* win, int row_dw, const uint32_t* curs,
                                          const uint32_t* csq, uint64_t* best, int* zero,
@@ -211,14 +237,61 @@ __device__ __forceinline__ void run_tile(const StreamDesc& s, const SearchTask& 
   const int dy0 = -s.L + b * V;
   uint64_t kbest = ~0ull;
   const long long tdim = s.table_dim;
-  uint32_t* trow = table ? table + s.table_base + (long long)mb * tdim * tdim : nullptr;
-  if (METRIC == kMetricSAD) {
+  uint32_t* trow = (TABLE && s.table_base_valid) ? table + s.table_base + (long long)mb * tdim * tdim
+                                                 : nullptr;
+  if (METRIC == kMetricSAD && TABLE && trow) {
     sad_tile<V>(wp, row_dw, c, [&](int v, uint32_t lo, uint32_t hi) {
       visit(dx0 + 0, dy0 + v, lo & 0xFFFF, xl, xr, yu, yd, s.L, trow, tdim, &zero[g], kbest);
       visit(dx0 + 1, dy0 + v, lo >> 16, xl, xr, yu, yd, s.L, trow, tdim, &zero[g], kbest);
       visit(dx0 + 2, dy0 + v, hi & 0xFFFF, xl, xr, yu, yd, s.L, trow, tdim, &zero[g], kbest);
       visit(dx0 + 3, dy0 + v, hi >> 16, xl, xr, yu, yd, s.L, trow, tdim, &zero[g], kbest);
     });
+  } else if (METRIC == kMetricSAD) {
+    // Argmin only, branch-free.  Each candidate gets a lane-local 32-bit key
+    //   (cost << 16) | code,  code = ((dist2 - d2min) << IB) | (2^IB - 1 - idx)
+    // with idx = 4 v + k its raster position in the tile and d2min the least
+    // dx^2 + dy^2 over the tile: within the lane this orders candidates as
+    // the full key does (cost, then distance to the zero predictor, then
+    // later raster first).  The host admits a tiling only if every tile's
+    // codes fit 16 bits (lane_keys_fit).  code splits into a column part
+    // Ck[k] and a row part Ev, so one row costs 3 ORs (window masks), 4 key
+    // builds, 3 mins and 2 adds; the full 64-bit key is built once per tile.
+    // Candidates outside the window get cost 0xFFFF, above any 16x16 SAD.
+    constexpr int IB = lane_key_bits(V);
+    const uint32_t cm_lo = (dx0 >= xl && dx0 <= xr ? 0u : 0xFFFFu) |
+                           (dx0 + 1 >= xl && dx0 + 1 <= xr ? 0u : 0xFFFF0000u);
+    const uint32_t cm_hi = (dx0 + 2 >= xl && dx0 + 2 <= xr ? 0u : 0xFFFFu) |
+                           (dx0 + 3 >= xl && dx0 + 3 <= xr ? 0u : 0xFFFF0000u);
+    const int ax = dx0 > 0 ? dx0 : (dx0 + 3 < 0 ? -(dx0 + 3) : 0);
+    const int ay = dy0 > 0 ? dy0 : (dy0 + V - 1 < 0 ? -(dy0 + V - 1) : 0);
+    uint32_t Ck[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      Ck[k] = ((uint32_t)((dx0 + k) * (dx0 + k) - ax * ax) << IB) + (uint32_t)(3 - k) + ((1u << IB) - 4u * V);
+    uint32_t Ev = ((uint32_t)(dy0 * dy0 - ay * ay) << IB) + 4u * (V - 1);
+    const uint32_t Dl = (uint32_t)(2 * dy0) << IB;
+    // rows of the tile inside [yu, yd]: bit v
+    const int vlo = max(yu - dy0, 0), vhi = min(yd - dy0, V - 1);
+    const uint32_t inv_rows = vlo > vhi ? 0xFFFFFFFFu : ~(((2u << vhi) - 1u) & ~((1u << vlo) - 1u));
+    uint32_t lbest = 0xFFFFFFFFu;
+    sad_tile<V>(wp, row_dw, c, [&](int v, uint32_t lo, uint32_t hi) {
+      const uint32_t rmask = (uint32_t)__builtin_amdgcn_sbfe((int)inv_rows, v, 1);
+      const uint32_t l2 = lo | cm_lo | rmask, h2 = hi | cm_hi | rmask;
+      const uint32_t k0 = (l2 << 16) | Ck[0];
+      const uint32_t k1 = (l2 & 0xFFFF0000u) | Ck[1];
+      const uint32_t k2 = (h2 << 16) | Ck[2];
+      const uint32_t k3 = (h2 & 0xFFFF0000u) | Ck[3];
+      lbest = min(lbest, min(min(k0, k1), min(k2, k3)) + Ev);
+      // Ev of row v+1: ((dy+1)^2 - dy^2) << IB, minus 4 for the raster part
+      Ev += Dl + (((2u * v + 1u) << IB) - 4u);
+    });
+    if (lbest < 0xFFFF0000u) {
+      const int idx = (1 << IB) - 1 - (int)(lbest & ((1u << IB) - 1u));
+      const int dx = dx0 + (idx & 3), dy = dy0 + (idx >> 2);
+      const uint32_t cost = lbest >> 16;
+      kbest = (dx == 0 && dy == 0) ? pack_key(cost, 0, kZeroRRank)
+                                   : pack_key(cost, (uint32_t)(dx * dx + dy * dy), raster_rrank(dy, dx));
+    }
   } else {
     const uint32_t c2 = csq[g];
     ssd_tile<V>(wp, row_dw, c, [&](int v, const uint32_t (&a)[4]) {
@@ -230,59 +303,121 @@ __device__ __forceinline__ void run_tile(const StreamDesc& s, const SearchTask& 
   atomicMin((unsigned long long*)&best[g], (unsigned long long)kbest);
 }
 
-template <int METRIC>
+// Staging of one task into an LDS buffer with direct global->LDS loads
+// (global_load_lds_dwordx4: lane l of a wave writes its 16 bytes at the
+// wave's LDS base + 16 l).  The window rows are padded to whole 16-byte
+// chunks; the blocks follow the window in the same buffer.
+// The window's first column, tile_dx0(L) pels left of the MB, is only
+// 4-byte aligned; staging reads from the 16-byte boundary below it and the
+// lanes skip the `shift` leading dwords.
+__device__ __forceinline__ int stage_shift(int L) { return ((tile_dx0(L) % 16) + 16) % 16 / 4; }
+
+struct Stage {
+  int rows, row_dw, row_dw4, win_chunks, cur_chunks, shift;
+};
+
+__device__ __forceinline__ Stage stage_shape(const StreamDesc& s, const SearchTask& t) {
+  Stage g;
+  g.rows = s.B * s.V + 15;
+  g.shift = stage_shift(s.L);
+  g.row_dw = 4 * (t.count - 1) + s.Q + 4;
+  g.row_dw4 = (g.row_dw + g.shift + 3) & ~3;
+  g.win_chunks = g.rows * (g.row_dw4 >> 2);
+  g.cur_chunks = t.count * 16;
+  return g;
+}
+
+__device__ __forceinline__ void issue_stage(const StreamDesc& s, const SearchTask& t, uint32_t* buf,
+                                            int tid) {
+  const Stage g = stage_shape(s, t);
+  const int col0 = t.mb % s.mbw, row = t.mb / s.mbw;
+  const uint8_t* wsrc =
+      s.ref + (long)(16 * row - s.L) * s.ref_stride + 16 * col0 + tile_dx0(s.L) - 4 * g.shift;
+  const int cpr = g.row_dw4 >> 2;  // chunks per window row
+  const int total = g.win_chunks + g.cur_chunks;
+  const int wave = tid >> 6, lane = tid & 63;
+  for (int base = wave * 64; base < total; base += kThreads) {
+    const int ch = base + lane;
+    const uint8_t* src;
+    if (ch < g.win_chunks) {
+      const int r = ch / cpr, c4 = ch - r * cpr;
+      src = wsrc + (long)r * s.ref_stride + 16 * c4;
+    } else {
+      const int k = ch - g.win_chunks, gm = k >> 4, r = k & 15;
+      src = s.cur + (long)(16 * row + r) * s.cur_stride + 16 * (col0 + gm);
+    }
+    if (ch < total)
+      __builtin_amdgcn_global_load_lds((const void*)src,
+                                       (__attribute__((address_space(3))) void*)(buf + 4 * base), 16, 0, 0);
+  }
+}
+
+// VFIX: tile height fixed at compile time, or 0 to take each stream's own
+// (mixed batches).  TABLE: write every cost to the table (filtered scan /
+// predictor modes) instead of only the argmin.
+// Persistent: workgroup w runs tasks w, w + grid, ... and streams the next
+// task's window into the other half of LDS while it computes the current one.
+template <int METRIC, int VFIX, bool TABLE>
 __global__ __launch_bounds__(kThreads) void me_search_kernel(const StreamDesc* __restrict__ streams,
                                                              const SearchTask* __restrict__ tasks,
                                                              int ntasks, IntResult* __restrict__ out,
-                                                             uint32_t* __restrict__ table) {
-  __shared__ uint32_t win[kWinDwords];
-  __shared__ __attribute__((aligned(16))) uint32_t curs[kMaxG * 64];
+                                                             uint32_t* __restrict__ table, int buf_dwords,
+                                                             int* __restrict__ tickets) {
+  extern __shared__ __attribute__((aligned(16))) uint32_t dyn[];
+  __shared__ int next_s;
   __shared__ uint32_t csq[kMaxG];
   __shared__ uint64_t best[kMaxG];
   __shared__ int zero[kMaxG];
   const int tid = threadIdx.x;
+  uint32_t* bufs[2] = {dyn, dyn + buf_dwords};
 
-  for (int ti = blockIdx.x; ti < ntasks; ti += gridDim.x) {
+  // tasks [0, grid) go one per workgroup; the rest are claimed through
+  // tickets[0] as workgroups free up; the last workgroup to finish zeroes
+  // both counters for the next launch (host keeps grid <= ntasks)
+  int ti = blockIdx.x;
+  {
+    const SearchTask t0 = tasks[ti];
+    issue_stage(streams[t0.stream], t0, bufs[0], tid);
+  }
+  if (tid < kMaxG) {
+    best[tid] = ~0ull;
+    zero[tid] = -1;
+  }
+  for (int it = 0; ti < ntasks; it++) {
     const SearchTask t = tasks[ti];
     const StreamDesc s = streams[t.stream];
-    const int col0 = t.mb % s.mbw, row = t.mb / s.mbw;
-    const int rows = s.B * s.V + 15;
-    const int row_dw = 4 * (t.count - 1) + s.Q + 4;
-    const uint8_t* wsrc = s.ref + (long)(16 * row - s.L) * s.ref_stride + 16 * col0 + tile_dx0(s.L);
-    for (int i = tid; i < rows * row_dw; i += kThreads) {
-      const int r = i / row_dw, cdw = i - r * row_dw;
-      win[i] = *(const uint32_t*)(wsrc + (long)r * s.ref_stride + 4 * cdw);
-    }
-    for (int i = tid; i < t.count * 64; i += kThreads) {
-      const int g = i >> 6, r = (i >> 2) & 15, k = i & 3;
-      curs[i] = *(const uint32_t*)(s.cur + (long)(16 * row + r) * s.cur_stride + 16 * (col0 + g) + 4 * k);
-    }
-    if (tid < t.count) {
-      best[tid] = ~0ull;
-      zero[tid] = -1;
-    }
+    const Stage g = stage_shape(s, t);
+    uint32_t* win = bufs[it & 1] + g.shift;
+    const uint32_t* curs = bufs[it & 1] + g.win_chunks * 4;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's staging loads have landed
+    __syncthreads();                // ... and every other wave's
+    if (tid == 0) next_s = (int)gridDim.x + atomicAdd(&tickets[0], 1);
     __syncthreads();
-    if (METRIC == kMetricSSD && tid < t.count) {
-      uint32_t a = 0;
-      for (int i = 0; i < 64; i++) a = __builtin_amdgcn_udot4(curs[tid * 64 + i], curs[tid * 64 + i], a, false);
-      csq[tid] = a;
+    const int tnext = next_s;
+    if (tnext < ntasks) {
+      const SearchTask tn = tasks[tnext];
+      issue_stage(streams[tn.stream], tn, bufs[(it + 1) & 1], tid);
     }
-    __syncthreads();
+    if (METRIC == kMetricSSD) {
+      if (tid < t.count) {
+        uint32_t a = 0;
+        for (int i = 0; i < 64; i++) a = __builtin_amdgcn_udot4(curs[tid * 64 + i], curs[tid * 64 + i], a, false);
+        csq[tid] = a;
+      }
+      __syncthreads();
+    }
     const int units = t.count * s.Q * s.B;
     for (int u = tid; u < units; u += kThreads) {
-      if (METRIC == kMetricSSD) {
-        switch (s.V) {
-          case 8: run_tile<METRIC, 8>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
-          case 11: run_tile<METRIC, 11>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
-          case 13: run_tile<METRIC, 13>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
-          default: run_tile<METRIC, 16>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
-        }
+      if constexpr (METRIC == kMetricSSD) {
+        run_tile<METRIC, 8, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table);  // SSD tiles are 8 rows
+      } else if constexpr (VFIX != 0) {
+        run_tile<METRIC, VFIX, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table);
       } else {
         switch (s.V) {
-          case 8: run_tile<METRIC, 8>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
-          case 11: run_tile<METRIC, 11>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
-          case 13: run_tile<METRIC, 13>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
-          default: run_tile<METRIC, 16>(s, t, u, win, row_dw, curs, csq, best, zero, table); break;
+          case 8: run_tile<METRIC, 8, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table); break;
+          case 11: run_tile<METRIC, 11, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table); break;
+          case 13: run_tile<METRIC, 13, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table); break;
+          default: run_tile<METRIC, 16, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table); break;
         }
       }
     }
@@ -297,32 +432,89 @@ __global__ __launch_bounds__(kThreads) void me_search_kernel(const StreamDesc* _
       r.cost = (int)key_cost(k);
       r.zero = zero[tid];
       out[s.mb_base + t.mb + tid] = r;
+      best[tid] = ~0ull;
+      zero[tid] = -1;
     }
-    __syncthreads();
+    ti = tnext;
+  }
+  if (tid == 0) {
+    __threadfence();
+    if (atomicAdd(&tickets[1], 1) == (int)gridDim.x - 1) {
+      atomicExch(&tickets[0], 0);
+      atomicExch(&tickets[1], 0);
+    }
   }
 }
 
 // ----------------------------------------------------------------- host side
 void launch_pad_plane(const uint8_t* src, int W, int H, int src_stride, uint8_t* dst_origin,
                       int dst_stride, hipStream_t st) {
-  const int pw = W + 2 * kPad;
-  dim3 grid((pw / 4 + 255) / 256, H + 2 * kPad);
-  hipLaunchKernelGGL(pad_plane_kernel, grid, dim3(256), 0, st, src, W, H, src_stride, dst_origin,
+  const int chunks = (W + 2 * kPad) / 16 * (H + 2 * kPad);
+  hipLaunchKernelGGL(pad_plane_kernel, dim3((chunks + 255) / 256), dim3(256), 0, st, src, W, H, src_stride, dst_origin,
                      dst_stride);
 }
 
-void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int grid,
-                   int metric, IntResult* d_out, uint32_t* d_table, hipStream_t st) {
-  if (grid <= 0 || grid > ntasks) grid = ntasks;
-  if (metric == kMetricSAD)
-    hipLaunchKernelGGL(me_search_kernel<kMetricSAD>, dim3(grid), dim3(kThreads), 0, st, d_streams,
-                       d_tasks, ntasks, d_out, d_table);
-  else
-    hipLaunchKernelGGL(me_search_kernel<kMetricSSD>, dim3(grid), dim3(kThreads), 0, st, d_streams,
-                       d_tasks, ntasks, d_out, d_table);
+// vfix: the common tile height of every task, or 0 when streams differ.
+// buf_dwords: one staging buffer (window + blocks) of the largest task; the
+// kernel holds two.
+void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, bool persistent,
+                   int metric, int vfix, bool table, int buf_dwords, IntResult* d_out,
+                   uint32_t* d_table, int* d_tickets, hipStream_t st) {
+  const size_t shmem = 2 * (size_t)buf_dwords * sizeof(uint32_t);
+  auto go = [&](auto kern) {
+    int grid = ntasks;
+    if (persistent) {
+      // as many workgroups as stay resident at once (registers and LDS),
+      // remembered per kernel and LDS size
+      static thread_local const void* last_kern = nullptr;
+      static thread_local size_t last_shmem = 0;
+      static thread_local int last_dev = -1, last_res = 0;
+      int dev = 0;
+      hipGetDevice(&dev);
+      if (last_kern != (const void*)kern || last_shmem != shmem || last_dev != dev) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kThreads, shmem) != hipSuccess)
+          per_cu = 1;
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        last_kern = (const void*)kern;
+        last_shmem = shmem;
+        last_dev = dev;
+        last_res = std::max(1, per_cu) * std::max(1, cus);
+      }
+      grid = std::min(ntasks, last_res);
+    }
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), shmem, st, d_streams, d_tasks, ntasks, d_out,
+                       d_table, buf_dwords, d_tickets);
+  };
+  if (metric == kMetricSSD) {
+    if (table) go(me_search_kernel<kMetricSSD, 8, true>);
+    else go(me_search_kernel<kMetricSSD, 8, false>);
+    return;
+  }
+  if (table) {
+    switch (vfix) {
+      case 8: go(me_search_kernel<kMetricSAD, 8, true>); break;
+      case 11: go(me_search_kernel<kMetricSAD, 11, true>); break;
+      case 13: go(me_search_kernel<kMetricSAD, 13, true>); break;
+      case 16: go(me_search_kernel<kMetricSAD, 16, true>); break;
+      default: go(me_search_kernel<kMetricSAD, 0, true>); break;
+    }
+  } else {
+    switch (vfix) {
+      case 8: go(me_search_kernel<kMetricSAD, 8, false>); break;
+      case 11: go(me_search_kernel<kMetricSAD, 11, false>); break;
+      case 13: go(me_search_kernel<kMetricSAD, 13, false>); break;
+      case 16: go(me_search_kernel<kMetricSAD, 16, false>); break;
+      default: go(me_search_kernel<kMetricSAD, 0, false>); break;
+    }
+  }
 }
 
 int search_max_window_dwords() { return kWinDwords; }
+int search_stage_dwords(int B, int V, int Q, int G) {
+  const int rows = B * V + 15, row_dw4 = (4 * (G - 1) + Q + 4 + 3 + 3) & ~3;  // + largest shift
+  return rows * row_dw4 + 64 * G;
+}
 int search_max_group() { return kMaxG; }
 
 }  // namespace icu

@@ -86,11 +86,14 @@ int icu_me_estimate_device(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref
 int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int16_t* d_mv,
                           int32_t* d_dist, void* stream, float* stage_ms);
 
-/* Tile choice in effect for this handle (after tuning). */
-int icu_me_get_tiling(icu_me* h, int* tile_rows, int* mbs_per_task);
+/* Tile choice in effect for this handle (after tuning): candidate rows per
+ * lane, MBs per workgroup task, and whether the search grid is persistent
+ * (workgroups claim tasks until none are left) or one task per workgroup. */
+int icu_me_get_tiling(icu_me* h, int* tile_rows, int* mbs_per_task, int* persistent);
 
-/* Time each candidate tiling on the given device frames and keep the
- * fastest (the tuner).  reps: timed repetitions per candidate. */
+/* Time each candidate tiling (rows per lane x MBs per task x grid mode) on
+ * the given device frames and keep the fastest (the tuner).  reps: timed
+ * repetitions per candidate.  No reference counterpart. */
 int icu_me_autotune(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int reps);
 
 /* Luma motion compensation + residual coding of one frame on the device:
@@ -105,6 +108,15 @@ int icu_me_autotune(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int r
 int icu_code_frame_device(icu_me* h, const uint8_t* d_cur, const int16_t* d_mv, int qp, int intra,
                           int cols_first, int16_t* d_levels, uint8_t* d_recon, int16_t* d_pred,
                           void* stream);
+
+/* icu_code_frame_device at nqp (1..8) QPs in one pass: the prediction and
+ * residual are formed once, then coded and reconstructed at each qps[i] into
+ * d_levels[i] / d_recon[i] (host arrays of device pointers).  No reference
+ * counterpart: the reference codes a frame at one QP per call; this is the
+ * same per-QP computation, batched. */
+int icu_code_frame_qps_device(icu_me* h, const uint8_t* d_cur, const int16_t* d_mv, int nqp,
+                              const int* qps, int intra, int cols_first, int16_t* const* d_levels,
+                              uint8_t* const* d_recon, int16_t* d_pred, void* stream);
 
 /* Batched 4x4 transforms on host buffers.  layout 0: contiguous 4x4 blocks
  * (16 int16 each, raster within the block); layout 1: 16x16 raster MBs of

@@ -145,6 +145,61 @@ def test_estimate_is_deterministic_and_rerunnable():
     np.testing.assert_array_equal(a[1], b[1])
 
 
+@pytest.mark.parametrize("tile_rows,mbs_per_task,R", [(8, 1, 24), (11, 5, 24), (13, 16, 24), (16, 3, 16),
+                                                      (8, 2, 64)])
+def test_explicit_tilings_match_oracle(tile_rows, mbs_per_task, R):
+    W, H = 352, 288
+    fp = synth.rect_motion_pair(W, H, 31, n_rects=8, max_mv=min(R - 4, 30))
+    modes = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=0)
+    est = capi.MotionEstimator(_cfg(W, H, R, tile_rows=tile_rows, mbs_per_task=mbs_per_task, **modes))
+    assert est.tiling()[:2] == (tile_rows, mbs_per_task)
+    mv, dist, _ = est.estimate(fp.cur.astype(np.int16), fp.ref.astype(np.int16))
+    omv, odist, _, _ = orc.me_estimate(fp.cur, fp.ref, R, **_oracle_modes(modes))
+    np.testing.assert_array_equal(mv, omv)
+    np.testing.assert_array_equal(dist, odist)
+
+
+def test_argmin_keys_tie_break_on_flat_and_periodic_content():
+    """Flat areas and a period-4 texture make many candidates cost the same:
+    the lane-local argmin keys must still pick the reference's candidate
+    (closest to the predictor, then latest in raster order)."""
+    W, H, R = 128, 96, 20
+    yy, xx = np.mgrid[0:H, 0:W]
+    cur = np.where((xx // 32 + yy // 32) % 2 == 0, 100, ((xx % 4) * 40 + (yy % 4) * 7)).astype(np.uint8)
+    ref = np.roll(cur, (3, -5), axis=(0, 1))
+    ref[:, :16] = 100
+    modes = dict(metric=1, window=1, filt=0, subpel=0, predictor=0, best_vector=0)
+    for V in (8, 11, 13):
+        est = capi.MotionEstimator(_cfg(W, H, R, tile_rows=V, **modes))
+        mv, dist, _ = est.estimate(cur.astype(np.int16), ref.astype(np.int16))
+        omv, odist, _, _ = orc.me_estimate(cur, ref, R, **_oracle_modes(modes))
+        np.testing.assert_array_equal(mv, omv)
+        np.testing.assert_array_equal(dist, odist)
+
+
+def test_too_tall_tile_for_exact_keys_is_rejected():
+    with pytest.raises(capi.IcuError) as e:
+        capi.MotionEstimator(_cfg(352, 288, 32, tile_rows=16, metric=1, window=1, filt=0, predictor=0))
+    assert e.value.code == capi.ICU_E_ARG
+
+
+def test_tuned_estimator_matches_oracle():
+    """The tuner retiles the handle (rows per lane, MBs per task, grid mode)
+    many times; results afterwards must not change."""
+    W, H, R = 352, 288, 16
+    fp = synth.rect_motion_pair(W, H, 32, n_rects=8, max_mv=14)
+    modes = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=0)
+    est = capi.MotionEstimator(_cfg(W, H, R, **modes))
+    d_cur, d_ref = capi.DeviceBuffer.from_array(fp.cur), capi.DeviceBuffer.from_array(fp.ref)
+    est.autotune(d_cur, d_ref, reps=1)
+    v, g, p = est.tiling()
+    assert v in (8, 11, 13, 16) and 1 <= g <= 16 and p in (0, 1)
+    mv, dist, _ = est.estimate(fp.cur.astype(np.int16), fp.ref.astype(np.int16))
+    omv, odist, _, _ = orc.me_estimate(fp.cur, fp.ref, R, **_oracle_modes(modes))
+    np.testing.assert_array_equal(mv, omv)
+    np.testing.assert_array_equal(dist, odist)
+
+
 def test_out_of_range_input_is_rejected():
     est = capi.MotionEstimator(_cfg(64, 48, 8))
     cur = np.full((48, 64), 300, np.int16)
@@ -192,6 +247,39 @@ def test_code_frame_matches_oracle(qp, cols_first):
     lv, rc = orc.code_residual(fp.cur, pred, qp, 0, cols_first)
     np.testing.assert_array_equal(d_lv.download(np.int16, (H, W)), lv)
     np.testing.assert_array_equal(d_rc.download(np.uint8, (H, W)).astype(np.int16), rc)
+
+
+@pytest.mark.parametrize("intra", [0, 1])
+def test_code_frame_at_several_qps_matches_oracle(intra):
+    """One prediction coded at QP 22..40 in one pass equals the oracle at
+    each QP, with quarter-pel vectors (every interpolation case)."""
+    W, H, R = 352, 288, 16
+    fp = synth.rect_motion_pair(W, H, 77 + intra, n_rects=10, max_mv=12)
+    est = capi.MotionEstimator(_cfg(W, H, R, metric=1, window=1, filt=0, subpel=2, predictor=0))
+    d_cur, d_ref = capi.DeviceBuffer.from_array(fp.cur), capi.DeviceBuffer.from_array(fp.ref)
+    n = est.nmb
+    d_mv, d_dist = capi.DeviceBuffer(4 * n), capi.DeviceBuffer(4 * n)
+    est.estimate_device(d_cur, d_ref, d_mv, d_dist)  # pads the reference the coder reads
+    # random quarter-pel vectors within 12 pels (the reference's padded
+    # border bounds what the oracle may read), every fraction pair present
+    rng = np.random.default_rng(5 + intra)
+    mv = rng.integers(-48, 49, size=(n, 2)).astype(np.int16)
+    mv[:16] = np.array([(fx + 4, fy - 8) for fx in range(4) for fy in range(4)], np.int16)
+    assert len({(int(x) % 4, int(y) % 4) for x, y in mv}) == 16
+    capi.lib.icu_stream_sync(None)
+    d_mv = capi.DeviceBuffer.from_array(mv)
+    qps = [22, 25, 28, 31, 34, 37, 40, 51]
+    d_lv = [capi.DeviceBuffer(2 * W * H) for _ in qps]
+    d_rc = [capi.DeviceBuffer(W * H) for _ in qps]
+    d_pr = capi.DeviceBuffer(2 * W * H)
+    est.code_frame_qps_device(d_cur, d_mv, qps, intra, 0, d_lv, d_rc, d_pr)
+    capi.lib.icu_stream_sync(None)
+    pred = orc.mc_luma(fp.ref, mv)
+    np.testing.assert_array_equal(d_pr.download(np.int16, (H, W)), pred)
+    for i, qp in enumerate(qps):
+        lv, rc = orc.code_residual(fp.cur, pred, qp, intra, 0)
+        np.testing.assert_array_equal(d_lv[i].download(np.int16, (H, W)), lv)
+        np.testing.assert_array_equal(d_rc[i].download(np.uint8, (H, W)).astype(np.int16), rc)
 
 
 # ------------------------------------------------------------- batching
