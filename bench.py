@@ -182,6 +182,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the planner's tiling")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first run one device-to-device copy of MIB MiB: a kernel of known traffic "
+                         "that calibrates hardware byte counters read in the same profile")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by "
@@ -190,6 +193,13 @@ def main():
     from imagecodingutils_amd import capi, synth
 
     info = capi.device_info()
+    if args.copy_probe > 0:
+        nb = args.copy_probe << 20
+        a, b = capi.DeviceBuffer(nb), capi.DeviceBuffer(nb)
+        capi.lib.icu_memset(a.ptr, 1, nb, None)
+        capi.lib.icu_memcpy_dtod(b.ptr, a.ptr, nb, None)
+        capi.lib.icu_stream_sync(None)
+        del a, b
     plain = run_plain(capi, synth, args.steps, args.warmup, tune=not args.no_tune)
     ms_per_step = plain["total_ms"] / args.steps
     value = plain["nmb"] / (ms_per_step / 1e3)

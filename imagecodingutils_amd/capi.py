@@ -27,7 +27,7 @@ class IcuError(RuntimeError):
 class MeConfig(C.Structure):
     _fields_ = [(n, C.c_int) for n in (
         "width", "height", "range", "metric", "window", "filter", "subpel",
-        "predictor", "best_vector", "tile_rows", "mbs_per_task")]
+        "predictor", "best_vector", "tile_rows", "mbs_per_task", "grid_mode")]
 
 
 _vp = C.c_void_p
@@ -61,6 +61,7 @@ _SIGS = {
     "icu_memcpy_htod": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
     "icu_memcpy_dtoh": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
     "icu_memset": (C.c_int, [_vp, C.c_int, C.c_size_t, _vp]),
+    "icu_memcpy_dtod": (C.c_int, [_vp, _vp, C.c_size_t, _vp]),
     "icu_stream_create": (_vp, []),
     "icu_stream_destroy": (C.c_int, [_vp]),
     "icu_stream_sync": (C.c_int, [_vp]),
@@ -188,9 +189,9 @@ def reference_config(width: int, height: int, motion_range_qpel: int) -> MeConfi
 
 
 def make_config(width, height, R, *, metric=0, window=0, filt=1, subpel=2, predictor=1,
-                best_vector=1, tile_rows=0, mbs_per_task=0) -> MeConfig:
+                best_vector=1, tile_rows=0, mbs_per_task=0, grid_mode=0) -> MeConfig:
     return MeConfig(width, height, R, metric, window, filt, subpel, predictor, best_vector,
-                    tile_rows, mbs_per_task)
+                    tile_rows, mbs_per_task, grid_mode)
 
 
 class MotionEstimator:
@@ -245,7 +246,7 @@ class MotionEstimator:
             stream.handle if stream else None))
 
     def tiling(self):
-        """(candidate rows per lane, MBs per task, persistent grid)."""
+        """(candidate rows per lane, MBs per task, search grid mode 0/1/2)."""
         v, g, p = C.c_int(0), C.c_int(0), C.c_int(0)
         _check(lib.icu_me_get_tiling(self.h, C.byref(v), C.byref(g), C.byref(p)))
         return v.value, g.value, p.value

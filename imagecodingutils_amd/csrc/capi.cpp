@@ -111,6 +111,10 @@ bool valid_cfg(const icu_me_config& c, std::string* why) {
     *why = "mbs_per_task must be 0..16";
     return false;
   }
+  if (c.grid_mode < 0 || c.grid_mode > 3) {
+    *why = "grid_mode must be 0..3";
+    return false;
+  }
   if (2 * c.range + 1 > 129) {
     *why = "range too large";
     return false;
@@ -159,11 +163,11 @@ struct Engine {
   std::vector<int> wave_off;
   int stage_dwords = 0;   // one LDS staging buffer of the search kernel
   int cus = 256;          // compute units of the device
-  // search grid mode (the tuner may switch it): persistent workgroups, as
-  // many as stay resident, claim tasks until none are left; otherwise one
-  // task per workgroup
-  bool persistent = true;
-  int ticket_off = 0;     // search task tickets (2 ints) inside d_wave
+  // search grid mode (the tuner may switch it): 1 persistent workgroups, as
+  // many as stay resident, claim tasks until none are left; 2 the same with
+  // the grid reduced so every workgroup runs equally many tasks; 0 one task
+  // per workgroup
+  int grid_mode = 1;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
 
@@ -224,6 +228,7 @@ struct Engine {
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     cus = prop.multiProcessorCount;
+    if (cfgs[0].grid_mode) grid_mode = cfgs[0].grid_mode - 1;
     sp.resize(n);
     total_mb = 0;
     table_total = 0;
@@ -252,8 +257,6 @@ struct Engine {
       wave_off.push_back(wave_total);
       wave_total += s.mbh + 2;
     }
-    ticket_off = wave_total;
-    wave_total += 2;
     build_tasks();
     for (auto& d : ds) {
       HIP_TRY(hipMalloc(&d.d_streams, sizeof(StreamDesc) * n));
@@ -367,8 +370,8 @@ struct Engine {
       if (x.t.V != vfix) vfix = 0;
       any_table |= x.table;
     }
-    launch_search(d_streams, d_tasks, (int)h_tasks.size(), persistent, sp[0].cfg.metric, vfix,
-                  any_table, stage_dwords, d_ires, d_table, d_wave + ticket_off, st);
+    launch_search(d_streams, d_tasks, (int)h_tasks.size(), grid_mode, sp[0].cfg.metric, vfix,
+                  any_table, stage_dwords, d_ires, d_table, st);
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
     if (!h_items.empty()) launch_decide_parallel(d_decide, d_items, (int)h_items.size(), st);
     for (size_t i = 0; i < sp.size(); i++) {
@@ -549,28 +552,52 @@ int icu_me_estimate(icu_me* h, const int16_t* cur, const int16_t* ref, const uin
   return ICU_OK;
 }
 
-int icu_me_get_tiling(icu_me* h, int* tile_rows, int* mbs_per_task, int* persistent) {
+int icu_me_get_tiling(icu_me* h, int* tile_rows, int* mbs_per_task, int* grid_mode) {
   if (!h) return fail(ICU_E_ARG, "null handle");
   if (tile_rows) *tile_rows = h->eng.sp[0].t.V;
   if (mbs_per_task) *mbs_per_task = h->eng.sp[0].t.G;
-  if (persistent) *persistent = h->eng.persistent ? 1 : 0;
+  if (grid_mode) *grid_mode = h->eng.grid_mode;
   return ICU_OK;
 }
+
+namespace {
+// Times `reps` estimates of the handle's current tiling on its own stream.
+int time_estimates(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int reps, hipEvent_t a,
+                   hipEvent_t b, float* ms) {
+  Engine& e = h->eng;
+  int rc = icu_me_estimate_device(h, d_cur, d_ref, h->d_mv, h->d_dist, nullptr, e.own);  // warm-up
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(a, e.own));
+  for (int r = 0; r < reps; r++) {
+    rc = icu_me_estimate_device(h, d_cur, d_ref, h->d_mv, h->d_dist, nullptr, e.own);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipEventRecord(b, e.own));
+  HIP_TRY(hipEventSynchronize(b));
+  HIP_TRY(hipEventElapsedTime(ms, a, b));
+  return ICU_OK;
+}
+}  // namespace
 
 int icu_me_autotune(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int reps) {
   if (!h || !d_cur || !d_ref) return fail(ICU_E_ARG, "null argument");
   Engine& e = h->eng;
   StreamPlan& s = e.sp[0];
   if (reps < 1) reps = 1;
-  hipEvent_t a, b;
+  hipEvent_t a = nullptr, b = nullptr;
   HIP_TRY(hipEventCreate(&a));
-  HIP_TRY(hipEventCreate(&b));
+  if (hipEventCreate(&b) != hipSuccess) {
+    hipEventDestroy(a);
+    return fail(ICU_E_DEVICE, "hipEventCreate failed");
+  }
   Tiling best = s.t;
-  bool best_persistent = e.persistent;
+  int best_mode = e.grid_mode;
   float best_ms = 1e30f;
+  int rc = ICU_OK;
   static const int kV[4] = {8, 11, 13, 16};
-  for (int pers = 0; pers < 2; pers++) {
+  for (int mode = 0; mode < 3 && rc == ICU_OK; mode++) {
     for (int v : kV) {
+      if (rc != ICU_OK) break;
       if (s.cfg.metric == ICU_METRIC_SSD && v != 8) continue;
       if ((2 * s.L + 1 + v - 1) / v * v - (2 * s.L + 1) > v / 2 + 2) continue;  // too much row waste
       for (int g = 1; g <= std::min(search_max_group(), s.mbw); g++) {
@@ -578,31 +605,22 @@ int icu_me_autotune(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int r
         if (t.V != v || t.G != g) continue;
         const int lanes = g * t.Q * t.B;
         if (lanes < 192 && g < s.mbw) continue;  // leaves most of a workgroup idle
-        e.persistent = pers != 0;
-        int rc = e.retile(0, t);
-        if (rc) return rc;
-        rc = icu_me_estimate_device(h, d_cur, d_ref, h->d_mv, h->d_dist, nullptr, e.own);
-        if (rc) return rc;
-        HIP_TRY(hipEventRecord(a, e.own));
-        for (int r = 0; r < reps; r++) {
-          rc = icu_me_estimate_device(h, d_cur, d_ref, h->d_mv, h->d_dist, nullptr, e.own);
-          if (rc) return rc;
-        }
-        HIP_TRY(hipEventRecord(b, e.own));
-        HIP_TRY(hipEventSynchronize(b));
+        e.grid_mode = mode;
         float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, a, b));
+        if ((rc = e.retile(0, t)) != ICU_OK || (rc = time_estimates(h, d_cur, d_ref, reps, a, b, &ms)) != ICU_OK)
+          break;
         if (ms < best_ms) {
           best_ms = ms;
           best = t;
-          best_persistent = e.persistent;
+          best_mode = mode;
         }
       }
     }
   }
-  e.persistent = best_persistent;
   hipEventDestroy(a);
   hipEventDestroy(b);
+  if (rc != ICU_OK) return rc;
+  e.grid_mode = best_mode;
   return e.retile(0, best);
 }
 
@@ -814,6 +832,11 @@ int icu_memcpy_dtoh(void* h, const void* d, size_t bytes, void* stream) {
   } else {
     HIP_TRY(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
   }
+  return ICU_OK;
+}
+
+int icu_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream) {
+  HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return ICU_OK;
 }
 

@@ -35,9 +35,9 @@ struct DecideItem {
 
 void launch_pad_plane(const uint8_t* src, int W, int H, int src_stride, uint8_t* dst_origin,
                       int dst_stride, hipStream_t st);
-void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, bool persistent,
+void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int grid_mode,
                    int metric, int vfix, bool table, int buf_dwords, IntResult* d_out,
-                   uint32_t* d_table, int* d_tickets, hipStream_t st);
+                   uint32_t* d_table, hipStream_t st);
 // Dwords of one staging buffer (window rows padded to 16 bytes + blocks).
 int search_stage_dwords(int B, int V, int Q, int G);
 int search_max_window_dwords();

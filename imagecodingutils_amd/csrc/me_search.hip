@@ -36,6 +36,12 @@ constexpr int kThreads = 256;
 constexpr int kWinDwords = 7680;
 constexpr int kMaxG = 16;         // MBs per task
 
+// LDS-qualified views of the staging buffers: reads through them compile to
+// ds_read (a generic pointer picked at run time would become a flat load).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
+
 // ---------------------------------------------------------------- padding
 // Edge-replicated copy of an 8-bit plane into the padded plane
 // (OverlayExtMem2Dv2::ExtendBoundary/FillBoundary,
@@ -74,48 +80,45 @@ __global__ __launch_bounds__(256) void pad_plane_kernel(const uint8_t* __restric
 }
 
 // ------------------------------------------------------------- tile bodies
-// An opaque zero that depends on `dep`: adding it to an address keeps the
-// load from being issued before `dep` is computed.
-__device__ __forceinline__ uint32_t opaque_zero(uint32_t dep) {
-  uint32_t z;
-  asm volatile("v_mov_b32 %0, 0" : "=v"(z) : "v"(dep));
-  return z;
-}
 
 // Rows are software-pipelined one ahead: the LDS loads of row rho+1 are
-// issued before the qsads of row rho.  Their address carries an opaque
-// dependency on row rho-1's sums, so the scheduler cannot hoist every row's
-// loads to the top of the tile (which would exhaust the VGPRs).
+// issued before the qsads of row rho.  The empty asm at the top of each row
+// takes the row's dwords as operands and clobbers memory: the wait for them
+// lands there (a whole row of qsads after their loads were issued), and no
+// later row's loads can be hoisted above it (which would exhaust the VGPRs).
+// The four 8-byte spans of a row (reference bytes 4k .. 4k+7, k = 0..3) are
+// loaded as separate 4-byte-aligned 64-bit values, straight into the VGPR
+// pairs v_qsad_pk_u16_u8 takes.
+typedef uint64_t __attribute__((aligned(4))) u64_a4;
+typedef __attribute__((address_space(3))) u64_a4 lds_u64_a4;
+
 template <int V, typename Done>
-__device__ __forceinline__ void sad_tile(const uint32_t* wp, int row_dw,
+__device__ __forceinline__ void sad_tile(const lds_u32* wp, int row_dw,
                                          const uint32_t (&c)[64], Done&& done) {
   uint64_t acc[V];
 #pragma unroll
   for (int v = 0; v < V; v++) acc[v] = 0;
-  uint32_t n0 = wp[0], n1 = wp[1], n2 = wp[2], n3 = wp[3], n4 = wp[4];
-  uint32_t dep = 0;
+  const lds_u32* rp = wp;
+  auto span = [&](int k) { return *(const lds_u64_a4*)(rp + k); };
+  uint64_t n0 = span(0), n1 = span(1), n2 = span(2), n3 = span(3);
 #pragma unroll
   for (int rho = 0; rho < V + 15; rho++) {
-    const uint32_t d0 = n0, d1 = n1, d2 = n2, d3 = n3, d4 = n4;
+    uint64_t p0 = n0, p1 = n1, p2 = n2, p3 = n3;
+    // also makes rp opaque, so the next address is one add, not recomputed
+    asm volatile("" : "+v"(p0), "+v"(p1), "+v"(p2), "+v"(p3), "+v"(rp) : : "memory");
     if (rho + 1 < V + 15) {
-      const uint32_t* rp = wp + (rho + 1) * row_dw + opaque_zero(dep);
-      n0 = rp[0]; n1 = rp[1]; n2 = rp[2]; n3 = rp[3]; n4 = rp[4];
+      rp += row_dw;
+      n0 = span(0); n1 = span(1); n2 = span(2); n3 = span(3);
     }
-    const uint64_t p0 = ((uint64_t)d1 << 32) | d0;
-    const uint64_t p1 = ((uint64_t)d2 << 32) | d1;
-    const uint64_t p2 = ((uint64_t)d3 << 32) | d2;
-    const uint64_t p3 = ((uint64_t)d4 << 32) | d3;
 #pragma unroll
-    for (int v = 0; v < V; v++) {
-      const int r = rho - v;
-      if (r >= 0 && r < 16) {
-        acc[v] = __builtin_amdgcn_qsad_pk_u16_u8(p0, c[4 * r + 0], acc[v]);
-        acc[v] = __builtin_amdgcn_qsad_pk_u16_u8(p1, c[4 * r + 1], acc[v]);
-        acc[v] = __builtin_amdgcn_qsad_pk_u16_u8(p2, c[4 * r + 2], acc[v]);
-        acc[v] = __builtin_amdgcn_qsad_pk_u16_u8(p3, c[4 * r + 3], acc[v]);
+    for (int k = 0; k < 4; k++) {
+      const uint64_t pk = k == 0 ? p0 : (k == 1 ? p1 : (k == 2 ? p2 : p3));
+#pragma unroll
+      for (int v = 0; v < V; v++) {
+        const int r = rho - v;
+        if (r >= 0 && r < 16) acc[v] = __builtin_amdgcn_qsad_pk_u16_u8(pk, c[4 * r + k], acc[v]);
       }
     }
-    dep = (uint32_t)acc[rho < V ? rho : V - 1];
     // accumulator rho-15 has seen all 16 block rows: retire it now so its
     // registers free up while the remaining rows run
     if (rho >= 15) done(rho - 15, (uint32_t)acc[rho - 15], (uint32_t)(acc[rho - 15] >> 32));
@@ -125,7 +128,7 @@ __device__ __forceinline__ void sad_tile(const uint32_t* wp, int row_dw,
 // SSD tile: acc[v][k] = sum over rows of (sum r^2 - 2 sum c*r) for column
 // offset k, in wrapping 32-bit arithmetic; the caller adds sum c^2.
 template <int V, typename Done>
-__device__ __forceinline__ void ssd_tile(const uint32_t* wp, int row_dw,
+__device__ __forceinline__ void ssd_tile(const lds_u32* wp, int row_dw,
                                          const uint32_t (&c)[64], Done&& done) {
   uint32_t acc[V][4];
 #pragma unroll
@@ -133,16 +136,18 @@ __device__ __forceinline__ void ssd_tile(const uint32_t* wp, int row_dw,
 #pragma unroll
     for (int k = 0; k < 4; k++) acc[v][k] = 0;
   uint32_t nx[5];
+  const lds_u32* rp = wp;
 #pragma unroll
-  for (int i = 0; i < 5; i++) nx[i] = wp[i];
-  uint32_t dep = 0;
+  for (int i = 0; i < 5; i++) nx[i] = rp[i];
 #pragma unroll
   for (int rho = 0; rho < V + 15; rho++) {
     uint32_t d[5];
 #pragma unroll
     for (int i = 0; i < 5; i++) d[i] = nx[i];
+    // same pipelining as sad_tile
+    asm volatile("" : "+v"(d[0]), "+v"(d[1]), "+v"(d[2]), "+v"(d[3]), "+v"(d[4]) : : "memory");
     if (rho + 1 < V + 15) {
-      const uint32_t* rp = wp + (rho + 1) * row_dw + opaque_zero(dep);
+      rp += row_dw;
 #pragma unroll
       for (int i = 0; i < 5; i++) nx[i] = rp[i];
     }
@@ -176,7 +181,6 @@ __device__ __forceinline__ void ssd_tile(const uint32_t* wp, int row_dw,
         }
       }
     }
-    dep = acc[rho < V ? rho : V - 1][0];
     if (rho >= 15) done(rho - 15, acc[rho - 15]);
   }
 }
@@ -211,23 +215,25 @@ __device__ __forceinline__ void visit(int dx, int dy, uint32_t cc, int xl, int x
 
 template <int METRIC, int V, bool TABLE>
 __device__ __forceinline__ void run_tile(const StreamDesc& s, const SearchTask& t, int u,
-                                         const uint32_t* win, int row_dw, const uint32_t* curs,
+                                         const lds_u32* win, int row_dw, const lds_u32* curs,
                                          const uint32_t* csq, uint64_t* best, int* zero,
                                          uint32_t* __restrict__ table) {
-  const int U = s.Q * s.B;
-  const int g = u / U, rem = u - g * U;
-  const int b = rem / s.Q, q = rem - b * s.Q;
+  // unit u: column quad fastest, then MB, then row tile.  Lanes of adjacent
+  // MBs read overlapping window dwords (MBs are 4 dwords apart), which the
+  // LDS serves as broadcasts instead of bank conflicts.
+  const int gq = u / s.Q, q = u - gq * s.Q;
+  const int b = gq / t.count, g = gq - b * t.count;
   uint32_t c[64];
-  const uint4* cp = (const uint4*)(curs + g * 64);
+  const lds_u32x4* cp = (const lds_u32x4*)(curs + g * 64);
 #pragma unroll
   for (int i = 0; i < 16; i++) {
-    uint4 x = cp[i];
+    const u32x4 x = cp[i];
     c[4 * i + 0] = x.x;
     c[4 * i + 1] = x.y;
     c[4 * i + 2] = x.z;
     c[4 * i + 3] = x.w;
   }
-  const uint32_t* wp = win + (b * V) * row_dw + 4 * g + q;
+  const lds_u32* wp = win + (b * V) * row_dw + 4 * g + q;
   const int mb = t.mb + g;
   const int col = mb % s.mbw, row = mb / s.mbw;
   const int n = col * 16, m = row * 16;
@@ -327,7 +333,7 @@ __device__ __forceinline__ Stage stage_shape(const StreamDesc& s, const SearchTa
   return g;
 }
 
-__device__ __forceinline__ void issue_stage(const StreamDesc& s, const SearchTask& t, uint32_t* buf,
+__device__ __forceinline__ void issue_stage(const StreamDesc& s, const SearchTask& t, lds_u32* buf,
                                             int tid) {
   const Stage g = stage_shape(s, t);
   const int col0 = t.mb % s.mbw, row = t.mb / s.mbw;
@@ -357,27 +363,36 @@ __device__ __forceinline__ void issue_stage(const StreamDesc& s, const SearchTas
 // predictor modes) instead of only the argmin.
 // Persistent: workgroup w runs tasks w, w + grid, ... and streams the next
 // task's window into the other half of LDS while it computes the current one.
+// Waves per SIMD the register allocation must allow: the SAD argmin kernel
+// with 8 rows per lane fits 128 VGPRs without spilling, so 4 waves; the
+// others keep the compiler's choice (taller tiles would spill).
+constexpr int search_min_waves(int metric, int vfix, bool table) {
+  return (metric == kMetricSAD && !table && vfix == 8) ? 4 : 1;
+}
+
 template <int METRIC, int VFIX, bool TABLE>
-__global__ __launch_bounds__(kThreads) void me_search_kernel(const StreamDesc* __restrict__ streams,
-                                                             const SearchTask* __restrict__ tasks,
-                                                             int ntasks, IntResult* __restrict__ out,
-                                                             uint32_t* __restrict__ table, int buf_dwords,
-                                                             int* __restrict__ tickets) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(search_min_waves(METRIC, VFIX, TABLE))))
+void me_search_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __restrict__ tasks,
+                      int ntasks, IntResult* __restrict__ out, uint32_t* __restrict__ table,
+                      int buf_dwords) {
   extern __shared__ __attribute__((aligned(16))) uint32_t dyn[];
-  __shared__ int next_s;
   __shared__ uint32_t csq[kMaxG];
   __shared__ uint64_t best[kMaxG];
   __shared__ int zero[kMaxG];
   const int tid = threadIdx.x;
-  uint32_t* bufs[2] = {dyn, dyn + buf_dwords};
+  lds_u32* const lds = (lds_u32*)dyn;
+  // staging buffer k of the pair
+  auto buf = [&](int k) { return lds + (k & 1) * buf_dwords; };
 
-  // tasks [0, grid) go one per workgroup; the rest are claimed through
-  // tickets[0] as workgroups free up; the last workgroup to finish zeroes
-  // both counters for the next launch (host keeps grid <= ntasks)
+  // Workgroup w runs tasks w, w + grid, w + 2 grid, ... (the host sorts the
+  // tasks longest first, so the rounds stay even).  A static schedule: a
+  // shared ticket counter costs one device-scope atomic per task on a single
+  // address, and those serialize (about 15 ns each, measured), which alone
+  // took longer than the search.
   int ti = blockIdx.x;
   {
     const SearchTask t0 = tasks[ti];
-    issue_stage(streams[t0.stream], t0, bufs[0], tid);
+    issue_stage(streams[t0.stream], t0, buf(0), tid);
   }
   if (tid < kMaxG) {
     best[tid] = ~0ull;
@@ -387,16 +402,14 @@ __global__ __launch_bounds__(kThreads) void me_search_kernel(const StreamDesc* _
     const SearchTask t = tasks[ti];
     const StreamDesc s = streams[t.stream];
     const Stage g = stage_shape(s, t);
-    uint32_t* win = bufs[it & 1] + g.shift;
-    const uint32_t* curs = bufs[it & 1] + g.win_chunks * 4;
+    const lds_u32* win = buf(it) + g.shift;
+    const lds_u32* curs = buf(it) + g.win_chunks * 4;
     __builtin_amdgcn_s_waitcnt(0);  // this wave's staging loads have landed
     __syncthreads();                // ... and every other wave's
-    if (tid == 0) next_s = (int)gridDim.x + atomicAdd(&tickets[0], 1);
-    __syncthreads();
-    const int tnext = next_s;
+    const int tnext = ti + (int)gridDim.x;
     if (tnext < ntasks) {
       const SearchTask tn = tasks[tnext];
-      issue_stage(streams[tn.stream], tn, bufs[(it + 1) & 1], tid);
+      issue_stage(streams[tn.stream], tn, buf(it + 1), tid);
     }
     if (METRIC == kMetricSSD) {
       if (tid < t.count) {
@@ -437,13 +450,6 @@ __global__ __launch_bounds__(kThreads) void me_search_kernel(const StreamDesc* _
     }
     ti = tnext;
   }
-  if (tid == 0) {
-    __threadfence();
-    if (atomicAdd(&tickets[1], 1) == (int)gridDim.x - 1) {
-      atomicExch(&tickets[0], 0);
-      atomicExch(&tickets[1], 0);
-    }
-  }
 }
 
 // ----------------------------------------------------------------- host side
@@ -457,13 +463,16 @@ void launch_pad_plane(const uint8_t* src, int W, int H, int src_stride, uint8_t*
 // vfix: the common tile height of every task, or 0 when streams differ.
 // buf_dwords: one staging buffer (window + blocks) of the largest task; the
 // kernel holds two.
-void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, bool persistent,
+// grid_mode: 0 one task per workgroup; 1 persistent, as many workgroups as
+// stay resident; 2 persistent, that many reduced so that every workgroup runs
+// the same number of tasks (no partly filled last round).
+void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int grid_mode,
                    int metric, int vfix, bool table, int buf_dwords, IntResult* d_out,
-                   uint32_t* d_table, int* d_tickets, hipStream_t st) {
+                   uint32_t* d_table, hipStream_t st) {
   const size_t shmem = 2 * (size_t)buf_dwords * sizeof(uint32_t);
   auto go = [&](auto kern) {
     int grid = ntasks;
-    if (persistent) {
+    if (grid_mode != 0) {
       // as many workgroups as stay resident at once (registers and LDS),
       // remembered per kernel and LDS size
       static thread_local const void* last_kern = nullptr;
@@ -482,9 +491,15 @@ void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int n
         last_res = std::max(1, per_cu) * std::max(1, cus);
       }
       grid = std::min(ntasks, last_res);
+      if (grid_mode == 2) {
+        const int rounds = (ntasks + grid - 1) / grid;
+        grid = (ntasks + rounds - 1) / rounds;
+      }
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), shmem, st, d_streams, d_tasks, ntasks, d_out,
-                       d_table, buf_dwords, d_tickets);
+    // a workgroup that runs one task never touches the second buffer
+    const size_t lds_bytes = grid >= ntasks ? shmem / 2 : shmem;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds_bytes, st, d_streams, d_tasks, ntasks, d_out,
+                       d_table, buf_dwords);
   };
   if (metric == kMetricSSD) {
     if (table) go(me_search_kernel<kMetricSSD, 8, true>);

@@ -45,6 +45,9 @@ typedef struct icu_me_config {
   int best_vector;   /* 1: GetBestVector over {best, predictor, zero} (reference); 0: keep best */
   int tile_rows;     /* 0: chosen by the tuner; else candidate rows per lane: 8, 11, 13 or 16 */
   int mbs_per_task;  /* 0: chosen by the tuner; else MBs staged per workgroup task (1..16) */
+  int grid_mode;     /* 0: chosen by the tuner (persistent until tuned); else 1 + the mode
+                        icu_me_get_tiling reports (1 one task per workgroup, 2 persistent,
+                        3 persistent balanced) */
 } icu_me_config;
 
 typedef struct icu_me icu_me;
@@ -87,9 +90,11 @@ int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref,
                           int32_t* d_dist, void* stream, float* stage_ms);
 
 /* Tile choice in effect for this handle (after tuning): candidate rows per
- * lane, MBs per workgroup task, and whether the search grid is persistent
- * (workgroups claim tasks until none are left) or one task per workgroup. */
-int icu_me_get_tiling(icu_me* h, int* tile_rows, int* mbs_per_task, int* persistent);
+ * lane, MBs per workgroup task, and the search grid mode: 0 one task per
+ * workgroup, 1 persistent (as many workgroups as stay resident claim tasks
+ * until none are left), 2 persistent with the grid reduced so every
+ * workgroup runs equally many tasks. */
+int icu_me_get_tiling(icu_me* h, int* tile_rows, int* mbs_per_task, int* grid_mode);
 
 /* Time each candidate tiling (rows per lane x MBs per task x grid mode) on
  * the given device frames and keep the fastest (the tuner).  reps: timed
@@ -165,6 +170,8 @@ void* icu_malloc(size_t bytes);
 int icu_free(void* d);
 int icu_memcpy_htod(void* d, const void* h, size_t bytes, void* stream);
 int icu_memcpy_dtoh(void* h, const void* d, size_t bytes, void* stream);
+/* Asynchronous device-to-device copy on `stream`. */
+int icu_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
 int icu_memset(void* d, int value, size_t bytes, void* stream);
 void* icu_stream_create(void);
 int icu_stream_destroy(void* stream);
