@@ -49,7 +49,8 @@ Tiling plan_tiling(int mbw, int L, int metric, bool argmin, int V_req, int G_req
   static const int kV[4] = {8, 11, 13, 16};
   auto allowed = [&](int v) {
     if (metric == ICU_METRIC_SSD) return v == 8;
-    return !(argmin && !lane_keys_fit(L, v));
+    if (argmin) return v != 16 && lane_keys_fit(L, v);
+    return true;
   };
   if (V_req && !allowed(V_req)) return t;
   int bestV = 8, bestWaste = 1 << 30;
@@ -149,7 +150,8 @@ struct Engine {
   struct DescSet {
     StreamDesc* d_streams = nullptr;
     DecideDesc* d_decide = nullptr;
-    std::vector<const uint8_t*> cur;
+    PadDesc* d_pad = nullptr;
+    std::vector<const uint8_t*> cur, ref;
     int16_t* mv = nullptr;
     int32_t* dist = nullptr;
     int32_t* integ = nullptr;
@@ -161,13 +163,16 @@ struct Engine {
   uint32_t* d_table = nullptr;
   int* d_wave = nullptr;  // per stream: mbh progress counters + 2 (ticket, error)
   std::vector<int> wave_off;
+  std::vector<int> pad_base;  // first padding workgroup of each stream
+  int pad_total = 0;          // padding workgroups of all streams
   int stage_dwords = 0;   // one LDS staging buffer of the search kernel
   int cus = 256;          // compute units of the device
-  // search grid mode (the tuner may switch it): 1 persistent workgroups, as
-  // many as stay resident, claim tasks until none are left; 2 the same with
-  // the grid reduced so every workgroup runs equally many tasks; 0 one task
-  // per workgroup
-  int grid_mode = 1;
+  // search grid mode (the tuner may switch it): 0 one task per workgroup
+  // (one LDS buffer, so the most workgroups fit); 1 persistent workgroups, as
+  // many as stay resident, run every grid-th task with the next task's window
+  // streaming into a second buffer; 2 the same with the grid reduced so every
+  // workgroup runs equally many tasks
+  int grid_mode = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
 
@@ -184,8 +189,10 @@ struct Engine {
     for (auto& d : ds) {
       hipFree(d.d_streams);
       hipFree(d.d_decide);
+      hipFree(d.d_pad);
       d.d_streams = nullptr;
       d.d_decide = nullptr;
+      d.d_pad = nullptr;
     }
     hipFree(d_tasks);
     hipFree(d_items);
@@ -256,12 +263,16 @@ struct Engine {
       s.refpad = s.refpad_base + (size_t)kPad * s.ref_stride + kPad;
       wave_off.push_back(wave_total);
       wave_total += s.mbh + 2;
+      pad_base.push_back(pad_total);
+      pad_total += pad_plane_workgroups(s.cfg.width, s.cfg.height);
     }
     build_tasks();
     for (auto& d : ds) {
       HIP_TRY(hipMalloc(&d.d_streams, sizeof(StreamDesc) * n));
       HIP_TRY(hipMalloc(&d.d_decide, sizeof(DecideDesc) * n));
+      HIP_TRY(hipMalloc(&d.d_pad, sizeof(PadDesc) * n));
       d.cur.assign(n, nullptr);
+      d.ref.assign(n, nullptr);
     }
     // every task holds at least one MB: total_mb entries fit any retiling
     HIP_TRY(hipMalloc(&d_tasks, sizeof(SearchTask) * total_mb));
@@ -282,18 +293,25 @@ struct Engine {
   }
 
   // (Re)write the descriptors when the caller's buffers change.
-  int bind(int set, const uint8_t* const* curs, int16_t* mv, int32_t* dist, int32_t* integ) {
+  int bind(int set, const uint8_t* const* curs, const uint8_t* const* refs, int16_t* mv, int32_t* dist,
+           int32_t* integ) {
     DescSet& D = ds[set];
     for (size_t i = 0; i < sp.size(); i++)
       if (((uintptr_t)curs[i] & 15) != 0)
         return fail(ICU_E_ARG, "device current planes must be 16-byte aligned");
     bool same = D.valid && mv == D.mv && dist == D.dist && integ == D.integ;
-    for (size_t i = 0; same && i < sp.size(); i++) same = D.cur[i] == curs[i];
+    for (size_t i = 0; same && i < sp.size(); i++) same = D.cur[i] == curs[i] && D.ref[i] == refs[i];
     if (same) return ICU_OK;
     // kernels of an earlier call may still read the old descriptors
     if (D.valid) HIP_TRY(hipDeviceSynchronize());
     h_streams.resize(sp.size());
     h_decide.resize(sp.size());
+    std::vector<PadDesc> h_pad(sp.size());
+    for (size_t i = 0; i < sp.size(); i++) {
+      const StreamPlan& s = sp[i];
+      h_pad[i] = PadDesc{refs[i], s.refpad, s.cfg.width, s.cfg.height, s.cfg.width, s.ref_stride, pad_base[i]};
+    }
+    HIP_TRY(hipMemcpy(D.d_pad, h_pad.data(), sizeof(PadDesc) * sp.size(), hipMemcpyHostToDevice));
     for (size_t i = 0; i < sp.size(); i++) {
       const StreamPlan& s = sp[i];
       StreamDesc& d = h_streams[i];
@@ -345,7 +363,10 @@ struct Engine {
     }
     HIP_TRY(hipMemcpy(D.d_streams, h_streams.data(), sizeof(StreamDesc) * sp.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(D.d_decide, h_decide.data(), sizeof(DecideDesc) * sp.size(), hipMemcpyHostToDevice));
-    for (size_t i = 0; i < sp.size(); i++) D.cur[i] = curs[i];
+    for (size_t i = 0; i < sp.size(); i++) {
+      D.cur[i] = curs[i];
+      D.ref[i] = refs[i];
+    }
     D.mv = mv;
     D.dist = dist;
     D.integ = integ;
@@ -355,14 +376,11 @@ struct Engine {
 
   // Enqueue the whole estimate on `st` (buffers already bound).
   // ev (nullable): 4 events recorded before padding, search, decision, end.
-  int enqueue(int set, const uint8_t* const* refs, hipStream_t st, hipEvent_t* ev = nullptr) {
+  int enqueue(int set, hipStream_t st, hipEvent_t* ev = nullptr) {
     StreamDesc* d_streams = ds[set].d_streams;
     DecideDesc* d_decide = ds[set].d_decide;
     if (ev) HIP_TRY(hipEventRecord(ev[0], st));
-    for (size_t i = 0; i < sp.size(); i++) {
-      const StreamPlan& s = sp[i];
-      launch_pad_plane(refs[i], s.cfg.width, s.cfg.height, s.cfg.width, s.refpad, s.ref_stride, st);
-    }
+    launch_pad_planes(ds[set].d_pad, (int)sp.size(), pad_total, st);
     if (ev) HIP_TRY(hipEventRecord(ev[1], st));
     int vfix = sp[0].t.V;
     bool any_table = false;
@@ -371,9 +389,10 @@ struct Engine {
       any_table |= x.table;
     }
     launch_search(d_streams, d_tasks, (int)h_tasks.size(), grid_mode, sp[0].cfg.metric, vfix,
-                  any_table, stage_dwords, d_ires, d_table, st);
+                  any_table, stage_dwords, d_ires, d_table, d_decide, st);
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
-    if (!h_items.empty()) launch_decide_parallel(d_decide, d_items, (int)h_items.size(), st);
+    // without cost tables the search kernel decides every MB itself
+    if (any_table && !h_items.empty()) launch_decide_parallel(d_decide, d_items, (int)h_items.size(), st);
     for (size_t i = 0; i < sp.size(); i++) {
       if (!sp[i].cfg.predictor) continue;
       launch_fill_i32(d_wave + wave_off[i], 0, sp[i].mbh + 2, st);
@@ -492,9 +511,9 @@ int icu_me_estimate_device(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref
   Engine& e = h->eng;
   const uint8_t* curs[1] = {d_cur};
   const uint8_t* refs[1] = {d_ref};
-  int rc = e.bind(0, curs, d_mv, d_dist, d_int);
+  int rc = e.bind(0, curs, refs, d_mv, d_dist, d_int);
   if (rc) return rc;
-  return e.enqueue(0, refs, pick(stream, e));
+  return e.enqueue(0, pick(stream, e));
 }
 
 int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int16_t* d_mv,
@@ -503,12 +522,12 @@ int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref,
   Engine& e = h->eng;
   const uint8_t* curs[1] = {d_cur};
   const uint8_t* refs[1] = {d_ref};
-  int rc = e.bind(0, curs, d_mv, d_dist, nullptr);
+  int rc = e.bind(0, curs, refs, d_mv, d_dist, nullptr);
   if (rc) return rc;
   hipEvent_t ev[4];
   for (auto& x : ev) HIP_TRY(hipEventCreate(&x));
   const hipStream_t st = pick(stream, e);
-  rc = e.enqueue(0, refs, st, ev);
+  rc = e.enqueue(0, st, ev);
   if (!rc) {
     HIP_TRY(hipEventSynchronize(ev[3]));
     for (int i = 0; i < 3; i++) HIP_TRY(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
@@ -752,23 +771,23 @@ int icu_me_batch_mb_base(icu_me_batch* b, int i) {
 int icu_me_batch_estimate_device(icu_me_batch* b, const uint8_t* const* curs, const uint8_t* const* refs,
                                  int16_t* d_mv, int32_t* d_dist, void* stream) {
   if (!b || !curs || !refs || !d_mv || !d_dist) return fail(ICU_E_ARG, "null argument");
-  int rc = b->eng.bind(0, curs, d_mv, d_dist, nullptr);
+  int rc = b->eng.bind(0, curs, refs, d_mv, d_dist, nullptr);
   if (rc) return rc;
-  return b->eng.enqueue(0, refs, pick(stream, b->eng));
+  return b->eng.enqueue(0, pick(stream, b->eng));
 }
 
 int icu_me_batch_capture(icu_me_batch* b, const uint8_t* const* curs, const uint8_t* const* refs,
                          int16_t* d_mv, int32_t* d_dist) {
   if (!b || !curs || !refs || !d_mv || !d_dist) return fail(ICU_E_ARG, "null argument");
   Engine& e = b->eng;
-  int rc = e.bind(1, curs, d_mv, d_dist, nullptr);
+  int rc = e.bind(1, curs, refs, d_mv, d_dist, nullptr);
   if (rc) return rc;
   if (e.gexec) hipGraphExecDestroy(e.gexec);
   if (e.graph) hipGraphDestroy(e.graph);
   e.gexec = nullptr;
   e.graph = nullptr;
   HIP_TRY(hipStreamBeginCapture(e.own, hipStreamCaptureModeThreadLocal));
-  rc = e.enqueue(1, refs, e.own);
+  rc = e.enqueue(1, e.own);
   hipGraph_t g = nullptr;
   hipError_t ce = hipStreamEndCapture(e.own, &g);
   if (rc) {

@@ -1,0 +1,425 @@
+// Per-macroblock decision, sub-pel refinement and final vector choice, as
+// device functions shared by the decision kernels and the search kernel
+// (which runs them on the MBs of a task right after its integer search).
+//
+// One wavefront (64 lanes) decides one MB.  It restates, in order, the rest
+// of MotionEstimatorH264ImplFull::Estimate
+// (src/CodecUtils/MotionEstimatorH264ImplFull.cpp:350-601):
+//   * the integer decision, either from the search kernel's packed argmin
+//     (zero predictor, filter off) or by replaying the reference's scan over
+//     the cost table: with the running-best filter (:410-411) the scan is
+//     path dependent, so lanes test 64 consecutive candidates at once against
+//     the current best, a ballot finds the first one the reference would
+//     accept, and the scan restarts after it;
+//   * the half-pel stage (LoadHalfQuartPelWindow :1095-1182, loop :452-483)
+//     from a 35x35 grid of G/b/h/j samples built in LDS;
+//   * the quarter-pel stage (LoadQuartPelWindow :1198-1222, loop :485-517)
+//     with the estimator's own pairwise averages;
+//   * the predictor distortion (:529-570) and GetBestVector
+//     (MotionEstimatorH264ImplFull.h:140-168) in correctly rounded doubles.
+// With the median predictor (H264MotionVectorPredictorImpl1 over
+// MacroBlockH264::GetMbMotionMedianPred, MacroBlockH264.cpp:757-816) an MB
+// depends on its left, above-left, above and above-right neighbours, so MB
+// rows run as a wavefront: each row is owned by one wave that waits on the
+// row above through a progress counter.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "icu_device.h"
+#include "icu_internal.h"
+#include "icu_qpel.h"
+
+namespace icu {
+namespace dec {
+
+
+constexpr int kHalf = 35;       // half-pel grid side
+
+
+struct WaveLds {
+  uint32_t cur4[64];                // the MB's input block, 16 rows of 4 dwords
+  uint8_t P[kPatch * kPatch];       // reference patch
+  int16_t hv[17 * kPatch];          // raw vertical 6-tap sums
+  int16_t Hg[kHalf * kHalf];        // G/b/h/j grid on the half-pel lattice
+};
+
+// Half-pel neighbour offsets in the reference's candidate order
+// (Full.cpp:452-483): row above left to right, left, right, row below.
+constexpr int kSubX[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+constexpr int kSubY[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+
+// The MB's cost-table window [yu..yd] x [xl..xr], read straight from the
+// table (each entry is visited about once, so it is not staged).
+struct TabView {
+  const uint32_t* t;
+  int dim, L, xl, yu, w;
+  __device__ __forceinline__ uint32_t operator[](int idx) const {
+    const int i = idx / w, j = idx - i * w;
+    return t[(i + yu + L) * dim + (j + xl + L)];
+  }
+};
+
+// LDS written by one lane of a wave and read by another, later in program
+// order: a wavefront-scope fence plus a wave barrier keeps the compiler from
+// reordering across it (a wave's LDS operations complete in order).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+// Eight wave sums at once: the shuffles of one step are independent, so
+// their latencies overlap.
+__device__ __forceinline__ void wave_sum8(int (&a)[8]) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int k = 0; k < 8; k++) a[k] += __shfl_xor(a[k], o, 64);
+}
+__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long w = __shfl_xor(v, o, 64);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ int pix_dist(int a, int b, int metric) {
+  const int d = a - b;
+  return metric == kMetricSAD ? (d < 0 ? -d : d) : d * d;
+}
+
+// Stage the 22x22 patch whose (3,3) is reference pel (x0, y0): 22 rows of 7
+// dwords read from the dword boundary at or left of x0-3 (the padded plane's
+// origin and stride are multiples of 16), every load issued before the first
+// LDS store.
+__device__ inline void load_patch(const DecideDesc& d, int x0, int y0, uint8_t* P, int lane) {
+  const int xa = x0 - 3, sh = xa & 3;  // 0..3 also for negative xa
+  const uint32_t* base = (const uint32_t*)(d.ref + (long)(y0 - 3) * d.ref_stride + (xa - sh));
+  const int sdw = d.ref_stride >> 2;
+  uint32_t w[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const int i = lane + 64 * j, r = i / 7, k = i - 7 * r;
+    w[j] = i < 7 * kPatch ? base[r * sdw + k] : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const int i = lane + 64 * j, r = i / 7, k = i - 7 * r;
+    if (i < 7 * kPatch) {
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const int c = 4 * k + b - sh;
+        if (c >= 0 && c < kPatch) P[r * kPatch + c] = (uint8_t)(w[j] >> (8 * b));
+      }
+    }
+  }
+}
+
+// Integer decision by argmin over the table with the predictor-distance key
+// (filter off): lower cost, then closer to (pXr, pYr), then later in raster.
+__device__ inline void table_argmin(const TabView& tab, int xl, int xr, int yu, int yd, int pXr, int pYr,
+                             int lane, int* mx, int* my, int* minD) {
+  const int w = xr - xl + 1, n = w * (yd - yu + 1);
+  unsigned long long best = ~0ull;
+  for (int idx = lane; idx < n; idx += 64) {
+    const int i = yu + idx / w, j = xl + idx % w;
+    const uint32_t c = tab[idx];
+    unsigned long long rr, d2;
+    if (i == 0 && j == 0) {
+      rr = 0x1FFFFull;
+    } else {
+      rr = 0xFFFFull - (unsigned long long)((i + 64) * 256 + (j + 64));
+    }
+    const int ex = j - pXr, ey = i - pYr;
+    d2 = (unsigned long long)(ex * ex + ey * ey);
+    const unsigned long long key = ((unsigned long long)c << 40) | (d2 << 17) | rr;
+    best = key < best ? key : best;
+  }
+  best = wave_min_u64(best);
+  const uint32_t rr = (uint32_t)(best & 0x1FFFF);
+  if (rr == 0x1FFFF) {
+    *mx = 0;
+    *my = 0;
+  } else {
+    const uint32_t rank = 0xFFFFu - rr;
+    *my = (int)(rank >> 8) - 64;
+    *mx = (int)(rank & 255) - 64;
+  }
+  *minD = (int)(best >> 40);
+}
+
+// The reference's raster scan with its running-best filter (Full.cpp:401-446).
+__device__ inline void table_replay(const TabView& tab, int xl, int xr, int yu, int yd, int pXr, int pYr,
+                             int lane, int* pmx, int* pmy, int* pminD) {
+  const int w = xr - xl + 1, n = w * (yd - yu + 1);
+  int mx = 0, my = 0;
+  int minD = (int)tab[(0 - yu) * w + (0 - xl)];
+  int sidx = 0;
+  while (sidx < n) {
+    const int idx = sidx + lane;
+    bool acc = false;
+    int i = 0, j = 0, c = 0;
+    if (idx < n) {
+      i = yu + idx / w;
+      j = xl + idx % w;
+      if ((i | j) != 0 && !((i + my) < yu || (i + my) > yd || (j + mx) < xl || (j + mx) > xr)) {
+        c = (int)tab[idx];
+        if (c < minD) {
+          acc = true;
+        } else if (c == minD) {
+          const int cx = mx - pXr, cy = my - pYr, nx = j - pXr, ny = i - pYr;
+          acc = (cy * cy + cx * cx) - (ny * ny + nx * nx) >= 0;
+        }
+      }
+    }
+    const unsigned long long mask = __ballot(acc);
+    if (mask == 0) {
+      sidx += 64;
+      continue;
+    }
+    const int f = __ffsll((long long)mask) - 1;
+    minD = __shfl(c, f, 64);
+    mx = __shfl(j, f, 64);
+    my = __shfl(i, f, 64);
+    sidx += f + 1;
+  }
+  *pmx = mx;
+  *pmy = my;
+  *pminD = minD;
+}
+
+// Build the half-pel lattice around full-pel origin (ox, oy) (patch staged).
+// Hg[hy][hx]: even/even G, even/odd b, odd/even h, odd/odd j.  Each lane
+// fills whole 2x2 cells (G, b, h, j of one full-pel position), so no lane
+// branches on the parity; fixed trip counts let the LDS reads of all
+// iterations overlap.
+__device__ inline void build_half_grid(WaveLds& s, int lane) {
+  constexpr int kHv = 17 * kPatch, kCells = 18 * 18;
+#pragma unroll
+  for (int j = 0; j < (kHv + 63) / 64; j++) {
+    const int i = min(lane + 64 * j, kHv - 1);
+    const int a = i / kPatch, x = i - a * kPatch;
+    s.hv[i] = (int16_t)vtap_p(s.P, x, a + 2);
+  }
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < (kCells + 63) / 64; j++) {
+    const int i = min(lane + 64 * j, kCells - 1);
+    const int cy = i / 18, cx = i - cy * 18;
+    // (b, h and j of the last row / column are not stored; their reads are
+    // clamped into the arrays)
+    const int cxi = min(cx, 16), cyi = min(cy, 16);
+    const int G = s.P[(cy + 2) * kPatch + cx + 2];
+    const int b = b_p(s.P, cxi + 2, cy + 2);
+    const int16_t* r = s.hv + cyi * kPatch;
+    const int h = clip255((r[cx + 2] + 16) >> 5);
+    const int t = ICU_TAP6((int)r[cxi], (int)r[cxi + 1], (int)r[cxi + 2], (int)r[cxi + 3],
+                           (int)r[cxi + 4], (int)r[cxi + 5]);
+    const int jv = clip255((t + 512) >> 10);
+    int16_t* o = s.Hg + (2 * cy) * kHalf + 2 * cx;
+    // cells on the last row / column have only their even samples inside
+    o[0] = (int16_t)G;
+    if (cx < 17) o[1] = (int16_t)b;
+    if (cy < 17) {
+      o[kHalf] = (int16_t)h;
+      if (cx < 17) o[kHalf + 1] = (int16_t)jv;
+    }
+  }
+  wave_sync();
+}
+
+struct Decision {
+  int mvx, mvy, dist;
+  int imx, imy, iminD;
+};
+
+__device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, int predX, int predY,
+                                     int lane, const IntResult* known = nullptr) {
+  const int col = mb % d.mbw, row = mb / d.mbw;
+  const int n = col * 16, m = row * 16;
+  const int slot = d.mb_base + mb;
+  // input block: one aligned dword per lane (current planes are 16-byte
+  // aligned with a stride that is a multiple of 16)
+  const uint32_t cw = *(const uint32_t*)(d.cur + (long)(m + (lane >> 2)) * d.cur_stride + n + 4 * (lane & 3));
+  s.cur4[lane] = cw;
+  const uint8_t* cur8 = (const uint8_t*)s.cur4;
+  // the zero vector's cost, which the argmin search does not report; only
+  // GetBestVector reads it (the padded plane's rows are 16-byte aligned)
+  int zero_direct = 0;
+  if (!d.table && d.bestvec) {
+    const uint32_t rw = *(const uint32_t*)(d.ref + (long)(m + (lane >> 2)) * d.ref_stride + n + 4 * (lane & 3));
+    int z = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) z += pix_dist((int)((cw >> (8 * i)) & 0xFF), (int)((rw >> (8 * i)) & 0xFF), d.metric);
+    zero_direct = wave_sum(z);
+  }
+
+  const int predXQ = predX % 4, predYQ = predY % 4;
+  int predX0 = predX / 4, predY0 = predY / 4;
+  const int pXr = predX < 0 ? (predX - 2) / 4 : (predX + 2) / 4;
+  const int pYr = predY < 0 ? (predY - 2) / 4 : (predY + 2) / 4;
+
+  int mx, my, minD, zeroD;
+  int xl, xr, yu, yd;
+  motion_window(n, m, d.W, d.H, d.L, &xl, &xr, &yu, &yd);
+  if (d.table) {
+    TabView tab;
+    tab.t = d.table + d.table_base + (long long)mb * d.table_dim * d.table_dim;
+    tab.dim = d.table_dim;
+    tab.L = d.L;
+    tab.xl = xl;
+    tab.yu = yu;
+    tab.w = xr - xl + 1;
+    zeroD = (int)tab[(0 - yu) * tab.w + (0 - xl)];
+    if (d.filter) table_replay(tab, xl, xr, yu, yd, pXr, pYr, lane, &mx, &my, &minD);
+    else table_argmin(tab, xl, xr, yu, yd, pXr, pYr, lane, &mx, &my, &minD);
+  } else {
+    const IntResult r = known ? *known : d.ires[slot];
+    mx = r.mx;
+    my = r.my;
+    minD = r.cost;
+    zeroD = zero_direct;
+  }
+  Decision out;
+  out.imx = mx;
+  out.imy = my;
+  out.iminD = minD;
+
+  int mvx = mx * 4, mvy = my * 4;
+  if (d.subpel >= 1) {
+    load_patch(d, n + mx, m + my, s.P, lane);
+    wave_sync();
+    build_half_grid(s, lane);
+    // half-pel candidates, all eight summed at once, then taken in the
+    // reference's order (strictly better only)
+    int hxh = 0, hyh = 0;
+    {
+      int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int p = lane; p < 256; p += 64) {
+        const int r = p >> 4, c = p & 15;
+        const int cv = cur8[p];
+        const int16_t* h = s.Hg + (2 + 2 * r) * kHalf + 2 + 2 * c;
+#pragma unroll
+        for (int k = 0; k < 8; k++) acc[k] += pix_dist(cv, h[kSubY[k] * kHalf + kSubX[k]], d.metric);
+      }
+      wave_sum8(acc);
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (acc[k] < minD) {
+          minD = acc[k];
+          hxh = kSubX[k];
+          hyh = kSubY[k];
+        }
+    }
+    int qmx = 2 * hxh, qmy = 2 * hyh;
+    if (d.subpel >= 2) {
+      // quarter candidates: the reference's pairwise averages of half-grid
+      // samples around the half-pel winner H = (cr, cc) (LoadQuartPelWindow)
+      const int hx = 2 * hxh, hy = 2 * hyh;
+      int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int p = lane; p < 256; p += 64) {
+        const int r = p >> 4, c = p & 15;
+        const int cv = cur8[p];
+        const int16_t* h = s.Hg + (2 + 2 * r + hyh) * kHalf + 2 + 2 * c + hxh;
+        const int uu = h[-kHalf - 1], u0 = h[-kHalf], l0 = h[-1], c0 = h[0], r0 = h[1], d0 = h[kHalf];
+        acc[0] += pix_dist(cv, (uu + c0 + 1) >> 1, d.metric);  // (-1,-1)
+        acc[1] += pix_dist(cv, (u0 + c0 + 1) >> 1, d.metric);  // ( 0,-1)
+        acc[2] += pix_dist(cv, (u0 + r0 + 1) >> 1, d.metric);  // ( 1,-1)
+        acc[3] += pix_dist(cv, (l0 + c0 + 1) >> 1, d.metric);  // (-1, 0)
+        acc[4] += pix_dist(cv, (r0 + c0 + 1) >> 1, d.metric);  // ( 1, 0)
+        acc[5] += pix_dist(cv, (l0 + d0 + 1) >> 1, d.metric);  // (-1, 1)
+        acc[6] += pix_dist(cv, (d0 + c0 + 1) >> 1, d.metric);  // ( 0, 1)
+        acc[7] += pix_dist(cv, (d0 + r0 + 1) >> 1, d.metric);  // ( 1, 1)
+      }
+      wave_sum8(acc);
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (acc[k] < minD) {
+          minD = acc[k];
+          qmx = hx + kSubX[k];
+          qmy = hy + kSubY[k];
+        }
+    }
+    mvx += qmx;
+    mvy += qmy;
+    wave_sync();
+  }
+
+  int outx = mvx, outy = mvy, outd = minD;
+  if (d.bestvec) {
+    if (predX0 + n > d.W) predX0 = 16;
+    if (predX0 + n < -16) predX0 = -16;
+    if (predY0 + m > d.H) predY0 = 16;
+    if (predY0 + m < -16) predY0 = -16;
+    const int pX = predX0 * 4 + predXQ, pY = predY0 * 4 + predYQ;
+    int predD;
+    if (predXQ == 0 && predYQ == 0 && predX0 == 0 && predY0 == 0) {
+      predD = zeroD;
+    } else {
+      // QuarterRead at the predictor: negative fractions reflect to the
+      // full pel to the left / above (OverlayMem2Dv2.cpp:709-724).
+      int fox = 0, fx = predXQ, foy = 0, fy = predYQ;
+      if (predXQ < 0) { fox = -1; fx = 4 + predXQ; }
+      if (predYQ < 0) { foy = -1; fy = 4 + predYQ; }
+      load_patch(d, predX0 + n + fox, predY0 + m + foy, s.P, lane);
+      wave_sync();
+      int acc = 0;
+      for (int p = lane; p < 256; p += 64) {
+        const int r = p >> 4, c = p & 15;
+        acc += pix_dist(cur8[p], qpel_p(s.P, 3 + c, 3 + r, fx, fy), d.metric);
+      }
+      predD = wave_sum(acc);
+      wave_sync();
+    }
+    // fifo [best, pred, zero, zero, zero]; strictly smaller cost wins
+    // (selects rather than an indexed array, which would live in scratch)
+    double cc = best_vector_cost(minD, (mvx - pX) * (mvx - pX) + (mvy - pY) * (mvy - pY));
+    const double cp = best_vector_cost(predD, 0);
+    if (cp < cc) {
+      cc = cp;
+      outx = pX;
+      outy = pY;
+      outd = predD;
+    }
+    const double cz = best_vector_cost(zeroD, pX * pX + pY * pY);
+    if (cz < cc) {
+      outx = 0;
+      outy = 0;
+      outd = zeroD;
+    }
+  }
+  out.mvx = outx;
+  out.mvy = outy;
+  out.dist = outd;
+  if (lane == 0) {
+    const uint32_t w = (uint32_t)(uint16_t)(int16_t)outx | ((uint32_t)(uint16_t)(int16_t)outy << 16);
+    __hip_atomic_store((uint32_t*)(d.mv_out + 2 * slot), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    d.dist_out[slot] = outd;
+    if (d.int_out) {
+      d.int_out[3 * slot] = out.imx;
+      d.int_out[3 * slot + 1] = out.imy;
+      d.int_out[3 * slot + 2] = out.iminD;
+    }
+  }
+  return out;
+}
+
+__device__ __forceinline__ int median3(int x, int y, int z) {
+  int mn, mx;
+  if (y - z < 0) { mn = y; mx = z; } else { mn = z; mx = y; }
+  if (x < mn) mn = x;
+  if (x > mx) mx = x;
+  return x + y + z - mn - mx;
+}
+
+}  // namespace dec
+}  // namespace icu

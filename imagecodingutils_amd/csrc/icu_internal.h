@@ -33,11 +33,22 @@ struct DecideItem {
   int mb;
 };
 
-void launch_pad_plane(const uint8_t* src, int W, int H, int src_stride, uint8_t* dst_origin,
-                      int dst_stride, hipStream_t st);
+// One reference plane to pad: an 8-bit W x H plane (stride src_stride) into
+// the padded plane whose pixel (0,0) is dst (stride dst_stride).  wg_base:
+// the plane's first workgroup in the shared launch.
+struct PadDesc {
+  const uint8_t* src;
+  uint8_t* dst;
+  int W, H, src_stride, dst_stride;
+  int wg_base;
+};
+// Workgroups padding one W x H plane.
+int pad_plane_workgroups(int W, int H);
+// Pads n planes in one launch of total_wgs workgroups.
+void launch_pad_planes(const PadDesc* d_pads, int n, int total_wgs, hipStream_t st);
 void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int grid_mode,
                    int metric, int vfix, bool table, int buf_dwords, IntResult* d_out,
-                   uint32_t* d_table, hipStream_t st);
+                   uint32_t* d_table, const DecideDesc* d_decide, hipStream_t st);
 // Dwords of one staging buffer (window rows padded to 16 bytes + blocks).
 int search_stage_dwords(int B, int V, int Q, int G);
 int search_max_window_dwords();

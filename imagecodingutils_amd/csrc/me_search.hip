@@ -27,6 +27,7 @@
 
 #include "icu_device.h"
 #include "icu_internal.h"
+#include "icu_decide.h"
 
 namespace icu {
 
@@ -48,20 +49,25 @@ typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 // src/ImageUtils/OverlayExtMem2Dv2.cpp:120-217, as a coordinate clamp).
 // 16 bytes per thread: chunks inside the picture copy one 16-byte load
 // (kPad, W and the source stride are multiples of 16), border chunks gather
-// clamped bytes.
-__global__ __launch_bounds__(256) void pad_plane_kernel(const uint8_t* __restrict__ src, int W,
-                                                        int H, int src_stride,
-                                                        uint8_t* __restrict__ dst_origin,
-                                                        int dst_stride) {
-  const int cpr = (W + 2 * kPad) / 16;  // 16-byte chunks per padded row
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= cpr * (H + 2 * kPad)) return;
+// clamped bytes.  Every plane of a batch pads in one launch: a workgroup finds
+// its plane by binary search over the planes' first workgroups.
+__global__ __launch_bounds__(256) void pad_planes_kernel(const PadDesc* __restrict__ pads, int n) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pads[mid].wg_base <= (int)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const PadDesc p = pads[lo];
+  const int cpr = (p.W + 2 * kPad) / 16;  // 16-byte chunks per padded row
+  const int idx = ((int)blockIdx.x - p.wg_base) * (int)blockDim.x + (int)threadIdx.x;
+  if (idx >= cpr * (p.H + 2 * kPad)) return;
   const int y = idx / cpr, x16 = (idx - y * cpr) * 16;
-  const int yy = min(max(y - kPad, 0), H - 1);
-  const uint8_t* row = src + (long)yy * src_stride;
+  const int yy = min(max(y - kPad, 0), p.H - 1);
+  const uint8_t* row = p.src + (long)yy * p.src_stride;
   const int x = x16 - kPad;
   uint4 v;
-  if (x >= 0 && x + 16 <= W && (((uintptr_t)row & 15) == 0)) {
+  if (x >= 0 && x + 16 <= p.W && (((uintptr_t)row & 15) == 0)) {
     v = *(const uint4*)(row + x);
   } else {
     uint32_t w[4];
@@ -70,13 +76,13 @@ __global__ __launch_bounds__(256) void pad_plane_kernel(const uint8_t* __restric
       w[q] = 0;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int xx = min(max(x + 4 * q + i, 0), W - 1);
+        const int xx = min(max(x + 4 * q + i, 0), p.W - 1);
         w[q] |= (uint32_t)row[xx] << (8 * i);
       }
     }
     v = make_uint4(w[0], w[1], w[2], w[3]);
   }
-  *(uint4*)(dst_origin - (long)kPad * dst_stride - kPad + (long)y * dst_stride + x16) = v;
+  *(uint4*)(p.dst - (long)kPad * p.dst_stride - kPad + (long)y * p.dst_stride + x16) = v;
 }
 
 // ------------------------------------------------------------- tile bodies
@@ -358,6 +364,15 @@ __device__ __forceinline__ void issue_stage(const StreamDesc& s, const SearchTas
   }
 }
 
+// The decision of one MB, kept out of line: inlined into the search kernel
+// its unrolled loops share one register allocation with the tile loop and
+// push the kernel past the VGPR budget of three waves per SIMD.
+__device__ __attribute__((noinline)) void decide_in_search(const DecideDesc* d, dec::WaveLds* s, int mb,
+                                                           int lane, int mx, int my, int cost) {
+  const IntResult r{mx, my, cost, -1};
+  dec::decide_mb(*d, *s, mb, 0, 0, lane, &r);
+}
+
 // VFIX: tile height fixed at compile time, or 0 to take each stream's own
 // (mixed batches).  TABLE: write every cost to the table (filtered scan /
 // predictor modes) instead of only the argmin.
@@ -374,7 +389,7 @@ template <int METRIC, int VFIX, bool TABLE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(search_min_waves(METRIC, VFIX, TABLE))))
 void me_search_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __restrict__ tasks,
                       int ntasks, IntResult* __restrict__ out, uint32_t* __restrict__ table,
-                      int buf_dwords) {
+                      int buf_dwords, const DecideDesc* __restrict__ decide) {
   extern __shared__ __attribute__((aligned(16))) uint32_t dyn[];
   __shared__ uint32_t csq[kMaxG];
   __shared__ uint64_t best[kMaxG];
@@ -430,45 +445,75 @@ void me_search_kernel(const StreamDesc* __restrict__ streams, const SearchTask* 
           case 8: run_tile<METRIC, 8, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table); break;
           case 11: run_tile<METRIC, 11, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table); break;
           case 13: run_tile<METRIC, 13, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table); break;
-          default: run_tile<METRIC, 16, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table); break;
+          default:
+            // 16-row tiles only with cost tables (argmin keys do not fit them
+            // beyond small ranges, and their registers would spill here)
+            if constexpr (TABLE) run_tile<METRIC, 16, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table);
+            break;
         }
       }
     }
     __syncthreads();
-    if (tid < t.count) {
-      const uint64_t k = best[tid];
-      int dx, dy;
-      key_vector(k, &dx, &dy);
-      IntResult r;
-      r.mx = dx;
-      r.my = dy;
-      r.cost = (int)key_cost(k);
-      r.zero = zero[tid];
-      out[s.mb_base + t.mb + tid] = r;
-      best[tid] = ~0ull;
-      zero[tid] = -1;
+    if constexpr (TABLE) {
+      // the decision kernels read the integer results (and the cost table)
+      if (tid < t.count) {
+        const uint64_t k = best[tid];
+        int dx, dy;
+        key_vector(k, &dx, &dy);
+        IntResult r;
+        r.mx = dx;
+        r.my = dy;
+        r.cost = (int)key_cost(k);
+        r.zero = zero[tid];
+        out[s.mb_base + t.mb + tid] = r;
+        best[tid] = ~0ull;
+        zero[tid] = -1;
+      }
+    } else {
+      // Decision in place: wave w finishes MBs w, w + 4, ... of the task
+      // (sub-pel refinement, final vector choice, outputs) while the block
+      // is still in LDS and the patch is in L2.  Each MB's key is read and
+      // reset by the one wave that decides it.
+      __shared__ dec::WaveLds ws[kThreads / 64];
+      const int wave = tid >> 6, lane = tid & 63;
+      for (int gg = wave; gg < t.count; gg += kThreads / 64) {
+        const uint64_t k = best[gg];
+        int mx, my;
+        key_vector(k, &mx, &my);
+        dec::wave_sync();
+        if (lane == 0) {
+          best[gg] = ~0ull;
+          zero[gg] = -1;
+        }
+        decide_in_search(decide + t.stream, ws + wave, t.mb + gg, lane, mx, my, (int)key_cost(k));
+      }
     }
     ti = tnext;
   }
 }
 
 // ----------------------------------------------------------------- host side
-void launch_pad_plane(const uint8_t* src, int W, int H, int src_stride, uint8_t* dst_origin,
-                      int dst_stride, hipStream_t st) {
-  const int chunks = (W + 2 * kPad) / 16 * (H + 2 * kPad);
-  hipLaunchKernelGGL(pad_plane_kernel, dim3((chunks + 255) / 256), dim3(256), 0, st, src, W, H, src_stride, dst_origin,
-                     dst_stride);
+int pad_plane_workgroups(int W, int H) {
+  const long chunks = (long)(W + 2 * kPad) / 16 * (H + 2 * kPad);
+  return (int)((chunks + 255) / 256);
+}
+
+void launch_pad_planes(const PadDesc* d_pads, int n, int total_wgs, hipStream_t st) {
+  if (n <= 0 || total_wgs <= 0) return;
+  hipLaunchKernelGGL(pad_planes_kernel, dim3(total_wgs), dim3(256), 0, st, d_pads, n);
 }
 
 // vfix: the common tile height of every task, or 0 when streams differ.
 // buf_dwords: one staging buffer (window + blocks) of the largest task; the
-// kernel holds two.
+// kernel holds two.  table: some stream keeps a cost table; then the
+// decision kernels run after the search, otherwise the search kernel decides
+// each MB itself from d_decide.
 // grid_mode: 0 one task per workgroup; 1 persistent, as many workgroups as
 // stay resident; 2 persistent, that many reduced so that every workgroup runs
 // the same number of tasks (no partly filled last round).
 void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int grid_mode,
                    int metric, int vfix, bool table, int buf_dwords, IntResult* d_out,
-                   uint32_t* d_table, hipStream_t st) {
+                   uint32_t* d_table, const DecideDesc* d_decide, hipStream_t st) {
   const size_t shmem = 2 * (size_t)buf_dwords * sizeof(uint32_t);
   auto go = [&](auto kern) {
     int grid = ntasks;
@@ -499,7 +544,7 @@ void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int n
     // a workgroup that runs one task never touches the second buffer
     const size_t lds_bytes = grid >= ntasks ? shmem / 2 : shmem;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds_bytes, st, d_streams, d_tasks, ntasks, d_out,
-                       d_table, buf_dwords);
+                       d_table, buf_dwords, d_decide);
   };
   if (metric == kMetricSSD) {
     if (table) go(me_search_kernel<kMetricSSD, 8, true>);
@@ -519,7 +564,6 @@ void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int n
       case 8: go(me_search_kernel<kMetricSAD, 8, false>); break;
       case 11: go(me_search_kernel<kMetricSAD, 11, false>); break;
       case 13: go(me_search_kernel<kMetricSAD, 13, false>); break;
-      case 16: go(me_search_kernel<kMetricSAD, 16, false>); break;
       default: go(me_search_kernel<kMetricSAD, 0, false>); break;
     }
   }

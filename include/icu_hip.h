@@ -45,7 +45,7 @@ typedef struct icu_me_config {
   int best_vector;   /* 1: GetBestVector over {best, predictor, zero} (reference); 0: keep best */
   int tile_rows;     /* 0: chosen by the tuner; else candidate rows per lane: 8, 11, 13 or 16 */
   int mbs_per_task;  /* 0: chosen by the tuner; else MBs staged per workgroup task (1..16) */
-  int grid_mode;     /* 0: chosen by the tuner (persistent until tuned); else 1 + the mode
+  int grid_mode;     /* 0: chosen by the tuner (one task per workgroup until tuned); else 1 + the mode
                         icu_me_get_tiling reports (1 one task per workgroup, 2 persistent,
                         3 persistent balanced) */
 } icu_me_config;

@@ -145,12 +145,14 @@ def test_estimate_is_deterministic_and_rerunnable():
     np.testing.assert_array_equal(a[1], b[1])
 
 
-@pytest.mark.parametrize("tile_rows,mbs_per_task,R", [(8, 1, 24), (11, 5, 24), (13, 16, 24), (16, 3, 16),
-                                                      (8, 2, 64)])
-def test_explicit_tilings_match_oracle(tile_rows, mbs_per_task, R):
+@pytest.mark.parametrize("tile_rows,mbs_per_task,R,filt", [(8, 1, 24, 0), (11, 5, 24, 0), (13, 16, 24, 0),
+                                                           (16, 3, 16, 1), (8, 2, 64, 0), (13, 4, 16, 1)])
+def test_explicit_tilings_match_oracle(tile_rows, mbs_per_task, R, filt):
+    """Argmin tilings (decided inside the search kernel) and cost-table
+    tilings (decided by the replay kernel)."""
     W, H = 352, 288
     fp = synth.rect_motion_pair(W, H, 31, n_rects=8, max_mv=min(R - 4, 30))
-    modes = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=0)
+    modes = dict(metric=1, window=1, filt=filt, subpel=1, predictor=0, best_vector=0)
     est = capi.MotionEstimator(_cfg(W, H, R, tile_rows=tile_rows, mbs_per_task=mbs_per_task, **modes))
     assert est.tiling()[:2] == (tile_rows, mbs_per_task)
     mv, dist, _ = est.estimate(fp.cur.astype(np.int16), fp.ref.astype(np.int16))
@@ -177,9 +179,10 @@ def test_argmin_keys_tie_break_on_flat_and_periodic_content():
         np.testing.assert_array_equal(dist, odist)
 
 
-def test_too_tall_tile_for_exact_keys_is_rejected():
+@pytest.mark.parametrize("tile_rows,R", [(16, 32), (16, 8), (13, 40)])
+def test_too_tall_tile_for_exact_keys_is_rejected(tile_rows, R):
     with pytest.raises(capi.IcuError) as e:
-        capi.MotionEstimator(_cfg(352, 288, 32, tile_rows=16, metric=1, window=1, filt=0, predictor=0))
+        capi.MotionEstimator(_cfg(352, 288, R, tile_rows=tile_rows, metric=1, window=1, filt=0, predictor=0))
     assert e.value.code == capi.ICU_E_ARG
 
 
