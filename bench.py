@@ -46,7 +46,7 @@ def pctl(xs, p):
     return float(np.percentile(np.asarray(xs), p))
 
 
-def run_plain(capi, synth, steps, warmup, tune=True):
+def run_plain(capi, synth, steps, warmup, tune=False):
     fp = synth.rect_motion_pair(W, H, 1080, n_rects=24, max_mv=32, sigma_cur=3.0)
     est = capi.MotionEstimator(capi.make_config(W, H, R, **MODES))
     s = capi.Stream()
@@ -181,7 +181,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
-    ap.add_argument("--no-tune", action="store_true", help="keep the planner's tiling")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the tile tuner first (the planner's tiling, 13 rows x 3 MBs per task, one task "
+                         "per workgroup, is what it picks for this workload)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB: a kernel of known traffic "
                          "that calibrates hardware byte counters read in the same profile")
@@ -200,7 +202,7 @@ def main():
         capi.lib.icu_memcpy_dtod(b.ptr, a.ptr, nb, None)
         capi.lib.icu_stream_sync(None)
         del a, b
-    plain = run_plain(capi, synth, args.steps, args.warmup, tune=not args.no_tune)
+    plain = run_plain(capi, synth, args.steps, args.warmup, tune=args.tune)
     ms_per_step = plain["total_ms"] / args.steps
     value = plain["nmb"] / (ms_per_step / 1e3)
     ops = sad_ops(W, H, R)

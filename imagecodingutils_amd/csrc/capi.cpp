@@ -389,7 +389,7 @@ struct Engine {
       any_table |= x.table;
     }
     launch_search(d_streams, d_tasks, (int)h_tasks.size(), grid_mode, sp[0].cfg.metric, vfix,
-                  any_table, stage_dwords, d_ires, d_table, d_decide, st);
+                  any_table, stage_dwords, d_ires, d_table, d_decide, sp.size() == 1, st);
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
     // without cost tables the search kernel decides every MB itself
     if (any_table && !h_items.empty()) launch_decide_parallel(d_decide, d_items, (int)h_items.size(), st);

@@ -48,7 +48,7 @@ int pad_plane_workgroups(int W, int H);
 void launch_pad_planes(const PadDesc* d_pads, int n, int total_wgs, hipStream_t st);
 void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int grid_mode,
                    int metric, int vfix, bool table, int buf_dwords, IntResult* d_out,
-                   uint32_t* d_table, const DecideDesc* d_decide, hipStream_t st);
+                   uint32_t* d_table, const DecideDesc* d_decide, bool band, hipStream_t st);
 // Dwords of one staging buffer (window rows padded to 16 bytes + blocks).
 int search_stage_dwords(int B, int V, int Q, int G);
 int search_max_window_dwords();

@@ -389,7 +389,7 @@ template <int METRIC, int VFIX, bool TABLE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(search_min_waves(METRIC, VFIX, TABLE))))
 void me_search_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __restrict__ tasks,
                       int ntasks, IntResult* __restrict__ out, uint32_t* __restrict__ table,
-                      int buf_dwords, const DecideDesc* __restrict__ decide) {
+                      int buf_dwords, const DecideDesc* __restrict__ decide, int band) {
   extern __shared__ __attribute__((aligned(16))) uint32_t dyn[];
   __shared__ uint32_t csq[kMaxG];
   __shared__ uint64_t best[kMaxG];
@@ -405,6 +405,15 @@ void me_search_kernel(const StreamDesc* __restrict__ streams, const SearchTask* 
   // address, and those serialize (about 15 ns each, measured), which alone
   // took longer than the search.
   int ti = blockIdx.x;
+  // XCD bands: workgroups are dispatched to the 8 XCDs in turn, and each XCD
+  // has its own L2.  With one task per workgroup and tasks of equal cost (one
+  // stream), XCD x runs a contiguous band of tasks, so neighbouring search
+  // windows are fetched from HBM into one L2 instead of into all eight.
+  if (band && (int)gridDim.x == ntasks) {
+    const int x = (int)blockIdx.x & 7, k = (int)blockIdx.x >> 3;
+    const int q = ntasks >> 3, r = ntasks & 7;
+    ti = x * q + min(x, r) + k;
+  }
   {
     const SearchTask t0 = tasks[ti];
     issue_stage(streams[t0.stream], t0, buf(0), tid);
@@ -513,7 +522,7 @@ void launch_pad_planes(const PadDesc* d_pads, int n, int total_wgs, hipStream_t 
 // the same number of tasks (no partly filled last round).
 void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int grid_mode,
                    int metric, int vfix, bool table, int buf_dwords, IntResult* d_out,
-                   uint32_t* d_table, const DecideDesc* d_decide, hipStream_t st) {
+                   uint32_t* d_table, const DecideDesc* d_decide, bool band, hipStream_t st) {
   const size_t shmem = 2 * (size_t)buf_dwords * sizeof(uint32_t);
   auto go = [&](auto kern) {
     int grid = ntasks;
@@ -544,7 +553,7 @@ void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int n
     // a workgroup that runs one task never touches the second buffer
     const size_t lds_bytes = grid >= ntasks ? shmem / 2 : shmem;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds_bytes, st, d_streams, d_tasks, ntasks, d_out,
-                       d_table, buf_dwords, d_decide);
+                       d_table, buf_dwords, d_decide, band ? 1 : 0);
   };
   if (metric == kMetricSSD) {
     if (table) go(me_search_kernel<kMetricSSD, 8, true>);
