@@ -26,6 +26,15 @@ sys.path.insert(0, str(ROOT))
 
 W, H, R = 1920, 1088, 32
 QPS = (22, 28, 34)
+
+# HBM traffic of one search launch at this workload, from two rocprofv3
+# --pmc runs of `bench.py --no-cpu --no-batched --steps 5 --warmup 1
+# --copy-probe 256` (KiB per dispatch), each counter scaled by what it reads
+# for the 256 MiB device copy / fill of --copy-probe: FETCH_SIZE reads half
+# the copy's 262144 KiB (x2.0), WRITE_SIZE reads it exactly (x1.0).  Measured
+# counters, not computed here: refresh them when the kernel changes.
+SEARCH_FETCH_KIB, FETCH_SCALE = 2870.0, 2.0
+SEARCH_WRITE_KIB, WRITE_SCALE = 287.0, 1.0
 MODES = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=0)
 
 
@@ -76,8 +85,10 @@ def run_plain(capi, synth, steps, warmup, tune=False):
     wall = time.perf_counter() - t0
     per = [ev[k].elapsed_ms(ev[k + 1]) for k in range(steps)]
     total_ms = ev[0].elapsed_ms(ev[steps])
-    # per-stage device times of the estimate on the same stream
-    stages = np.array([est.estimate_timed(d_cur, d_ref, d_mv, d_dist, stream=s) for _ in range(5)])
+    # per-stage device times of the estimate on the same stream; the search
+    # stage is the mean of 20 back-to-back launches between one event pair
+    stages = np.array([est.estimate_timed(d_cur, d_ref, d_mv, d_dist, stream=s, search_reps=20)
+                       for _ in range(5)])
     return dict(nmb=n, total_ms=total_ms, wall_s=wall, per_ms=per, stages_ms=np.median(stages, 0).tolist(),
                 tiling=est.tiling(), est=est, frames=fp)
 
@@ -231,8 +242,12 @@ def main():
         "p99_ms": round(pctl(plain["per_ms"], 99), 4),
         "stage_ms": {"pad": plain["stages_ms"][0], "search": search_ms, "decide_subpel": plain["stages_ms"][2]},
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": round(peak, 2),
-                     "unit": "T SAD-op/s", "frac": round(achieved / peak, 4), "traffic": None,
-                     "kernel": "me_search_kernel<SAD>", "sad_ops_per_launch": ops},
+                     "unit": "T SAD-op/s", "frac": round(achieved / peak, 4),
+                     "traffic": round((SEARCH_FETCH_KIB * FETCH_SCALE + SEARCH_WRITE_KIB * WRITE_SCALE) * 1024),
+                     # padded reference + current plane read once, 8 B of results per MB
+                     "traffic_algorithmic": (W + 160) * (H + 160) + W * H + W * H // 256 * 8,
+                     "kernel": "me_search_kernel<SAD, 13 rows> (integer search + in-kernel decision)",
+                     "sad_ops_per_launch": ops, "search_ms": round(search_ms, 5)},
         "device": info,
     }
     if not args.no_batched:

@@ -37,7 +37,7 @@ _SIGS = {
     "icu_me_destroy": (C.c_int, [_vp]),
     "icu_me_estimate": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_long)]),
     "icu_me_estimate_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "icu_me_estimate_timed": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_float)]),
+    "icu_me_estimate_timed": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_float)]),
     "icu_me_get_tiling": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_me_autotune": (C.c_int, [_vp, _vp, _vp, C.c_int]),
     "icu_code_frame_device": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
@@ -223,11 +223,12 @@ class MotionEstimator:
         _check(lib.icu_me_estimate_device(self.h, _p(d_cur), _p(d_ref), _p(d_mv), _p(d_dist),
                                           _p(d_int), stream.handle if stream else None))
 
-    def estimate_timed(self, d_cur, d_ref, d_mv, d_dist, stream: Stream | None = None):
-        """Returns per-stage device times in ms: (pad, search, decide)."""
+    def estimate_timed(self, d_cur, d_ref, d_mv, d_dist, stream: Stream | None = None, search_reps: int = 1):
+        """Per-stage device times in ms: (pad, search, decide); the search time
+        is the mean of search_reps back-to-back launches."""
         ms = (C.c_float * 3)()
         _check(lib.icu_me_estimate_timed(self.h, _p(d_cur), _p(d_ref), _p(d_mv), _p(d_dist),
-                                         stream.handle if stream else None, ms))
+                                         stream.handle if stream else None, search_reps, ms))
         return tuple(float(x) for x in ms)
 
     def code_frame_device(self, d_cur, d_mv, qp, intra, cols_first, d_levels, d_recon, d_pred=None,

@@ -376,7 +376,10 @@ struct Engine {
 
   // Enqueue the whole estimate on `st` (buffers already bound).
   // ev (nullable): 4 events recorded before padding, search, decision, end.
-  int enqueue(int set, hipStream_t st, hipEvent_t* ev = nullptr) {
+  // search_reps > 1 (timing only): the search launch is repeated; it reads
+  // only the padded references and the current planes, so every repeat
+  // writes the same results.
+  int enqueue(int set, hipStream_t st, hipEvent_t* ev = nullptr, int search_reps = 1) {
     StreamDesc* d_streams = ds[set].d_streams;
     DecideDesc* d_decide = ds[set].d_decide;
     if (ev) HIP_TRY(hipEventRecord(ev[0], st));
@@ -388,8 +391,9 @@ struct Engine {
       if (x.t.V != vfix) vfix = 0;
       any_table |= x.table;
     }
-    launch_search(d_streams, d_tasks, (int)h_tasks.size(), grid_mode, sp[0].cfg.metric, vfix,
-                  any_table, stage_dwords, d_ires, d_table, d_decide, sp.size() == 1, st);
+    for (int r = 0; r < search_reps; r++)
+      launch_search(d_streams, d_tasks, (int)h_tasks.size(), grid_mode, sp[0].cfg.metric, vfix,
+                    any_table, stage_dwords, d_ires, d_table, d_decide, sp.size() == 1, st);
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
     // without cost tables the search kernel decides every MB itself
     if (any_table && !h_items.empty()) launch_decide_parallel(d_decide, d_items, (int)h_items.size(), st);
@@ -517,8 +521,9 @@ int icu_me_estimate_device(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref
 }
 
 int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int16_t* d_mv,
-                          int32_t* d_dist, void* stream, float* stage_ms) {
+                          int32_t* d_dist, void* stream, int search_reps, float* stage_ms) {
   if (!h || !d_cur || !d_ref || !d_mv || !d_dist || !stage_ms) return fail(ICU_E_ARG, "null argument");
+  if (search_reps < 1 || search_reps > 1000) return fail(ICU_E_ARG, "search_reps must be 1..1000");
   Engine& e = h->eng;
   const uint8_t* curs[1] = {d_cur};
   const uint8_t* refs[1] = {d_ref};
@@ -527,10 +532,11 @@ int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref,
   hipEvent_t ev[4];
   for (auto& x : ev) HIP_TRY(hipEventCreate(&x));
   const hipStream_t st = pick(stream, e);
-  rc = e.enqueue(0, st, ev);
+  rc = e.enqueue(0, st, ev, search_reps);
   if (!rc) {
     HIP_TRY(hipEventSynchronize(ev[3]));
     for (int i = 0; i < 3; i++) HIP_TRY(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
+    stage_ms[1] /= (float)search_reps;
   }
   for (auto& x : ev) hipEventDestroy(x);
   return rc;

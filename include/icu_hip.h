@@ -83,11 +83,14 @@ int icu_me_estimate_device(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref
 
 /* icu_me_estimate_device with HIP events between its stages, recorded on
  * the stream the kernels run on; waits for completion and writes the
- * duration of each stage in ms: [0] reference padding, [1] integer search,
- * [2] decision + sub-pel refinement.  No reference counterpart (the
- * reference has no timers); used by the benchmark. */
+ * duration of each stage in ms: [0] reference padding, [1] integer search
+ * (with the decision when no cost table is kept), [2] decision kernels.
+ * search_reps (1..1000) back-to-back search launches share one event pair
+ * and [1] is their mean, which takes the events' own dispatch gap out of a
+ * single launch's time (the search rewrites the same results each time).
+ * No reference counterpart (the reference has no timers); for benchmarks. */
 int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int16_t* d_mv,
-                          int32_t* d_dist, void* stream, float* stage_ms);
+                          int32_t* d_dist, void* stream, int search_reps, float* stage_ms);
 
 /* Tile choice in effect for this handle (after tuning): candidate rows per
  * lane, MBs per workgroup task, and the search grid mode: 0 one task per
