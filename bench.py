@@ -149,6 +149,88 @@ def run_batched(capi, synth, steps, warmup):
                 sad_ops=ops, streams=len(cfgs))
 
 
+SERVE_STREAMS, SERVE_RATE, SERVE_QP, SERVE_BUDGET = 32, 2000.0, 28, 64 << 30
+SERVE_RING = 3  # distinct frames generated per stream, submitted in turn
+
+
+def run_serving(capi, synth, seconds, lanes=4, max_inflight=32):
+    """Serving: 32 streams (CIF:720p:1080p drawn 4:2:1), frames arriving as
+    a Poisson process at 2000 frames/s in aggregate, each coded through the
+    server (+-32 full search + half-pel + MC + IT/quant at QP 28) with its
+    stream's reference resident on the device, under a 64 GB budget.
+    Latency of a frame: from its scheduled arrival to the poll that returns
+    it.  Then a saturated pass (every stream always has a frame queued) for
+    the server's throughput ceiling."""
+    rng = np.random.default_rng(4242)
+    names = rng.choice(["CIF", "720p", "1080p"], size=SERVE_STREAMS, p=[4 / 7, 2 / 7, 1 / 7])
+    srv = capi.Server(lanes=lanes, max_inflight=max_inflight, mem_budget=SERVE_BUDGET, qp=SERVE_QP)
+    ring, nmb, ids = [], [], []
+    for k, name in enumerate(names):
+        w, h = synth.SIZES[str(name)]
+        src = synth.StreamSource(w, h, 7000 + k, n_rects=int(rng.integers(8, 25)), max_mv=8)
+        ring.append([src.frame(t) for t in range(SERVE_RING)])
+        nmb.append((w // 16) * (h // 16))
+        ids.append(srv.open_stream(capi.make_config(w, h, R, **MODES)))
+    sent = [0] * SERVE_STREAMS
+
+    def submit(k, tag):
+        srv.submit(ids[k], ring[k][sent[k] % SERVE_RING], tag=tag)
+        sent[k] += 1
+
+    # warm-up: a few frames per stream (first frames, kernels, buffer pools)
+    for _ in range(2):
+        for k in range(SERVE_STREAMS):
+            submit(k, -1)
+    done = 0
+    while done < 2 * SERVE_STREAMS:
+        done += len(srv.poll(wait=True))
+    # Poisson arrivals
+    n = max(1, int(SERVE_RATE * seconds))
+    arrive = np.cumsum(rng.exponential(1.0 / SERVE_RATE, n))
+    who = rng.integers(0, SERVE_STREAMS, n)
+    lat = np.zeros(n)
+    mbs, got, i = 0, 0, 0
+    t0 = capi.Server.clock()
+    last_done = t0
+    while got < n:
+        now = capi.Server.clock() - t0
+        while i < n and arrive[i] <= now:
+            submit(int(who[i]), i)
+            i += 1
+        for r in srv.poll(wait=(i >= n)):
+            lat[r["tag"]] = r["done_s"] - (t0 + arrive[r["tag"]])
+            mbs += nmb[r["stream"]]
+            last_done = max(last_done, r["done_s"])
+            got += 1
+    span = last_done - t0
+    st = srv.stats()
+    # saturated: keep two frames queued per stream for about a second
+    sat_n = 0
+    sat_mbs = 0
+    for k in range(SERVE_STREAMS):
+        submit(k, -1)
+        submit(k, -1)
+    t1 = capi.Server.clock()
+    while capi.Server.clock() - t1 < min(2.0, max(0.5, seconds / 5)):
+        for r in srv.poll(wait=True):
+            sat_n += 1
+            sat_mbs += nmb[r["stream"]]
+            submit(r["stream"], -1)
+    sat_s = capi.Server.clock() - t1
+    srv.close()  # waits for the frames in flight, drops the queued ones
+    mix = {str(x): int((names == x).sum()) for x in ("CIF", "720p", "1080p")}
+    return {"workload": f"{SERVE_STREAMS} streams {mix}, Poisson {SERVE_RATE:.0f} frames/s for {seconds:g} s, "
+                        f"+-{R} full search + half-pel + MC + IT/quant at QP {SERVE_QP}, "
+                        f"{SERVE_BUDGET >> 30} GB budget, {lanes} lanes, up to {max_inflight} frames in flight",
+            "frames": n, "mbs_per_s": round(mbs / span, 1), "frames_per_s": round(n / span, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
+            "max_ms": round(float(lat.max()) * 1e3, 4),
+            "device_bytes_peak": st["peak_bytes"],
+            "saturated_mbs_per_s": round(sat_mbs / sat_s, 1),
+            "saturated_frames_per_s": round(sat_n / sat_s, 1)}
+
+
 def cpu_baseline(synth, budget_s=12.0):
     """The compiled reference class on the host, one thread: as shipped
     (SSD, running-best filter, +-31, half+quarter pel, median predictor),
@@ -195,6 +277,8 @@ def main():
     ap.add_argument("--tune", action="store_true",
                     help="run the tile tuner first (the planner's tiling, 13 rows x 3 MBs per task, one task "
                          "per workgroup, is what it picks for this workload)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="length of the serving leg's Poisson arrival run (0: skip the leg)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB: a kernel of known traffic "
                          "that calibrates hardware byte counters read in the same profile")
@@ -258,6 +342,8 @@ def main():
                           "value": round(bt["mbps"], 1), "plain_same_streams_mbps": round(bt["plain_mbps"], 1),
                           "batched_vs_plain": round(bt["mbps"] / bt["plain_mbps"], 3),
                           "achieved_T_sad_ops": round(bt["sad_ops"] / (bt["ms"] / 1e3) / 1e12, 2)}
+    if args.serve_seconds > 0:
+        out["serving"] = run_serving(capi, synth, args.serve_seconds)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(synth)
     print(json.dumps(out), flush=True)

@@ -30,6 +30,18 @@ class MeConfig(C.Structure):
         "predictor", "best_vector", "tile_rows", "mbs_per_task", "grid_mode")]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("lanes", C.c_int), ("max_inflight", C.c_int), ("mem_budget", C.c_longlong),
+                ("qp", C.c_int)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("tag", C.c_longlong), ("stream", C.c_int), ("frame", C.c_int), ("nmb", C.c_int),
+                ("submit_s", C.c_double), ("done_s", C.c_double),
+                ("levels_sum", C.c_ulonglong), ("recon_sum", C.c_ulonglong),
+                ("mv", C.POINTER(C.c_int16))]
+
+
 _vp = C.c_void_p
 _SIGS = {
     "icu_me_config_reference": (C.c_int, [C.POINTER(MeConfig), C.c_int, C.c_int, C.c_int]),
@@ -54,6 +66,14 @@ _SIGS = {
     "icu_me_batch_estimate_device": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), _vp, _vp, _vp]),
     "icu_me_batch_capture": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), _vp, _vp]),
     "icu_me_batch_graph_launch": (C.c_int, [_vp, _vp]),
+    "icu_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
+    "icu_server_destroy": (C.c_int, [_vp]),
+    "icu_server_open_stream": (C.c_int, [_vp, C.POINTER(MeConfig), C.POINTER(C.c_int)]),
+    "icu_server_submit": (C.c_int, [_vp, C.c_int, _vp, C.c_longlong]),
+    "icu_server_poll": (C.c_int, [_vp, C.POINTER(ServerResult), C.c_int, C.c_int]),
+    "icu_server_stats": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                   C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "icu_server_clock": (C.c_double, []),
     "icu_last_error": (C.c_char_p, []),
     "icu_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_malloc": (_vp, [C.c_size_t]),
@@ -298,6 +318,68 @@ class MotionEstimatorBatch:
     def close(self):
         if getattr(self, "h", None):
             lib.icu_me_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+# ------------------------------------------------------------------ serving
+def server_checksum(a: np.ndarray) -> int:
+    """The checksum icu_server_result carries for an array's bytes: the sum
+    over bytes b[i] of (b[i] + 1) * ((i * 0x9E3779B1 mod 2^32) | 1), mod 2^64."""
+    b = np.ascontiguousarray(a).view(np.uint8).ravel().astype(np.uint64)
+    i = np.arange(b.size, dtype=np.uint64)
+    w = ((i * np.uint64(0x9E3779B1)) & np.uint64(0xFFFFFFFF)) | np.uint64(1)
+    return int(np.sum((b + np.uint64(1)) * w, dtype=np.uint64))
+
+
+class Server:
+    """Request loop with continuous batching over many streams (icu_server_*).
+
+    submit() queues a frame and starts what can start; poll() returns the
+    finished frames as dicts (tag, stream, frame, submit_s, done_s,
+    levels_sum, recon_sum, mv as an (nmb, 2) int16 array)."""
+
+    def __init__(self, lanes: int = 4, max_inflight: int = 16, mem_budget: int = 64 << 30, qp: int = 28):
+        cfg = ServerConfig(lanes, max_inflight, mem_budget, qp)
+        h = _vp()
+        _check(lib.icu_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._res = (ServerResult * 256)()
+
+    def open_stream(self, cfg: MeConfig) -> int:
+        sid = C.c_int()
+        _check(lib.icu_server_open_stream(self.h, C.byref(cfg), C.byref(sid)))
+        return sid.value
+
+    def submit(self, stream: int, frame: np.ndarray, tag: int = 0) -> None:
+        f = np.ascontiguousarray(frame, dtype=np.uint8)
+        _check(lib.icu_server_submit(self.h, stream, f.ctypes.data, tag))
+
+    def poll(self, wait: bool = False) -> list[dict]:
+        n = lib.icu_server_poll(self.h, self._res, len(self._res), int(wait))
+        if n < 0:
+            _check(n)
+        out = []
+        for r in self._res[:n]:
+            mv = np.ctypeslib.as_array(r.mv, shape=(2 * r.nmb,)).reshape(r.nmb, 2).copy()
+            out.append(dict(tag=r.tag, stream=r.stream, frame=r.frame, submit_s=r.submit_s,
+                            done_s=r.done_s, levels_sum=r.levels_sum, recon_sum=r.recon_sum, mv=mv))
+        return out
+
+    def stats(self) -> dict:
+        b, pk, q, f = C.c_longlong(), C.c_longlong(), C.c_int(), C.c_int()
+        _check(lib.icu_server_stats(self.h, C.byref(b), C.byref(pk), C.byref(q), C.byref(f)))
+        return dict(bytes_in_use=b.value, peak_bytes=pk.value, queued=q.value, inflight=f.value)
+
+    @staticmethod
+    def clock() -> float:
+        return lib.icu_server_clock()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.icu_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -23,6 +23,12 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+}  // namespace
+
+int icu::icu_fail(int code, const std::string& msg) { return fail(code, msg); }
+
+namespace {
+
 #define HIP_TRY(expr)                                                                     \
   do {                                                                                    \
     hipError_t e_ = (expr);                                                               \
@@ -141,13 +147,18 @@ struct Engine {
   std::vector<StreamPlan> sp;
   int total_mb = 0;
   long long table_total = 0;
-  std::vector<StreamDesc> h_streams;
-  std::vector<DecideDesc> h_decide;
   std::vector<SearchTask> h_tasks;
   std::vector<DecideItem> h_items;
-  // Descriptor sets: [0] for eager calls, [1] owned by a captured graph, so
-  // eager calls with other buffers never redirect a graph's replays.
+  // Descriptor slots.  kEager slots serve eager calls: a call uses the slot
+  // already bound to its buffers, or refills the least recently used one
+  // after waiting for that slot's last launch (an event, not a device-wide
+  // synchronisation) and uploads it on the call's stream from a pinned
+  // mirror.  Slot kGraph belongs to a captured graph, so eager calls never
+  // redirect a graph's replays.
+  static constexpr int kEager = 3, kGraph = kEager;
   struct DescSet {
+    void* d_block = nullptr;  // StreamDesc[n] | DecideDesc[n] | PadDesc[n]
+    void* h_block = nullptr;  // pinned host mirror of d_block
     StreamDesc* d_streams = nullptr;
     DecideDesc* d_decide = nullptr;
     PadDesc* d_pad = nullptr;
@@ -156,7 +167,13 @@ struct Engine {
     int32_t* dist = nullptr;
     int32_t* integ = nullptr;
     bool valid = false;
-  } ds[2];
+    bool used = false;        // `done` records this slot's last launch
+    hipEvent_t done = nullptr;
+    unsigned long long stamp = 0;
+  } ds[kEager + 1];
+  unsigned long long use_clock = 0;
+  size_t desc_bytes = 0;
+  long long dev_bytes = 0;  // device memory this engine allocated
   SearchTask* d_tasks = nullptr;
   DecideItem* d_items = nullptr;
   IntResult* d_ires = nullptr;
@@ -187,12 +204,11 @@ struct Engine {
       if (s.refpad_base) hipFree(s.refpad_base);
     sp.clear();
     for (auto& d : ds) {
-      hipFree(d.d_streams);
-      hipFree(d.d_decide);
-      hipFree(d.d_pad);
-      d.d_streams = nullptr;
-      d.d_decide = nullptr;
-      d.d_pad = nullptr;
+      if (d.used) hipEventSynchronize(d.done);
+      hipFree(d.d_block);
+      hipHostFree(d.h_block);
+      if (d.done) hipEventDestroy(d.done);
+      d = DescSet{};
     }
     hipFree(d_tasks);
     hipFree(d_items);
@@ -260,6 +276,7 @@ struct Engine {
       s.ref_stride = ((s.cfg.width + 2 * kPad) + 255) & ~255;
       const size_t bytes = (size_t)s.ref_stride * (s.cfg.height + 2 * kPad);
       HIP_TRY(hipMalloc(&s.refpad_base, bytes));
+      dev_bytes += bytes;
       s.refpad = s.refpad_base + (size_t)kPad * s.ref_stride + kPad;
       wave_off.push_back(wave_total);
       wave_total += s.mbh + 2;
@@ -267,10 +284,18 @@ struct Engine {
       pad_total += pad_plane_workgroups(s.cfg.width, s.cfg.height);
     }
     build_tasks();
+    // descriptor block: three arrays, each starting on a 256-byte boundary
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_dec = up(sizeof(StreamDesc) * n), o_pad = o_dec + up(sizeof(DecideDesc) * n);
+    desc_bytes = o_pad + up(sizeof(PadDesc) * n);
     for (auto& d : ds) {
-      HIP_TRY(hipMalloc(&d.d_streams, sizeof(StreamDesc) * n));
-      HIP_TRY(hipMalloc(&d.d_decide, sizeof(DecideDesc) * n));
-      HIP_TRY(hipMalloc(&d.d_pad, sizeof(PadDesc) * n));
+      HIP_TRY(hipMalloc(&d.d_block, desc_bytes));
+      dev_bytes += desc_bytes;
+      HIP_TRY(hipHostMalloc(&d.h_block, desc_bytes, hipHostMallocDefault));
+      HIP_TRY(hipEventCreateWithFlags(&d.done, hipEventDisableTiming));
+      d.d_streams = (StreamDesc*)d.d_block;
+      d.d_decide = (DecideDesc*)((char*)d.d_block + o_dec);
+      d.d_pad = (PadDesc*)((char*)d.d_block + o_pad);
       d.cur.assign(n, nullptr);
       d.ref.assign(n, nullptr);
     }
@@ -278,8 +303,10 @@ struct Engine {
     HIP_TRY(hipMalloc(&d_tasks, sizeof(SearchTask) * total_mb));
     HIP_TRY(hipMalloc(&d_ires, sizeof(IntResult) * total_mb));
     HIP_TRY(hipMalloc(&d_wave, sizeof(int) * wave_total));
+    dev_bytes += (sizeof(SearchTask) + sizeof(IntResult)) * total_mb + sizeof(int) * wave_total;
     HIP_TRY(hipMemset(d_wave, 0, sizeof(int) * wave_total));
     if (table_total) HIP_TRY(hipMalloc(&d_table, sizeof(uint32_t) * table_total));
+    dev_bytes += sizeof(uint32_t) * table_total;
     HIP_TRY(hipMemcpy(d_tasks, h_tasks.data(), sizeof(SearchTask) * h_tasks.size(), hipMemcpyHostToDevice));
     h_items.clear();
     for (int i = 0; i < n; i++)
@@ -287,31 +314,71 @@ struct Engine {
         for (int mb = 0; mb < sp[i].nmb; mb++) h_items.push_back(DecideItem{i, mb});
     if (!h_items.empty()) {
       HIP_TRY(hipMalloc(&d_items, sizeof(DecideItem) * h_items.size()));
+      dev_bytes += sizeof(DecideItem) * h_items.size();
       HIP_TRY(hipMemcpy(d_items, h_items.data(), sizeof(DecideItem) * h_items.size(), hipMemcpyHostToDevice));
     }
     return ICU_OK;
   }
 
-  // (Re)write the descriptors when the caller's buffers change.
-  int bind(int set, const uint8_t* const* curs, const uint8_t* const* refs, int16_t* mv, int32_t* dist,
+  bool bound_to(const DescSet& D, const uint8_t* const* curs, const uint8_t* const* refs, int16_t* mv,
+                int32_t* dist, int32_t* integ) const {
+    bool same = D.valid && mv == D.mv && dist == D.dist && integ == D.integ;
+    for (size_t i = 0; same && i < sp.size(); i++) same = D.cur[i] == curs[i] && D.ref[i] == refs[i];
+    return same;
+  }
+
+  // Slot for an eager call on stream st: the one bound to these buffers, or
+  // the least recently used eager slot refilled (after its last launch) and
+  // uploaded on st ahead of the launches that read it.
+  int bind_eager(const uint8_t* const* curs, const uint8_t* const* refs, int16_t* mv, int32_t* dist,
+                 int32_t* integ, hipStream_t st, int* slot) {
+    int pick_slot = 0;
+    for (int k = 0; k < kEager; k++) {
+      if (bound_to(ds[k], curs, refs, mv, dist, integ)) {
+        ds[k].stamp = ++use_clock;
+        *slot = k;
+        return ICU_OK;
+      }
+      if (ds[k].stamp < ds[pick_slot].stamp) pick_slot = k;
+    }
+    DescSet& D = ds[pick_slot];
+    if (D.used) HIP_TRY(hipEventSynchronize(D.done));  // its launches and its last upload are done
+    int rc = fill(D, curs, refs, mv, dist, integ);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(D.d_block, D.h_block, desc_bytes, hipMemcpyHostToDevice, st));
+    // the mirror stays in use until the copy ran (the launches re-record it)
+    HIP_TRY(hipEventRecord(D.done, st));
+    D.used = true;
+    D.stamp = ++use_clock;
+    *slot = pick_slot;
+    return ICU_OK;
+  }
+
+  // The graph's slot, written synchronously before capture (replays read it).
+  int bind_graph(const uint8_t* const* curs, const uint8_t* const* refs, int16_t* mv, int32_t* dist) {
+    DescSet& D = ds[kGraph];
+    if (bound_to(D, curs, refs, mv, dist, nullptr)) return ICU_OK;
+    HIP_TRY(hipDeviceSynchronize());  // earlier replays may still read it
+    int rc = fill(D, curs, refs, mv, dist, nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(D.d_block, D.h_block, desc_bytes, hipMemcpyHostToDevice));
+    return ICU_OK;
+  }
+
+  // Write a slot's descriptors into its pinned host mirror.
+  int fill(DescSet& D, const uint8_t* const* curs, const uint8_t* const* refs, int16_t* mv, int32_t* dist,
            int32_t* integ) {
-    DescSet& D = ds[set];
     for (size_t i = 0; i < sp.size(); i++)
       if (((uintptr_t)curs[i] & 15) != 0)
         return fail(ICU_E_ARG, "device current planes must be 16-byte aligned");
-    bool same = D.valid && mv == D.mv && dist == D.dist && integ == D.integ;
-    for (size_t i = 0; same && i < sp.size(); i++) same = D.cur[i] == curs[i] && D.ref[i] == refs[i];
-    if (same) return ICU_OK;
-    // kernels of an earlier call may still read the old descriptors
-    if (D.valid) HIP_TRY(hipDeviceSynchronize());
-    h_streams.resize(sp.size());
-    h_decide.resize(sp.size());
-    std::vector<PadDesc> h_pad(sp.size());
+    const size_t o_dec = (char*)D.d_decide - (char*)D.d_block, o_pad = (char*)D.d_pad - (char*)D.d_block;
+    StreamDesc* h_streams = (StreamDesc*)D.h_block;
+    DecideDesc* h_decide = (DecideDesc*)((char*)D.h_block + o_dec);
+    PadDesc* h_pad = (PadDesc*)((char*)D.h_block + o_pad);
     for (size_t i = 0; i < sp.size(); i++) {
       const StreamPlan& s = sp[i];
       h_pad[i] = PadDesc{refs[i], s.refpad, s.cfg.width, s.cfg.height, s.cfg.width, s.ref_stride, pad_base[i]};
     }
-    HIP_TRY(hipMemcpy(D.d_pad, h_pad.data(), sizeof(PadDesc) * sp.size(), hipMemcpyHostToDevice));
     for (size_t i = 0; i < sp.size(); i++) {
       const StreamPlan& s = sp[i];
       StreamDesc& d = h_streams[i];
@@ -361,8 +428,6 @@ struct Engine {
       e.progress = d_wave + wave_off[i];
       e.ticket = d_wave + wave_off[i] + s.mbh;
     }
-    HIP_TRY(hipMemcpy(D.d_streams, h_streams.data(), sizeof(StreamDesc) * sp.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(D.d_decide, h_decide.data(), sizeof(DecideDesc) * sp.size(), hipMemcpyHostToDevice));
     for (size_t i = 0; i < sp.size(); i++) {
       D.cur[i] = curs[i];
       D.ref[i] = refs[i];
@@ -404,6 +469,10 @@ struct Engine {
     }
     if (ev) HIP_TRY(hipEventRecord(ev[3], st));
     HIP_TRY(hipGetLastError());
+    if (set != kGraph) {
+      HIP_TRY(hipEventRecord(ds[set].done, st));
+      ds[set].used = true;
+    }
     return ICU_OK;
   }
 
@@ -424,8 +493,7 @@ struct Engine {
     // kernels of an earlier call may still read the task list
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(d_tasks, h_tasks.data(), sizeof(SearchTask) * h_tasks.size(), hipMemcpyHostToDevice));
-    ds[0].valid = false;
-    ds[1].valid = false;
+    for (auto& d : ds) d.valid = false;
     return ICU_OK;
   }
 };
@@ -452,6 +520,11 @@ struct icu_me {
 struct icu_me_batch {
   Engine eng;
 };
+
+long long icu::me_device_bytes(const icu_me* h) {
+  const long long px = (long long)h->eng.sp[0].cfg.width * h->eng.sp[0].cfg.height;
+  return h->eng.dev_bytes + 2 * px + 8 * (px / 256);
+}
 
 static hipStream_t pick(void* stream, Engine& e) { return stream ? (hipStream_t)stream : e.own; }
 
@@ -515,9 +588,11 @@ int icu_me_estimate_device(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref
   Engine& e = h->eng;
   const uint8_t* curs[1] = {d_cur};
   const uint8_t* refs[1] = {d_ref};
-  int rc = e.bind(0, curs, refs, d_mv, d_dist, d_int);
+  const hipStream_t st = pick(stream, e);
+  int slot = 0;
+  int rc = e.bind_eager(curs, refs, d_mv, d_dist, d_int, st, &slot);
   if (rc) return rc;
-  return e.enqueue(0, pick(stream, e));
+  return e.enqueue(slot, st);
 }
 
 int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int16_t* d_mv,
@@ -527,12 +602,13 @@ int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref,
   Engine& e = h->eng;
   const uint8_t* curs[1] = {d_cur};
   const uint8_t* refs[1] = {d_ref};
-  int rc = e.bind(0, curs, refs, d_mv, d_dist, nullptr);
+  const hipStream_t st = pick(stream, e);
+  int slot = 0;
+  int rc = e.bind_eager(curs, refs, d_mv, d_dist, nullptr, st, &slot);
   if (rc) return rc;
   hipEvent_t ev[4];
   for (auto& x : ev) HIP_TRY(hipEventCreate(&x));
-  const hipStream_t st = pick(stream, e);
-  rc = e.enqueue(0, st, ev, search_reps);
+  rc = e.enqueue(slot, st, ev, search_reps);
   if (!rc) {
     HIP_TRY(hipEventSynchronize(ev[3]));
     for (int i = 0; i < 3; i++) HIP_TRY(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
@@ -777,23 +853,25 @@ int icu_me_batch_mb_base(icu_me_batch* b, int i) {
 int icu_me_batch_estimate_device(icu_me_batch* b, const uint8_t* const* curs, const uint8_t* const* refs,
                                  int16_t* d_mv, int32_t* d_dist, void* stream) {
   if (!b || !curs || !refs || !d_mv || !d_dist) return fail(ICU_E_ARG, "null argument");
-  int rc = b->eng.bind(0, curs, refs, d_mv, d_dist, nullptr);
+  const hipStream_t st = pick(stream, b->eng);
+  int slot = 0;
+  int rc = b->eng.bind_eager(curs, refs, d_mv, d_dist, nullptr, st, &slot);
   if (rc) return rc;
-  return b->eng.enqueue(0, pick(stream, b->eng));
+  return b->eng.enqueue(slot, st);
 }
 
 int icu_me_batch_capture(icu_me_batch* b, const uint8_t* const* curs, const uint8_t* const* refs,
                          int16_t* d_mv, int32_t* d_dist) {
   if (!b || !curs || !refs || !d_mv || !d_dist) return fail(ICU_E_ARG, "null argument");
   Engine& e = b->eng;
-  int rc = e.bind(1, curs, refs, d_mv, d_dist, nullptr);
+  int rc = e.bind_graph(curs, refs, d_mv, d_dist);
   if (rc) return rc;
   if (e.gexec) hipGraphExecDestroy(e.gexec);
   if (e.graph) hipGraphDestroy(e.graph);
   e.gexec = nullptr;
   e.graph = nullptr;
   HIP_TRY(hipStreamBeginCapture(e.own, hipStreamCaptureModeThreadLocal));
-  rc = e.enqueue(1, e.own);
+  rc = e.enqueue(Engine::kGraph, e.own);
   hipGraph_t g = nullptr;
   hipError_t ce = hipStreamEndCapture(e.own, &g);
   if (rc) {

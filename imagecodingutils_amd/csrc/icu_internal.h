@@ -3,9 +3,19 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "icu_device.h"
 
+struct icu_me;
+
 namespace icu {
+
+// Device bytes an estimator handle holds.
+long long me_device_bytes(const ::icu_me* h);
+
+// Records msg as the calling thread's last error and returns code.
+int icu_fail(int code, const std::string& msg);
 
 // Per-stream parameters of the decision / sub-pel stage.
 struct DecideDesc {
@@ -59,6 +69,8 @@ void launch_decide_parallel(const DecideDesc* d_descs, const DecideItem* d_items
 void launch_decide_wavefront(const DecideDesc* d_desc, int* d_progress_and_ticket, int mbh,
                              hipStream_t st);
 void launch_fill_i32(int* p, int v, int n, hipStream_t st);
+// Adds an order-free checksum of `bytes` bytes at p (4-byte aligned) to *out.
+void launch_checksum(const void* p, long bytes, unsigned long long* out, hipStream_t st);
 
 // Transform / coding launchers (it_kernels.hip).
 void launch_fwd4x4(const int16_t* in, int16_t* out, long n, int qp, int intra, int mode,

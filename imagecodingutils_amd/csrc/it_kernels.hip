@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "icu_device.h"
 #include "icu_internal.h"
 #include "icu_qpel.h"
@@ -310,6 +312,44 @@ void launch_inv4x4(const int16_t* in, int16_t* out, long n, int qp, int mode, in
   if (n <= 0) return;
   hipLaunchKernelGGL(inv_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, out, n, qp,
                      mode, layout, cols_first, d_weights);
+}
+
+namespace {
+// Order-free checksum of a byte array: the wrapping 64-bit sum over i of
+// (p[i] + 1) * (odd 32-bit hash of i), so it does not depend on which thread
+// adds which part, and a changed or moved byte changes it.
+__global__ __launch_bounds__(256) void checksum_kernel(const uint8_t* __restrict__ p, long n,
+                                                       unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long part[4];
+  unsigned long long s = 0;
+  const long stride = (long)gridDim.x * blockDim.x;
+  const long ndw = n >> 2;  // whole dwords (p is 4-byte aligned)
+  const uint32_t* pw = (const uint32_t*)p;
+  for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < ndw; k += stride) {
+    const uint32_t w = pw[k];
+    const uint32_t i0 = (uint32_t)k * 4u;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      s += (unsigned long long)(((w >> (8 * j)) & 255u) + 1u) * (((i0 + j) * 0x9E3779B1u) | 1u);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+    const long i = (ndw << 2) + threadIdx.x;
+    s += (unsigned long long)(p[i] + 1u) * (((uint32_t)i * 0x9E3779B1u) | 1u);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
+}
+}  // namespace
+
+void launch_checksum(const void* p, long bytes, unsigned long long* out, hipStream_t st) {
+  // one atomic per workgroup: few workgroups, each thread over many dwords
+  const long blocks = std::min<long>(128, (bytes / 4 + 255) / 256);
+  if (bytes <= 0) return;
+  hipLaunchKernelGGL(checksum_kernel, dim3((unsigned)std::max<long>(1, blocks)), dim3(256), 0, st,
+                     (const uint8_t*)p, bytes, out);
 }
 
 void launch_code_frame(const uint8_t* cur, int cur_stride, const uint8_t* ref_padded, int ref_stride,

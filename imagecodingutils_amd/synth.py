@@ -127,3 +127,40 @@ def shifted_copies(w: int, h: int, seed: int, n: int, max_mv: int = 16,
         refs.append(_to_u8(tex.eval(xs - dx, ys - dy) + rng.normal(0.0, s, (h, w))))
         mvs.append((dx, dy))
     return cur, refs, mvs
+
+
+class StreamSource:
+    """Frames of one video stream: a seeded texture under a drifting
+    piecewise-constant motion field.  The background pans by ``pan`` pels per
+    frame; each of ``n_rects`` rectangles moves by its own integer velocity in
+    [-max_mv, max_mv]^2 per frame, carrying its content with it (so the true
+    motion of a block inside rectangle k between frames t-1 and t is minus
+    that velocity).  Frame t adds fresh sensor noise N(0, sigma) seeded by
+    (seed, t), so ``frame(t)`` is the same on every call."""
+
+    def __init__(self, w: int, h: int, seed: int, n_rects: int = 12, max_mv: int = 8,
+                 pan: tuple[int, int] | None = None, sigma: float = 3.0):
+        rng = np.random.default_rng(seed)
+        self.w, self.h, self.seed, self.sigma = w, h, seed, sigma
+        self.tex = Texture.make(rng)
+        self.pan = pan if pan is not None else (int(rng.integers(-2, 3)), int(rng.integers(-2, 3)))
+        self.rects = []
+        for _ in range(n_rects):
+            rw = int(rng.integers(w // 8, w // 3 + 1))
+            rh = int(rng.integers(h // 8, h // 3 + 1))
+            self.rects.append((rw, rh, int(rng.integers(0, w - rw + 1)), int(rng.integers(0, h - rh + 1)),
+                               int(rng.integers(-max_mv, max_mv + 1)), int(rng.integers(-max_mv, max_mv + 1))))
+
+    def frame(self, t: int) -> np.ndarray:
+        w, h = self.w, self.h
+        ys, xs = np.mgrid[0:h, 0:w]
+        dx = np.full((h, w), t * self.pan[0], dtype=np.int64)
+        dy = np.full((h, w), t * self.pan[1], dtype=np.int64)
+        for rw, rh, x0, y0, vx, vy in self.rects:
+            # the rectangle's position wraps inside the picture
+            px = (x0 + t * vx) % (w - rw + 1)
+            py = (y0 + t * vy) % (h - rh + 1)
+            dx[py:py + rh, px:px + rw] = t * vx
+            dy[py:py + rh, px:px + rw] = t * vy
+        noise = np.random.default_rng([self.seed, t]).normal(0.0, self.sigma, (h, w))
+        return _to_u8(self.tex.eval(xs - dx, ys - dy) + noise)

@@ -166,6 +166,60 @@ int icu_me_batch_capture(icu_me_batch* b, const uint8_t* const* d_curs, const ui
                          int16_t* d_mv, int32_t* d_dist);
 int icu_me_batch_graph_launch(icu_me_batch* b, void* stream);
 
+/* ---------------------------------------------------------------- serving
+ * A request loop over many streams with continuous batching.  No reference
+ * counterpart: the reference estimates and codes one frame per call on the
+ * caller's thread (MotionEstimatorH264ImplFull::Estimate,
+ * src/CodecUtils/MotionEstimatorH264ImplFull.cpp:330); the server runs the
+ * same per-frame work (icu_me_estimate_device, then icu_code_frame_device at
+ * the server's QP, inter, rows-first inverse) for many streams at once.
+ *
+ * Each stream keeps its reference frame (the previous frame's
+ * reconstruction) resident in device memory; the first frame of a stream is
+ * its own reference.  A submitted frame waits until its stream's previous
+ * frame is done, fewer than max_inflight frames are on the device and its
+ * transient buffers fit mem_budget; admitted frames run on `lanes` HIP
+ * streams, so work of different streams overlaps and frames that arrive
+ * while others run start as slots and memory free up.  Results depend only
+ * on the stream's own frames, never on what else runs. */
+typedef struct icu_server icu_server;
+typedef struct {
+  int lanes;           /* HIP streams work is spread over, 1..16 */
+  int max_inflight;    /* frames on the device at once, >= 1 */
+  long long mem_budget;/* device bytes for resident streams + frames in flight */
+  int qp;              /* residual coding QP, 0..51 */
+} icu_server_config;
+typedef struct {
+  long long tag;       /* the caller's tag from icu_server_submit */
+  int stream, frame;   /* stream id, frame index within the stream */
+  int nmb;             /* macroblocks (mv holds 2*nmb values) */
+  double submit_s, done_s; /* icu_server_clock() at submit / at completion seen */
+  unsigned long long levels_sum, recon_sum; /* checksums of the levels (int16
+                        W*H, as bytes) and reconstruction (8-bit W*H): the sum
+                        over bytes b[i] of (b[i] + 1) * ((i * 0x9E3779B1 mod 2^32) | 1),
+                        mod 2^64 */
+  const int16_t* mv;   /* quarter-pel vectors, valid until the next poll */
+} icu_server_result;
+
+int icu_server_create(const icu_server_config* cfg, icu_server** out);
+/* Waits for frames in flight; queued frames are dropped. */
+int icu_server_destroy(icu_server* s);
+/* Opens a stream with estimator configuration cfg.  ICU_E_RANGE when its
+ * resident state does not fit the remaining budget. */
+int icu_server_open_stream(icu_server* s, const icu_me_config* cfg, int* stream_id);
+/* Queues one 8-bit width*height luma frame (copied before return) and
+ * admits what can start. */
+int icu_server_submit(icu_server* s, int stream_id, const uint8_t* frame, long long tag);
+/* Collects up to max_out finished frames (any order across streams, frame
+ * order within a stream) and admits queued ones.  wait != 0: blocks until at
+ * least one frame finishes when any is in flight.  Returns the number
+ * collected (>= 0) or an error code. */
+int icu_server_poll(icu_server* s, icu_server_result* out, int max_out, int wait);
+int icu_server_stats(icu_server* s, long long* bytes_in_use, long long* peak_bytes, int* queued,
+                     int* inflight);
+/* Seconds on the server's monotonic clock. */
+double icu_server_clock(void);
+
 /* Errors and a minimal device runtime for callers without their own. */
 const char* icu_last_error(void);
 int icu_device_info(char* name, int name_len, int* compute_units, int* clock_khz);

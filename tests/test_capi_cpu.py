@@ -62,3 +62,26 @@ def test_batch_rejects_predictor_and_mixed_metrics():
     c = capi.make_config(64, 48, 8, metric=0, filt=0, predictor=0)
     with pytest.raises(capi.IcuError):
         capi.MotionEstimatorBatch([b, c])
+
+
+@pytest.mark.parametrize("bad", [dict(lanes=0), dict(lanes=17), dict(max_inflight=0), dict(mem_budget=0),
+                                 dict(qp=-1), dict(qp=52)])
+def test_server_rejects_bad_config(bad):
+    kw = dict(lanes=4, max_inflight=8, mem_budget=1 << 30, qp=28)
+    kw.update(bad)
+    with pytest.raises(capi.IcuError) as e:
+        capi.Server(**kw)
+    assert e.value.code == capi.ICU_E_ARG
+
+
+def test_server_checksum_definition():
+    # sum over bytes of (b + 1) * ((i * 0x9E3779B1 mod 2^32) | 1), mod 2^64
+    a = np.array([0, 255, 7], np.uint8)
+    w = [1, 0x9E3779B1, (2 * 0x9E3779B1) % 2**32 | 1]
+    assert capi.server_checksum(a) == (1 * w[0] + 256 * w[1] + 8 * w[2]) % 2**64
+    b = np.array([-1, 2], np.int16)  # bytes ff ff 02 00
+    wb = [(i * 0x9E3779B1) % 2**32 | 1 for i in range(4)]
+    assert capi.server_checksum(b) == (256 * wb[0] + 256 * wb[1] + 3 * wb[2] + 1 * wb[3]) % 2**64
+    big = np.full(1 << 16, 255, np.uint8)
+    ref = sum(256 * ((i * 0x9E3779B1) % 2**32 | 1) for i in range(1 << 16)) % 2**64
+    assert capi.server_checksum(big) == ref

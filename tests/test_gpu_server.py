@@ -1,0 +1,184 @@
+"""Serving loop (icu_server_*): every frame's result is the same alone, in a
+mix of streams with overlapping work, under a tight memory budget, and on
+the plain per-frame path; the first coded frames also match the oracle.
+Integer work, so every comparison is exact."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+capi = pytest.importorskip("imagecodingutils_amd.capi")
+from imagecodingutils_amd import synth  # noqa: E402
+
+QP = 28
+SERVE = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=1)
+
+
+def _streams():
+    # (W, H, R, modes); the last stream runs the shipped configuration
+    # (SSD, running-best filter, median predictor, quarter pel)
+    return [
+        (176, 144, 8, SERVE),
+        (352, 288, 16, SERVE),
+        (352, 288, 32, SERVE),
+        (176, 144, 16, dict(metric=0, window=0, filt=1, subpel=2, predictor=1, best_vector=1)),
+    ]
+
+
+def _cfg(w, h, R, m):
+    return capi.make_config(w, h, R, **m)
+
+
+def _frames(n_frames):
+    return [[synth.StreamSource(w, h, 900 + i, n_rects=6, max_mv=min(R, 6)).frame(t) for t in range(n_frames)]
+            for i, (w, h, R, _) in enumerate(_streams())]
+
+
+def _key(r):
+    return (r["stream"], r["frame"])
+
+
+def _drain(srv, expect):
+    out = {}
+    while len(out) < expect:
+        for r in srv.poll(wait=True):
+            assert _key(r) not in out
+            out[_key(r)] = r
+    return out
+
+
+def _run_alone(frames):
+    """Each stream on a server of its own, one frame at a time."""
+    out = {}
+    for i, (w, h, R, m) in enumerate(_streams()):
+        srv = capi.Server(lanes=1, max_inflight=1, qp=QP)
+        sid = srv.open_stream(_cfg(w, h, R, m))
+        for t, f in enumerate(frames[i]):
+            srv.submit(sid, f, tag=t)
+            (r,) = _drain(srv, 1).values()
+            out[(i, r["frame"])] = r
+        srv.close()
+    return out
+
+
+def _run_mix(frames, **kw):
+    srv = capi.Server(qp=QP, **kw)
+    ids = [srv.open_stream(_cfg(w, h, R, m)) for (w, h, R, m) in _streams()]
+    n = sum(len(f) for f in frames)
+    peak_inflight = 0
+    got = {}
+    # all frames submitted up front, interleaved across streams; frames of
+    # one stream wait for their predecessor, the others overlap
+    for t in range(len(frames[0])):
+        for i, sid in enumerate(ids):
+            srv.submit(sid, frames[i][t], tag=1000 * i + t)
+            peak_inflight = max(peak_inflight, srv.stats()["inflight"])
+    while len(got) < n:
+        for r in srv.poll(wait=True):
+            got[_key(r)] = r
+        peak_inflight = max(peak_inflight, srv.stats()["inflight"])
+    st = srv.stats()
+    srv.close()
+    return {(ids.index(s), f): r for (s, f), r in got.items()}, peak_inflight, st
+
+
+def _assert_same(a, b, what):
+    assert a.keys() == b.keys()
+    for k in a:
+        np.testing.assert_array_equal(a[k]["mv"], b[k]["mv"], err_msg=f"{what} {k}")
+        assert a[k]["levels_sum"] == b[k]["levels_sum"], f"{what} {k}"
+        assert a[k]["recon_sum"] == b[k]["recon_sum"], f"{what} {k}"
+
+
+def test_server_result_is_the_same_alone_and_in_the_mix():
+    frames = _frames(4)
+    alone = _run_alone(frames)
+    mix, peak, st = _run_mix(frames, lanes=4, max_inflight=8)
+    assert peak > 1, "frames of different streams never overlapped"
+    _assert_same(alone, mix, "mix")
+    assert st["bytes_in_use"] > 0 and st["queued"] == 0 and st["inflight"] == 0
+    # frame order within each stream, tags carried through
+    for (i, t), r in mix.items():
+        assert r["tag"] == 1000 * i + t
+        assert r["done_s"] >= r["submit_s"]
+
+
+def test_server_matches_plain_path_and_oracle():
+    """Per stream: estimate + code each frame with the plain entry points,
+    the previous reconstruction as reference (the first frame is its own),
+    and compare vectors and checksums; the first two frames of the small
+    streams also against the oracle."""
+    frames = _frames(3)
+    mix, _, _ = _run_mix(frames, lanes=2, max_inflight=3)
+    for i, (w, h, R, m) in enumerate(_streams()):
+        est = capi.MotionEstimator(_cfg(w, h, R, m))
+        n = est.nmb
+        d_ref = capi.DeviceBuffer.from_array(frames[i][0])
+        d_mv, d_dist = capi.DeviceBuffer(4 * n), capi.DeviceBuffer(4 * n)
+        d_lv, d_rc = capi.DeviceBuffer(2 * w * h), capi.DeviceBuffer(w * h)
+        ref_host = frames[i][0]
+        for t, f in enumerate(frames[i]):
+            d_cur = capi.DeviceBuffer.from_array(f)
+            est.estimate_device(d_cur, d_cur if t == 0 else d_ref, d_mv, d_dist)
+            est.code_frame_device(d_cur, d_mv, QP, 0, 0, d_lv, d_rc)
+            capi.lib.icu_stream_sync(None)
+            mv = d_mv.download(np.int16, (n, 2))
+            lv = d_lv.download(np.int16, (h, w))
+            rc = d_rc.download(np.uint8, (h, w))
+            r = mix[(i, t)]
+            np.testing.assert_array_equal(r["mv"], mv, err_msg=f"stream {i} frame {t}")
+            assert r["levels_sum"] == capi.server_checksum(lv)
+            assert r["recon_sum"] == capi.server_checksum(rc)
+            if t < 2 and w <= 176:
+                omv, _, _, _ = orc.me_estimate(f, ref_host, R, metric=m["metric"], window=m["window"],
+                                               filter=m["filt"], subpel=m["subpel"], pred=m["predictor"],
+                                               bestvec=m["best_vector"])
+                np.testing.assert_array_equal(mv, omv)
+                pred = orc.mc_luma(ref_host, omv)
+                olv, orc_rc = orc.code_residual(f, pred, QP, 0, 0)
+                np.testing.assert_array_equal(lv, olv)
+                np.testing.assert_array_equal(rc.astype(np.int16), orc_rc)
+            ref_host = rc
+            d_ref = d_rc
+            d_rc = capi.DeviceBuffer(w * h)
+
+
+def test_server_admits_by_memory_budget():
+    frames = _frames(3)
+    alone = _run_alone(frames)
+    # find the resident size of the streams, then allow them plus the
+    # transient buffers of about one frame in flight
+    probe = capi.Server(qp=QP)
+    for (w, h, R, m) in _streams():
+        probe.open_stream(_cfg(w, h, R, m))
+    resident = probe.stats()["bytes_in_use"]
+    probe.close()
+    budget = resident + 352 * 288 + 8 * 396  # one CIF frame, or two QCIF
+    mix, peak, st = _run_mix(frames, lanes=4, max_inflight=8, mem_budget=budget)
+    assert peak <= 2
+    assert st["peak_bytes"] <= budget
+    _assert_same(alone, mix, "budget")
+    # a stream beyond the budget is refused without disturbing the others
+    srv = capi.Server(qp=QP, mem_budget=resident)
+    for (w, h, R, m) in _streams():
+        srv.open_stream(_cfg(w, h, R, m))
+    with pytest.raises(capi.IcuError) as e:
+        srv.open_stream(_cfg(176, 144, 8, SERVE))
+    assert e.value.code == capi.ICU_E_RANGE
+    assert srv.stats()["bytes_in_use"] == resident
+    # no room for any frame: refused at submission rather than queued forever
+    with pytest.raises(capi.IcuError) as e:
+        srv.submit(0, frames[0][0])
+    assert e.value.code == capi.ICU_E_RANGE
+    srv.close()
+
+
+def test_server_rejects_bad_requests():
+    srv = capi.Server(qp=QP)
+    with pytest.raises(capi.IcuError) as e:
+        srv.submit(3, np.zeros((144, 176), np.uint8))
+    assert e.value.code == capi.ICU_E_ARG
+    assert srv.poll() == []
+    srv.close()
