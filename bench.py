@@ -149,6 +149,79 @@ def run_batched(capi, synth, steps, warmup):
                 sad_ops=ops, streams=len(cfgs))
 
 
+VBS_W, VBS_H, VBS_R, VBS_REFS, VBS_LAMBDA = 3840, 2160, 16, 4, 4
+VBS_QPS = tuple(range(22, 41))
+
+
+def vbs_sad_ops(W, H, R, nrefs):
+    """Candidates in each MB's clamped +-R window x references x 256 pels."""
+    tot = 0
+    for m in range(0, H, 16):
+        yu = -R if m - R >= -16 else -(m + 16)
+        yd = R if m + R < H else H - m
+        for n in range(0, W, 16):
+            xl = -R if n - R >= -16 else -(n + 16)
+            xr = R if n + R < W else W - n
+            tot += (xr - xl + 1) * (yd - yu + 1)
+    return tot * nrefs * 256
+
+
+def run_multiref_4k(capi, synth, steps, warmup):
+    """4K, 4 references (shifted, noised copies of one texture), +-16
+    exhaustive search with 16x16/16x8/8x16/8x8 partitions from 4x4 SADs and
+    the rate term 4 (|vx| + |vy|), then the residual coded at QP 22..40."""
+    W, H = VBS_W, VBS_H
+    cur, refs, _ = synth.shifted_copies(W, H, 4096, VBS_REFS, max_mv=16)
+    est = capi.VbsEstimator(W, H, VBS_R, VBS_REFS, VBS_LAMBDA)
+    d_cur = capi.DeviceBuffer.from_array(cur)
+    d_refs = [capi.DeviceBuffer.from_array(r) for r in refs]
+    d_mode, d_cost = capi.DeviceBuffer(est.nmb), capi.DeviceBuffer(4 * est.nmb)
+    d_mv8, d_ref8 = capi.DeviceBuffer(4 * est.nb8), capi.DeviceBuffer(est.nb8)
+    d_lv = [capi.DeviceBuffer(2 * W * H) for _ in VBS_QPS]
+    d_rc = [capi.DeviceBuffer(W * H) for _ in VBS_QPS]
+    s = capi.Stream()
+
+    def step():
+        est.estimate_device(d_cur, d_refs, d_mode, d_mv8, d_ref8, d_cost, s)
+        est.code_device(d_cur, d_mv8, d_ref8, list(VBS_QPS), 0, 0, d_lv, d_rc, None, s)
+
+    for _ in range(warmup):
+        step()
+    s.sync()
+    ev = [capi.Event() for _ in range(3)]
+    ev[0].record(s)
+    for _ in range(steps):
+        step()
+    ev[1].record(s)
+    s.sync()
+    ms = ev[0].elapsed_ms(ev[1]) / steps
+    # the two stages apart: back-to-back launches of each
+    ev[0].record(s)
+    for _ in range(steps):
+        est.estimate_device(d_cur, d_refs, d_mode, d_mv8, d_ref8, d_cost, s)
+    ev[1].record(s)
+    for _ in range(steps):
+        est.code_device(d_cur, d_mv8, d_ref8, list(VBS_QPS), 0, 0, d_lv, d_rc, None, s)
+    ev[2].record(s)
+    s.sync()
+    est_ms = ev[0].elapsed_ms(ev[1]) / steps
+    code_ms = ev[1].elapsed_ms(ev[2]) / steps
+    mode = d_mode.download(np.uint8, est.nmb)
+    ops = vbs_sad_ops(W, H, VBS_R, VBS_REFS)
+    # coder traffic: current + prediction source read once, levels (2 B) and
+    # reconstruction (1 B) written per pel per QP, vectors and indices
+    code_bytes = W * H * (2 + 3 * len(VBS_QPS)) + est.nb8 * 5
+    return {"workload": f"4K {W}x{H}, {VBS_REFS} references (shifted copies, sigma 2..6), "
+                        f"+-{VBS_R} exhaustive, 16x16/16x8/8x16/8x8 from 4x4 SADs, rate {VBS_LAMBDA}(|vx|+|vy|), "
+                        f"residual at QP 22..40 ({len(VBS_QPS)} QPs)",
+            "mbs_per_step": est.nmb, "ms_per_step": round(ms, 4), "value": round(est.nmb / (ms / 1e3), 1),
+            "unit": "MB/s", "estimate_ms": round(est_ms, 4), "code_ms": round(code_ms, 4),
+            "search_T_sad_ops": round(ops / (est_ms / 1e3) / 1e12, 2), "sad_ops_per_frame": ops,
+            "code_GBps": round(code_bytes / (code_ms / 1e3) / 1e9, 1), "code_bytes": code_bytes,
+            "tile_rows": est.plan()[0],
+            "modes": {k: int((mode == i).sum()) for i, k in enumerate(("16x16", "16x8", "8x16", "8x8"))}}
+
+
 SERVE_STREAMS, SERVE_RATE, SERVE_QP, SERVE_BUDGET = 32, 2000.0, 28, 64 << 30
 SERVE_RING = 3  # distinct frames generated per stream, submitted in turn
 
@@ -277,6 +350,7 @@ def main():
     ap.add_argument("--tune", action="store_true",
                     help="run the tile tuner first (the planner's tiling, 13 rows x 3 MBs per task, one task "
                          "per workgroup, is what it picks for this workload)")
+    ap.add_argument("--no-4k", action="store_true", help="skip the 4K multi-reference leg")
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="length of the serving leg's Poisson arrival run (0: skip the leg)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -342,6 +416,8 @@ def main():
                           "value": round(bt["mbps"], 1), "plain_same_streams_mbps": round(bt["plain_mbps"], 1),
                           "batched_vs_plain": round(bt["mbps"] / bt["plain_mbps"], 3),
                           "achieved_T_sad_ops": round(bt["sad_ops"] / (bt["ms"] / 1e3) / 1e12, 2)}
+    if not args.no_4k:
+        out["multiref_4k"] = run_multiref_4k(capi, synth, max(5, args.steps // 5), 2)
     if args.serve_seconds > 0:
         out["serving"] = run_serving(capi, synth, args.serve_seconds)
     if not args.no_cpu:

@@ -42,6 +42,10 @@ class ServerResult(C.Structure):
                 ("mv", C.POINTER(C.c_int16))]
 
 
+class VbsConfig(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("width", "height", "range", "nrefs", "lambda_")]
+
+
 _vp = C.c_void_p
 _SIGS = {
     "icu_me_config_reference": (C.c_int, [C.POINTER(MeConfig), C.c_int, C.c_int, C.c_int]),
@@ -74,6 +78,12 @@ _SIGS = {
     "icu_server_stats": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_server_clock": (C.c_double, []),
+    "icu_vbs_create": (C.c_int, [C.POINTER(VbsConfig), C.POINTER(_vp)]),
+    "icu_vbs_destroy": (C.c_int, [_vp]),
+    "icu_vbs_estimate_device": (C.c_int, [_vp, _vp, C.POINTER(_vp), _vp, _vp, _vp, _vp, _vp]),
+    "icu_vbs_code_device": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int,
+                                      C.POINTER(_vp), C.POINTER(_vp), _vp, _vp]),
+    "icu_vbs_get_plan": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_last_error": (C.c_char_p, []),
     "icu_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_malloc": (_vp, [C.c_size_t]),
@@ -318,6 +328,47 @@ class MotionEstimatorBatch:
     def close(self):
         if getattr(self, "h", None):
             lib.icu_me_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+# ---------------------------------------- multi-reference, partitioned search
+class VbsEstimator:
+    """Multi-reference variable-block-size search and its residual coder
+    (icu_vbs_*).  Device buffers in, device buffers out."""
+
+    def __init__(self, width: int, height: int, R: int, nrefs: int, lam: int = 4):
+        cfg = VbsConfig(width, height, R, nrefs, lam)
+        h = _vp()
+        _check(lib.icu_vbs_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.W, self.H, self.R, self.nrefs, self.lam = width, height, R, nrefs, lam
+        self.nmb = (width // 16) * (height // 16)
+        self.nb8 = (width // 8) * (height // 8)
+
+    def plan(self) -> tuple[int, int]:
+        v, lds = C.c_int(), C.c_int()
+        _check(lib.icu_vbs_get_plan(self.h, C.byref(v), C.byref(lds)))
+        return v.value, lds.value
+
+    def estimate_device(self, d_cur, d_refs, d_mode, d_mv8, d_ref8, d_cost, stream: Stream | None = None):
+        refs = (_vp * len(d_refs))(*[_p(r) for r in d_refs])
+        _check(lib.icu_vbs_estimate_device(self.h, _p(d_cur), refs, _p(d_mode), _p(d_mv8), _p(d_ref8),
+                                           _p(d_cost), stream.handle if stream else None))
+
+    def code_device(self, d_cur, d_mv8, d_ref8, qps, intra, cols_first, d_levels, d_recons, d_pred=None,
+                    stream: Stream | None = None):
+        n = len(qps)
+        _check(lib.icu_vbs_code_device(self.h, _p(d_cur), _p(d_mv8), _p(d_ref8), n, (C.c_int * n)(*qps),
+                                       intra, cols_first, (_vp * n)(*[_p(x) for x in d_levels]),
+                                       (_vp * n)(*[_p(x) for x in d_recons]), _p(d_pred),
+                                       stream.handle if stream else None))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.icu_vbs_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -729,7 +729,7 @@ int icu_code_frame_qps_device(icu_me* h, const uint8_t* d_cur, const int16_t* d_
                               const int* qps, int intra, int cols_first, int16_t* const* d_levels,
                               uint8_t* const* d_recon, int16_t* d_pred, void* stream) {
   if (!h || !d_cur || !d_mv || !qps || !d_levels || !d_recon) return fail(ICU_E_ARG, "null argument");
-  if (nqp < 1 || nqp > kMaxQpSet) return fail(ICU_E_ARG, "nqp must be 1..8");
+  if (nqp < 1 || nqp > kMaxQpSet) return fail(ICU_E_ARG, "nqp must be 1..20");
   QpSet qs{};
   qs.n = nqp;
   // the coder reads and writes whole rows of 4 samples

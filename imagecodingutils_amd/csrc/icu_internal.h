@@ -72,13 +72,38 @@ void launch_fill_i32(int* p, int v, int n, hipStream_t st);
 // Adds an order-free checksum of `bytes` bytes at p (4-byte aligned) to *out.
 void launch_checksum(const void* p, long bytes, unsigned long long* out, hipStream_t st);
 
+// Multi-reference variable-block-size search (me_vbs.hip).
+constexpr int kVbsMaxRefs = 4;
+struct VbsParams {
+  const uint8_t* cur;                  // W x H, stride W
+  const uint8_t* ref[kVbsMaxRefs];     // padded planes at pixel (0,0)
+  int ref_stride;
+  int W, H, mbw, nmb;
+  int R, N, nrefs, lambda;
+  int dx0, Q, shift, V, Gr, rows, row_dw;  // tile plan (vbs_plan)
+  uint8_t* mode;                       // per MB
+  int16_t* mv8;                        // 2 per 8x8 block, quarter pel
+  uint8_t* ref8;                       // per 8x8 block
+  int32_t* cost;                       // per MB
+};
+// Fills the tile plan fields of p for range R and nrefs references.
+void vbs_plan(int R, int nrefs, VbsParams* p);
+int vbs_lds_bytes(const VbsParams& p);
+void launch_vbs_search(const VbsParams& p, hipStream_t st);
+// Pads up to kVbsMaxRefs planes in one launch, descriptors passed by value.
+struct PadSet {
+  PadDesc d[kVbsMaxRefs];
+  int n;
+};
+void launch_pad_set(const PadSet& s, int total_wgs, hipStream_t st);
+
 // Transform / coding launchers (it_kernels.hip).
 void launch_fwd4x4(const int16_t* in, int16_t* out, long n, int qp, int intra, int mode,
                    int layout, hipStream_t st);
 void launch_inv4x4(const int16_t* in, int16_t* out, long n, int qp, int mode, int layout,
                    int cols_first, const int* d_weights, hipStream_t st);
 // QPs a frame's residual is coded at in one pass, with an output pair each.
-constexpr int kMaxQpSet = 8;
+constexpr int kMaxQpSet = 20;
 struct QpSet {
   int n;
   int qp[kMaxQpSet];
@@ -88,6 +113,16 @@ struct QpSet {
 void launch_code_frame(const uint8_t* cur, int cur_stride, const uint8_t* ref_padded,
                        int ref_stride, int W, int H, const int16_t* mv, const QpSet& qs, int intra,
                        int cols_first, int16_t* pred_out, hipStream_t st);
+// Residual coding of a frame predicted per 8x8 block from one of up to
+// kVbsMaxRefs padded references at an integer vector (mv8 in quarter pel,
+// multiples of 4; ref8 the reference index).
+struct VbsRefs {
+  const uint8_t* ref[kVbsMaxRefs];
+  int stride;
+};
+void launch_code_vbs(const uint8_t* cur, const VbsRefs& refs, int W, int H, const int16_t* mv8,
+                     const uint8_t* ref8, const QpSet& qs, int intra, int cols_first,
+                     int16_t* pred_out, hipStream_t st);
 void launch_sqrt_check(const int* m, double* out, int n, hipStream_t st);
 
 }  // namespace icu

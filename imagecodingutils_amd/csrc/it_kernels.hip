@@ -315,6 +315,82 @@ void launch_inv4x4(const int16_t* in, int16_t* out, long n, int qp, int mode, in
 }
 
 namespace {
+// Residual coding of a frame predicted per 8x8 block at an integer vector
+// from one of several padded references (the multi-reference search's
+// output): one lane per 4x4 block.  The prediction is a copy
+// (MotionCompensatorH264ImplStd::Compensate at a full-pel vector,
+// src/CodecUtils/MotionCompensatorH264ImplStd.cpp:410-437); the coding at
+// each QP of the set is code_frame_kernel's.
+__global__ __launch_bounds__(256) void code_vbs_kernel(const uint8_t* __restrict__ cur, const VbsRefs refs,
+                                                       int W, int H, const int16_t* __restrict__ mv8,
+                                                       const uint8_t* __restrict__ ref8, const QpSet qs,
+                                                       int intra, int cols_first,
+                                                       int16_t* __restrict__ pred_out) {
+  const int bw4 = W >> 2;
+  const long blk = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blk >= (long)bw4 * (H >> 2)) return;
+  const int by = (int)(blk / bw4), bx = (int)(blk - (long)by * bw4);
+  const int x = 4 * bx, y = 4 * by;
+  const long b8 = (long)(by >> 1) * (W >> 3) + (bx >> 1);
+  const int ri = ref8[b8];
+  const uint8_t* rb = ri == 0 ? refs.ref[0] : (ri == 1 ? refs.ref[1] : (ri == 2 ? refs.ref[2] : refs.ref[3]));
+  // Full-pel source of the block (vectors floored to full pels), clamped
+  // into the padding: beyond the picture the padded plane repeats its edge,
+  // so a block clamped there reads what the far one would.  4 prediction
+  // rows of 4 bytes at a byte offset within 2 aligned dwords (the padded
+  // planes' pixel (0,0) is 16-byte aligned).
+  const int xs = min(max(x + (mv8[2 * b8] >> 2), -kPad + 4), W + kPad - 8);
+  const int ys = min(max(y + (mv8[2 * b8 + 1] >> 2), -kPad), H + kPad - 4);
+  const int sh = xs & 3;
+  uint32_t pw[4], cw[4];
+#pragma unroll
+  for (int rr = 0; rr < 4; rr++) {
+    const uint32_t* a = (const uint32_t*)(rb + (long)(ys + rr) * refs.stride + (xs - sh));
+    pw[rr] = __builtin_amdgcn_alignbyte(a[1], a[0], sh);
+    cw[rr] = *(const uint32_t*)(cur + (long)(y + rr) * W + x);
+  }
+  int16_t b[16], pr[16];
+#pragma unroll
+  for (int p = 0; p < 16; p++) {
+    pr[p] = (int16_t)((pw[p >> 2] >> (8 * (p & 3))) & 0xFF);
+    b[p] = (int16_t)((int)((cw[p >> 2] >> (8 * (p & 3))) & 0xFF) - pr[p]);
+  }
+  if (pred_out) {
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      uint2 pv;
+      pv.x = (uint32_t)(uint16_t)pr[4 * rr] | ((uint32_t)(uint16_t)pr[4 * rr + 1] << 16);
+      pv.y = (uint32_t)(uint16_t)pr[4 * rr + 2] | ((uint32_t)(uint16_t)pr[4 * rr + 3] << 16);
+      *(uint2*)(pred_out + (long)(y + rr) * W + x) = pv;
+    }
+  }
+#pragma unroll 1
+  for (int iq = 0; iq < qs.n; iq++) {
+    const int qp = qs.qp[iq];
+    int16_t t[16];
+#pragma unroll
+    for (int p = 0; p < 16; p++) t[p] = b[p];
+    fwd4x4(t, qp, intra, 0);
+    int16_t* levels = qs.levels[iq];
+    uint8_t* recon = qs.recon[iq];
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      uint2 lv;
+      lv.x = (uint32_t)(uint16_t)t[4 * rr] | ((uint32_t)(uint16_t)t[4 * rr + 1] << 16);
+      lv.y = (uint32_t)(uint16_t)t[4 * rr + 2] | ((uint32_t)(uint16_t)t[4 * rr + 3] << 16);
+      *(uint2*)(levels + (long)(y + rr) * W + x) = lv;
+    }
+    inv4x4(t, qp, 0, cols_first, nullptr);
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      uint32_t rc = 0;
+#pragma unroll
+      for (int cc = 0; cc < 4; cc++) rc |= (uint32_t)clip255(pr[4 * rr + cc] + t[4 * rr + cc]) << (8 * cc);
+      *(uint32_t*)(recon + (long)(y + rr) * W + x) = rc;
+    }
+  }
+}
+
 // Order-free checksum of a byte array: the wrapping 64-bit sum over i of
 // (p[i] + 1) * (odd 32-bit hash of i), so it does not depend on which thread
 // adds which part, and a changed or moved byte changes it.
@@ -350,6 +426,14 @@ void launch_checksum(const void* p, long bytes, unsigned long long* out, hipStre
   if (bytes <= 0) return;
   hipLaunchKernelGGL(checksum_kernel, dim3((unsigned)std::max<long>(1, blocks)), dim3(256), 0, st,
                      (const uint8_t*)p, bytes, out);
+}
+
+void launch_code_vbs(const uint8_t* cur, const VbsRefs& refs, int W, int H, const int16_t* mv8,
+                     const uint8_t* ref8, const QpSet& qs, int intra, int cols_first,
+                     int16_t* pred_out, hipStream_t st) {
+  const long nblk = (long)(W / 4) * (H / 4);
+  hipLaunchKernelGGL(code_vbs_kernel, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, st, cur, refs, W,
+                     H, mv8, ref8, qs, intra, cols_first, pred_out);
 }
 
 void launch_code_frame(const uint8_t* cur, int cur_stride, const uint8_t* ref_padded, int ref_stride,

@@ -51,14 +51,7 @@ typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 // (kPad, W and the source stride are multiples of 16), border chunks gather
 // clamped bytes.  Every plane of a batch pads in one launch: a workgroup finds
 // its plane by binary search over the planes' first workgroups.
-__global__ __launch_bounds__(256) void pad_planes_kernel(const PadDesc* __restrict__ pads, int n) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (pads[mid].wg_base <= (int)blockIdx.x) lo = mid;
-    else hi = mid - 1;
-  }
-  const PadDesc p = pads[lo];
+__device__ __forceinline__ void pad_chunk(const PadDesc& p) {
   const int cpr = (p.W + 2 * kPad) / 16;  // 16-byte chunks per padded row
   const int idx = ((int)blockIdx.x - p.wg_base) * (int)blockDim.x + (int)threadIdx.x;
   if (idx >= cpr * (p.H + 2 * kPad)) return;
@@ -83,6 +76,26 @@ __global__ __launch_bounds__(256) void pad_planes_kernel(const PadDesc* __restri
     v = make_uint4(w[0], w[1], w[2], w[3]);
   }
   *(uint4*)(p.dst - (long)kPad * p.dst_stride - kPad + (long)y * p.dst_stride + x16) = v;
+}
+
+__global__ __launch_bounds__(256) void pad_planes_kernel(const PadDesc* __restrict__ pads, int n) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pads[mid].wg_base <= (int)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  pad_chunk(pads[lo]);
+}
+
+// The same over a handful of planes whose descriptors come as a kernel
+// argument (no descriptor upload).
+__global__ __launch_bounds__(256) void pad_set_kernel(const PadSet s) {
+  PadDesc d = s.d[0];
+#pragma unroll
+  for (int i = 1; i < kVbsMaxRefs; i++)
+    if (i < s.n && s.d[i].wg_base <= (int)blockIdx.x) d = s.d[i];
+  pad_chunk(d);
 }
 
 // ------------------------------------------------------------- tile bodies
@@ -505,6 +518,11 @@ void me_search_kernel(const StreamDesc* __restrict__ streams, const SearchTask* 
 int pad_plane_workgroups(int W, int H) {
   const long chunks = (long)(W + 2 * kPad) / 16 * (H + 2 * kPad);
   return (int)((chunks + 255) / 256);
+}
+
+void launch_pad_set(const PadSet& s, int total_wgs, hipStream_t st) {
+  if (s.n <= 0 || total_wgs <= 0) return;
+  hipLaunchKernelGGL(pad_set_kernel, dim3(total_wgs), dim3(256), 0, st, s);
 }
 
 void launch_pad_planes(const PadDesc* d_pads, int n, int total_wgs, hipStream_t st) {

@@ -1,0 +1,274 @@
+// Multi-reference, variable-block-size integer motion search on gfx950.
+//
+// Per 16x16 MB and per reference frame, every full-pel candidate of the MB's
+// window (GetMotionRange with half-width R,
+// src/CodecUtils/MotionEstimatorH264ImplFull.cpp:1029-1081, exhaustive form)
+// gets the SADs of the MB's four 8x8 quadrants, each the sum of four 4x4
+// SADs (OverlayMem2Dv2::Tad4x4 / Tad8x8, src/ImageUtils/OverlayMem2Dv2.cpp:
+// 2555-2654).  The quadrant SADs are merged into the nine partitions of the
+// 16x16 / 16x8 / 8x16 / 8x8 modes, a rate term lambda * (|vx| + |vy|) is
+// added (no reference counterpart: the reference searches 16x16 blocks of
+// one reference), and each partition keeps its least cost; ties go to the
+// lower reference index, then to the earlier candidate in raster order.  The
+// MB then takes the mode whose partitions cost least in total, ties to the
+// mode with fewer partitions.
+//
+// Work decomposition: one workgroup per MB.  The windows of all references
+// (2R+1 candidate rows + 15, dword rows) are staged in LDS with
+// global_load_lds; the MB's block is read once with scalar loads (every lane
+// of the workgroup uses the same 64 dwords, so they live in SGPRs).  A lane
+// task is one reference x one 4-candidate column quad x V candidate rows: it
+// walks the V+15 reference rows its tile touches once, and each (reference
+// row, block row) pair issues 4 v_qsad_pk_u16_u8 into the accumulators of
+// the quadrant that block row and column half belong to (an 8x8 SAD is at
+// most 16320, so the packed u16 sums never wrap).  Each finished candidate
+// row merges the quadrants into the nine partitions and folds them into nine
+// lane-local minimum keys (cost << 15 | tie-break index); the workgroup then
+// reduces the keys across lanes and waves and decides the mode.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "icu_device.h"
+#include "icu_internal.h"
+
+namespace icu {
+
+namespace {
+
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef uint64_t __attribute__((aligned(4))) u64_a4;
+typedef __attribute__((address_space(3))) u64_a4 lds_u64_a4;
+
+constexpr int kThreads = 256;
+
+// Partition order: 0 16x16; 1, 2 16x8 top / bottom; 3, 4 8x16 left / right;
+// 5..8 8x8 in raster order.  Quadrants: 0 top-left, 1 top-right,
+// 2 bottom-left, 3 bottom-right.
+__device__ __forceinline__ int part_of_quadrant(int mode, int k) {
+  return mode == 0 ? 0 : (mode == 1 ? 1 + (k >> 1) : (mode == 2 ? 3 + (k & 1) : 5 + k));
+}
+
+template <int V>
+__global__ __launch_bounds__(kThreads) void vbs_search_kernel(const VbsParams p) {
+  extern __shared__ __attribute__((aligned(16))) uint32_t dyn[];
+  __shared__ uint32_t red[kThreads / 64][9];
+  lds_u32* const win = (lds_u32*)dyn;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+  // XCD bands: workgroups go to the 8 XCDs in turn; XCD x runs a contiguous
+  // band of MBs, so neighbouring windows are fetched into one L2.
+  int mb;
+  {
+    const int x = (int)blockIdx.x & 7, k = (int)blockIdx.x >> 3;
+    const int q = p.nmb >> 3, r = p.nmb & 7;
+    mb = x * q + min(x, r) + k;
+  }
+  const int mcol = mb % p.mbw, mrow = mb / p.mbw;
+  const int n = 16 * mcol, m = 16 * mrow;
+
+  // stage every reference's window: rows m-R .. m-R+rows-1, 16-byte chunks
+  // from the 16-byte boundary at or left of column n + dx0
+  {
+    const int cpr = p.row_dw >> 2;
+    const int per_ref = p.rows * cpr;
+    const int total = p.nrefs * per_ref;
+    for (int base = wave * 64; base < total; base += kThreads) {
+      const int ch = base + lane;
+      if (ch < total) {
+        const int r = ch / per_ref, rem = ch - r * per_ref;
+        const int row = rem / cpr, c4 = rem - row * cpr;
+        const uint8_t* rb = r == 0 ? p.ref[0] : (r == 1 ? p.ref[1] : (r == 2 ? p.ref[2] : p.ref[3]));
+        const uint8_t* src = rb + (long)(m - p.R + row) * p.ref_stride + n + p.dx0 - 4 * p.shift + 16 * c4;
+        __builtin_amdgcn_global_load_lds((const void*)src,
+                                         (__attribute__((address_space(3))) void*)(win + 4 * base), 16, 0, 0);
+      }
+    }
+  }
+  // the MB's block: uniform addresses, so scalar loads into SGPRs
+  uint32_t c[64];
+  {
+    const uint32_t* cw = (const uint32_t*)(p.cur + (long)m * p.W + n);
+    const int sdw = p.W >> 2;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) c[4 * i + k] = __builtin_amdgcn_readfirstlane(cw[i * sdw + k]);
+  }
+  int xl, xr, yu, yd;
+  motion_window(n, m, p.W, p.H, p.R, &xl, &xr, &yu, &yd);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+
+  uint32_t best[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) best[i] = ~0u;
+  const int QG = p.Q * p.Gr;
+  const int T = p.nrefs * QG;
+  const int NN = p.N * p.N;
+  for (int t = tid; t < T; t += kThreads) {
+    const int r = t / QG, rem = t - r * QG;
+    const int g = rem / p.Q, q = rem - g * p.Q;
+    const lds_u32* rp = win + (r * p.rows + g * V) * p.row_dw + p.shift + q;
+    const int dx0 = p.dx0 + 4 * q, dy0 = -p.R + g * V;
+    // Key offsets (rate << 15) + tie-break index, split into a column part
+    // (fixed per task) and a row part (per candidate row).  Candidates
+    // outside the MB's window get the offset 2^31 instead: keys stay below
+    // 2^32 (cost < 2^17), and the window always holds the zero vector, whose
+    // key is below 2^31, so they never win.
+    uint32_t offc[4];
+    bool colok[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int dx = dx0 + j;
+      colok[j] = dx >= xl && dx <= xr;
+      offc[j] = ((uint32_t)(p.lambda * (dx < 0 ? -dx : dx)) << 15) + (uint32_t)(dx + p.R);
+    }
+    const uint32_t rowbase = (uint32_t)(r * NN + (dy0 + p.R) * p.N);
+    uint64_t acc[4][V];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int v = 0; v < V; v++) acc[k][v] = 0;
+    auto span = [&](int k) { return *(const lds_u64_a4*)(rp + k); };
+    uint64_t n0 = span(0), n1 = span(1), n2 = span(2), n3 = span(3);
+#pragma unroll
+    for (int rho = 0; rho < V + 15; rho++) {
+      uint64_t p0 = n0, p1 = n1, p2 = n2, p3 = n3;
+      // the wait for this row's spans lands here, a row of qsads after
+      // their loads were issued; later rows' loads stay below it
+      asm volatile("" : "+v"(p0), "+v"(p1), "+v"(p2), "+v"(p3), "+v"(rp) : : "memory");
+      if (rho + 1 < V + 15) {
+        rp += p.row_dw;
+        n0 = span(0); n1 = span(1); n2 = span(2); n3 = span(3);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint64_t pk = k == 0 ? p0 : (k == 1 ? p1 : (k == 2 ? p2 : p3));
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+          const int br = rho - v;  // block row against reference row rho of this tile
+          if (br >= 0 && br < 16) {
+            const int quad = (br >= 8 ? 2 : 0) + (k >> 1);
+            acc[quad][v] = __builtin_amdgcn_qsad_pk_u16_u8(pk, c[4 * br + k], acc[quad][v]);
+          }
+        }
+      }
+      // candidate row rho-15 has seen all 16 block rows.  Candidates outside
+      // the MB's window get the key offset 2^31 instead of their rate and
+      // index: keys stay below 2^32 (cost < 2^17), and the window always
+      // holds the zero vector, whose key is below 2^31, so they never win.
+      if (rho >= 15) {
+        const int v = rho - 15;
+        const int dy = dy0 + v;
+        const bool row_ok = dy >= yu && dy <= yd;
+        const uint32_t offr = (__umul24((uint32_t)p.lambda, (uint32_t)(dy < 0 ? -dy : dy)) << 15) +
+                              rowbase + (uint32_t)(v * p.N);
+        uint32_t k[9][4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const uint32_t off = (row_ok && colok[j]) ? offr + offc[j] : 0x80000000u;
+          const uint32_t s0 = (uint32_t)(acc[0][v] >> (16 * j)) & 0xFFFFu;
+          const uint32_t s1 = (uint32_t)(acc[1][v] >> (16 * j)) & 0xFFFFu;
+          const uint32_t s2 = (uint32_t)(acc[2][v] >> (16 * j)) & 0xFFFFu;
+          const uint32_t s3 = (uint32_t)(acc[3][v] >> (16 * j)) & 0xFFFFu;
+          const uint32_t top = s0 + s1, bot = s2 + s3;
+          k[0][j] = ((top + bot) << 15) + off;
+          k[1][j] = (top << 15) + off;
+          k[2][j] = (bot << 15) + off;
+          k[3][j] = ((s0 + s2) << 15) + off;
+          k[4][j] = ((s1 + s3) << 15) + off;
+          k[5][j] = (s0 << 15) + off;
+          k[6][j] = (s1 << 15) + off;
+          k[7][j] = (s2 << 15) + off;
+          k[8][j] = (s3 << 15) + off;
+        }
+#pragma unroll
+        for (int i = 0; i < 9; i++)
+          best[i] = min(min(min(min(best[i], k[i][0]), k[i][1]), k[i][2]), k[i][3]);
+      }
+    }
+  }
+  // workgroup minimum of each partition's key
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    uint32_t k = best[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) k = min(k, (uint32_t)__shfl_xor((int)k, o, 64));
+    if (lane == 0) red[wave][i] = k;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t key[9];
+    uint32_t J[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      key[i] = min(min(red[0][i], red[1][i]), min(red[2][i], red[3][i]));
+      J[i] = key[i] >> 15;
+    }
+    const uint32_t mj[4] = {J[0], J[1] + J[2], J[3] + J[4], J[5] + J[6] + J[7] + J[8]};
+    int mode = 0;
+#pragma unroll
+    for (int k = 1; k < 4; k++)
+      if (mj[k] < mj[mode]) mode = k;
+    p.mode[mb] = (uint8_t)mode;
+    p.cost[mb] = (int32_t)mj[mode];
+    const int bw = p.W >> 3;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t idx = key[part_of_quadrant(mode, k)] & 0x7FFFu;
+      const int r = (int)idx / NN, rem = (int)idx - r * NN;
+      const int dy = rem / p.N - p.R, dx = rem - (rem / p.N) * p.N - p.R;
+      const long b = (long)(2 * mrow + (k >> 1)) * bw + 2 * mcol + (k & 1);
+      p.mv8[2 * b] = (int16_t)(4 * dx);
+      p.mv8[2 * b + 1] = (int16_t)(4 * dy);
+      p.ref8[b] = (uint8_t)r;
+    }
+  }
+}
+
+}  // namespace
+
+void vbs_plan(int R, int nrefs, VbsParams* p) {
+  const int N = 2 * R + 1;
+  p->R = R;
+  p->N = N;
+  p->nrefs = nrefs;
+  p->dx0 = tile_dx0(R);
+  p->Q = (R - p->dx0) / 4 + 1;
+  // the column of n + dx0 within its 16-byte chunk (n is a multiple of 16)
+  p->shift = ((p->dx0 % 16) + 16) % 16 / 4;
+  // rows per lane tile: fewest task rounds of the workgroup, then the least
+  // work per round (V qsad rows + the 15 extra rows every tile walks)
+  int bestV = 4, bestCost = 1 << 30;
+  for (int V = 4; V <= 8; V++) {
+    const int Gr = (N + V - 1) / V;
+    const int rounds = (nrefs * p->Q * Gr + kThreads - 1) / kThreads;
+    const int cost = rounds * (V + 3);
+    if (cost < bestCost) {
+      bestCost = cost;
+      bestV = V;
+    }
+  }
+  p->V = bestV;
+  p->Gr = (N + bestV - 1) / bestV;
+  p->rows = p->Gr * bestV + 15;
+  p->row_dw = ((p->shift + p->Q + 4) + 3) & ~3;
+}
+
+int vbs_lds_bytes(const VbsParams& p) { return p.nrefs * p.rows * p.row_dw * 4; }
+
+void launch_vbs_search(const VbsParams& p, hipStream_t st) {
+  const size_t shmem = (size_t)vbs_lds_bytes(p);
+  const dim3 grid((unsigned)p.nmb), block(kThreads);
+  switch (p.V) {
+    case 4: hipLaunchKernelGGL(vbs_search_kernel<4>, grid, block, shmem, st, p); break;
+    case 5: hipLaunchKernelGGL(vbs_search_kernel<5>, grid, block, shmem, st, p); break;
+    case 6: hipLaunchKernelGGL(vbs_search_kernel<6>, grid, block, shmem, st, p); break;
+    case 7: hipLaunchKernelGGL(vbs_search_kernel<7>, grid, block, shmem, st, p); break;
+    default: hipLaunchKernelGGL(vbs_search_kernel<8>, grid, block, shmem, st, p); break;
+  }
+}
+
+}  // namespace icu

@@ -117,7 +117,7 @@ int icu_code_frame_device(icu_me* h, const uint8_t* d_cur, const int16_t* d_mv, 
                           int cols_first, int16_t* d_levels, uint8_t* d_recon, int16_t* d_pred,
                           void* stream);
 
-/* icu_code_frame_device at nqp (1..8) QPs in one pass: the prediction and
+/* icu_code_frame_device at nqp (1..20) QPs in one pass: the prediction and
  * residual are formed once, then coded and reconstructed at each qps[i] into
  * d_levels[i] / d_recon[i] (host arrays of device pointers).  No reference
  * counterpart: the reference codes a frame at one QP per call; this is the
@@ -219,6 +219,45 @@ int icu_server_stats(icu_server* s, long long* bytes_in_use, long long* peak_byt
                      int* inflight);
 /* Seconds on the server's monotonic clock. */
 double icu_server_clock(void);
+
+/* ------------------------------------------ multi-reference, partitioned
+ * Integer motion search over up to 4 reference frames with 16x16, 16x8,
+ * 8x16 and 8x8 partitions, and the residual coding of its prediction.  No
+ * reference counterpart: the reference's estimators search 16x16 blocks of
+ * one reference.  Built from its pieces: 4x4 / 8x8 SADs as
+ * OverlayMem2Dv2::Tad4x4 / Tad8x8 (src/ImageUtils/OverlayMem2Dv2.cpp:
+ * 2555-2654), each MB's window as GetMotionRange with half-width range
+ * (src/CodecUtils/MotionEstimatorH264ImplFull.cpp:1029-1081, exhaustive
+ * form), shared by the MB's partitions.  Per partition p, reference r and
+ * full-pel vector v: J = SAD_p(r, v) + lambda * (|vx| + |vy|); the least J
+ * wins, ties to the lower r, then the earlier v in raster order.  The MB
+ * mode is the least total J over its partitions, ties to fewer partitions. */
+typedef struct icu_vbs icu_vbs;
+typedef struct {
+  int width, height; /* luma, multiples of 16 */
+  int range;         /* full-pel half-width R, 1..32 */
+  int nrefs;         /* reference frames, 1..4 */
+  int lambda;        /* rate weight, >= 0, lambda * 2 * range < 65792 */
+} icu_vbs_config;
+int icu_vbs_create(const icu_vbs_config* cfg, icu_vbs** out);
+int icu_vbs_destroy(icu_vbs* h);
+/* d_refs: host array of nrefs device planes (8-bit, stride = width).
+ * Outputs (device): d_mode per MB (0 16x16, 1 16x8, 2 8x16, 3 8x8);
+ * d_mv8 2 int16 per 8x8 block (quarter pel, multiples of 4) and d_ref8 the
+ * reference index per 8x8 block, both in 8x8-block raster order over the
+ * picture; d_cost the chosen mode's total J per MB. */
+int icu_vbs_estimate_device(icu_vbs* h, const uint8_t* d_cur, const uint8_t* const* d_refs,
+                            uint8_t* d_mode, int16_t* d_mv8, uint8_t* d_ref8, int32_t* d_cost,
+                            void* stream);
+/* Residual coding at nqp (1..20) QPs of the prediction that copies each 8x8
+ * block from reference d_ref8[b] (as padded by the last estimate) at
+ * d_mv8[b] floored to full pels: levels / recon / pred as in
+ * icu_code_frame_qps_device. */
+int icu_vbs_code_device(icu_vbs* h, const uint8_t* d_cur, const int16_t* d_mv8, const uint8_t* d_ref8,
+                        int nqp, const int* qps, int intra, int cols_first, int16_t* const* d_levels,
+                        uint8_t* const* d_recon, int16_t* d_pred, void* stream);
+/* Candidate rows per lane tile and workgroup LDS bytes of the search. */
+int icu_vbs_get_plan(icu_vbs* h, int* tile_rows, int* lds_bytes);
 
 /* Errors and a minimal device runtime for callers without their own. */
 const char* icu_last_error(void);

@@ -56,6 +56,17 @@ void orc_mc_luma(const int16_t *ref, int W, int H, const int16_t *mv, int16_t *p
 void orc_code_residual(const int16_t *cur, const int16_t *pred, int W, int H, int qp,
                        int intra, int cols_first, int16_t *levels_out, int16_t *recon_out);
 
+/* Multi-reference variable-block-size integer search (vbs_oracle.c).
+ * refs: nrefs contiguous 8-bit W*H planes.  mode_out: nMB (0 16x16, 1 16x8,
+ * 2 8x16, 3 8x8); mv8_out: 2 int16 quarter-pel per 8x8 block, 8x8-block
+ * raster over the picture; ref8_out: reference index per 8x8 block;
+ * cost_out: the chosen mode's total J per MB. */
+int orc_vbs_estimate(const uint8_t *cur, const uint8_t *refs, int nrefs, int W, int H, int R,
+                     int lambda, uint8_t *mode_out, int16_t *mv8_out, uint8_t *ref8_out,
+                     int32_t *cost_out);
+void orc_vbs_pred(const uint8_t *refs, int W, int H, const int16_t *mv8, const uint8_t *ref8,
+                  int16_t *pred_out);
+
 #ifdef __cplusplus
 }
 #endif

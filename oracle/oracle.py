@@ -69,6 +69,10 @@ def lib():
         L.orc_block_dist.argtypes = [_i16p, _i16p, C.c_int]
         L.orc_code_residual.argtypes = [_i16p, _i16p, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, _i16p, _i16p]
+        L.orc_vbs_estimate.argtypes = [_u8p, _u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       _u8p, _i16p, _u8p, _i32p]
+        L.orc_vbs_estimate.restype = C.c_int
+        L.orc_vbs_pred.argtypes = [_u8p, C.c_int, C.c_int, _i16p, _u8p, _i16p]
         _lib = L
     return _lib
 
@@ -170,3 +174,30 @@ def code_residual(cur, pred, qp, intra=0, cols_first=0):
 def block_dist(a, b, metric):
     return lib().orc_block_dist(np.ascontiguousarray(a, dtype=np.int16).reshape(-1),
                                 np.ascontiguousarray(b, dtype=np.int16).reshape(-1), metric)
+
+
+def vbs_estimate(cur, refs, R, lam=4):
+    """Multi-reference variable-block-size search.  refs: list of (H, W)
+    uint8 planes.  Returns (mode (mbh, mbw) uint8, mv8 (H/8, W/8, 2) int16
+    quarter-pel, ref8 (H/8, W/8) uint8, cost (mbh, mbw) int32)."""
+    cur = np.ascontiguousarray(cur, dtype=np.uint8)
+    H, W = cur.shape
+    rs = np.ascontiguousarray(np.stack([np.asarray(r, np.uint8) for r in refs]))
+    mode = np.zeros((H // 16) * (W // 16), np.uint8)
+    cost = np.zeros_like(mode, dtype=np.int32)
+    mv8 = np.zeros(2 * (H // 8) * (W // 8), np.int16)
+    ref8 = np.zeros((H // 8) * (W // 8), np.uint8)
+    rc = lib().orc_vbs_estimate(cur, rs.reshape(-1), len(refs), W, H, R, lam, mode, mv8, ref8, cost)
+    if rc != 0:
+        raise ValueError(f"oracle vbs_estimate failed ({rc})")
+    return (mode.reshape(H // 16, W // 16), mv8.reshape(H // 8, W // 8, 2), ref8.reshape(H // 8, W // 8),
+            cost.reshape(H // 16, W // 16))
+
+
+def vbs_pred(refs, mv8, ref8):
+    rs = np.ascontiguousarray(np.stack([np.asarray(r, np.uint8) for r in refs]))
+    H, W = rs.shape[1:]
+    out = np.zeros((H, W), np.int16)
+    lib().orc_vbs_pred(rs.reshape(-1), W, H, np.ascontiguousarray(mv8, np.int16).reshape(-1),
+                       np.ascontiguousarray(ref8, np.uint8).reshape(-1), out)
+    return out

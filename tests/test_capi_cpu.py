@@ -85,3 +85,12 @@ def test_server_checksum_definition():
     big = np.full(1 << 16, 255, np.uint8)
     ref = sum(256 * ((i * 0x9E3779B1) % 2**32 | 1) for i in range(1 << 16)) % 2**64
     assert capi.server_checksum(big) == ref
+
+
+def test_vbs_rejects_bad_configs():
+    # checked before any device call
+    for args in ((30, 32, 8, 1, 4), (32, 32, 0, 1, 4), (32, 32, 33, 1, 4), (32, 32, 8, 0, 4),
+                 (32, 32, 8, 5, 4), (32, 32, 32, 1, 1100), (32, 32, 8, 1, -1)):
+        with pytest.raises(capi.IcuError) as e:
+            capi.VbsEstimator(*args)
+        assert e.value.code == capi.ICU_E_ARG

@@ -1,0 +1,171 @@
+"""GPU parity of the multi-reference variable-block-size search and its
+residual coder (icu_vbs_*) against the oracle.  Integer work: every
+comparison is exact."""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+capi = pytest.importorskip("imagecodingutils_amd.capi")
+from imagecodingutils_amd import synth  # noqa: E402
+
+
+def _run(cur, refs, R, lam, stream=None):
+    H, W = cur.shape
+    est = capi.VbsEstimator(W, H, R, len(refs), lam)
+    d_cur = capi.DeviceBuffer.from_array(cur)
+    d_refs = [capi.DeviceBuffer.from_array(r) for r in refs]
+    d_mode = capi.DeviceBuffer(est.nmb)
+    d_mv8 = capi.DeviceBuffer(4 * est.nb8)
+    d_ref8 = capi.DeviceBuffer(est.nb8)
+    d_cost = capi.DeviceBuffer(4 * est.nmb)
+    est.estimate_device(d_cur, d_refs, d_mode, d_mv8, d_ref8, d_cost, stream)
+    capi.lib.icu_stream_sync(None)
+    capi.DeviceBuffer.sync()
+    if stream is not None:
+        stream.sync()
+    out = (d_mode.download(np.uint8, (H // 16, W // 16)), d_mv8.download(np.int16, (H // 8, W // 8, 2)),
+           d_ref8.download(np.uint8, (H // 8, W // 8)), d_cost.download(np.int32, (H // 16, W // 16)))
+    return est, (d_cur, d_refs, d_mv8, d_ref8), out
+
+
+def _check(got, want, what):
+    for name, g, w in zip(("mode", "mv8", "ref8", "cost"), got, want):
+        np.testing.assert_array_equal(g, w, err_msg=f"{what}: {name}")
+
+
+CASES = [  # (W, H, R, nrefs, lambda, seed)
+    (176, 144, 8, 1, 0, 1),
+    (176, 144, 16, 4, 4, 2),
+    (352, 288, 16, 2, 4, 3),
+    (352, 288, 32, 4, 4, 4),
+    (64, 48, 13, 3, 17, 5),
+    (96, 64, 5, 4, 1, 6),
+    (48, 32, 32, 2, 7, 7),
+    (176, 144, 27, 4, 1000, 8),
+]
+
+
+@pytest.mark.parametrize("W,H,R,nrefs,lam,seed", CASES)
+def test_vbs_search_matches_oracle(W, H, R, nrefs, lam, seed):
+    cur, refs, _ = synth.shifted_copies(W, H, seed, nrefs, max_mv=min(R, 16))
+    _, _, got = _run(cur, refs, R, lam)
+    _check(got, orc.vbs_estimate(cur, refs, R, lam), f"{W}x{H} R{R} n{nrefs} l{lam}")
+
+
+@pytest.mark.parametrize("lam", [0, 3])
+def test_vbs_ties_on_flat_and_periodic_content(lam):
+    W, H, R = 96, 64, 9
+    flat = np.full((H, W), 77, np.uint8)
+    ys, xs = np.mgrid[0:H, 0:W]
+    stripes = ((xs // 2 + ys // 3) % 2 * 200).astype(np.uint8)  # period 4 x 6
+    for cur, refs in ((flat, [flat, flat, flat]), (stripes, [stripes, np.roll(stripes, 1, 1)]),
+                      (stripes, [np.roll(stripes, 2, 0), stripes])):
+        _, _, got = _run(cur, refs, R, lam)
+        _check(got, orc.vbs_estimate(cur, refs, R, lam), f"ties lam {lam}")
+
+
+def test_vbs_known_layouts_pick_each_mode():
+    """Parts of the current frame copied from different references at
+    different vectors: the exact answer on every layout, as in the oracle's
+    own known-answer pin."""
+    W, H, R, lam = 128, 96, 6, 3
+    rng = np.random.default_rng(9)
+    texs = [rng.integers(0, 256, (H + 20, W + 20)).astype(np.uint8) for _ in range(4)]
+    vs = [(3, -2), (-4, 1), (0, 5), (-2, -3)]
+    refs = [np.ascontiguousarray(t[10:10 + H, 10:10 + W]) for t in texs]
+    parts = [np.ascontiguousarray(t[10 + dy:10 + dy + H, 10 + dx:10 + dx + W]) for t, (dx, dy) in zip(texs, vs)]
+    ys, xs = np.mgrid[0:H, 0:W]
+    top, left = (ys % 16) < 8, (xs % 16) < 8
+    for lay in (np.zeros((H, W), int), np.where(top, 0, 1), np.where(left, 2, 3),
+                np.where(top, np.where(left, 0, 1), np.where(left, 2, 3))):
+        cur = np.zeros((H, W), np.uint8)
+        for k in range(4):
+            cur[lay == k] = parts[k][lay == k]
+        _, _, got = _run(cur, refs, R, lam)
+        _check(got, orc.vbs_estimate(cur, refs, R, lam), "layout")
+
+
+def test_vbs_code_at_qp_sweep_matches_oracle():
+    """Prediction by block copy from the chosen references, then the
+    residual coded at QP 22..40 in one pass (19 QPs)."""
+    W, H, R = 352, 288, 16
+    cur, refs, _ = synth.shifted_copies(W, H, 31, 4, max_mv=12)
+    est, (d_cur, d_refs, d_mv8, d_ref8), (mode, mv8, ref8, cost) = _run(cur, refs, R, 4)
+    qps = list(range(22, 41))
+    d_lv = [capi.DeviceBuffer(2 * W * H) for _ in qps]
+    d_rc = [capi.DeviceBuffer(W * H) for _ in qps]
+    d_pr = capi.DeviceBuffer(2 * W * H)
+    est.code_device(d_cur, d_mv8, d_ref8, qps, 0, 0, d_lv, d_rc, d_pr)
+    capi.lib.icu_stream_sync(None)
+    pred = orc.vbs_pred(refs, mv8, ref8)
+    np.testing.assert_array_equal(d_pr.download(np.int16, (H, W)), pred)
+    for i, qp in enumerate(qps):
+        lv, rc = orc.code_residual(cur, pred, qp, 0, 0)
+        np.testing.assert_array_equal(d_lv[i].download(np.int16, (H, W)), lv, err_msg=f"qp {qp}")
+        np.testing.assert_array_equal(d_rc[i].download(np.uint8, (H, W)).astype(np.int16), rc)
+
+
+@pytest.mark.parametrize("intra,cols_first", [(1, 0), (0, 1)])
+def test_vbs_code_arbitrary_vectors(intra, cols_first):
+    """Any vectors and reference indices, including far outside the picture
+    and with quarter-pel parts (floored to full pels)."""
+    W, H = 176, 144
+    rng = np.random.default_rng(40 + intra)
+    refs = [rng.integers(0, 256, (H, W)).astype(np.uint8) for _ in range(3)]
+    cur = rng.integers(0, 256, (H, W)).astype(np.uint8)
+    est = capi.VbsEstimator(W, H, 8, 3, 4)
+    d_cur = capi.DeviceBuffer.from_array(cur)
+    d_refs = [capi.DeviceBuffer.from_array(r) for r in refs]
+    d_mode, d_cost = capi.DeviceBuffer(est.nmb), capi.DeviceBuffer(4 * est.nmb)
+    d_tmv, d_tref = capi.DeviceBuffer(4 * est.nb8), capi.DeviceBuffer(est.nb8)
+    est.estimate_device(d_cur, d_refs, d_mode, d_tmv, d_tref, d_cost)  # pads the references
+    mv8 = rng.integers(-1200, 1201, (H // 8, W // 8, 2)).astype(np.int16)
+    ref8 = rng.integers(0, 3, (H // 8, W // 8)).astype(np.uint8)
+    capi.lib.icu_stream_sync(None)
+    d_mv8, d_ref8 = capi.DeviceBuffer.from_array(mv8), capi.DeviceBuffer.from_array(ref8)
+    qps = [0, 28, 51]
+    d_lv = [capi.DeviceBuffer(2 * W * H) for _ in qps]
+    d_rc = [capi.DeviceBuffer(W * H) for _ in qps]
+    d_pr = capi.DeviceBuffer(2 * W * H)
+    est.code_device(d_cur, d_mv8, d_ref8, qps, intra, cols_first, d_lv, d_rc, d_pr)
+    capi.lib.icu_stream_sync(None)
+    pred = orc.vbs_pred(refs, mv8, ref8)
+    np.testing.assert_array_equal(d_pr.download(np.int16, (H, W)), pred)
+    for i, qp in enumerate(qps):
+        lv, rc = orc.code_residual(cur, pred, qp, intra, cols_first)
+        np.testing.assert_array_equal(d_lv[i].download(np.int16, (H, W)), lv)
+        np.testing.assert_array_equal(d_rc[i].download(np.uint8, (H, W)).astype(np.int16), rc)
+
+
+def test_vbs_4k_interior_matches_oracle_on_crops():
+    """The 4K four-reference workload at full size: MBs whose windows lie
+    inside a crop decide the same in the crop as in the full picture, so the
+    oracle on two crops checks the full-size run there; the run is also
+    repeatable."""
+    W, H, R = 3840, 2160, 16
+    cur, refs, _ = synth.shifted_copies(W, H, 2024, 4, max_mv=16)
+    stream = capi.Stream()
+    _, _, got = _run(cur, refs, R, 4, stream)
+    _, _, again = _run(cur, refs, R, 4)
+    _check(again, got, "rerun")
+    for (x0, y0) in ((0, 0), (1664, 960)):
+        cw, ch = 256, 192
+        crop = lambda a: np.ascontiguousarray(a[y0:y0 + ch, x0:x0 + cw])
+        mode, mv8, ref8, cost = orc.vbs_estimate(crop(cur), [crop(r) for r in refs], R, 4)
+        # MBs at least R + 16 pels from a crop edge that is not a picture edge
+        k = (R + 15) // 16 + 1
+        m0, n0 = y0 // 16, x0 // 16
+        r_lo = 0 if y0 == 0 else k
+        c_lo = 0 if x0 == 0 else k
+        r_hi, c_hi = ch // 16 - k, cw // 16 - k
+        sl = (slice(r_lo, r_hi), slice(c_lo, c_hi))
+        gl = (slice(m0 + r_lo, m0 + r_hi), slice(n0 + c_lo, n0 + c_hi))
+        np.testing.assert_array_equal(got[0][gl], mode[sl])
+        np.testing.assert_array_equal(got[3][gl], cost[sl])
+        s8 = (slice(2 * r_lo, 2 * r_hi), slice(2 * c_lo, 2 * c_hi))
+        g8 = (slice(2 * (m0 + r_lo), 2 * (m0 + r_hi)), slice(2 * (n0 + c_lo), 2 * (n0 + c_hi)))
+        np.testing.assert_array_equal(got[1][g8], mv8[s8])
+        np.testing.assert_array_equal(got[2][g8], ref8[s8])
