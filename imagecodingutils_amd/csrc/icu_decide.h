@@ -235,6 +235,73 @@ __device__ inline void build_half_grid(WaveLds& s, int lane) {
   wave_sync();
 }
 
+// The eight half-pel candidates around full-pel (x0, y0) without the LDS
+// lattice: lane l owns the 4 pixels of block row l / 4, columns 4 (l % 4)
+// .. +3 (its dword cw of the input block).  Their samples need reference
+// rows y0 + r - 3 .. y0 + r + 3 and columns x0 + c0 - 3 .. x0 + c0 + 6, read
+// as 7 x 4 aligned dwords from the padded plane (L2-resident after the
+// search); every G/b/h/j sample is then filtered in registers with the same
+// arithmetic as build_half_grid (vertical 6-tap sums first, j from them).
+// acc[k] receives this lane's distortion of candidate k (kSubX/kSubY order).
+__device__ __forceinline__ void half_pel_direct(const DecideDesc& d, int x0, int y0, uint32_t cw,
+                                                int lane, int (&acc)[8]) {
+  const int r = lane >> 2, c0 = 4 * (lane & 3);
+  const int xs = x0 - 3 + c0, sh = xs & 3;  // 0..3 also for negative xs
+  const uint32_t* base = (const uint32_t*)(d.ref + (long)(y0 - 3 + r) * d.ref_stride + (xs - sh));
+  const int sdw = d.ref_stride >> 2;
+  // 10 bytes from byte sh of a row span 4 dwords when sh == 3
+  uint32_t w[7][4];
+#pragma unroll
+  for (int k = 0; k < 7; k++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[k][q] = base[k * sdw + q];
+  // p[k][i]: reference (x0 + c0 - 3 + i, y0 + r - 3 + k), i = 0..9
+  int p[7][10];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    const uint32_t a0 = __builtin_amdgcn_alignbyte(w[k][1], w[k][0], sh);
+    const uint32_t a1 = __builtin_amdgcn_alignbyte(w[k][2], w[k][1], sh);
+    const uint32_t a2 = __builtin_amdgcn_alignbyte(w[k][3], w[k][2], sh);
+#pragma unroll
+    for (int i = 0; i < 10; i++) {
+      const uint32_t a = i < 4 ? a0 : (i < 8 ? a1 : a2);
+      p[k][i] = (int)((a >> (8 * (i & 3))) & 0xFFu);
+    }
+  }
+  // raw vertical sums between rows y0 + r - 1, y0 + r (a = 0) and y0 + r,
+  // y0 + r + 1 (a = 1)
+  int v[2][10];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int i = 0; i < 10; i++)
+      v[a][i] = ICU_TAP6(p[a][i], p[a + 1][i], p[a + 2][i], p[a + 3][i], p[a + 4][i], p[a + 5][i]);
+  int B[5], J[2][5], Hh[2][4];
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    B[i] = clip255((ICU_TAP6(p[3][i], p[3][i + 1], p[3][i + 2], p[3][i + 3], p[3][i + 4], p[3][i + 5]) + 16) >> 5);
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+      J[a][i] = clip255((ICU_TAP6(v[a][i], v[a][i + 1], v[a][i + 2], v[a][i + 3], v[a][i + 4], v[a][i + 5]) + 512) >> 10);
+  }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) Hh[a][j] = clip255((v[a][j + 3] + 16) >> 5);
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int cv = (int)((cw >> (8 * j)) & 0xFFu);
+    acc[0] += pix_dist(cv, J[0][j], d.metric);      // (-1,-1)
+    acc[1] += pix_dist(cv, Hh[0][j], d.metric);     // ( 0,-1)
+    acc[2] += pix_dist(cv, J[0][j + 1], d.metric);  // ( 1,-1)
+    acc[3] += pix_dist(cv, B[j], d.metric);         // (-1, 0)
+    acc[4] += pix_dist(cv, B[j + 1], d.metric);     // ( 1, 0)
+    acc[5] += pix_dist(cv, J[1][j], d.metric);      // (-1, 1)
+    acc[6] += pix_dist(cv, Hh[1][j], d.metric);     // ( 0, 1)
+    acc[7] += pix_dist(cv, J[1][j + 1], d.metric);  // ( 1, 1)
+  }
+}
+
 struct Decision {
   int mvx, mvy, dist;
   int imx, imy, iminD;
@@ -293,7 +360,22 @@ __device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, in
   out.iminD = minD;
 
   int mvx = mx * 4, mvy = my * 4;
-  if (d.subpel >= 1) {
+  if (d.subpel == 1) {
+    // half pel only: the eight candidates filtered in registers
+    int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    half_pel_direct(d, n + mx, m + my, cw, lane, acc);
+    wave_sum8(acc);
+    int hxh = 0, hyh = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (acc[k] < minD) {
+        minD = acc[k];
+        hxh = kSubX[k];
+        hyh = kSubY[k];
+      }
+    mvx += 2 * hxh;
+    mvy += 2 * hyh;
+  } else if (d.subpel >= 2) {
     load_patch(d, n + mx, m + my, s.P, lane);
     wave_sync();
     build_half_grid(s, lane);
