@@ -85,15 +85,17 @@ __host__ __device__ inline void motion_window(int n, int m, int W, int H, int L,
 // multiple of 4 so every lane's 4-candidate quad starts on a dword.
 __host__ __device__ inline int tile_dx0(int L) { return -4 * ((L + 3) / 4); }
 
-// Bits of a lane tile's raster index (4 V candidates) in the search kernel's
-// lane-local argmin keys.
-__host__ __device__ constexpr int lane_key_bits(int V) { return 4 * V <= 32 ? 5 : 6; }
+// Lane tiles of the search kernel's argmin keys: tiles taller than 8 rows
+// keep one key per half, rows [0, V1) and [V1, V).
+__host__ __device__ constexpr int lane_key_split(int V) { return V > 8 ? (V + 1) / 2 : V; }
+// Bits of a tile half's raster index (4 V1 candidates) in those keys.
+__host__ __device__ constexpr int lane_key_bits(int V) { return 4 * lane_key_split(V) <= 32 ? 5 : 6; }
 
-// True when every lane tile of the window (half-width L, tile rows V) keeps
-// its argmin codes ((dist2 - least dist2 of the tile) << bits | raster part)
-// below 2^16, so that (cost << 16) | code orders the tile exactly.
+// True when every lane tile half of the window (half-width L, tile rows V)
+// keeps its argmin codes ((dist2 - least dist2 of the half) << bits | raster
+// part) below 2^16, so that (cost << 16) | code orders the half exactly.
 inline bool lane_keys_fit(int L, int V) {
-  const int ib = lane_key_bits(V);
+  const int ib = lane_key_bits(V), V1 = lane_key_split(V);
   const int N = 2 * L + 1, B = (N + V - 1) / V, Q = (L - tile_dx0(L)) / 4 + 1;
   auto span = [](int a, int len, int* lo2, int* hi2) {  // min / max of x^2 on [a, a+len)
     const int b = a + len - 1;
@@ -103,13 +105,14 @@ inline bool lane_keys_fit(int L, int V) {
     *hi2 = M * M;
   };
   for (int b = 0; b < B; b++)
-    for (int q = 0; q < Q; q++) {
-      int xl2, xh2, yl2, yh2;
-      span(tile_dx0(L) + 4 * q, 4, &xl2, &xh2);
-      span(-L + b * V, V, &yl2, &yh2);
-      const long code = ((long)(xh2 + yh2 - xl2 - yl2) << ib) + (1 << ib) - 1;
-      if (code > 0xFFFF) return false;
-    }
+    for (int h = 0; h < (V1 < V ? 2 : 1); h++)
+      for (int q = 0; q < Q; q++) {
+        int xl2, xh2, yl2, yh2;
+        span(tile_dx0(L) + 4 * q, 4, &xl2, &xh2);
+        span(-L + b * V + h * V1, h ? V - V1 : V1, &yl2, &yh2);
+        const long code = ((long)(xh2 + yh2 - xl2 - yl2) << ib) + (1 << ib) - 1;
+        if (code > 0xFFFF) return false;
+      }
   return true;
 }
 

@@ -233,7 +233,7 @@ __device__ __forceinline__ void visit(int dx, int dy, uint32_t cc, int xl, int x
 }
 
 template <int METRIC, int V, bool TABLE>
-__device__ __forceinline__ void run_tile(const StreamDesc& s, const SearchTask& t, int u,
+__device__ __forceinline__ void run_tile(const StreamDesc& s, const SearchTask& t, int u, int b_lo,
                                          const lds_u32* win, int row_dw, const lds_u32* curs,
                                          const uint32_t* csq, uint64_t* best, int* zero,
                                          uint32_t* __restrict__ table) {
@@ -241,7 +241,8 @@ __device__ __forceinline__ void run_tile(const StreamDesc& s, const SearchTask& 
   // MBs read overlapping window dwords (MBs are 4 dwords apart), which the
   // LDS serves as broadcasts instead of bank conflicts.
   const int gq = u / s.Q, q = u - gq * s.Q;
-  const int b = gq / t.count, g = gq - b * t.count;
+  const int bi = gq / t.count, g = gq - bi * t.count;
+  const int b = b_lo + bi;
   uint32_t c[64];
   const lds_u32x4* cp = (const lds_u32x4*)(curs + g * 64);
 #pragma unroll
@@ -274,49 +275,81 @@ __device__ __forceinline__ void run_tile(const StreamDesc& s, const SearchTask& 
   } else if (METRIC == kMetricSAD) {
     // Argmin only, branch-free.  Each candidate gets a lane-local 32-bit key
     //   (cost << 16) | code,  code = ((dist2 - d2min) << IB) | (2^IB - 1 - idx)
-    // with idx = 4 v + k its raster position in the tile and d2min the least
-    // dx^2 + dy^2 over the tile: within the lane this orders candidates as
+    // with idx = 4 v + k its raster position in the tile (half) and d2min the
+    // least dx^2 + dy^2 over the tile (half): within the lane this orders candidates as
     // the full key does (cost, then distance to the zero predictor, then
     // later raster first).  The host admits a tiling only if every tile's
     // codes fit 16 bits (lane_keys_fit).  code splits into a column part
     // Ck[k] and a row part Ev, so one row costs 3 ORs (window masks), 4 key
     // builds, 3 mins and 2 adds; the full 64-bit key is built once per tile.
     // Candidates outside the window get cost 0xFFFF, above any 16x16 SAD.
+    // Tiles taller than 8 rows keep two keys, one per half (rows [0, V1)
+    // and [V1, V)), each with its own least dist2: the halves' codes stay
+    // within 16 bits at every range up to 64.
+    constexpr int V1 = lane_key_split(V);
     constexpr int IB = lane_key_bits(V);
     const uint32_t cm_lo = (dx0 >= xl && dx0 <= xr ? 0u : 0xFFFFu) |
                            (dx0 + 1 >= xl && dx0 + 1 <= xr ? 0u : 0xFFFF0000u);
     const uint32_t cm_hi = (dx0 + 2 >= xl && dx0 + 2 <= xr ? 0u : 0xFFFFu) |
                            (dx0 + 3 >= xl && dx0 + 3 <= xr ? 0u : 0xFFFF0000u);
     const int ax = dx0 > 0 ? dx0 : (dx0 + 3 < 0 ? -(dx0 + 3) : 0);
-    const int ay = dy0 > 0 ? dy0 : (dy0 + V - 1 < 0 ? -(dy0 + V - 1) : 0);
+    // least |dy| over rows [a, a + n)
+    auto min_abs = [](int a, int n) { return a > 0 ? a : (a + n - 1 < 0 ? -(a + n - 1) : 0); };
+    const int ay = min_abs(dy0, V1);
     uint32_t Ck[4];
 #pragma unroll
     for (int k = 0; k < 4; k++)
-      Ck[k] = ((uint32_t)((dx0 + k) * (dx0 + k) - ax * ax) << IB) + (uint32_t)(3 - k) + ((1u << IB) - 4u * V);
-    uint32_t Ev = ((uint32_t)(dy0 * dy0 - ay * ay) << IB) + 4u * (V - 1);
+      Ck[k] = ((uint32_t)((dx0 + k) * (dx0 + k) - ax * ax) << IB) + (uint32_t)(3 - k) + ((1u << IB) - 4u * V1);
+    // row part of the code: ((dy^2 - least dy^2 of the half) << IB) +
+    // 4 (V1 - 1 - row within the half), affine in v within each half
+    const int dyb = dy0 + V1, ayb = min_abs(dyb, V - V1);
+    const uint32_t EvA = ((uint32_t)(dy0 * dy0 - ay * ay) << IB) + 4u * (V1 - 1);
+    const uint32_t EvB = ((uint32_t)(dyb * dyb - ayb * ayb) << IB) + 4u * (V1 - 1);
     const uint32_t Dl = (uint32_t)(2 * dy0) << IB;
+    const uint32_t DlB = (uint32_t)(2 * dyb) << IB;
     // rows of the tile inside [yu, yd]: bit v
     const int vlo = max(yu - dy0, 0), vhi = min(yd - dy0, V - 1);
     const uint32_t inv_rows = vlo > vhi ? 0xFFFFFFFFu : ~(((2u << vhi) - 1u) & ~((1u << vlo) - 1u));
-    uint32_t lbest = 0xFFFFFFFFu;
+    uint32_t lbest = 0xFFFFFFFFu, lbest0 = 0xFFFFFFFFu;
+    uint32_t Evr = EvA;  // one-key tiles: the row part, updated row by row
     sad_tile<V>(wp, row_dw, c, [&](int v, uint32_t lo, uint32_t hi) {
+      const bool second = v >= V1;
+      const uint32_t u = (uint32_t)(second ? v - V1 : v);
+      // (dy+u)^2 - dy^2 = 2 dy u + u^2
+      const uint32_t Ev = V1 < V ? (second ? EvB + u * DlB : EvA + u * Dl) + ((u * u) << IB) - 4u * u : Evr;
       const uint32_t rmask = (uint32_t)__builtin_amdgcn_sbfe((int)inv_rows, v, 1);
       const uint32_t l2 = lo | cm_lo | rmask, h2 = hi | cm_hi | rmask;
       const uint32_t k0 = (l2 << 16) | Ck[0];
       const uint32_t k1 = (l2 & 0xFFFF0000u) | Ck[1];
       const uint32_t k2 = (h2 << 16) | Ck[2];
       const uint32_t k3 = (h2 & 0xFFFF0000u) | Ck[3];
-      lbest = min(lbest, min(min(k0, k1), min(k2, k3)) + Ev);
-      // Ev of row v+1: ((dy+1)^2 - dy^2) << IB, minus 4 for the raster part
-      Ev += Dl + (((2u * v + 1u) << IB) - 4u);
+      const uint32_t m = min(min(k0, k1), min(k2, k3)) + Ev;
+      if constexpr (V1 < V) {
+        if (second) lbest = min(lbest, m);
+        else lbest0 = min(lbest0, m);
+        // keeps each row's key folding in place (left free, the scheduler
+        // hoists it and the tile runs out of registers)
+        asm volatile("" : "+v"(lbest), "+v"(lbest0));
+      } else {
+        lbest0 = min(lbest0, m);
+        // Ev of row v+1: ((dy+1)^2 - dy^2) << IB, minus 4 for the raster part
+        Evr += Dl + (((2u * v + 1u) << IB) - 4u);
+      }
     });
-    if (lbest < 0xFFFF0000u) {
-      const int idx = (1 << IB) - 1 - (int)(lbest & ((1u << IB) - 1u));
-      const int dx = dx0 + (idx & 3), dy = dy0 + (idx >> 2);
-      const uint32_t cost = lbest >> 16;
-      kbest = (dx == 0 && dy == 0) ? pack_key(cost, 0, kZeroRRank)
-                                   : pack_key(cost, (uint32_t)(dx * dx + dy * dy), raster_rrank(dy, dx));
-    }
+    auto fold = [&](uint32_t lb, int vbase) {
+      if (lb < 0xFFFF0000u) {
+        const int idx = (1 << IB) - 1 - (int)(lb & ((1u << IB) - 1u));
+        const int dx = dx0 + (idx & 3), dy = dy0 + vbase + (idx >> 2);
+        const uint32_t cost = lb >> 16;
+        const uint64_t key = (dx == 0 && dy == 0)
+                                 ? pack_key(cost, 0, kZeroRRank)
+                                 : pack_key(cost, (uint32_t)(dx * dx + dy * dy), raster_rrank(dy, dx));
+        if constexpr (V1 < V) kbest = key < kbest ? key : kbest;
+        else kbest = key;  // the tile's only key
+      }
+    };
+    fold(lbest0, 0);
+    if constexpr (V1 < V) fold(lbest, V1);
   } else {
     const uint32_t c2 = csq[g];
     ssd_tile<V>(wp, row_dw, c, [&](int v, const uint32_t (&a)[4]) {
@@ -392,10 +425,11 @@ __device__ __attribute__((noinline)) void decide_in_search(const DecideDesc* d, 
 // Persistent: workgroup w runs tasks w, w + grid, ... and streams the next
 // task's window into the other half of LDS while it computes the current one.
 // Waves per SIMD the register allocation must allow: the SAD argmin kernel
-// with 8 rows per lane fits 128 VGPRs without spilling, so 4 waves; the
-// others keep the compiler's choice (taller tiles would spill).
+// with 8 rows per lane fits 128 VGPRs without spilling, so 4 waves; with
+// taller or mixed tiles 3 (168 VGPRs); the cost-table kernels keep the
+// compiler's choice.
 constexpr int search_min_waves(int metric, int vfix, bool table) {
-  return (metric == kMetricSAD && !table && vfix == 8) ? 4 : 1;
+  return (metric == kMetricSAD && !table) ? (vfix == 8 ? 4 : 3) : 1;
 }
 
 template <int METRIC, int VFIX, bool TABLE>
@@ -456,21 +490,31 @@ void me_search_kernel(const StreamDesc* __restrict__ streams, const SearchTask* 
       }
       __syncthreads();
     }
-    const int units = t.count * s.Q * s.B;
+    // row tiles that meet the task's window rows (one MB row: the same
+    // clamped [yu, yd] for every MB of the task)
+    int b_lo, nb;
+    {
+      const int m = 16 * (t.mb / s.mbw);
+      const int yu = (m - s.L >= -16) ? -s.L : -(m + 16);
+      const int yd = (m + s.L < s.H) ? s.L : s.H - m;
+      b_lo = (yu + s.L) / s.V;
+      nb = min((yd + s.L) / s.V, s.B - 1) - b_lo + 1;
+    }
+    const int units = t.count * s.Q * nb;
     for (int u = tid; u < units; u += kThreads) {
       if constexpr (METRIC == kMetricSSD) {
-        run_tile<METRIC, 8, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table);  // SSD tiles are 8 rows
+        run_tile<METRIC, 8, TABLE>(s, t, u, b_lo, win, g.row_dw4, curs, csq, best, zero, table);  // SSD tiles are 8 rows
       } else if constexpr (VFIX != 0) {
-        run_tile<METRIC, VFIX, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table);
+        run_tile<METRIC, VFIX, TABLE>(s, t, u, b_lo, win, g.row_dw4, curs, csq, best, zero, table);
       } else {
         switch (s.V) {
-          case 8: run_tile<METRIC, 8, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table); break;
-          case 11: run_tile<METRIC, 11, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table); break;
-          case 13: run_tile<METRIC, 13, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table); break;
+          case 8: run_tile<METRIC, 8, TABLE>(s, t, u, b_lo, win, g.row_dw4, curs, csq, best, zero, table); break;
+          case 11: run_tile<METRIC, 11, TABLE>(s, t, u, b_lo, win, g.row_dw4, curs, csq, best, zero, table); break;
+          case 13: run_tile<METRIC, 13, TABLE>(s, t, u, b_lo, win, g.row_dw4, curs, csq, best, zero, table); break;
           default:
             // 16-row tiles only with cost tables (argmin keys do not fit them
             // beyond small ranges, and their registers would spill here)
-            if constexpr (TABLE) run_tile<METRIC, 16, TABLE>(s, t, u, win, g.row_dw4, curs, csq, best, zero, table);
+            if constexpr (TABLE) run_tile<METRIC, 16, TABLE>(s, t, u, b_lo, win, g.row_dw4, curs, csq, best, zero, table);
             break;
         }
       }

@@ -146,7 +146,9 @@ def test_estimate_is_deterministic_and_rerunnable():
 
 
 @pytest.mark.parametrize("tile_rows,mbs_per_task,R,filt", [(8, 1, 24, 0), (11, 5, 24, 0), (13, 16, 24, 0),
-                                                           (16, 3, 16, 1), (8, 2, 64, 0), (13, 4, 16, 1)])
+                                                           (16, 3, 16, 1), (8, 2, 64, 0), (13, 4, 16, 1),
+                                                           (13, 2, 64, 0), (11, 3, 64, 0), (13, 3, 40, 0),
+                                                           (11, 1, 47, 0)])
 def test_explicit_tilings_match_oracle(tile_rows, mbs_per_task, R, filt):
     """Argmin tilings (decided inside the search kernel) and cost-table
     tilings (decided by the replay kernel)."""
@@ -179,7 +181,25 @@ def test_argmin_keys_tie_break_on_flat_and_periodic_content():
         np.testing.assert_array_equal(dist, odist)
 
 
-@pytest.mark.parametrize("tile_rows,R", [(16, 32), (16, 8), (13, 40)])
+@pytest.mark.parametrize("R", [64, 37])
+def test_split_argmin_keys_tie_break_at_large_ranges(R):
+    """Tiles taller than 8 rows keep one key per half; far from the window
+    centre the halves' distances differ most.  Flat and periodic content with
+    wide windows makes the tie-breaks decide."""
+    W, H = 352, 288
+    yy, xx = np.mgrid[0:H, 0:W]
+    cur = np.where((xx // 48 + yy // 48) % 2 == 0, 90, ((xx % 4) * 30 + (yy % 8) * 9)).astype(np.uint8)
+    ref = np.roll(cur, (-R + 3, R - 5), axis=(0, 1))
+    modes = dict(metric=1, window=1, filt=0, subpel=0, predictor=0, best_vector=0)
+    omv, odist, _, _ = orc.me_estimate(cur, ref, R, **_oracle_modes(modes))
+    for V in (8, 11, 13):
+        est = capi.MotionEstimator(_cfg(W, H, R, tile_rows=V, mbs_per_task=2, **modes))
+        mv, dist, _ = est.estimate(cur.astype(np.int16), ref.astype(np.int16))
+        np.testing.assert_array_equal(mv, omv, err_msg=f"V {V}")
+        np.testing.assert_array_equal(dist, odist)
+
+
+@pytest.mark.parametrize("tile_rows,R", [(16, 32), (16, 8), (16, 64)])
 def test_too_tall_tile_for_exact_keys_is_rejected(tile_rows, R):
     with pytest.raises(capi.IcuError) as e:
         capi.MotionEstimator(_cfg(352, 288, R, tile_rows=tile_rows, metric=1, window=1, filt=0, predictor=0))
