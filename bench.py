@@ -28,12 +28,12 @@ W, H, R = 1920, 1088, 32
 QPS = (22, 28, 34)
 
 # HBM traffic of one search launch at this workload, from two rocprofv3
-# --pmc runs of `bench.py --no-cpu --no-batched --steps 5 --warmup 1
-# --copy-probe 256` (KiB per dispatch), each counter scaled by what it reads
+# --pmc runs of `bench.py --no-cpu --no-batched --no-4k --serve-seconds 0
+# --steps 5 --warmup 1 --copy-probe 256` (KiB per dispatch), each counter scaled by what it reads
 # for the 256 MiB device copy / fill of --copy-probe: FETCH_SIZE reads half
 # the copy's 262144 KiB (x2.0), WRITE_SIZE reads it exactly (x1.0).  Measured
 # counters, not computed here: refresh them when the kernel changes.
-SEARCH_FETCH_KIB, FETCH_SCALE = 2870.0, 2.0
+SEARCH_FETCH_KIB, FETCH_SCALE = 2853.6, 2.0
 SEARCH_WRITE_KIB, WRITE_SCALE = 287.0, 1.0
 MODES = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=0)
 
