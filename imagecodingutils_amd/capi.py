@@ -84,6 +84,10 @@ _SIGS = {
     "icu_vbs_code_device": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int,
                                       C.POINTER(_vp), C.POINTER(_vp), _vp, _vp]),
     "icu_vbs_get_plan": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "icu_mc_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_vp)]),
+    "icu_mc_destroy": (C.c_int, [_vp]),
+    "icu_mc_compensate": (C.c_int, [_vp, _vp, _vp]),
+    "icu_mc_compensate_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "icu_last_error": (C.c_char_p, []),
     "icu_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_malloc": (_vp, [C.c_size_t]),
@@ -328,6 +332,39 @@ class MotionEstimatorBatch:
     def close(self):
         if getattr(self, "h", None):
             lib.icu_me_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+# ------------------------------------------------- YUV motion compensation
+class MotionCompensator:
+    """Whole-frame YUV 4:2:0 compensation (icu_mc_*)."""
+
+    def __init__(self, width: int, height: int):
+        h = _vp()
+        _check(lib.icu_mc_create(width, height, C.byref(h)))
+        self.h, self.W, self.H = h, width, height
+
+    def compensate(self, y, u, v, mv):
+        """Host planes in, compensated host planes out (uint8)."""
+        yuv = np.concatenate([np.asarray(p, np.int16).ravel() for p in (y, u, v)])
+        _check(lib.icu_mc_compensate(self.h, yuv.ctypes.data,
+                                     np.ascontiguousarray(mv, np.int16).ctypes.data))
+        ny, nc = self.W * self.H, (self.W // 2) * (self.H // 2)
+        return (yuv[:ny].reshape(self.H, self.W).astype(np.uint8),
+                yuv[ny:ny + nc].reshape(self.H // 2, self.W // 2).astype(np.uint8),
+                yuv[ny + nc:].reshape(self.H // 2, self.W // 2).astype(np.uint8))
+
+    def compensate_device(self, d_ref, d_mv, d_out, stream: Stream | None = None):
+        """d_ref, d_out: (Y, U, V) device planes; d_out may be d_ref."""
+        _check(lib.icu_mc_compensate_device(self.h, *[_p(x) for x in d_ref], _p(d_mv),
+                                            *[_p(x) for x in d_out], stream.handle if stream else None))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.icu_mc_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -123,6 +123,10 @@ struct VbsRefs {
 void launch_code_vbs(const uint8_t* cur, const VbsRefs& refs, int W, int H, const int16_t* mv8,
                      const uint8_t* ref8, const QpSet& qs, int intra, int cols_first,
                      int16_t* pred_out, hipStream_t st);
+// Whole-frame YUV 4:2:0 compensation from padded planes (luma stride ys,
+// chroma stride cs, pointers at pixel (0,0)) into 8-bit output planes.
+void launch_mc_yuv(const uint8_t* ref_y, int ys, const uint8_t* ref_u, const uint8_t* ref_v, int cs, int W,
+                   int H, const int16_t* mv, uint8_t* out_y, uint8_t* out_u, uint8_t* out_v, hipStream_t st);
 void launch_sqrt_check(const int* m, double* out, int n, hipStream_t st);
 
 }  // namespace icu

@@ -189,6 +189,53 @@ __global__ __launch_bounds__(256) void inv_kernel(const int16_t* __restrict__ in
 // Fused coder: one workgroup of 256 lanes codes 16 MBs, one lane per 4x4.
 constexpr int kMbPerWg = 16;
 
+// The 22 x 22 luma patch of one MB for quarter-pel vector (mvx, mvy), staged
+// by the MB's 16 lanes: full / quarter split with C truncation, negative
+// fractions reflected to the full pel to the left / above; the patch origin
+// is clamped into the padding, where rows and columns past the picture
+// repeat its edge, so a clamped patch predicts what the far vector would.
+// The dword reads span [ox-6, ox+24] x [oy-3, oy+18]: 22 rows x 7 dwords from
+// the dword boundary at or left of ox-3 (the padded plane's origin and
+// stride are multiples of 16), ten loads per lane, issued by load() and
+// written to LDS by store() so that other loads can go in flight between.
+struct LumaPatch {
+  uint32_t w[10];
+  int sh, qx, qy;
+  __device__ __forceinline__ void load(const uint8_t* ref, int ref_stride, int W, int H, int n, int m,
+                                       int mvx, int mvy, int blk) {
+    const int fx = mvx / 4, fy = mvy / 4;
+    qx = mvx % 4;
+    qy = mvy % 4;
+    int fox = 0, foy = 0;
+    if (qx < 0) { fox = -1; qx += 4; }
+    if (qy < 0) { foy = -1; qy += 4; }
+    const int ox = min(max(n + fx + fox, -kPad + 3), W + kPad - 25);
+    const int oy = min(max(m + fy + foy, -kPad + 3), H + kPad - 19);
+    const int xa = ox - 3;
+    sh = xa & 3;
+    const uint32_t* base = (const uint32_t*)(ref + (long)(oy - 3) * ref_stride + (xa - sh));
+    const int sdw = ref_stride >> 2;
+#pragma unroll
+    for (int j = 0; j < 10; j++) {
+      const int i = blk + 16 * j, r = i / 7, k = i - 7 * r;
+      w[j] = i < 7 * kPatch ? base[r * sdw + k] : 0u;
+    }
+  }
+  __device__ __forceinline__ void store(uint8_t* patch, int blk) const {
+#pragma unroll
+    for (int j = 0; j < 10; j++) {
+      const int i = blk + 16 * j, r = i / 7, k = i - 7 * r;
+      if (i < 7 * kPatch) {
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+          const int c = 4 * k + b - sh;
+          if (c >= 0 && c < kPatch) patch[r * kPatch + c] = (uint8_t)(w[j] >> (8 * b));
+        }
+      }
+    }
+  }
+};
+
 __global__ __launch_bounds__(256) void code_frame_kernel(const uint8_t* __restrict__ cur, int cur_stride,
                                                          const uint8_t* __restrict__ ref, int ref_stride,
                                                          int W, int H, const int16_t* __restrict__ mv,
@@ -201,51 +248,16 @@ __global__ __launch_bounds__(256) void code_frame_kernel(const uint8_t* __restri
   const bool live = mb < nmb;
   const int n = (mb % mbw) * 16, m = (mb / mbw) * 16;
   const int bi = blk >> 2, bj = blk & 3;
-  int qx = 0, qy = 0, ox = n, oy = m;
-  uint32_t w[10], cw[4];
-  int sh = 0;
+  LumaPatch lp;
+  lp.qx = lp.qy = 0;
+  uint32_t cw[4];
   if (live) {
-    const int mvx = mv[2 * mb], mvy = mv[2 * mb + 1];
-    // full / quarter split with C truncation, negative fractions reflected
-    // to the full pel to the left / above
-    const int fx = mvx / 4, fy = mvy / 4;
-    qx = mvx % 4;
-    qy = mvy % 4;
-    int fox = 0, foy = 0;
-    if (qx < 0) { fox = -1; qx += 4; }
-    if (qy < 0) { foy = -1; qy += 4; }
-    // Clamped into the padding, where rows and columns past the picture
-    // repeat its edge, so a clamped patch predicts what the far vector
-    // would; the dword reads below span [ox-6, ox+24] x [oy-3, oy+18].
-    ox = min(max(n + fx + fox, -kPad + 3), W + kPad - 25);
-    oy = min(max(m + fy + foy, -kPad + 3), H + kPad - 19);
-    // 22 rows x 7 dwords from the dword boundary at or left of ox-3 (the
-    // padded plane's origin and stride are multiples of 16): ten loads per
-    // lane, plus the four rows of this lane's 4x4 input block, all in flight
-    // before the first use
-    const int xa = ox - 3;
-    sh = xa & 3;
-    const uint32_t* base = (const uint32_t*)(ref + (long)(oy - 3) * ref_stride + (xa - sh));
-    const int sdw = ref_stride >> 2;
-#pragma unroll
-    for (int j = 0; j < 10; j++) {
-      const int i = blk + 16 * j, r = i / 7, k = i - 7 * r;
-      w[j] = i < 7 * kPatch ? base[r * sdw + k] : 0u;
-    }
+    lp.load(ref, ref_stride, W, H, n, m, mv[2 * mb], mv[2 * mb + 1], blk);
+    // this lane's 4x4 input block, in flight with the patch loads
 #pragma unroll
     for (int rr = 0; rr < 4; rr++)
       cw[rr] = *(const uint32_t*)(cur + (long)(m + 4 * bi + rr) * cur_stride + n + 4 * bj);
-#pragma unroll
-    for (int j = 0; j < 10; j++) {
-      const int i = blk + 16 * j, r = i / 7, k = i - 7 * r;
-      if (i < 7 * kPatch) {
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-          const int c = 4 * k + b - sh;
-          if (c >= 0 && c < kPatch) patch[lmb][r * kPatch + c] = (uint8_t)(w[j] >> (8 * b));
-        }
-      }
-    }
+    lp.store(patch[lmb], blk);
   }
   __syncthreads();
   if (!live) return;
@@ -253,7 +265,7 @@ __global__ __launch_bounds__(256) void code_frame_kernel(const uint8_t* __restri
 #pragma unroll
   for (int p = 0; p < 16; p++) {
     const int r = 4 * bi + (p >> 2), c = 4 * bj + (p & 3);
-    pr[p] = (int16_t)qpel_p(patch[lmb], 3 + c, 3 + r, qx, qy);
+    pr[p] = (int16_t)qpel_p(patch[lmb], 3 + c, 3 + r, lp.qx, lp.qy);
     b[p] = (int16_t)((int)((cw[p >> 2] >> (8 * (p & 3))) & 0xFF) - pr[p]);
   }
   // 4 samples per row: 8-byte level / prediction stores, 4-byte recon stores
@@ -391,6 +403,90 @@ __global__ __launch_bounds__(256) void code_vbs_kernel(const uint8_t* __restrict
   }
 }
 
+// Whole-frame YUV 4:2:0 compensation, MotionCompensatorH264ImplStd::Compensate
+// (src/CodecUtils/MotionCompensatorH264ImplStd.cpp:323-473): 16 MBs per
+// workgroup.  Lane blk of an MB predicts its 4x4 luma block at the quarter-pel
+// vector (QuarterRead from the staged patch) and 4 samples of row blk / 2 of
+// each 8x8 chroma block at the same vector read as eighth pels
+// (OverlayMem2Dv2::EighthRead, src/ImageUtils/OverlayMem2Dv2.cpp:1063-1114:
+// C-truncated full / eighth split, negative fractions one pel left / above
+// with 8 + f, bilinear ((8-x)(8-y)A + x(8-y)B + (8-x)yC + xyD + 32) >> 6,
+// zero fractions copy).  Sources are the padded copies, so the outputs may
+// be the reference planes themselves (the reference compensates in place
+// from a copy).
+__global__ __launch_bounds__(256) void mc_yuv_kernel(const uint8_t* __restrict__ ref_y, int ys,
+                                                     const uint8_t* __restrict__ ref_u,
+                                                     const uint8_t* __restrict__ ref_v, int cs, int W, int H,
+                                                     const int16_t* __restrict__ mv, uint8_t* out_y,
+                                                     uint8_t* out_u, uint8_t* out_v) {
+  __shared__ uint8_t patch[kMbPerWg][kPatch * kPatch];
+  const int mbw = W / 16, nmb = mbw * (H / 16);
+  const int lmb = threadIdx.x >> 4, blk = threadIdx.x & 15;
+  const int mb = blockIdx.x * kMbPerWg + lmb;
+  const bool live = mb < nmb;
+  const int n = (mb % mbw) * 16, m = (mb / mbw) * 16;
+  LumaPatch lp;
+  lp.qx = lp.qy = 0;
+  uint32_t cu[4], cv[4];
+  int cfx = 0, cfy = 0, csh = 0;
+  if (live) {
+    const int mvx = mv[2 * mb], mvy = mv[2 * mb + 1];
+    lp.load(ref_y, ys, W, H, n, m, mvx, mvy, blk);
+    // chroma: rows oy + r, oy + r + 1, columns ox + c0 .. + 4
+    const int Wc = W >> 1, Hc = H >> 1;
+    cfx = mvx % 8;
+    cfy = mvy % 8;
+    int ox = (n >> 1) + mvx / 8, oy = (m >> 1) + mvy / 8;
+    if (cfx < 0) { ox -= 1; cfx += 8; }
+    if (cfy < 0) { oy -= 1; cfy += 8; }
+    // clamped into the padding: beyond it every read would repeat the edge
+    ox = min(max(ox, -kPad + 3), Wc + kPad - 12);
+    oy = min(max(oy, -kPad), Hc + kPad - 9);
+    const int r = blk >> 1, xs = ox + 4 * (blk & 1);
+    csh = xs & 3;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const long o = (long)(oy + r + k) * cs + (xs - csh);
+      cu[2 * k] = *(const uint32_t*)(ref_u + o);
+      cu[2 * k + 1] = *(const uint32_t*)(ref_u + o + 4);
+      cv[2 * k] = *(const uint32_t*)(ref_v + o);
+      cv[2 * k + 1] = *(const uint32_t*)(ref_v + o + 4);
+    }
+    lp.store(patch[lmb], blk);
+  }
+  __syncthreads();
+  if (!live) return;
+  // luma 4x4 block
+  const int bi = blk >> 2, bj = blk & 3;
+#pragma unroll
+  for (int rr = 0; rr < 4; rr++) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int cc = 0; cc < 4; cc++)
+      o |= (uint32_t)qpel_p(patch[lmb], 3 + 4 * bj + cc, 3 + 4 * bi + rr, lp.qx, lp.qy) << (8 * cc);
+    *(uint32_t*)(out_y + (long)(m + 4 * bi + rr) * W + n + 4 * bj) = o;
+  }
+  // chroma: 4 samples of one row per plane
+  const long co = (long)((m >> 1) + (blk >> 1)) * (W >> 1) + (n >> 1) + 4 * (blk & 1);
+  auto chroma = [&](const uint32_t (&w)[4]) {
+    const uint64_t r0 = ((uint64_t)w[1] << 32 | w[0]) >> (8 * csh);
+    const uint64_t r1 = ((uint64_t)w[3] << 32 | w[2]) >> (8 * csh);
+    uint32_t o = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int A = (int)((r0 >> (8 * c)) & 255), B = (int)((r0 >> (8 * c + 8)) & 255);
+      const int C = (int)((r1 >> (8 * c)) & 255), D = (int)((r1 >> (8 * c + 8)) & 255);
+      const int v = (cfx | cfy) ? ((8 - cfx) * (8 - cfy) * A + cfx * (8 - cfy) * B + (8 - cfx) * cfy * C +
+                                   cfx * cfy * D + 32) >> 6
+                                : A;
+      o |= (uint32_t)v << (8 * c);
+    }
+    return o;
+  };
+  *(uint32_t*)(out_u + co) = chroma(cu);
+  *(uint32_t*)(out_v + co) = chroma(cv);
+}
+
 // Order-free checksum of a byte array: the wrapping 64-bit sum over i of
 // (p[i] + 1) * (odd 32-bit hash of i), so it does not depend on which thread
 // adds which part, and a changed or moved byte changes it.
@@ -426,6 +522,13 @@ void launch_checksum(const void* p, long bytes, unsigned long long* out, hipStre
   if (bytes <= 0) return;
   hipLaunchKernelGGL(checksum_kernel, dim3((unsigned)std::max<long>(1, blocks)), dim3(256), 0, st,
                      (const uint8_t*)p, bytes, out);
+}
+
+void launch_mc_yuv(const uint8_t* ref_y, int ys, const uint8_t* ref_u, const uint8_t* ref_v, int cs, int W,
+                   int H, const int16_t* mv, uint8_t* out_y, uint8_t* out_u, uint8_t* out_v, hipStream_t st) {
+  const int nmb = (W / 16) * (H / 16);
+  hipLaunchKernelGGL(mc_yuv_kernel, dim3((nmb + kMbPerWg - 1) / kMbPerWg), dim3(256), 0, st, ref_y, ys, ref_u,
+                     ref_v, cs, W, H, mv, out_y, out_u, out_v);
 }
 
 void launch_code_vbs(const uint8_t* cur, const VbsRefs& refs, int W, int H, const int16_t* mv8,

@@ -424,12 +424,11 @@ __device__ __attribute__((noinline)) void decide_in_search(const DecideDesc* d, 
 // predictor modes) instead of only the argmin.
 // Persistent: workgroup w runs tasks w, w + grid, ... and streams the next
 // task's window into the other half of LDS while it computes the current one.
-// Waves per SIMD the register allocation must allow: the SAD argmin kernel
-// with 8 rows per lane fits 128 VGPRs without spilling, so 4 waves; with
-// taller or mixed tiles 3 (168 VGPRs); the cost-table kernels keep the
-// compiler's choice.
+// Waves per SIMD the register allocation must allow: the SAD argmin kernels
+// 3 (168 VGPRs; the 8-row one needs 129, one over the 4-wave budget); the
+// cost-table kernels keep the compiler's choice.
 constexpr int search_min_waves(int metric, int vfix, bool table) {
-  return (metric == kMetricSAD && !table) ? (vfix == 8 ? 4 : 3) : 1;
+  return (metric == kMetricSAD && !table) ? 3 : 1;
 }
 
 template <int METRIC, int VFIX, bool TABLE>

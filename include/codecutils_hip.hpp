@@ -14,6 +14,8 @@
 //       include/CodecUtils/IForwardTransform.h:56-126
 //   IInverseTransform / FastInverse4x4ITImpl1 / FastInverse4x4On16x16ITImpl1
 //       include/CodecUtils/IInverseTransform.h:56-115
+//   IMotionCompensator / MotionCompensatorH264ImplStd
+//       include/CodecUtils/IMotionCompensator.h, MotionCompensatorH264ImplStd.h:55-110
 // They live in namespace icu_codecutils; INTEGRATION.md shows the adapter a
 // codec adds to bind them to its own CodecUtils interface types instead.
 //
@@ -310,5 +312,48 @@ class FastInverse4x4ITHip : public IInverseTransform {
 };
 typedef FastInverse4x4ITHip<0> FastInverse4x4ITImpl1;
 typedef FastInverse4x4ITHip<1> FastInverse4x4On16x16ITImpl1;
+
+// ----------------------------------------------------- motion compensation
+class IMotionCompensator {
+ public:
+  virtual ~IMotionCompensator() {}
+  virtual int Create(void* ref, int imgWidth, int imgHeight, int macroBlkWidth, int macroBlkHeight) = 0;
+  virtual void Destroy(void) = 0;
+  virtual void Reset(void) = 0;
+  virtual void Compensate(void* pMotionList) = 0;
+};
+
+// Compensates the YUV 4:2:0 reference buffer given to Create() in place, as
+// the reference class: `range` only sized the reference's extended borders
+// (here every vector is valid: samples past the picture repeat its edge).
+// 16x16 macroblocks only.
+class MotionCompensatorH264ImplStd : public IMotionCompensator {
+ public:
+  explicit MotionCompensatorH264ImplStd(int range) : _range(range) {}
+  ~MotionCompensatorH264ImplStd() override { Destroy(); }
+  int Create(void* ref, int imgWidth, int imgHeight, int macroBlkWidth, int macroBlkHeight) override {
+    Destroy();
+    if (!ref || macroBlkWidth != 16 || macroBlkHeight != 16) return 0;
+    if (icu_mc_create(imgWidth, imgHeight, &_h) != ICU_OK) return 0;
+    _ref = (int16_t*)ref;
+    return 1;
+  }
+  void Destroy(void) override {
+    if (_h) icu_mc_destroy(_h);
+    _h = nullptr;
+  }
+  void Reset(void) override {}
+  void Compensate(void* pMotionList) override {
+    VectorStructList* l = (VectorStructList*)pMotionList;
+    if (!_h || !l || l->GetLength() == 0 || l->GetType() != VectorStructList::SIMPLE2D) return;
+    icu_mc_compensate(_h, _ref, l->Data());
+  }
+  const char* LastError(void) const { return icu_last_error(); }
+
+ private:
+  int _range;
+  icu_mc* _h = nullptr;
+  int16_t* _ref = nullptr;
+};
 
 }  // namespace icu_codecutils

@@ -166,6 +166,27 @@ int icu_me_batch_capture(icu_me_batch* b, const uint8_t* const* d_curs, const ui
                          int16_t* d_mv, int32_t* d_dist);
 int icu_me_batch_graph_launch(icu_me_batch* b, void* stream);
 
+/* ------------------------------------------------- YUV motion compensation
+ * Whole-frame YUV 4:2:0 compensation from a 16x16 quarter-pel vector list:
+ * replaces MotionCompensatorH264ImplStd (ctor + Create + Compensate(pMotionList),
+ * src/CodecUtils/MotionCompensatorH264ImplStd.cpp:76-240,323-473).  Luma per
+ * MB: copy at integer vectors, else QuarterRead; chroma per 8x8 block at the
+ * same vector in eighth pels: copy, else EighthRead's bilinear
+ * (src/ImageUtils/OverlayMem2Dv2.cpp:1063-1114).  Reference samples outside
+ * the picture repeat its edge. */
+typedef struct icu_mc icu_mc;
+int icu_mc_create(int width, int height, icu_mc** out);
+int icu_mc_destroy(icu_mc* h);
+/* In place on a host buffer laid out as the reference class's: int16 luma
+ * W*H, then U and V (W/2)*(H/2), samples 0..255 (ICU_E_RANGE otherwise);
+ * mv: 2 int16 per MB, raster order (VectorStructList SIMPLE2D). */
+int icu_mc_compensate(icu_mc* h, int16_t* yuv, const int16_t* mv);
+/* Device form on 8-bit planes (stride = plane width): the compensated
+ * planes go to d_y/d_u/d_v, which may be the reference planes themselves. */
+int icu_mc_compensate_device(icu_mc* h, const uint8_t* d_ref_y, const uint8_t* d_ref_u,
+                             const uint8_t* d_ref_v, const int16_t* d_mv, uint8_t* d_y, uint8_t* d_u,
+                             uint8_t* d_v, void* stream);
+
 /* ---------------------------------------------------------------- serving
  * A request loop over many streams with continuous batching.  No reference
  * counterpart: the reference estimates and codes one frame per call on the

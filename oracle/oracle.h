@@ -56,6 +56,10 @@ void orc_mc_luma(const int16_t *ref, int W, int H, const int16_t *mv, int16_t *p
 void orc_code_residual(const int16_t *cur, const int16_t *pred, int W, int H, int qp,
                        int intra, int cols_first, int16_t *levels_out, int16_t *recon_out);
 
+/* Chroma compensation of one 4:2:0 chroma plane (Wc x Hc) from the per-MB
+ * quarter-pel luma vectors (mc_oracle.c). */
+void orc_mc_chroma(const uint8_t *ref, int Wc, int Hc, const int16_t *mv, uint8_t *out);
+
 /* Multi-reference variable-block-size integer search (vbs_oracle.c).
  * refs: nrefs contiguous 8-bit W*H planes.  mode_out: nMB (0 16x16, 1 16x8,
  * 2 8x16, 3 8x8); mv8_out: 2 int16 quarter-pel per 8x8 block, 8x8-block

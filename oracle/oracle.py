@@ -73,6 +73,7 @@ def lib():
                                        _u8p, _i16p, _u8p, _i32p]
         L.orc_vbs_estimate.restype = C.c_int
         L.orc_vbs_pred.argtypes = [_u8p, C.c_int, C.c_int, _i16p, _u8p, _i16p]
+        L.orc_mc_chroma.argtypes = [_u8p, C.c_int, C.c_int, _i16p, _u8p]
         _lib = L
     return _lib
 
@@ -93,6 +94,7 @@ def ref_lib():
         L.ref_block_dist.argtypes = [_i16p, _i16p, C.c_int]
         L.ref_block_dist_lt.argtypes = [_i16p, _i16p, C.c_int, C.c_int]
         L.ref_mc_luma.argtypes = [_i16p, C.c_int, C.c_int, C.c_int, _i16p, _i16p]
+        L.ref_mc_yuv.argtypes = [_i16p, C.c_int, C.c_int, C.c_int, _i16p]
         _ref = L
     return _ref
 
@@ -200,4 +202,14 @@ def vbs_pred(refs, mv8, ref8):
     out = np.zeros((H, W), np.int16)
     lib().orc_vbs_pred(rs.reshape(-1), W, H, np.ascontiguousarray(mv8, np.int16).reshape(-1),
                        np.ascontiguousarray(ref8, np.uint8).reshape(-1), out)
+    return out
+
+
+def mc_chroma(ref_c, mv):
+    """One 4:2:0 chroma plane compensated with the per-MB quarter-pel luma
+    vectors (mv: (nMB, 2))."""
+    ref_c = np.ascontiguousarray(ref_c, dtype=np.uint8)
+    out = np.zeros_like(ref_c)
+    lib().orc_mc_chroma(ref_c, ref_c.shape[1], ref_c.shape[0],
+                        np.ascontiguousarray(mv, dtype=np.int16).reshape(-1), out)
     return out

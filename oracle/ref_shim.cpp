@@ -196,4 +196,20 @@ int ref_mc_luma(const short* ref, int W, int H, int range, const short* mv, shor
   return 0;
 }
 
+// Whole-frame YUV 4:2:0 compensation in place (luma then both chroma planes),
+// MotionCompensatorH264ImplStd::Compensate(pMotionList).
+int ref_mc_yuv(short* yuv, int W, int H, int range, const short* mv) {
+  MotionCompensatorH264ImplStd mc(range);
+  if (!mc.Create(yuv, W, H, 16, 16)) return -1;
+  int nmb = (W / 16) * (H / 16);
+  VectorStructList l(VectorStructList::SIMPLE2D);
+  l.SetLength(nmb);
+  for (int i = 0; i < nmb; i++) {
+    l.SetSimpleElement(i, 0, mv[2 * i]);
+    l.SetSimpleElement(i, 1, mv[2 * i + 1]);
+  }
+  mc.Compensate(&l);
+  return 0;
+}
+
 }  // extern "C"

@@ -107,6 +107,29 @@ int main(int argc, char** argv) {
     save(argv[7], x.data(), n);
     return 0;
   }
-  std::fprintf(stderr, "usage: dropin_test me|fwd|inv ...\n");
+  if (cmd == "mc" && argc >= 6) {
+    // mc W H yuv.raw mv.raw: compensates the YUV buffer in place, as
+    // MotionCompensatorH264ImplStd::Compensate(pMotionList) does
+    const int W = std::atoi(argv[2]), H = std::atoi(argv[3]);
+    const size_t n = (size_t)W * H * 3 / 2;
+    auto yuv = load<short>(argv[4], n);
+    const int nmb = (W / 16) * (H / 16);
+    auto mv = load<short>(argv[5], 2 * (size_t)nmb);
+    VectorStructList l(VectorStructList::SIMPLE2D);
+    l.SetLength(nmb);
+    for (int i = 0; i < nmb; i++) {
+      l.SetSimpleElement(i, 0, mv[2 * i]);
+      l.SetSimpleElement(i, 1, mv[2 * i + 1]);
+    }
+    MotionCompensatorH264ImplStd mc(32);
+    if (!mc.Create(yuv.data(), W, H, 16, 16)) {
+      std::fprintf(stderr, "Create failed: %s\n", mc.LastError());
+      return 4;
+    }
+    mc.Compensate(&l);
+    save(argv[4], yuv.data(), n);
+    return 0;
+  }
+  std::fprintf(stderr, "usage: dropin_test me|fwd|inv|mc ...\n");
   return 1;
 }

@@ -158,6 +158,33 @@ def make_mc(R_lib):
     print("mc")
 
 
+def make_mc_yuv(R_lib):
+    """Whole-frame YUV 4:2:0 compensation: every (mvx % 8, mvy % 8) pair,
+    negative remainders included, so every chroma eighth-pel case occurs."""
+    W, H = 352, 288
+    rng = np.random.default_rng(150)
+    fp = synth.rect_motion_pair(W, H, 151, n_rects=8, max_mv=12)
+    y = fp.ref.astype(np.int16)
+    u = rng.integers(0, 256, (H // 2, W // 2)).astype(np.int16)
+    v = rng.integers(0, 256, (H // 2, W // 2)).astype(np.int16)
+    n = (W // 16) * (H // 16)
+    mv = rng.integers(-48, 49, (n, 2)).astype(np.int16)
+    pairs = [(fx, fy) for fx in range(-7, 8) for fy in range(-7, 8)]
+    for i, (fx, fy) in enumerate(pairs[:n]):
+        mv[i] = (fx + (8 if fx >= 0 else -8) * (i % 3), fy - (8 if fy >= 0 else -8) * (i % 2))
+    mv[n - 3] = (0, 0)
+    yuv = np.concatenate([y.ravel(), u.ravel(), v.ravel()]).astype(np.int16)
+    out = yuv.copy()
+    assert R_lib.ref_mc_yuv(out, W, H, 16 + 16, mv.reshape(-1)) == 0
+    ny, nc = W * H, (W // 2) * (H // 2)
+    np.savez_compressed(OUT / "mc_yuv_cif.npz", y=y.astype(np.uint8), u=u.astype(np.uint8),
+                        v=v.astype(np.uint8), mv=mv,
+                        out_y=out[:ny].reshape(H, W).astype(np.uint8),
+                        out_u=out[ny:ny + nc].reshape(H // 2, W // 2).astype(np.uint8),
+                        out_v=out[ny + nc:].reshape(H // 2, W // 2).astype(np.uint8))
+    print("mc yuv")
+
+
 def main():
     if not orc.build_ref():
         sys.exit("reference sources not available; fixtures are committed")
@@ -166,6 +193,7 @@ def main():
     make_transforms(R_lib)
     make_qpel_and_metrics(R_lib)
     make_mc(R_lib)
+    make_mc_yuv(R_lib)
 
 
 if __name__ == "__main__":

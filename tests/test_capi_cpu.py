@@ -94,3 +94,10 @@ def test_vbs_rejects_bad_configs():
         with pytest.raises(capi.IcuError) as e:
             capi.VbsEstimator(*args)
         assert e.value.code == capi.ICU_E_ARG
+
+
+def test_compensator_rejects_bad_sizes():
+    for w, h in ((30, 32), (32, 0), (0, 16), (48, 40)):
+        with pytest.raises(capi.IcuError) as e:
+            capi.MotionCompensator(w, h)
+        assert e.value.code == capi.ICU_E_ARG

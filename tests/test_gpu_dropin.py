@@ -58,3 +58,17 @@ def test_dropin_transforms_match_reference(tmp_path, qp):
         _run("inv", 1, qp, 0, mode, tmp_path / "in.raw", tmp_path / "out.raw", 4)
         np.testing.assert_array_equal(np.fromfile(tmp_path / "out.raw", np.int16).reshape(-1, 256),
                                       fx["inv_mb"][qp, mode])
+
+
+def test_dropin_compensator_matches_reference(tmp_path):
+    """MotionCompensatorH264ImplStd: Create on a YUV buffer, Compensate(list)
+    in place, against the compiled reference's output."""
+    fx = np.load(GOLDEN / "mc_yuv_cif.npz")
+    H, W = fx["y"].shape
+    yuv = np.concatenate([fx[k].astype(np.int16).ravel() for k in ("y", "u", "v")])
+    yuv.tofile(tmp_path / "yuv.raw")
+    fx["mv"].astype(np.int16).tofile(tmp_path / "mv.raw")
+    _run("mc", W, H, tmp_path / "yuv.raw", tmp_path / "mv.raw")
+    out = np.fromfile(tmp_path / "yuv.raw", np.int16)
+    want = np.concatenate([fx[k].astype(np.int16).ravel() for k in ("out_y", "out_u", "out_v")])
+    np.testing.assert_array_equal(out, want)

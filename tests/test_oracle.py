@@ -214,3 +214,45 @@ def test_round_trip_tolerance_like_reference_fragment():
         qstep = 0.625 * 2 ** (qp / 6)
         # inter rounding offset 1/6 leaves errors up to ~(5/6)*qstep per coefficient
         assert np.abs(rec.astype(np.int64) - x).max() <= 2 * qstep + 1
+
+
+def test_mc_yuv_matches_reference():
+    """Whole-frame YUV compensation by the compiled reference
+    (MotionCompensatorH264ImplStd::Compensate): every eighth-pel chroma case,
+    negative remainders included."""
+    fx = np.load(GOLDEN / "mc_yuv_cif.npz")
+    np.testing.assert_array_equal(orc.mc_luma(fx["y"], fx["mv"]).astype(np.uint8), fx["out_y"])
+    np.testing.assert_array_equal(orc.mc_chroma(fx["u"], fx["mv"]), fx["out_u"])
+    np.testing.assert_array_equal(orc.mc_chroma(fx["v"], fx["mv"]), fx["out_v"])
+
+
+def test_mc_chroma_closed_forms():
+    rng = np.random.default_rng(77)
+    Wc, Hc = 48, 32
+    n = (Wc // 8) * (Hc // 8)
+    ref = rng.integers(0, 256, (Hc, Wc)).astype(np.uint8)
+    # whole-pel chroma vectors (luma quarter-pel multiples of 8) copy
+    mv = (rng.integers(-3, 4, (n, 2)) * 8).astype(np.int16)
+    out = orc.mc_chroma(ref, mv)
+    for i in range(n):
+        r, c = divmod(i, Wc // 8)
+        ys = np.clip(np.arange(8 * r, 8 * r + 8) + mv[i, 1] // 8, 0, Hc - 1)
+        xs = np.clip(np.arange(8 * c, 8 * c + 8) + mv[i, 0] // 8, 0, Wc - 1)
+        np.testing.assert_array_equal(out[8 * r:8 * r + 8, 8 * c:8 * c + 8], ref[np.ix_(ys, xs)])
+    # a constant plane stays constant at every fraction
+    flat = np.full((Hc, Wc), 131, np.uint8)
+    mv = rng.integers(-60, 61, (n, 2)).astype(np.int16)
+    assert (orc.mc_chroma(flat, mv) == 131).all()
+    # a horizontal ramp at +x/8 (x > 0) is the exact interpolation
+    ramp = np.tile((np.arange(Wc) * 4).astype(np.uint8), (Hc, 1))
+    for fxv in range(1, 8):
+        mv = np.tile(np.array([[fxv, 0]], np.int16), (n, 1))
+        out = orc.mc_chroma(ramp, mv).astype(int)
+        inner = out[:, :Wc - 8]
+        want = (ramp[:, :Wc - 8].astype(int) * 64 + 4 * fxv * 8 + 32) >> 6
+        np.testing.assert_array_equal(inner, want)
+    # -1/8 reads one pel to the left with fraction 7/8
+    mv = np.tile(np.array([[-1, 0]], np.int16), (n, 1))
+    out = orc.mc_chroma(ramp, mv).astype(int)
+    want = ((ramp[:, 7:Wc - 9].astype(int) * 64) + 4 * 7 * 8 + 32) >> 6
+    np.testing.assert_array_equal(out[:, 8:Wc - 8], want)
