@@ -88,6 +88,13 @@ _SIGS = {
     "icu_mc_destroy": (C.c_int, [_vp]),
     "icu_mc_compensate": (C.c_int, [_vp, _vp, _vp]),
     "icu_mc_compensate_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "icu_yuv_file_open": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "icu_yuv_file_next_unit": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(C.c_int)]),
+    "icu_yuv_file_put_unit": (C.c_int, [_vp, _vp, C.c_int]),
+    "icu_yuv_file_close": (C.c_int, [_vp]),
+    "icu_yuv_unpack_device": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "icu_mv_dump": (C.c_int, [C.c_char_p, C.c_char_p, _vp, C.c_int]),
+    "icu_mv_load": (C.c_int, [C.c_char_p, _vp, C.c_int, C.POINTER(C.c_int)]),
     "icu_last_error": (C.c_char_p, []),
     "icu_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_malloc": (_vp, [C.c_size_t]),
@@ -336,6 +343,61 @@ class MotionEstimatorBatch:
 
     def __del__(self):
         self.close()
+
+
+# ------------------------------------------------------ raw YUV / MV files
+YUV420_8P, YUV420_NV12, YUV420_16P = 0, 1, 16
+
+
+class YuvFile:
+    """YuvRawFileHandler over icu_yuv_file_* (read: next_unit(); write:
+    put_unit(), written on close)."""
+
+    def __init__(self, path, width, height, in_type=YUV420_8P, out_type=YUV420_16P, write=False):
+        h = _vp()
+        _check(lib.icu_yuv_file_open(str(path).encode(), int(write), width, height, in_type, out_type,
+                                     C.byref(h)))
+        self.h, self.W, self.H, self.out_type = h, width, height, out_type
+
+    def next_unit(self):
+        """(samples of the whole unit buffer as a copy, unit length)."""
+        p, n = _vp(), C.c_int()
+        _check(lib.icu_yuv_file_next_unit(self.h, C.byref(p), C.byref(n)))
+        total = self.W * self.H * 3 // 2
+        ct = C.c_int16 if self.out_type == YUV420_16P else C.c_uint8
+        buf = np.ctypeslib.as_array(C.cast(p, C.POINTER(ct)), shape=(total,)).copy()
+        return buf, n.value
+
+    def put_unit(self, data: bytes | np.ndarray):
+        b = np.ascontiguousarray(np.frombuffer(bytes(data), np.uint8) if isinstance(data, bytes) else data)
+        _check(lib.icu_yuv_file_put_unit(self.h, b.ctypes.data, b.nbytes))
+
+    def close(self):
+        if getattr(self, "h", None):
+            h, self.h = self.h, None
+            _check(lib.icu_yuv_file_close(h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.icu_yuv_file_close(self.h)
+            self.h = None
+
+
+def yuv_unpack_device(d_raw, in_type, width, height, d_y, d_u, d_v, stream: Stream | None = None):
+    _check(lib.icu_yuv_unpack_device(_p(d_raw), in_type, width, height, _p(d_y), _p(d_u), _p(d_v),
+                                     stream.handle if stream else None))
+
+
+def mv_dump(path, mv: np.ndarray, title: str = ""):
+    m = np.ascontiguousarray(mv, np.int16)
+    _check(lib.icu_mv_dump(str(path).encode(), title.encode(), m.ctypes.data, m.size // 2))
+
+
+def mv_load(path, max_n: int) -> np.ndarray:
+    out = np.zeros((max_n, 2), np.int16)
+    n = C.c_int()
+    _check(lib.icu_mv_load(str(path).encode(), out.ctypes.data, max_n, C.byref(n)))
+    return out[:n.value]
 
 
 # ------------------------------------------------- YUV motion compensation

@@ -14,6 +14,7 @@
 //       include/CodecUtils/IForwardTransform.h:56-126
 //   IInverseTransform / FastInverse4x4ITImpl1 / FastInverse4x4On16x16ITImpl1
 //       include/CodecUtils/IInverseTransform.h:56-115
+//   YuvRawFileHandler   include/CodecUtils/YuvRawFileHandler.h:53-98
 //   IMotionCompensator / MotionCompensatorH264ImplStd
 //       include/CodecUtils/IMotionCompensator.h, MotionCompensatorH264ImplStd.h:55-110
 // They live in namespace icu_codecutils; INTEGRATION.md shows the adapter a
@@ -44,6 +45,11 @@ class VectorStructList {
   int GetSimpleElement(int pos, int element) const { return _v[2 * (size_t)pos + element]; }
   void SetSimpleElement(int pos, int element, int value) { _v[2 * (size_t)pos + element] = (int16_t)value; }
   int16_t* Data(void) { return _v.data(); }
+  // VectorStructList::Dump: "mv x,mv y" then one "x,y" line per vector (the
+  // reference writes no title line)
+  void Dump(char* filename, const char* title) {
+    if (_type == SIMPLE2D) icu_mv_dump(filename, title, _v.data(), GetLength());
+  }
 
  private:
   int _type;
@@ -354,6 +360,54 @@ class MotionCompensatorH264ImplStd : public IMotionCompensator {
   int _range;
   icu_mc* _h = nullptr;
   int16_t* _ref = nullptr;
+};
+
+// ---------------------------------------------------------------- raw YUV
+// Reads a raw 4:2:0 file unit by unit (YUV4208P or YUV420NV12 in, YUV42016P
+// or YUV4208P out) or collects units to write on Close(), as the reference
+// handler; rw 0 reads, 1 writes.
+class YuvRawFileHandler {
+ public:
+  static const int YUV4208P = ICU_YUV420_8P;
+  static const int YUV420NV12 = ICU_YUV420_NV12;
+  static const int YUV42016P = ICU_YUV420_16P;
+  ~YuvRawFileHandler() { Close(); }
+  void SetWidth(int width) { _w = width; }
+  void SetHeight(int height) { _h = height; }
+  void SetType(int type) { _in = type; }
+  void SetType(int intype, int outtype) {
+    _in = intype;
+    _out = outtype;
+  }
+  int Open(const char* filename, int rw) {
+    Close();
+    return icu_yuv_file_open(filename, rw, _w, _h, _in, _out, &_f) == ICU_OK ? 1 : 0;
+  }
+  void Close(void) {
+    if (_f) icu_yuv_file_close(_f);
+    _f = nullptr;
+    _unit = nullptr;
+    _len = 0;
+  }
+  void* GetNextUnit(int* unitLen) {
+    *unitLen = 0;
+    if (!_f || icu_yuv_file_next_unit(_f, (const void**)&_unit, unitLen) != ICU_OK) return nullptr;
+    _len = *unitLen;
+    return _unit;
+  }
+  void* GetCurrUnit(int* unitLen) {
+    *unitLen = _w * _h * 3 / 2;
+    return _unit;
+  }
+  int PutNextUnit(void* unit, int unitLen) {
+    return _f && icu_yuv_file_put_unit(_f, unit, unitLen) == ICU_OK ? 1 : 0;
+  }
+
+ private:
+  int _w = 352, _h = 288, _in = YUV4208P, _out = YUV42016P;
+  icu_yuv_file* _f = nullptr;
+  void* _unit = nullptr;
+  int _len = 0;
 };
 
 }  // namespace icu_codecutils

@@ -187,6 +187,40 @@ int icu_mc_compensate_device(icu_mc* h, const uint8_t* d_ref_y, const uint8_t* d
                              const uint8_t* d_ref_v, const int16_t* d_mv, uint8_t* d_y, uint8_t* d_u,
                              uint8_t* d_v, void* stream);
 
+/* --------------------------------------------------- raw YUV and MV files
+ * YuvRawFileHandler (include/CodecUtils/YuvRawFileHandler.h:53-98,
+ * src/CodecUtils/YuvRawFileHandler.cpp:78-184 over RawFileHandlerBase,
+ * src/CodecUtils/RawFileHandlerBase.cpp:82-180): a read handle maps the
+ * file; each unit is one 4:2:0 frame from the current position, wrapping
+ * to the start once the position reaches the last byte; a short last unit
+ * reports how many samples it read (NV12: plus W*H/4, as the reference
+ * counts) and keeps the previous unit's samples beyond them.  Types: the
+ * handler's YUV4208P, YUV420NV12 (in only) and YUV42016P (out only). */
+#define ICU_YUV420_8P 0
+#define ICU_YUV420_NV12 1
+#define ICU_YUV420_16P 16
+typedef struct icu_yuv_file icu_yuv_file;
+/* rw 0 read, 1 write (written on close, as the handler's delayed write). */
+int icu_yuv_file_open(const char* path, int rw, int width, int height, int in_type, int out_type,
+                      icu_yuv_file** out);
+/* GetNextUnit: *unit points at the handle's buffer (int16 for 16P, uint8
+ * for 8P) of W*H*3/2 samples, valid until the next call or close. */
+int icu_yuv_file_next_unit(icu_yuv_file* f, const void** unit, int* unit_len);
+/* PutNextUnit: appends bytes to the output stream. */
+int icu_yuv_file_put_unit(icu_yuv_file* f, const void* unit, int bytes);
+int icu_yuv_file_close(icu_yuv_file* f);
+/* A raw frame already on the device (I420 or NV12 bytes) into the 8-bit
+ * Y/U/V planes the estimator and compensator take (asynchronous). */
+int icu_yuv_unpack_device(const uint8_t* d_raw, int in_type, int width, int height, uint8_t* d_y,
+                          uint8_t* d_u, uint8_t* d_v, void* stream);
+/* VectorStructList::Dump (src/CodecUtils/VectorStructList.cpp:242-270) of a
+ * SIMPLE2D list: "mv x,mv y" then one "x,y" line per vector.  The title is
+ * accepted and, as in the reference (which sets it before its table
+ * exists), not written.  icu_mv_load reads such a file back into at most
+ * max_n vectors (*n receives the count). */
+int icu_mv_dump(const char* path, const char* title, const int16_t* mv, int n);
+int icu_mv_load(const char* path, int16_t* mv, int max_n, int* n);
+
 /* ---------------------------------------------------------------- serving
  * A request loop over many streams with continuous batching.  No reference
  * counterpart: the reference estimates and codes one frame per call on the

@@ -95,6 +95,8 @@ def ref_lib():
         L.ref_block_dist_lt.argtypes = [_i16p, _i16p, C.c_int, C.c_int]
         L.ref_mc_luma.argtypes = [_i16p, C.c_int, C.c_int, C.c_int, _i16p, _i16p]
         L.ref_mc_yuv.argtypes = [_i16p, C.c_int, C.c_int, C.c_int, _i16p]
+        L.ref_yuv_read.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _i32p]
+        L.ref_mv_dump.argtypes = [C.c_char_p, C.c_char_p, _i16p, C.c_int]
         _ref = L
     return _ref
 

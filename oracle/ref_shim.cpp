@@ -10,6 +10,8 @@
 //   FastForward4x4ITImpl1/ITImpl2/On16x16ITImpl1, FastInverse4x4ITImpl1/On16x16ITImpl1
 //   OverlayMem2Dv2 / OverlayExtMem2Dv2   src/ImageUtils/OverlayMem2Dv2.cpp:704,1623,2655
 //   MotionCompensatorH264ImplStd         src/CodecUtils/MotionCompensatorH264ImplStd.cpp:323-473
+//   YuvRawFileHandler                    src/CodecUtils/YuvRawFileHandler.cpp:78-184
+//   VectorStructList::Dump               src/CodecUtils/VectorStructList.cpp:242-270
 #include <math.h>  // the estimator header calls sqrt without including it
 
 #include <chrono>
@@ -29,6 +31,7 @@
 #include "OverlayExtMem2Dv2.h"
 #include "OverlayMem2Dv2.h"
 #include "VectorStructList.h"
+#include "YuvRawFileHandler.h"
 
 namespace {
 
@@ -209,6 +212,38 @@ int ref_mc_yuv(short* yuv, int W, int H, int range, const short* mv) {
     l.SetSimpleElement(i, 1, mv[2 * i + 1]);
   }
   mc.Compensate(&l);
+  return 0;
+}
+
+// Reads `units` units with YuvRawFileHandler::GetNextUnit; out receives
+// units x (frame elements) samples as int (16-bit or 8-bit outputs widened),
+// lens the unit lengths.
+int ref_yuv_read(const char* path, int W, int H, int in_type, int out_type, int units, int* out, int* lens) {
+  YuvRawFileHandler f;
+  f.SetWidth(W);
+  f.SetHeight(H);
+  f.SetType(in_type, out_type);
+  if (!f.Open(path, 0)) return -1;
+  const int n = W * H * 3 / 2;
+  for (int u = 0; u < units; u++) {
+    int len = 0;
+    void* p = f.GetNextUnit(&len);
+    lens[u] = len;
+    for (int i = 0; i < n; i++)
+      out[(long)u * n + i] = out_type == YuvRawFileHandler::YUV42016P ? ((short*)p)[i] : ((unsigned char*)p)[i];
+  }
+  f.Close();
+  return 0;
+}
+
+int ref_mv_dump(const char* path, const char* title, const short* mv, int n) {
+  VectorStructList l(VectorStructList::SIMPLE2D);
+  l.SetLength(n);
+  for (int i = 0; i < n; i++) {
+    l.SetSimpleElement(i, 0, mv[2 * i]);
+    l.SetSimpleElement(i, 1, mv[2 * i + 1]);
+  }
+  l.Dump((char*)path, title);
   return 0;
 }
 

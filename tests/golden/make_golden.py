@@ -185,6 +185,36 @@ def make_mc_yuv(R_lib):
     print("mc yuv")
 
 
+def make_io(R_lib):
+    """YuvRawFileHandler units (I420 and NV12 in, 16-bit or 8-bit planar out,
+    a short last unit, the wrap to the start) and VectorStructList::Dump
+    text from the compiled reference."""
+    import tempfile
+    W, H = 32, 16
+    fb = W * H * 3 // 2
+    rng = np.random.default_rng(160)
+    data = rng.integers(0, 256, 2 * fb + fb // 2).astype(np.uint8)
+    out = {"raw": data, "W": np.int32(W), "H": np.int32(H)}
+    with tempfile.TemporaryDirectory() as d:
+        path = f"{d}/in.yuv"
+        data.tofile(path)
+        units = 5
+        for name, it, ot in (("i420_16", 0, 16), ("i420_8", 0, 0), ("nv12_16", 1, 16)):
+            buf = np.zeros(units * fb, np.int32)
+            lens = np.zeros(units, np.int32)
+            assert R_lib.ref_yuv_read(path.encode(), W, H, it, ot, units, buf, lens) == 0
+            out[name] = buf.reshape(units, fb)
+            out[name + "_len"] = lens
+        mv = np.array([[0, 0], [4, -8], [-13, 7], [255, -256], [-1, 1], [32767, -32768], [2, 3]], np.int16)
+        out["mv"] = mv
+        for tag, title in (("titled", "frame 7 vectors"), ("untitled", "")):
+            p = f"{d}/mv_{tag}.csv"
+            assert R_lib.ref_mv_dump(p.encode(), title.encode(), mv.reshape(-1), len(mv)) == 0
+            out["dump_" + tag] = np.frombuffer(open(p, "rb").read(), np.uint8)
+    np.savez_compressed(OUT / "io.npz", **out)
+    print("io")
+
+
 def main():
     if not orc.build_ref():
         sys.exit("reference sources not available; fixtures are committed")
@@ -194,6 +224,7 @@ def main():
     make_qpel_and_metrics(R_lib)
     make_mc(R_lib)
     make_mc_yuv(R_lib)
+    make_io(R_lib)
 
 
 if __name__ == "__main__":
