@@ -256,6 +256,59 @@ def run_mc_yuv(capi, synth, steps, warmup):
             "GBps_algorithmic": round(nbytes / (ms / 1e3) / 1e9, 1), "bytes_algorithmic": nbytes}
 
 
+def run_cavlc(capi, synth, steps, warmup):
+    """CAVLC of one 1080p frame's luma levels: the 130560 4x4 blocks that
+    the search + MC + IT/quant at QP 28 produce, each with its nC from its
+    left and upper neighbours' coefficient counts (H.264 9.2.1), encoded into
+    one stream (count, scan, one length read back, write), then decoded
+    from the offsets."""
+    fp = synth.rect_motion_pair(W, H, 1080, n_rects=24, max_mv=32, sigma_cur=3.0)
+    est = capi.MotionEstimator(capi.make_config(W, H, R, **MODES))
+    d_cur, d_ref = capi.DeviceBuffer.from_array(fp.cur), capi.DeviceBuffer.from_array(fp.ref)
+    n_mb = (W // 16) * (H // 16)
+    d_mv, d_dist = capi.DeviceBuffer(4 * n_mb), capi.DeviceBuffer(4 * n_mb)
+    s = capi.Stream()
+    est.estimate_device(d_cur, d_ref, d_mv, d_dist, stream=s)
+    d_lv, d_rec = capi.DeviceBuffer(2 * W * H), capi.DeviceBuffer(W * H)
+    est.code_frame_device(d_cur, d_mv, 28, 0, 0, d_lv, d_rec, stream=s)
+    s.sync()
+    # picture layout -> raster 4x4 blocks, block-major (input preparation)
+    lv = d_lv.download(np.int16, (H, W))
+    blocks = np.ascontiguousarray(lv.reshape(H // 4, 4, W // 4, 4).transpose(0, 2, 1, 3).reshape(-1, 16))
+    n = len(blocks)
+    tc = np.count_nonzero(blocks, axis=1).reshape(H // 4, W // 4)
+    na = np.full_like(tc, -1)
+    nb = np.full_like(tc, -1)
+    na[:, 1:], nb[1:, :] = tc[:, :-1], tc[:-1, :]
+    nc = np.where((na >= 0) & (nb >= 0), (na + nb + 1) >> 1, np.maximum(np.maximum(na, nb), 0))
+    d_coef = capi.DeviceBuffer.from_array(blocks)
+    d_nc = capi.DeviceBuffer.from_array(nc.reshape(-1).astype(np.int8))
+    cap = n * 160
+    d_stream, d_off = capi.DeviceBuffer(cap), capi.DeviceBuffer(8 * (n + 1))
+    d_dec, d_status = capi.DeviceBuffer(2 * 16 * n), capi.DeviceBuffer(4 * n)
+    dev = capi.CavlcDevice()
+    for _ in range(warmup):
+        bits = dev.encode(d_coef, n, 1, d_nc, 0, 0, d_stream, cap, d_off, stream=s)
+        dev.decode(d_stream, cap, d_off, n, 1, d_nc, 0, 0, d_dec, d_status, stream=s)
+    s.sync()
+    a, b, c = capi.Event(), capi.Event(), capi.Event()
+    a.record(s)
+    for _ in range(steps):
+        dev.encode(d_coef, n, 1, d_nc, 0, 0, d_stream, cap, d_off, stream=s)
+    b.record(s)
+    for _ in range(steps):
+        dev.decode(d_stream, cap, d_off, n, 1, d_nc, 0, 0, d_dec, d_status, stream=s)
+    c.record(s)
+    s.sync()
+    enc_ms, dec_ms = a.elapsed_ms(b) / steps, b.elapsed_ms(c) / steps
+    ok = bool((d_dec.download(np.int16, (n, 16)) == blocks).all())
+    return {"workload": "CAVLC of a 1080p frame's luma levels at QP 28 (130560 4x4 blocks, nC from neighbours)",
+            "encode_ms": round(enc_ms, 4), "decode_ms": round(dec_ms, 4),
+            "encode_mbs_per_s": round(n_mb / (enc_ms / 1e3), 1),
+            "decode_mbs_per_s": round(n_mb / (dec_ms / 1e3), 1),
+            "stream_bits": int(bits), "nonzero_blocks": int(np.count_nonzero(tc)), "round_trip_exact": ok}
+
+
 SERVE_STREAMS, SERVE_RATE, SERVE_QP, SERVE_BUDGET = 32, 2000.0, 28, 64 << 30
 SERVE_RING = 3  # distinct frames generated per stream, submitted in turn
 
@@ -451,6 +504,7 @@ def main():
                           "batched_vs_plain": round(bt["mbps"] / bt["plain_mbps"], 3),
                           "achieved_T_sad_ops": round(bt["sad_ops"] / (bt["ms"] / 1e3) / 1e12, 2)}
     out["mc_yuv_1080p"] = run_mc_yuv(capi, synth, 20, 3)
+    out["cavlc_1080p"] = run_cavlc(capi, synth, 20, 3)
     if not args.no_4k:
         out["multiref_4k"] = run_multiref_4k(capi, synth, max(5, args.steps // 5), 2)
     if args.serve_seconds > 0:

@@ -95,6 +95,16 @@ _SIGS = {
     "icu_yuv_unpack_device": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
     "icu_mv_dump": (C.c_int, [C.c_char_p, C.c_char_p, _vp, C.c_int]),
     "icu_mv_load": (C.c_int, [C.c_char_p, _vp, C.c_int, C.POINTER(C.c_int)]),
+    "icu_cavlc_encode": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_long, C.POINTER(C.c_long),
+                                   C.POINTER(C.c_int)]),
+    "icu_cavlc_decode": (C.c_int, [_vp, C.c_long, C.POINTER(C.c_long), C.c_int, C.c_int, C.c_int, _vp,
+                                   C.POINTER(C.c_int)]),
+    "icu_cavlc_create": (C.c_int, [C.POINTER(_vp)]),
+    "icu_cavlc_destroy": (C.c_int, [_vp]),
+    "icu_cavlc_encode_device": (C.c_int, [_vp, _vp, C.c_long, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_long, _vp,
+                                          C.POINTER(C.c_longlong), _vp]),
+    "icu_cavlc_decode_device": (C.c_int, [_vp, _vp, C.c_long, _vp, C.c_long, C.c_int, _vp, C.c_int, C.c_int, _vp,
+                                          _vp, _vp]),
     "icu_last_error": (C.c_char_p, []),
     "icu_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_malloc": (_vp, [C.c_size_t]),
@@ -339,6 +349,74 @@ class MotionEstimatorBatch:
     def close(self):
         if getattr(self, "h", None):
             lib.icu_me_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+# -------------------------------------------------------------------- CAVLC
+CAVLC_2X2, CAVLC_4X4 = 0, 1
+
+
+def cavlc_encode(blocks: np.ndarray, mode: int, ncs, dc_skip: int = 0):
+    """Host encode of blocks (n x 16 or n x 4 int16) one after another into
+    one MSB-first stream.  Returns (stream bytes, bits per block)."""
+    b = np.ascontiguousarray(blocks, np.int16)
+    n = b.shape[0]
+    ncs = np.broadcast_to(np.asarray(ncs, np.int64), (n,))
+    cap = 64 + 1200 * n
+    out = np.zeros((cap + 7) // 8, np.uint8)
+    pos = C.c_long(0)
+    bits = np.zeros(n, np.int32)
+    nb = C.c_int()
+    for i in range(n):
+        _check(lib.icu_cavlc_encode(b[i].ctypes.data, mode, int(ncs[i]), dc_skip, out.ctypes.data, cap,
+                                    C.byref(pos), C.byref(nb)))
+        bits[i] = nb.value
+    return out[:(pos.value + 7) // 8], bits
+
+
+def cavlc_decode(stream: np.ndarray, nblocks: int, mode: int, ncs, dc_skip: int = 0):
+    s = np.ascontiguousarray(stream, np.uint8)
+    ncs = np.broadcast_to(np.asarray(ncs, np.int64), (nblocks,))
+    size = 4 if mode == CAVLC_2X2 else 16
+    out = np.zeros((nblocks, size), np.int16)
+    bits = np.zeros(nblocks, np.int32)
+    pos = C.c_long(0)
+    nb = C.c_int()
+    for i in range(nblocks):
+        _check(lib.icu_cavlc_decode(s.ctypes.data, s.size * 8, C.byref(pos), mode, int(ncs[i]), dc_skip,
+                                    out[i].ctypes.data, C.byref(nb)))
+        bits[i] = nb.value
+    return out, bits
+
+
+class CavlcDevice:
+    """Batched device CAVLC (icu_cavlc_*_device)."""
+
+    def __init__(self):
+        h = _vp()
+        _check(lib.icu_cavlc_create(C.byref(h)))
+        self.h = h
+
+    def encode(self, d_coef, nblocks, mode, d_nc, nc, dc_skip, d_stream, stream_bytes, d_offsets,
+               stream: Stream | None = None) -> int:
+        tot = C.c_longlong()
+        _check(lib.icu_cavlc_encode_device(self.h, _p(d_coef), nblocks, mode, _p(d_nc), nc, dc_skip, _p(d_stream),
+                                           stream_bytes, _p(d_offsets), C.byref(tot),
+                                           stream.handle if stream else None))
+        return tot.value
+
+    def decode(self, d_stream, stream_bytes, d_offsets, nblocks, mode, d_nc, nc, dc_skip, d_coef, d_status=None,
+               stream: Stream | None = None):
+        _check(lib.icu_cavlc_decode_device(self.h, _p(d_stream), stream_bytes, _p(d_offsets), nblocks, mode,
+                                           _p(d_nc), nc, dc_skip, _p(d_coef), _p(d_status),
+                                           stream.handle if stream else None))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.icu_cavlc_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -187,6 +187,44 @@ int icu_mc_compensate_device(icu_mc* h, const uint8_t* d_ref_y, const uint8_t* d
                              const uint8_t* d_ref_v, const int16_t* d_mv, uint8_t* d_y, uint8_t* d_u,
                              uint8_t* d_v, void* stream);
 
+/* ------------------------------------------------------------------ CAVLC
+ * Context-adaptive VLC of one block of quantised levels (H.264 9.2):
+ * replaces CAVLCH264Impl::Encode / Decode with its VLC coders
+ * (src/CodecUtils/CAVLCH264Impl.cpp:134-577; SetMode Mode2x2 / Mode4x4,
+ * SetParameter NUM_TOT_NEIGHBOR_COEFF_ID = nc, DC_SKIP_FLAG_ID = dc_skip)
+ * over an MSB-first bit stream (BitStreamWriterMSB / BitStreamReaderMSB).
+ * Blocks are 16 int16 levels in raster order (mode ICU_CAVLC_4X4) or 4
+ * (ICU_CAVLC_2X2, chroma DC; nc -1 selects the 4:2:0 chroma DC
+ * coeff_token table, -2 the 4:2:2 one). */
+#define ICU_CAVLC_2X2 0
+#define ICU_CAVLC_4X4 1
+/* Writes the block's code at *bit_pos of stream (stream_bits long; bits are
+ * set or cleared, others untouched) and advances *bit_pos; stream NULL only
+ * counts.  *nbits: bits written.  ICU_E_RANGE when the stream is full. */
+int icu_cavlc_encode(const int16_t* coef, int mode, int nc, int dc_skip, uint8_t* stream, long stream_bits,
+                     long* bit_pos, int* nbits);
+/* Reads one block from *bit_pos (all mode-size outputs written; zeros
+ * where no level), advances *bit_pos.  ICU_E_RANGE: the stream ended;
+ * ICU_E_STATE: no such code. */
+int icu_cavlc_decode(const uint8_t* stream, long stream_bits, long* bit_pos, int mode, int nc, int dc_skip,
+                     int16_t* coef, int* nbits);
+/* Batches on the device.  Encode nblocks blocks (block b with nC d_nc[b],
+ * or nc for all when d_nc is NULL) into one MSB-first stream in block
+ * order; d_offsets (nblocks + 1 int64) receives each block's first bit and,
+ * last, the total, also returned in *total_bits.  Waits for the stream
+ * length (one device-to-host read) before writing; ICU_E_RANGE when it
+ * exceeds stream_bytes.  Decode takes such offsets; d_status (nullable)
+ * receives each block's bits or a negative error. */
+typedef struct icu_cavlc icu_cavlc;
+int icu_cavlc_create(icu_cavlc** out);
+int icu_cavlc_destroy(icu_cavlc* h);
+int icu_cavlc_encode_device(icu_cavlc* h, const int16_t* d_coef, long nblocks, int mode, const int8_t* d_nc,
+                            int nc, int dc_skip, uint8_t* d_stream, long stream_bytes, int64_t* d_offsets,
+                            long long* total_bits, void* stream);
+int icu_cavlc_decode_device(icu_cavlc* h, const uint8_t* d_stream, long stream_bytes, const int64_t* d_offsets,
+                            long nblocks, int mode, const int8_t* d_nc, int nc, int dc_skip, int16_t* d_coef,
+                            int32_t* d_status, void* stream);
+
 /* --------------------------------------------------- raw YUV and MV files
  * YuvRawFileHandler (include/CodecUtils/YuvRawFileHandler.h:53-98,
  * src/CodecUtils/YuvRawFileHandler.cpp:78-184 over RawFileHandlerBase,

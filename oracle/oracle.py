@@ -97,6 +97,7 @@ def ref_lib():
         L.ref_mc_yuv.argtypes = [_i16p, C.c_int, C.c_int, C.c_int, _i16p]
         L.ref_yuv_read.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _i32p]
         L.ref_mv_dump.argtypes = [C.c_char_p, C.c_char_p, _i16p, C.c_int]
+        L.ref_cavlc.argtypes = [_i16p, C.c_int, C.c_int, _i32p, C.c_int, _u8p, C.c_int, _i32p, _i16p, _i32p]
         _ref = L
     return _ref
 

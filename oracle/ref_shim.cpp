@@ -12,6 +12,7 @@
 //   MotionCompensatorH264ImplStd         src/CodecUtils/MotionCompensatorH264ImplStd.cpp:323-473
 //   YuvRawFileHandler                    src/CodecUtils/YuvRawFileHandler.cpp:78-184
 //   VectorStructList::Dump               src/CodecUtils/VectorStructList.cpp:242-270
+//   CAVLCH264Impl + its VLC coders       src/CodecUtils/CAVLCH264Impl.cpp:134-577
 #include <math.h>  // the estimator header calls sqrt without including it
 
 #include <chrono>
@@ -32,6 +33,19 @@
 #include "OverlayMem2Dv2.h"
 #include "VectorStructList.h"
 #include "YuvRawFileHandler.h"
+#include "BitStreamReaderMSB.h"
+#include "BitStreamWriterMSB.h"
+#include "CAVLCH264Impl.h"
+#include "CoeffTokenH264VlcDecoder.h"
+#include "CoeffTokenH264VlcEncoder.h"
+#include "PrefixH264VlcDecoderImpl1.h"
+#include "PrefixH264VlcEncoderImpl1.h"
+#include "RunBeforeH264VlcDecoder.h"
+#include "RunBeforeH264VlcEncoder.h"
+#include "TotalZeros2x2H264VlcDecoder.h"
+#include "TotalZeros2x2H264VlcEncoder.h"
+#include "TotalZeros4x4H264VlcDecoder.h"
+#include "TotalZeros4x4H264VlcEncoder.h"
 
 namespace {
 
@@ -244,6 +258,57 @@ int ref_mv_dump(const char* path, const char* title, const short* mv, int n) {
     l.SetSimpleElement(i, 1, mv[2 * i + 1]);
   }
   l.Dump((char*)path, title);
+  return 0;
+}
+
+// CAVLC of n blocks (4 or 16 shorts each) into one MSB-first stream, block
+// b with nC nc[b]; bits[b] receives Encode's return value.  Then the stream
+// is decoded block by block into dec (dec_bits: Decode's return value).
+int ref_cavlc(const short* blocks, int n, int mode, const int* nc, int dc_skip, unsigned char* out,
+              int out_bytes, int* bits, short* dec, int* dec_bits) {
+  PrefixH264VlcEncoderImpl1 pe;
+  PrefixH264VlcDecoderImpl1 pd;
+  CoeffTokenH264VlcEncoder ce;
+  CoeffTokenH264VlcDecoder cd;
+  TotalZeros4x4H264VlcEncoder t4e;
+  TotalZeros4x4H264VlcDecoder t4d;
+  TotalZeros2x2H264VlcEncoder t2e;
+  TotalZeros2x2H264VlcDecoder t2d;
+  RunBeforeH264VlcEncoder re;
+  RunBeforeH264VlcDecoder rd;
+  CAVLCH264Impl c;
+  c.SetTokenCoeffVlcEncoder(&ce);
+  c.SetTokenCoeffVlcDecoder(&cd);
+  c.SetPrefixVlcEncoder(&pe);
+  c.SetPrefixVlcDecoder(&pd);
+  c.SetRunBeforeVlcEncoder(&re);
+  c.SetRunBeforeVlcDecoder(&rd);
+  if (mode == CAVLCH264Impl::Mode2x2) {
+    c.SetTotalZerosVlcEncoder(&t2e);
+    c.SetTotalZerosVlcDecoder(&t2d);
+  } else {
+    c.SetTotalZerosVlcEncoder(&t4e);
+    c.SetTotalZerosVlcDecoder(&t4d);
+  }
+  c.SetMode(mode);
+  const int size = mode == CAVLCH264Impl::Mode2x2 ? 4 : 16;
+  std::memset(out, 0, out_bytes);
+  BitStreamWriterMSB w;
+  w.SetStream(out, out_bytes * 8);
+  for (int b = 0; b < n; b++) {
+    c.SetParameter(CAVLCH264Impl::NUM_TOT_NEIGHBOR_COEFF_ID, nc[b]);
+    c.SetParameter(CAVLCH264Impl::DC_SKIP_FLAG_ID, dc_skip);
+    // the coder casts its void* back to the interface: pass the interface
+    // subobject (the writer derives from more than one base)
+    bits[b] = c.Encode((void*)(blocks + (long)b * size), (void*)static_cast<IBitStreamWriter*>(&w));
+  }
+  BitStreamReaderMSB r;
+  r.SetStream(out, out_bytes * 8);
+  for (int b = 0; b < n; b++) {
+    c.SetParameter(CAVLCH264Impl::NUM_TOT_NEIGHBOR_COEFF_ID, nc[b]);
+    c.SetParameter(CAVLCH264Impl::DC_SKIP_FLAG_ID, dc_skip);
+    dec_bits[b] = c.Decode((void*)static_cast<IBitStreamReader*>(&r), (void*)(dec + (long)b * size));
+  }
   return 0;
 }
 

@@ -215,6 +215,105 @@ def make_io(R_lib):
     print("io")
 
 
+ZZ4 = [0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15]
+
+
+def cavlc_blocks(mode, dc_skip, rng):
+    """Blocks that reach every coeff_token (TotalCoeff, TrailingOnes),
+    total_zeros and run_before entry, levels in every prefix / suffix regime
+    (escapes 14, 15 and 16+ included), plus random blocks shaped as in the
+    reference's own CAVLC test (levels flattened towards high frequency)."""
+    size = 4 if mode == 0 else 16
+    zz = list(range(4)) if mode == 0 else ZZ4
+    span = size - dc_skip
+    out = []
+
+    def place(levels, gaps, lead):
+        # levels high frequency first; gaps[i] zeros between levels i and i+1;
+        # lead zeros below the last level (towards DC)
+        blk = np.zeros(size, np.int16)
+        pos = dc_skip + lead + sum(gaps) + len(levels) - 1
+        for i, lv in enumerate(levels):
+            blk[zz[pos]] = lv
+            if i < len(gaps):
+                pos -= gaps[i] + 1
+        return blk
+
+    for tc in range(0, span + 1):
+        for t1 in range(0, min(3, tc) + 1):
+            for rep in range(3):
+                mags = rng.choice([2, 3, 5, 9, 17, 40, 300, 3000], size=tc)
+                lv = [int(rng.choice([-1, 1])) for _ in range(t1)]
+                lv += [int(m) * int(rng.choice([-1, 1])) for m in mags[t1:]]
+                tz = int(rng.integers(0, span - tc + 1)) if tc else 0
+                gaps = list(rng.multinomial(tz, [1.0 / tc] * tc)) if tc else []
+                out.append(place(lv, gaps[:-1] if tc else [], gaps[-1] if tc else 0))
+    # every (TotalCoeff, total_zeros) and, with two levels, every run split
+    for tc in range(1, span + 1):
+        for tz in range(0, span - tc + 1):
+            lv = [int(rng.integers(2, 9)) * int(rng.choice([-1, 1])) for _ in range(tc)]
+            gaps = list(rng.multinomial(tz, [1.0 / tc] * tc))
+            out.append(place(lv, gaps[:-1], gaps[-1]))
+    for tz in range(0, span - 1):
+        for r in range(0, tz + 1):
+            out.append(place([int(rng.integers(-4, 5)) or 2, 3], [r], tz - r))
+    # large levels and escapes at every suffix length
+    for big in (14, 15, 16, 29, 30, 31, 100, 4095, 4096, 4200, 8200, 20000, 32767, -32768):
+        for tc in (1, 2, 5, 11, min(16, span)):
+            if tc > span:
+                continue
+            lv = [int(rng.integers(1, 60)) * int(rng.choice([-1, 1])) for _ in range(tc)]
+            lv[int(rng.integers(0, tc))] = big
+            out.append(place(lv, [0] * (tc - 1), 0))
+    # reference-style random blocks
+    for k in range(200):
+        a = np.round(rng.random(size) * 1023.0 - 256.0).astype(np.int32)
+        a[1:] = (a[1:] / (2.0 * np.arange(1, size))).astype(np.int32)
+        nz = k % (size + 1)
+        a[rng.choice(size, nz, replace=False)] = 0
+        ones = k % 4
+        for i in range(size - 1, -1, -1):
+            if ones and a[i] != 0:
+                a[i] = 1 if a[i] > 0 else -1
+                ones -= 1
+        if dc_skip:
+            a[0] = 0
+        out.append(a.astype(np.int16))
+    return np.stack(out)
+
+
+def make_cavlc(R_lib):
+    rng = np.random.default_rng(170)
+    out = {}
+    for mode, ncs in ((1, [0, 1, 2, 3, 4, 5, 7, 8, 16]), (0, [-1, -2, 0, 2, 4, 8])):
+        for dc_skip in (0, 1):
+            one = cavlc_blocks(mode, dc_skip, rng)
+            # every block under one nC of each coeff_token table, then the
+            # whole set again under nC drawn at random
+            classes = [0, 2, 4, 8] if mode == 1 else [-1, -2, 0, 2, 4, 8]
+            blocks = np.concatenate([one] * (len(classes) + 1))
+            nc = np.concatenate([np.full(len(one), c, np.int32) for c in classes] +
+                                [rng.choice(ncs, size=len(one)).astype(np.int32)])
+            n = len(blocks)
+            stream = np.zeros(n * 256, np.uint8)
+            bits = np.zeros(n, np.int32)
+            dec = np.zeros_like(blocks)
+            dbits = np.zeros(n, np.int32)
+            assert R_lib.ref_cavlc(np.ascontiguousarray(blocks).reshape(-1), n, mode, nc, dc_skip, stream,
+                                   stream.size, bits, dec.reshape(-1), dbits) == 0
+            assert (bits > 0).all(), "reference rejected a block"
+            total = int(bits.sum())
+            key = f"m{mode}_s{dc_skip}"
+            out[key + "_blocks"] = blocks
+            out[key + "_nc"] = nc.astype(np.int8)
+            out[key + "_bits"] = bits
+            out[key + "_stream"] = stream[:(total + 7) // 8]
+            out[key + "_dec"] = dec
+            out[key + "_dec_bits"] = dbits
+            print("cavlc", key, n, "blocks", total, "bits")
+    np.savez_compressed(OUT / "cavlc.npz", **out)
+
+
 def main():
     if not orc.build_ref():
         sys.exit("reference sources not available; fixtures are committed")
@@ -225,6 +324,7 @@ def main():
     make_mc(R_lib)
     make_mc_yuv(R_lib)
     make_io(R_lib)
+    make_cavlc(R_lib)
 
 
 if __name__ == "__main__":

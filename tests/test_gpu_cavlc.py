@@ -1,0 +1,133 @@
+"""GPU parity of the batched CAVLC coder (icu_cavlc_*_device): the device
+stream, per-block offsets and decode against the compiled reference's
+coder (fixture), then a picture-sized batch against the host coder and
+as a decode round trip.  Bit-exact."""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+capi = pytest.importorskip("imagecodingutils_amd.capi")
+
+KEYS = ["m1_s0", "m1_s1", "m0_s0", "m0_s1"]
+
+
+def _encode(dev, blocks, mode, nc, dc, cap_bytes=None):
+    """(stream bytes, offsets) of a device encode; nc an array (per block)
+    or an int (every block)."""
+    n = len(blocks)
+    d_coef = capi.DeviceBuffer.from_array(np.ascontiguousarray(blocks, np.int16))
+    d_nc = capi.DeviceBuffer.from_array(np.asarray(nc, np.int8)) if isinstance(nc, np.ndarray) else None
+    cap = cap_bytes or (n * 160 + 64) // 4 * 4
+    d_stream = capi.DeviceBuffer(cap)
+    d_off = capi.DeviceBuffer(8 * (n + 1))
+    tot = dev.encode(d_coef, n, mode, d_nc, 0 if d_nc else int(nc), dc, d_stream, cap, d_off)
+    capi.lib.icu_stream_sync(None)
+    off = d_off.download(np.int64, (n + 1,))
+    assert off[-1] == tot
+    return d_stream.download(np.uint8, ((tot + 7) // 8,)), off, (d_stream, cap, d_off, d_nc)
+
+
+def _decode(dev, bufs, n, mode, nc, dc):
+    d_stream, cap, d_off, d_nc = bufs
+    size = 4 if mode == 0 else 16
+    d_coef = capi.DeviceBuffer(n * size * 2)
+    d_status = capi.DeviceBuffer(4 * n)
+    dev.decode(d_stream, cap, d_off, n, mode, d_nc, 0 if d_nc else int(nc), dc, d_coef, d_status)
+    capi.lib.icu_stream_sync(None)
+    return d_coef.download(np.int16, (n, size)), d_status.download(np.int32, (n,))
+
+
+@pytest.mark.parametrize("key", KEYS)
+def test_device_stream_matches_reference(key):
+    fx = np.load(GOLDEN / "cavlc.npz")
+    mode, dc = int(key[1]), int(key[4])
+    blocks, nc, bits = fx[key + "_blocks"], fx[key + "_nc"], fx[key + "_bits"]
+    dev = capi.CavlcDevice()
+    stream, off, bufs = _encode(dev, blocks, mode, nc, dc)
+    np.testing.assert_array_equal(np.diff(off), bits)
+    assert off[0] == 0
+    np.testing.assert_array_equal(stream, fx[key + "_stream"])
+    # decode the reference's own stream at the offsets
+    bufs[0].upload(fx[key + "_stream"])
+    dec, status = _decode(dev, bufs, len(blocks), mode, nc, dc)
+    np.testing.assert_array_equal(dec, fx[key + "_dec"])
+    np.testing.assert_array_equal(status, fx[key + "_dec_bits"])
+
+
+@pytest.mark.parametrize("mode,nc", [(1, 0), (1, 3), (1, 9), (0, -1), (0, -2)])
+def test_one_nc_for_every_block_matches_host(mode, nc):
+    fx = np.load(GOLDEN / "cavlc.npz")
+    blocks = fx[f"m{mode}_s0_blocks"]
+    dev = capi.CavlcDevice()
+    stream, off, bufs = _encode(dev, blocks, mode, nc, 0)
+    want, bits = capi.cavlc_encode(blocks, mode, nc, 0)
+    np.testing.assert_array_equal(np.diff(off), bits)
+    np.testing.assert_array_equal(stream, want)
+    dec, status = _decode(dev, bufs, len(blocks), mode, nc, 0)
+    np.testing.assert_array_equal(dec, blocks)
+    np.testing.assert_array_equal(status, bits)
+
+
+def _picture_blocks(W, H, seed):
+    """Quantised-residual-like 4x4 levels of a W x H picture, block-major in
+    raster block order, and each block's nC from its left and upper
+    neighbours' coefficient counts (H.264 9.2.1)."""
+    rng = np.random.default_rng(seed)
+    bw, bh = W // 4, H // 4
+    # Laplacian levels, larger at low frequency, about one block in three empty
+    scale = np.array([3.0, 1.6, 0.9, 0.5] * 4, np.float32).reshape(4, 4)
+    scale = (scale + scale.T) / 2
+    lv = rng.laplace(0, 1, size=(bh * bw, 16)).astype(np.float32) * scale.reshape(16) * \
+        rng.choice([0.0, 0.6, 2.5, 40.0], size=(bh * bw, 1), p=[0.33, 0.4, 0.25, 0.02])
+    blocks = np.clip(np.round(lv), -2000, 2000).astype(np.int16)
+    tc = np.count_nonzero(blocks, axis=1).reshape(bh, bw)
+    na = np.full((bh, bw), -1)
+    nb = np.full((bh, bw), -1)
+    na[:, 1:] = tc[:, :-1]
+    nb[1:, :] = tc[:-1, :]
+    nc = np.where((na >= 0) & (nb >= 0), (na + nb + 1) >> 1, np.maximum(np.maximum(na, nb), 0))
+    return blocks, nc.reshape(-1).astype(np.int8)
+
+
+@pytest.mark.parametrize("W,H", [(1920, 1088)])
+def test_picture_batch_matches_host_and_round_trips(W, H):
+    blocks, nc = _picture_blocks(W, H, 7)
+    n = len(blocks)
+    dev = capi.CavlcDevice()
+    stream, off, bufs = _encode(dev, blocks, 1, nc, 0)
+    # the host coder on every block
+    want, bits = capi.cavlc_encode(blocks, 1, nc.astype(np.int64), 0)
+    np.testing.assert_array_equal(np.diff(off), bits)
+    np.testing.assert_array_equal(stream, want)
+    dec, status = _decode(dev, bufs, n, 1, nc, 0)
+    np.testing.assert_array_equal(dec, blocks)
+    np.testing.assert_array_equal(status, bits)
+    # again with the DC positions left to a separate 2x2 / DC path
+    stream1, off1, bufs1 = _encode(dev, blocks, 1, nc, 1)
+    dec1, _ = _decode(dev, bufs1, n, 1, nc, 1)
+    want1 = blocks.copy()
+    want1[:, 0] = 0
+    np.testing.assert_array_equal(dec1, want1)
+
+
+def test_device_errors():
+    dev = capi.CavlcDevice()
+    blocks = np.full((64, 16), 5, np.int16)
+    n = len(blocks)
+    d_coef = capi.DeviceBuffer.from_array(blocks)
+    d_stream = capi.DeviceBuffer(64)
+    d_off = capi.DeviceBuffer(8 * (n + 1))
+    with pytest.raises(capi.IcuError) as e:
+        dev.encode(d_coef, n, 1, None, 0, 0, d_stream, 64, d_off)
+    assert e.value.code == capi.ICU_E_RANGE
+    for mode, nc, dc in ((3, 0, 0), (1, -1, 0), (1, 0, 5)):
+        with pytest.raises(capi.IcuError) as e:
+            dev.encode(d_coef, n, mode, None, nc, dc, d_stream, 64, d_off)
+        assert e.value.code == capi.ICU_E_ARG
+    # empty batch: offsets [0], no stream
+    tot = dev.encode(d_coef, 0, 1, None, 0, 0, d_stream, 64, d_off)
+    capi.lib.icu_stream_sync(None)
+    assert tot == 0 and d_off.download(np.int64, (1,))[0] == 0
