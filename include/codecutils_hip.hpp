@@ -17,6 +17,9 @@
 //   YuvRawFileHandler   include/CodecUtils/YuvRawFileHandler.h:53-98
 //   IMotionCompensator / MotionCompensatorH264ImplStd
 //       include/CodecUtils/IMotionCompensator.h, MotionCompensatorH264ImplStd.h:55-110
+//   IContextAwareRunLevelCodec / CAVLCH264Impl, BitStreamWriterMSB / BitStreamReaderMSB
+//       include/CodecUtils/IContextAwareRunLevelCodec.h:49-103, CAVLCH264Impl.h:59-150,
+//       BitStreamBaseMSB.h:62-87
 // They live in namespace icu_codecutils; INTEGRATION.md shows the adapter a
 // codec adds to bind them to its own CodecUtils interface types instead.
 //
@@ -408,6 +411,161 @@ class YuvRawFileHandler {
   icu_yuv_file* _f = nullptr;
   void* _unit = nullptr;
   int _len = 0;
+};
+
+// ------------------------------------------------------- MSB-first streams
+// A cursor over a caller's byte array, bits taken from the MSB of each byte
+// down.  The cursor is kept linear; GetStreamBitPos() and Seek() exchange
+// it in the reference base class's own encoding, (byte << 3) + bit with bit
+// counting 7 (MSB) down to 0, so that Seek(GetStreamBitPos()) returns to the
+// same place as with the reference (include/CodecUtils/BitStreamBaseMSB.h:62-87).
+class BitStreamBaseMSB {
+ public:
+  virtual ~BitStreamBaseMSB() {}
+  virtual void SetStream(void* stream, int bitSize) {
+    _s = (uint8_t*)stream;
+    _size = bitSize;
+    _pos = 0;
+  }
+  virtual void* GetStream(void) { return _s; }
+  virtual void Reset(void) { _pos = 0; }
+  virtual int Seek(int streamBitPos) {
+    if (streamBitPos >= _size) return 0;
+    _pos = (long)(streamBitPos / 8) * 8 + 7 - streamBitPos % 8;
+    return 1;
+  }
+  virtual int GetStreamBitPos(void) { return (int)((_pos >> 3) << 3) + 7 - (int)(_pos & 7); }
+  virtual int GetStreamBytePos(void) { return (int)(_pos >> 3); }
+  virtual void SetStreamBitSize(int bitSize) { _size = bitSize; }
+  virtual int GetStreamBitSize(void) { return _size; }
+  virtual int GetStreamBitsRemaining(void) { return _size - (int)_pos; }
+  // the linear position, for the entropy coder
+  long& Cursor(void) { return _pos; }
+  uint8_t* Bytes(void) { return _s; }
+
+ protected:
+  uint8_t* _s = nullptr;
+  int _size = 0;
+  long _pos = 0;
+};
+
+// include/CodecUtils/BitStreamWriterMSB.h (src/CodecUtils/BitStreamWriterMSB.cpp:73-180):
+// each written bit is set or cleared, the rest of the byte kept.
+class BitStreamWriterMSB : public BitStreamBaseMSB {
+ public:
+  void Write(int val) { Write(1, val); }
+  void Write(int numBits, int val) {
+    for (int i = numBits - 1; i >= 0; i--, _pos++) put(_pos, (val >> i) & 1);
+  }
+  void Write(int numBits, int valH, int valL) {
+    if (numBits > 32) {
+      Write(numBits - 32, valH);
+      Write(32, valL);
+    } else {
+      Write(numBits, valL);
+    }
+  }
+  void Poke(int bitLoc, int numBits, int val) {
+    // bitLoc in the same encoding as Seek()
+    long p = (long)(bitLoc / 8) * 8 + 7 - bitLoc % 8;
+    for (int i = numBits - 1; i >= 0; i--, p++) put(p, (val >> i) & 1);
+  }
+
+ private:
+  void put(long p, int b) {
+    const uint8_t m = (uint8_t)(0x80u >> (p & 7));
+    if (b) _s[p >> 3] |= m;
+    else _s[p >> 3] &= (uint8_t)~m;
+  }
+};
+
+// include/CodecUtils/BitStreamReaderMSB.h (src/CodecUtils/BitStreamReaderMSB.cpp:71-160).
+class BitStreamReaderMSB : public BitStreamBaseMSB {
+ public:
+  int Read(void) { return Read(1); }
+  int Read(int numBits) {
+    int v = 0;
+    for (int i = 0; i < numBits; i++, _pos++) v = (v << 1) | ((_s[_pos >> 3] >> (7 - (_pos & 7))) & 1);
+    return v;
+  }
+  int Peek(int bitLoc, int numBits) {
+    long p = (long)(bitLoc / 8) * 8 + 7 - bitLoc % 8;
+    int v = 0;
+    for (int i = 0; i < numBits; i++, p++) v = (v << 1) | ((_s[p >> 3] >> (7 - (p & 7))) & 1);
+    return v;
+  }
+};
+
+// ------------------------------------------------------------------ CAVLC
+// IContextAwareRunLevelCodec / CAVLCH264Impl
+// (include/CodecUtils/IContextAwareRunLevelCodec.h:49-103,
+// include/CodecUtils/CAVLCH264Impl.h:59-150): Encode(short block[], writer)
+// and Decode(reader, short block[]) return the bits coded or
+// VLC_SYMBOL_NOT_RECOGNISED / STREAM_ACCESS_DENIED; the stream arguments are
+// this header's BitStreamWriterMSB / BitStreamReaderMSB.  Mode8x8 is not
+// coded (H.264 codes an 8x8 transform's levels as four 4x4 blocks) and
+// returns VLC_SYMBOL_NOT_RECOGNISED.
+class IContextAwareRunLevelCodec {
+ public:
+  virtual ~IContextAwareRunLevelCodec() {}
+  virtual int Encode(void* in, void* stream) = 0;
+  virtual int Decode(void* stream, void* out) = 0;
+  virtual void SetMode(int mode) = 0;
+  virtual int GetMode(void) = 0;
+  virtual void SetParameter(int paramID, int paramVal) = 0;
+  virtual int GetParameter(int paramID) = 0;
+  static const int Mode2x2 = 0;
+  static const int Mode4x4 = 1;
+  static const int Mode8x8 = 2;
+  static const int NUM_TOT_NEIGHBOR_COEFF_ID = 0;
+  static const int DC_SKIP_FLAG_ID = 1;
+  static const int NUM_TOT_COEFF_ID = 2;
+};
+
+class CAVLCH264Impl : public IContextAwareRunLevelCodec {
+ public:
+  static const int VLC_SYMBOL_NOT_RECOGNISED = -1;
+  static const int STREAM_ACCESS_DENIED = -2;
+  int Encode(void* in, void* stream) override {
+    BitStreamWriterMSB* w = (BitStreamWriterMSB*)stream;
+    if (_mode == Mode8x8) return VLC_SYMBOL_NOT_RECOGNISED;
+    int nbits = 0;
+    const int rc = icu_cavlc_encode((const int16_t*)in, _mode, _nc, _dc, w->Bytes(), w->GetStreamBitSize(),
+                                    &w->Cursor(), &nbits);
+    if (rc != ICU_OK) return rc == ICU_E_RANGE ? STREAM_ACCESS_DENIED : VLC_SYMBOL_NOT_RECOGNISED;
+    _num = count((const int16_t*)in);
+    return nbits;
+  }
+  int Decode(void* stream, void* out) override {
+    BitStreamReaderMSB* r = (BitStreamReaderMSB*)stream;
+    if (_mode == Mode8x8) return VLC_SYMBOL_NOT_RECOGNISED;
+    int nbits = 0;
+    const int rc = icu_cavlc_decode(r->Bytes(), r->GetStreamBitSize(), &r->Cursor(), _mode, _nc, _dc,
+                                    (int16_t*)out, &nbits);
+    if (rc != ICU_OK) return rc == ICU_E_RANGE ? STREAM_ACCESS_DENIED : VLC_SYMBOL_NOT_RECOGNISED;
+    _num = count((const int16_t*)out);
+    return nbits;
+  }
+  // unknown modes select 4x4, as the reference
+  void SetMode(int mode) override { _mode = (mode == Mode2x2 || mode == Mode8x8) ? mode : Mode4x4; }
+  int GetMode(void) override { return _mode; }
+  void SetParameter(int paramID, int paramVal) override {
+    if (paramID == NUM_TOT_NEIGHBOR_COEFF_ID) _nc = paramVal;
+    else if (paramID == DC_SKIP_FLAG_ID) _dc = paramVal;
+  }
+  int GetParameter(int paramID) override {
+    return paramID == NUM_TOT_NEIGHBOR_COEFF_ID ? _nc : paramID == DC_SKIP_FLAG_ID ? _dc
+                                                    : paramID == NUM_TOT_COEFF_ID ? _num : -1;
+  }
+
+ private:
+  // levels coded in the block: every position but a skipped DC
+  int count(const int16_t* b) const {
+    int n = 0;
+    for (int i = _dc ? 1 : 0; i < (_mode == Mode2x2 ? 4 : 16); i++) n += b[i] != 0;
+    return n;
+  }
+  int _mode = Mode4x4, _nc = 0, _dc = 0, _num = 0;
 };
 
 }  // namespace icu_codecutils

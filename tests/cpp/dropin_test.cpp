@@ -130,6 +130,49 @@ int main(int argc, char** argv) {
     save(argv[4], yuv.data(), n);
     return 0;
   }
-  std::fprintf(stderr, "usage: dropin_test me|fwd|inv|mc ...\n");
+  if (cmd == "cavlc" && argc >= 9) {
+    // cavlc mode dc n blocks.raw nc.raw stream.raw out.raw: every block
+    // through CAVLCH264Impl::Encode into one BitStreamWriterMSB stream
+    // (SetParameter(NUM_TOT_NEIGHBOR_COEFF_ID) per block), then Decode from
+    // a BitStreamReaderMSB; out.raw holds per block the encoded bits, the
+    // decoded bits, NUM_TOT_COEFF after each and the decoded levels.
+    const int mode = std::atoi(argv[2]), dc = std::atoi(argv[3]), n = std::atoi(argv[4]);
+    const int size = mode == IContextAwareRunLevelCodec::Mode2x2 ? 4 : 16;
+    auto blocks = load<short>(argv[5], (size_t)n * size);
+    auto nc = load<signed char>(argv[6], (size_t)n);
+    std::vector<unsigned char> stream((size_t)n * 128 + 16, 0xA5);
+    CAVLCH264Impl codec;
+    codec.SetMode(mode);
+    codec.SetParameter(IContextAwareRunLevelCodec::DC_SKIP_FLAG_ID, dc);
+    BitStreamWriterMSB w;
+    w.SetStream(stream.data(), (int)stream.size() * 8);
+    std::vector<int> out;
+    for (int i = 0; i < n; i++) {
+      codec.SetParameter(IContextAwareRunLevelCodec::NUM_TOT_NEIGHBOR_COEFF_ID, nc[i]);
+      out.push_back(codec.Encode(&blocks[(size_t)i * size], &w));
+      out.push_back(codec.GetParameter(IContextAwareRunLevelCodec::NUM_TOT_COEFF_ID));
+    }
+    const int total = (int)(stream.size() * 8) - w.GetStreamBitsRemaining();
+    save(argv[7], stream.data(), (size_t)(total + 7) / 8);
+    BitStreamReaderMSB r;
+    r.SetStream(stream.data(), total);
+    std::vector<short> dec(size);
+    for (int i = 0; i < n; i++) {
+      codec.SetParameter(IContextAwareRunLevelCodec::NUM_TOT_NEIGHBOR_COEFF_ID, nc[i]);
+      out.push_back(codec.Decode(&r, dec.data()));
+      out.push_back(codec.GetParameter(IContextAwareRunLevelCodec::NUM_TOT_COEFF_ID));
+      for (int k = 0; k < size; k++) out.push_back(dec[k]);
+    }
+    // the cursor round trip of the reference's position encoding
+    w.Reset();
+    w.Write(13, 0x1abc);
+    const int p = w.GetStreamBitPos();
+    w.Write(5, 0);
+    out.push_back(w.Seek(p));
+    out.push_back(w.GetStreamBitsRemaining());
+    save(argv[8], out.data(), out.size());
+    return 0;
+  }
+  std::fprintf(stderr, "usage: dropin_test me|fwd|inv|mc|cavlc ...\n");
   return 1;
 }

@@ -70,3 +70,43 @@ def test_decode_of_a_truncated_stream_fails_cleanly(fx):
     with pytest.raises(capi.IcuError) as e:
         capi.cavlc_decode(s, 50, 1, fx[key + "_nc"][:50].astype(int), 0)
     assert e.value.code in (capi.ICU_E_RANGE, capi.ICU_E_STATE)
+
+
+EXE = GOLDEN.parent / "cpp" / "dropin_test"
+
+
+@pytest.mark.parametrize("key", KEYS)
+def test_cpp_dropin_codec_matches_reference(fx, key, tmp_path):
+    """CAVLCH264Impl over BitStreamWriterMSB / BitStreamReaderMSB from
+    include/codecutils_hip.hpp, driven as codec code drives the reference
+    classes (the host coder: no device needed)."""
+    import subprocess
+    if not EXE.exists():
+        pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
+    mode, dc = int(key[1]), int(key[4])
+    blocks, nc = fx[key + "_blocks"], fx[key + "_nc"]
+    n, size = blocks.shape
+    blocks.astype(np.int16).tofile(tmp_path / "b.raw")
+    nc.astype(np.int8).tofile(tmp_path / "nc.raw")
+    r = subprocess.run([str(EXE), "cavlc", str(mode), str(dc), str(n), str(tmp_path / "b.raw"),
+                        str(tmp_path / "nc.raw"), str(tmp_path / "s.raw"), str(tmp_path / "o.raw")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    # the buffer starts as 0xA5 bytes (each written bit is set or cleared);
+    # past the stream's last bit the fill remains
+    want = fx[key + "_stream"].copy()
+    spare = (-int(fx[key + "_bits"].sum())) % 8
+    want[-1] |= 0xA5 & ((1 << spare) - 1)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "s.raw", np.uint8), want)
+    out = np.fromfile(tmp_path / "o.raw", np.int32)
+    enc = out[:2 * n].reshape(n, 2)
+    dec = out[2 * n:2 * n + n * (2 + size)].reshape(n, 2 + size)
+    tail = out[2 * n + n * (2 + size):]
+    coded = blocks[:, dc:] if dc else blocks
+    np.testing.assert_array_equal(enc[:, 0], fx[key + "_bits"])
+    np.testing.assert_array_equal(enc[:, 1], np.count_nonzero(coded, axis=1))
+    np.testing.assert_array_equal(dec[:, 0], fx[key + "_dec_bits"])
+    np.testing.assert_array_equal(dec[:, 1], np.count_nonzero(coded, axis=1))
+    np.testing.assert_array_equal(dec[:, 2:], fx[key + "_dec"])
+    # Seek(GetStreamBitPos()) after 13 bits: back to bit 13
+    assert tail[0] == 1 and tail[1] == (n * 128 + 16) * 8 - 13
