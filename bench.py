@@ -423,8 +423,54 @@ def cpu_baseline(synth, budget_s=12.0):
         t_it = 0.0
         sample = "1/8 of the MBs of a 1080p frame pair with the oracle in the benchmark mode"
     per_frame = t_me + t_it
-    return {"value": nmb / per_frame, "unit": "MB/s", "cores": 1, "kind": kind, "sample": sample,
-            "s_per_frame": per_frame}
+    out = {"value": nmb / per_frame, "unit": "MB/s", "cores": 1, "kind": kind, "sample": sample,
+           "s_per_frame": per_frame}
+    model, nproc, avail = _cpu_info()
+    out.update(cpu_model=model, nproc=nproc, cpus_available=avail)
+    # the same mode as the GPU (SAD, exhaustive +-32, filter off, zero
+    # predictor, half-pel) through the oracle on 1/8 of the MBs: the reference
+    # class cannot be built in that mode
+    t0 = time.perf_counter()
+    orc.me_estimate(cur, ref, R, step=8, metric=1, window=1, filter=0, subpel=1, pred=0, bestvec=0)
+    t_same = (time.perf_counter() - t0) * 8
+    out["same_mode_as_gpu"] = {"value": round(nmb / t_same, 1), "unit": "MB/s", "cores": 1, "kind": "port",
+                               "sample": "1/8 of the MBs of a 1080p frame pair, oracle in the benchmark's "
+                                         "search mode (IT/quant excluded)"}
+    if ref_lib is not None:
+        # one estimator instance per thread over independent frame pairs (the
+        # reference has no threading of its own); the calls release the GIL
+        import threading
+        nthr = max(1, min(16, avail))
+        per = 2
+
+        def work():
+            ref_lib.ref_me_time(W, H, 4 * R, 1, cur, ref, per)
+
+        ths = [threading.Thread(target=work) for _ in range(nthr)]
+        t0 = time.perf_counter()
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        wall = time.perf_counter() - t0
+        out["threads"] = {"value": round(nthr * per * nmb / wall, 1), "unit": "MB/s", "cores": nthr,
+                          "kind": "reference",
+                          "sample": f"{nthr} threads x {per} 1080p frame pairs, reference estimator as shipped "
+                                    "(IT/quant excluded)"}
+    return out
+
+
+def _cpu_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count(), len(os.sched_getaffinity(0))
 
 
 def main():
