@@ -105,6 +105,11 @@ _SIGS = {
                                           C.POINTER(C.c_longlong), _vp]),
     "icu_cavlc_decode_device": (C.c_int, [_vp, _vp, C.c_long, _vp, C.c_long, C.c_int, _vp, C.c_int, C.c_int, _vp,
                                           _vp, _vp]),
+    "icu_umhs_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "icu_umhs_destroy": (C.c_int, [_vp]),
+    "icu_umhs_estimate": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_long)]),
+    "icu_umhs_estimate_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "icu_umhs_check": (C.c_int, [_vp]),
     "icu_last_error": (C.c_char_p, []),
     "icu_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_malloc": (_vp, [C.c_size_t]),
@@ -505,6 +510,55 @@ class MotionCompensator:
     def close(self):
         if getattr(self, "h", None):
             lib.icu_mc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+# ------------------------------------------------------------ UMHS estimator
+class UmhsEstimator:
+    """MotionEstimatorH264ImplUMHS on the device (icu_umhs_*).  motion_range
+    in quarter pels as the reference's constructor; mode 0 quarter, 1 half,
+    2 full pel."""
+
+    def __init__(self, width, height, motion_range, mode=0):
+        h = _vp()
+        _check(lib.icu_umhs_create(width, height, motion_range, mode, C.byref(h)))
+        self.h, self.W, self.H = h, width, height
+        self.nmb = (width // 16) * (height // 16)
+
+    def estimate(self, cur, ref, prev_mv=None, prev_intra=None, include=None):
+        """Host planes (int16, 0..255).  Returns (mv (nMB, 2) int16, dist
+        (nMB,) int32, avg over include)."""
+        c = np.ascontiguousarray(cur, np.int16)
+        r = np.ascontiguousarray(ref, np.int16)
+        mv = np.zeros((self.nmb, 2), np.int16)
+        dist = np.zeros(self.nmb, np.int32)
+        avg = C.c_long(0)
+        pm = pi = inc = None
+        if prev_mv is not None:
+            pm = np.ascontiguousarray(prev_mv, np.int16)
+            pi = np.ascontiguousarray(prev_intra if prev_intra is not None else np.zeros(self.nmb), np.uint8)
+        if include is not None:
+            inc = np.ascontiguousarray(include, np.uint8)
+        _check(lib.icu_umhs_estimate(self.h, c.ctypes.data, r.ctypes.data, None if pm is None else pm.ctypes.data,
+                                     None if pi is None else pi.ctypes.data,
+                                     None if inc is None else inc.ctypes.data, mv.ctypes.data, dist.ctypes.data,
+                                     C.byref(avg)))
+        return mv, dist, avg.value
+
+    def estimate_device(self, d_cur, d_ref, d_mv, d_dist, d_prev_mv=None, d_prev_intra=None,
+                        stream: Stream | None = None):
+        _check(lib.icu_umhs_estimate_device(self.h, _p(d_cur), _p(d_ref), _p(d_prev_mv), _p(d_prev_intra),
+                                            _p(d_mv), _p(d_dist), stream.handle if stream else None))
+
+    def check(self):
+        _check(lib.icu_umhs_check(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.icu_umhs_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -302,64 +302,16 @@ __device__ __forceinline__ void half_pel_direct(const DecideDesc& d, int x0, int
   }
 }
 
-struct Decision {
-  int mvx, mvy, dist;
-  int imx, imy, iminD;
-};
-
-__device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, int predX, int predY,
-                                     int lane, const IntResult* known = nullptr) {
-  const int col = mb % d.mbw, row = mb / d.mbw;
-  const int n = col * 16, m = row * 16;
-  const int slot = d.mb_base + mb;
-  // input block: one aligned dword per lane (current planes are 16-byte
-  // aligned with a stride that is a multiple of 16)
-  const uint32_t cw = *(const uint32_t*)(d.cur + (long)(m + (lane >> 2)) * d.cur_stride + n + 4 * (lane & 3));
-  s.cur4[lane] = cw;
+// Half-pel then (subpel >= 2) quarter-pel refinement around full-pel
+// (mx, my) of the MB at (n, m), strictly better distortions only
+// (Full.cpp:452-517; UMHS.cpp:795-862 is the same stage).  cw: this lane's
+// dword of the input block (row lane / 4, columns 4 (lane % 4)), also
+// staged in s.cur4.  Updates minD; returns the vector in quarter pels.
+__device__ inline void refine_subpel(const DecideDesc& d, WaveLds& s, int n, int m, int mx, int my, uint32_t cw,
+                                     int lane, int& minD, int& mvx, int& mvy) {
   const uint8_t* cur8 = (const uint8_t*)s.cur4;
-  // the zero vector's cost, which the argmin search does not report; only
-  // GetBestVector reads it (the padded plane's rows are 16-byte aligned)
-  int zero_direct = 0;
-  if (!d.table && d.bestvec) {
-    const uint32_t rw = *(const uint32_t*)(d.ref + (long)(m + (lane >> 2)) * d.ref_stride + n + 4 * (lane & 3));
-    int z = 0;
-#pragma unroll
-    for (int i = 0; i < 4; i++) z += pix_dist((int)((cw >> (8 * i)) & 0xFF), (int)((rw >> (8 * i)) & 0xFF), d.metric);
-    zero_direct = wave_sum(z);
-  }
-
-  const int predXQ = predX % 4, predYQ = predY % 4;
-  int predX0 = predX / 4, predY0 = predY / 4;
-  const int pXr = predX < 0 ? (predX - 2) / 4 : (predX + 2) / 4;
-  const int pYr = predY < 0 ? (predY - 2) / 4 : (predY + 2) / 4;
-
-  int mx, my, minD, zeroD;
-  int xl, xr, yu, yd;
-  motion_window(n, m, d.W, d.H, d.L, &xl, &xr, &yu, &yd);
-  if (d.table) {
-    TabView tab;
-    tab.t = d.table + d.table_base + (long long)mb * d.table_dim * d.table_dim;
-    tab.dim = d.table_dim;
-    tab.L = d.L;
-    tab.xl = xl;
-    tab.yu = yu;
-    tab.w = xr - xl + 1;
-    zeroD = (int)tab[(0 - yu) * tab.w + (0 - xl)];
-    if (d.filter) table_replay(tab, xl, xr, yu, yd, pXr, pYr, lane, &mx, &my, &minD);
-    else table_argmin(tab, xl, xr, yu, yd, pXr, pYr, lane, &mx, &my, &minD);
-  } else {
-    const IntResult r = known ? *known : d.ires[slot];
-    mx = r.mx;
-    my = r.my;
-    minD = r.cost;
-    zeroD = zero_direct;
-  }
-  Decision out;
-  out.imx = mx;
-  out.imy = my;
-  out.iminD = minD;
-
-  int mvx = mx * 4, mvy = my * 4;
+  mvx = mx * 4;
+  mvy = my * 4;
   if (d.subpel == 1) {
     // half pel only: the eight candidates filtered in registers
     int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -436,6 +388,88 @@ __device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, in
     wave_sync();
   }
 
+}
+
+// Distortion of the input block against the quarter-pel read at full pel
+// (x0, y0) + fractions (fx, fy) in -3..3 (OverlayMem2Dv2::QuarterRead; a
+// negative fraction reads from the pel to the left / above with 4 + f,
+// OverlayMem2Dv2.cpp:709-724).
+__device__ inline int quarter_dist(const DecideDesc& d, WaveLds& s, int x0, int y0, int fx, int fy, int lane) {
+  const uint8_t* cur8 = (const uint8_t*)s.cur4;
+  int fox = 0, foy = 0;
+  if (fx < 0) { fox = -1; fx = 4 + fx; }
+  if (fy < 0) { foy = -1; fy = 4 + fy; }
+  load_patch(d, x0 + fox, y0 + foy, s.P, lane);
+  wave_sync();
+  int acc = 0;
+  for (int p = lane; p < 256; p += 64) {
+    const int r = p >> 4, c = p & 15;
+    acc += pix_dist(cur8[p], qpel_p(s.P, 3 + c, 3 + r, fx, fy), d.metric);
+  }
+  const int v = wave_sum(acc);
+  wave_sync();
+  return v;
+}
+
+struct Decision {
+  int mvx, mvy, dist;
+  int imx, imy, iminD;
+};
+
+__device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, int predX, int predY,
+                                     int lane, const IntResult* known = nullptr) {
+  const int col = mb % d.mbw, row = mb / d.mbw;
+  const int n = col * 16, m = row * 16;
+  const int slot = d.mb_base + mb;
+  // input block: one aligned dword per lane (current planes are 16-byte
+  // aligned with a stride that is a multiple of 16)
+  const uint32_t cw = *(const uint32_t*)(d.cur + (long)(m + (lane >> 2)) * d.cur_stride + n + 4 * (lane & 3));
+  s.cur4[lane] = cw;
+  // the zero vector's cost, which the argmin search does not report; only
+  // GetBestVector reads it (the padded plane's rows are 16-byte aligned)
+  int zero_direct = 0;
+  if (!d.table && d.bestvec) {
+    const uint32_t rw = *(const uint32_t*)(d.ref + (long)(m + (lane >> 2)) * d.ref_stride + n + 4 * (lane & 3));
+    int z = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) z += pix_dist((int)((cw >> (8 * i)) & 0xFF), (int)((rw >> (8 * i)) & 0xFF), d.metric);
+    zero_direct = wave_sum(z);
+  }
+
+  const int predXQ = predX % 4, predYQ = predY % 4;
+  int predX0 = predX / 4, predY0 = predY / 4;
+  const int pXr = predX < 0 ? (predX - 2) / 4 : (predX + 2) / 4;
+  const int pYr = predY < 0 ? (predY - 2) / 4 : (predY + 2) / 4;
+
+  int mx, my, minD, zeroD;
+  int xl, xr, yu, yd;
+  motion_window(n, m, d.W, d.H, d.L, &xl, &xr, &yu, &yd);
+  if (d.table) {
+    TabView tab;
+    tab.t = d.table + d.table_base + (long long)mb * d.table_dim * d.table_dim;
+    tab.dim = d.table_dim;
+    tab.L = d.L;
+    tab.xl = xl;
+    tab.yu = yu;
+    tab.w = xr - xl + 1;
+    zeroD = (int)tab[(0 - yu) * tab.w + (0 - xl)];
+    if (d.filter) table_replay(tab, xl, xr, yu, yd, pXr, pYr, lane, &mx, &my, &minD);
+    else table_argmin(tab, xl, xr, yu, yd, pXr, pYr, lane, &mx, &my, &minD);
+  } else {
+    const IntResult r = known ? *known : d.ires[slot];
+    mx = r.mx;
+    my = r.my;
+    minD = r.cost;
+    zeroD = zero_direct;
+  }
+  Decision out;
+  out.imx = mx;
+  out.imy = my;
+  out.iminD = minD;
+
+  int mvx, mvy;
+  refine_subpel(d, s, n, m, mx, my, cw, lane, minD, mvx, mvy);
+
   int outx = mvx, outy = mvy, outd = minD;
   if (d.bestvec) {
     if (predX0 + n > d.W) predX0 = 16;
@@ -447,20 +481,7 @@ __device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, in
     if (predXQ == 0 && predYQ == 0 && predX0 == 0 && predY0 == 0) {
       predD = zeroD;
     } else {
-      // QuarterRead at the predictor: negative fractions reflect to the
-      // full pel to the left / above (OverlayMem2Dv2.cpp:709-724).
-      int fox = 0, fx = predXQ, foy = 0, fy = predYQ;
-      if (predXQ < 0) { fox = -1; fx = 4 + predXQ; }
-      if (predYQ < 0) { foy = -1; fy = 4 + predYQ; }
-      load_patch(d, predX0 + n + fox, predY0 + m + foy, s.P, lane);
-      wave_sync();
-      int acc = 0;
-      for (int p = lane; p < 256; p += 64) {
-        const int r = p >> 4, c = p & 15;
-        acc += pix_dist(cur8[p], qpel_p(s.P, 3 + c, 3 + r, fx, fy), d.metric);
-      }
-      predD = wave_sum(acc);
-      wave_sync();
+      predD = quarter_dist(d, s, predX0 + n, predY0 + m, predXQ, predYQ, lane);
     }
     // fifo [best, pred, zero, zero, zero]; strictly smaller cost wins
     // (selects rather than an indexed array, which would live in scratch)

@@ -225,6 +225,36 @@ int icu_cavlc_decode_device(icu_cavlc* h, const uint8_t* d_stream, long stream_b
                             long nblocks, int mode, const int8_t* d_nc, int nc, int dc_skip, int16_t* d_coef,
                             int32_t* d_status, void* stream);
 
+/* ---------------------------------------------------------- UMHS estimator
+ * MotionEstimatorH264ImplUMHS (include/CodecUtils/MotionEstimatorH264ImplUMHS.h:78-126,
+ * src/CodecUtils/MotionEstimatorH264ImplUMHS.cpp:446-940): 16x16 SSD search
+ * from the H.264 median predictor with distortion, the zero vector and the
+ * previous frame's co-located vector; early-termination tests; unsymmetrical
+ * cross, 5x5, uneven multi-hexagon, extended hexagon and diamond stages;
+ * half / quarter refinement; a final test against the predictor.
+ * motion_range: quarter pels, as the reference's constructor (8..256; full
+ * pel window +-(motion_range/4 - 1) within the 16-pel border).  mode: the
+ * reference's SetMode, 0 quarter, 1 half, 2 full pel.  The predictor is the
+ * reference's H264MotionVectorPredictorImpl1 over the picture as one slice. */
+typedef struct icu_umhs icu_umhs;
+int icu_umhs_create(int width, int height, int motion_range, int mode, icu_umhs** out);
+int icu_umhs_destroy(icu_umhs* h);
+/* Host planes (int16 0..255, W*H).  prev_mv (2*nMB quarter-pel) and
+ * prev_intra (nMB flags) are the previous frame's macroblocks (both NULL:
+ * none, as if every one were intra; the reference class needs them).  A
+ * previous vector must keep its block within the 16-pel border, as the
+ * reference's own vectors do.  include (nullable) selects the MBs of *avg.
+ * dist (nullable): per MB the distortion the predictor keeps. */
+int icu_umhs_estimate(icu_umhs* h, const int16_t* cur, const int16_t* ref, const int16_t* prev_mv,
+                      const uint8_t* prev_intra, const uint8_t* include, int16_t* mv, int32_t* dist, long* avg);
+/* Device form: 8-bit planes (d_cur 16-byte aligned, stride W), asynchronous
+ * on stream.  One wave per MB row runs the rows as a wavefront; calls that
+ * overlap on one handle race.  icu_umhs_check waits for the device and
+ * reports a wavefront that timed out (ICU_E_STATE; not expected). */
+int icu_umhs_estimate_device(icu_umhs* h, const uint8_t* d_cur, const uint8_t* d_ref, const int16_t* d_prev_mv,
+                             const uint8_t* d_prev_intra, int16_t* d_mv, int32_t* d_dist, void* stream);
+int icu_umhs_check(icu_umhs* h);
+
 /* --------------------------------------------------- raw YUV and MV files
  * YuvRawFileHandler (include/CodecUtils/YuvRawFileHandler.h:53-98,
  * src/CodecUtils/YuvRawFileHandler.cpp:78-184 over RawFileHandlerBase,

@@ -360,3 +360,257 @@ int orc_block_dist(const int16_t *a256, const int16_t *b256, int metric) {
   for (int i = 0; i < 256; i++) blk[i] = b256[i];
   return blk_dist(a256, 16, 0, 0, blk, metric);
 }
+
+/* ------------------------------------------------------------------------
+ * MotionEstimatorH264ImplUMHS::Estimate (src/CodecUtils/MotionEstimatorH264ImplUMHS.cpp:446-940),
+ * SSD as shipped (:78 #undef MEH264IUMHS_ABS_DIFF), restated step for step:
+ * starting points (predictor, zero, the previous frame's co-located vector),
+ * early-termination tests, unsymmetrical cross, 5x5, uneven multi-hexagon,
+ * extended hexagon and diamond loops, then the sub-pel stages shared with the
+ * full-search estimator and the final test against the predictor.
+ * The predictor is H264MotionVectorPredictorImpl1's distortion form
+ * (MacroBlockH264::GetMbMotionMedianPred, MacroBlockH264.cpp:818-915),
+ * including its comparison of the y median with C's x component (:893).
+ * ---------------------------------------------------------------------- */
+
+/* Pattern tables (UMHS.cpp:80-122). */
+static const int U_SQ5[24][2] = {{-1, 0}, {1, 0},  {0, -1},  {0, 1},  {-1, -1}, {1, -1}, {-1, 1}, {1, 1},
+                                 {0, -2}, {-2, 0}, {2, 0},   {0, 2},  {-1, -2}, {1, -2}, {-2, -1}, {2, -1},
+                                 {-2, 1}, {2, 1},  {-1, 2},  {1, 2},  {-2, -2}, {2, -2}, {-2, 2}, {2, 2}};
+static const int U_CROSS[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+static const int U_HEX16[16][2] = {{-4, 0}, {4, 0},  {-4, 1}, {4, 1},  {-4, -1}, {4, -1}, {-4, 2}, {4, 2},
+                                   {-4, -2}, {4, -2}, {-2, 3}, {2, 3}, {-2, -3}, {2, -3}, {0, 4},  {0, -4}};
+static const int U_EXTHEX[6][2] = {{-2, 0}, {-1, -2}, {1, -2}, {2, 0}, {1, 2}, {-1, 2}};
+
+/* How often each branch ran since the last orc_umhs_stats call (so tests can
+ * show that a set of inputs reaches every stage): 0 cross, 1 5x5, 2 hexagon,
+ * 3 hexagon early break, 4 extended-hexagon moves, 5 diamond moves,
+ * 6 previous-frame candidate tested, 7 predictor truncated, 8 predictor
+ * chosen at the end, 9 exit to the diamond, 10 exit to the hexagon before
+ * the cross, 11 exit after the cross, 12 exit after the 5x5. */
+static long umhs_stat[13];
+void orc_umhs_stats(long *out) {
+  for (int i = 0; i < 13; i++) { out[i] = umhs_stat[i]; umhs_stat[i] = 0; }
+}
+
+/* MEH264IUMHS_COST (UMHS.cpp:167): distortion / 256 plus 1.05 x the
+ * Euclidean distance to (xr, yr), truncated to int. */
+static int umhs_cost(int d, int x, int y, int xr, int yr) {
+  const double e = sqrt((double)((x - xr) * (x - xr) + (y - yr) * (y - yr)));
+  return (int)(((double)d / 256.0) + (1.05 * e));
+}
+
+/* Median prediction with its distortion from MBs decided in this call. */
+void orc_umhs_pred(const int16_t *mv, const int32_t *dist, int mbw, int col, int row, int *px, int *py,
+                   int *pd) {
+  int Ax = 0, Ay = 0, Ad = 0, Bx = 0, By = 0, Bd = 0, Cx = 0, Cy = 0, Cd = 0;
+  const int hasB = row > 0, hasC = row > 0 && col + 1 < mbw;
+  if (col > 0) { int k = row * mbw + col - 1; Ax = mv[2 * k]; Ay = mv[2 * k + 1]; Ad = dist[k]; }
+  if (hasB) { int k = (row - 1) * mbw + col; Bx = mv[2 * k]; By = mv[2 * k + 1]; Bd = dist[k]; }
+  if (hasC) {
+    int k = (row - 1) * mbw + col + 1; Cx = mv[2 * k]; Cy = mv[2 * k + 1]; Cd = dist[k];
+  } else if (col > 0 && row > 0) {
+    int k = (row - 1) * mbw + col - 1; Cx = mv[2 * k]; Cy = mv[2 * k + 1]; Cd = dist[k];
+  }
+  if (!hasB && !hasC) { Bx = Cx = Ax; By = Cy = Ay; Bd = Cd = Ad; }
+  const int x = median3(Ax, Bx, Cx), y = median3(Ay, By, Cy);
+  int xd = Ad, yd = Ad;
+  if ((x == Bx) && (Bd < xd)) xd = Bd;
+  if ((x == Cx) && (Cd < xd)) xd = Cd;
+  if ((y == By) && (Bd < yd)) yd = Bd;
+  if ((y == Cx) && (Cd < yd)) yd = Cd; /* the reference compares with Cx here */
+  *px = x;
+  *py = y;
+  *pd = yd < xd ? yd : xd;
+}
+
+typedef struct {
+  const int16_t *cur;
+  const plane_t *ref;
+  int W, n, m;
+  int xl, xr, yu, yd;
+  int pX0, pY0; /* cost reference: the truncated full-pel predictor */
+  int minD, minC, rmx, rmy;
+} umhs_t;
+
+/* One candidate at offset (j, i) from (mx, my) (the loops' common body). */
+static void umhs_try(umhs_t *u, int mx, int my, int j, int i) {
+  if ((i + my) < u->yu || (i + my) > u->yd || (j + mx) < u->xl || (j + mx) > u->xr) return;
+  int blk[256];
+  read_full(u->ref, u->n + mx + j, u->m + my + i, blk);
+  const int d = blk_dist(u->cur, u->W, u->n, u->m, blk, 0);
+  if (d <= u->minD) {
+    const int c = umhs_cost(d, mx + j, my + i, u->pX0, u->pY0);
+    if (c < u->minC) { u->minD = d; u->minC = c; u->rmx = j; u->rmy = i; }
+  }
+}
+
+int orc_umhs_estimate(const int16_t *cur, const int16_t *refp, int W, int H, int qrange, int mode,
+                      const int16_t *prev_mv, const uint8_t *prev_intra, const uint8_t *include,
+                      int16_t *mv_out, int32_t *dist_out, long *avg_out) {
+  if (W <= 0 || H <= 0 || (W & 15) || (H & 15) || qrange < 4) return -1;
+  const int mbw = W / 16, mbh = H / 16, mRng = qrange / 4;
+  plane_t ref = {refp, W, H};
+  qwin_t *win = (qwin_t *)malloc(sizeof(qwin_t));
+  int32_t *pdist = (int32_t *)calloc((size_t)mbw * mbh, sizeof(int32_t));
+  if (!win || !pdist) { free(win); free(pdist); return -2; }
+  memset(win, 0, sizeof(*win));
+  int blk[256];
+  long total = 0;
+  int included = 0;
+  for (int row = 0; row < mbh; row++)
+    for (int col = 0; col < mbw; col++) {
+      const int n = col * 16, m = row * 16, vec = row * mbw + col;
+      int predX, predY, predD;
+      orc_umhs_pred(mv_out, pdist, mbw, col, row, &predX, &predY, &predD);
+      const int predXQ = predX % 4, predYQ = predY % 4;
+      int predX0 = predX / 4, predY0 = predY / 4;
+      const int oX0 = predX0, oY0 = predY0;
+      if (predX0 + n > W) predX0 = 16;
+      if (predX0 + n < -16) predX0 = -16;
+      if (predY0 + m > H) predY0 = 16;
+      if (predY0 + m < -16) predY0 = -16;
+      const int trunc = (oX0 != predX0) || (oY0 != predY0);
+      umhs_stat[7] += trunc;
+      umhs_t u;
+      u.cur = cur; u.ref = &ref; u.W = W; u.n = n; u.m = m;
+      motion_range(n, m, W, H, mRng, 0, &u.xl, &u.xr, &u.yu, &u.yd);
+      u.pX0 = predX0; u.pY0 = predY0;
+      read_full(&ref, n + predX0, m + predY0, blk);
+      int predVecD = blk_dist(cur, W, n, m, blk, 0);
+      int mx = predX0, my = predY0;
+      u.minD = predVecD;
+      u.minC = predVecD / 256;
+      u.rmx = u.rmy = 0;
+      if (predX0 || predY0) {
+        read_full(&ref, n, m, blk);
+        const int zd = blk_dist(cur, W, n, m, blk, 0);
+        const int zc = umhs_cost(zd, 0, 0, predX0, predY0);
+        if (zc < u.minC) { u.minD = zd; u.minC = zc; mx = 0; my = 0; }
+      }
+      if (prev_mv && !prev_intra[vec]) {
+        const int qX0 = prev_mv[2 * vec] / 4, qY0 = prev_mv[2 * vec + 1] / 4;
+        if ((qX0 || qY0) && (qX0 != predX0) && (qY0 != predY0)) {
+          read_full(&ref, n + qX0, m + qY0, blk);
+          const int qd = blk_dist(cur, W, n, m, blk, 0);
+          const int qc = umhs_cost(qd, qX0, qY0, predX0, predY0);
+          if (qc < u.minC) { u.minD = qd; u.minC = qc; mx = qX0; my = qY0; }
+          umhs_stat[6]++;
+        }
+      }
+      int stage; /* 0: cross, 1: extended hexagon, 2: diamond */
+      if (u.minD < 1000) stage = 2;
+      else if ((u.minD < 4000) ||
+               ((u.minD == predVecD) && (u.minD < (predD * 12 / 10)) && (u.minD > (predD * 8 / 10))))
+        stage = 1;
+      else stage = 0;
+      umhs_stat[stage == 2 ? 9 : (stage == 1 ? 10 : 0)]++;
+      if (stage == 0) {
+        /* unsymmetrical cross (:582-612) */
+        u.rmx = u.rmy = 0;
+        int prior = u.minD;
+        for (int w = 1; w < 16; w += 2)
+          for (int x = 0; x < 4; x++) {
+            const int i = w * U_CROSS[x][1], j = w * U_CROSS[x][0];
+            if ((i < mRng / 2) && (i > -mRng / 2)) umhs_try(&u, mx, my, j, i);
+          }
+        mx += u.rmx; my += u.rmy;
+        if (u.minD < 1000) stage = 2;
+        else if ((u.minD < 2000) || ((u.minD < (prior * 9 / 10)) && (u.rmx || u.rmy) && (abs(u.rmx) <= 3) &&
+                                     (abs(u.rmy) <= 3)))
+          stage = 1;
+        if (stage != 0) umhs_stat[11]++;
+        if (stage == 0) {
+          umhs_stat[1]++;
+          /* 5x5 (:624-653); its winner moves the centre only through the
+             hexagon stage below */
+          const int sx = u.rmx, sy = u.rmy;
+          u.rmx = u.rmy = 0;
+          for (int w = 0; w < 24; w++) {
+            const int i = U_SQ5[w][1], j = U_SQ5[w][0];
+            const int done = (sx || sy) && (((sy == 0) && ((j == 2) || (j == -2))) ||
+                                            ((sx == 0) && ((i == 2) || (i == -2))));
+            if (!done) umhs_try(&u, mx, my, j, i);
+          }
+          if (u.minD < 1000) {
+            stage = 2;
+            umhs_stat[12]++;
+          } else {
+            umhs_stat[2]++;
+            /* uneven multi-hexagon (:663-698) */
+            prior = u.minD;
+            for (int w = 1; w < 4; w++) {
+              for (int x = 0; x < 16; x++) umhs_try(&u, mx, my, w * U_HEX16[x][0], w * U_HEX16[x][1]);
+              if ((u.rmx || u.rmy) && ((u.minD < 2000) || (u.minD < (prior / 5)))) { umhs_stat[3]++; break; }
+            }
+            mx += u.rmx; my += u.rmy;
+            stage = 1;
+          }
+        }
+      }
+      if (stage == 1) {
+        /* extended hexagon until its centre wins (:706-737) */
+        do {
+          u.rmx = u.rmy = 0;
+          for (int x = 0; x < 6; x++) umhs_try(&u, mx, my, U_EXTHEX[x][0], U_EXTHEX[x][1]);
+          mx += u.rmx; my += u.rmy;
+          umhs_stat[4] += (u.rmx || u.rmy);
+        } while (u.rmx || u.rmy);
+      }
+      /* diamond until its centre wins (:740-770) */
+      do {
+        u.rmx = u.rmy = 0;
+        for (int x = 0; x < 4; x++) umhs_try(&u, mx, my, U_CROSS[x][0], U_CROSS[x][1]);
+        mx += u.rmx; my += u.rmy;
+        umhs_stat[5] += (u.rmx || u.rmy);
+      } while (u.rmx || u.rmy);
+
+      /* sub-pel stages (:795-862), as the full-search estimator's */
+      int minD = u.minD;
+      int mvx = mx * 4, mvy = my * 4;
+      if (mode != 2) {
+        int hmx = 0, hmy = 0;
+        load_half_window(win, &ref, n + mx, m + my);
+        for (int k = 0; k < 8; k++) {
+          const int ox = 2 * SUBPOS[k][0], oy = 2 * SUBPOS[k][1];
+          window_read(win, ox, oy, blk);
+          const int d = blk_dist(cur, W, n, m, blk, 0);
+          if (d < minD) { minD = d; hmx = ox; hmy = oy; }
+        }
+        int qmx = hmx, qmy = hmy;
+        if (mode != 1) {
+          load_quarter_window(win, hmx, hmy);
+          for (int k = 0; k < 8; k++) {
+            const int ox = hmx + SUBPOS[k][0], oy = hmy + SUBPOS[k][1];
+            window_read(win, ox, oy, blk);
+            const int d = blk_dist(cur, W, n, m, blk, 0);
+            if (d < minD) { minD = d; qmx = ox; qmy = oy; }
+          }
+        }
+        mvx += qmx;
+        mvy += qmy;
+      }
+      /* the predictor at quarter pel (:866-903) */
+      const int rX = predX0 * 4 + predXQ, rY = predY0 * 4 + predYQ;
+      if (predXQ || predYQ) {
+        orc_quarter_read(refp, W, H, predX0 + n, predY0 + m, predXQ, predYQ, blk);
+        predVecD = blk_dist(cur, W, n, m, blk, 0);
+      } else if (trunc) {
+        read_full(&ref, predX0 + n, predY0 + m, blk);
+        predVecD = blk_dist(cur, W, n, m, blk, 0);
+      }
+      if (umhs_cost(predVecD, rX, rY, predX, predY) <= umhs_cost(minD, mvx, mvy, predX, predY)) {
+        minD = predVecD; mvx = rX; mvy = rY;
+        umhs_stat[8]++;
+      }
+      if (include && include[vec]) { included++; total += minD; }
+      mv_out[2 * vec] = (int16_t)mvx;
+      mv_out[2 * vec + 1] = (int16_t)mvy;
+      pdist[vec] = minD;
+      if (dist_out) dist_out[vec] = minD;
+    }
+  free(win);
+  free(pdist);
+  if (avg_out) *avg_out = included ? total / included : 0;
+  return 0;
+}

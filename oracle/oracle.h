@@ -36,6 +36,18 @@ void orc_quarter_read(const int16_t *ref, int W, int H, int x, int y, int qx, in
                       int *blk);
 void orc_median_pred(const int16_t *mv, int mbw, int col, int row, int *px, int *py);
 
+/* MotionEstimatorH264ImplUMHS (me_oracle.c).  motion_range in quarter pels
+ * as the reference's constructor; mode 0 quarter, 1 half, 2 full pel.
+ * prev_mv / prev_intra: the previous frame's co-located 16x16 vectors and
+ * intra flags (NULL: no previous-frame candidate).  dist_out: the chosen
+ * vector's SSD, which the predictor also keeps. */
+int orc_umhs_estimate(const int16_t *cur, const int16_t *ref, int W, int H, int motion_range, int mode,
+                      const int16_t *prev_mv, const uint8_t *prev_intra, const uint8_t *include,
+                      int16_t *mv_out, int32_t *dist_out, long *avg_out);
+void orc_umhs_stats(long *out13);
+void orc_umhs_pred(const int16_t *mv, const int32_t *dist, int mbw, int col, int row, int *px, int *py,
+                   int *pd);
+
 /* 4x4 transforms.  mode: 0 IT+Q, 1 IT only, 2 Q only. */
 void orc_fwd4x4(int16_t *blk, int qp, int intra, int mode);
 void orc_inv4x4(int16_t *blk, int qp, int mode, const int *weight16);

@@ -73,6 +73,9 @@ def lib():
                                        _u8p, _i16p, _u8p, _i32p]
         L.orc_vbs_estimate.restype = C.c_int
         L.orc_vbs_pred.argtypes = [_u8p, C.c_int, C.c_int, _i16p, _u8p, _i16p]
+        L.orc_umhs_stats.argtypes = [C.c_void_p]
+        L.orc_umhs_estimate.argtypes = [_i16p, _i16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, _i16p, C.c_void_p, C.POINTER(C.c_long)]
         L.orc_mc_chroma.argtypes = [_u8p, C.c_int, C.c_int, _i16p, _u8p]
         _lib = L
     return _lib
@@ -98,6 +101,9 @@ def ref_lib():
         L.ref_yuv_read.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _i32p]
         L.ref_mv_dump.argtypes = [C.c_char_p, C.c_char_p, _i16p, C.c_int]
         L.ref_cavlc.argtypes = [_i16p, C.c_int, C.c_int, _i32p, C.c_int, _u8p, C.c_int, _i32p, _i16p, _i32p]
+        L.ref_umhs_estimate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i16p, _i16p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_long), C.c_int,
+                                        C.POINTER(C.c_double)]
         _ref = L
     return _ref
 
@@ -197,6 +203,66 @@ def vbs_estimate(cur, refs, R, lam=4):
         raise ValueError(f"oracle vbs_estimate failed ({rc})")
     return (mode.reshape(H // 16, W // 16), mv8.reshape(H // 8, W // 8, 2), ref8.reshape(H // 8, W // 8),
             cost.reshape(H // 16, W // 16))
+
+
+def umhs_estimate(cur, ref, qrange, mode=0, prev_mv=None, prev_intra=None, include=None):
+    """MotionEstimatorH264ImplUMHS through the oracle.  qrange: the
+    reference's motionRange (quarter pels).  Returns (mv (nMB, 2) int16,
+    dist (nMB,) int32, avg)."""
+    cur = np.ascontiguousarray(cur, dtype=np.int16)
+    ref = np.ascontiguousarray(ref, dtype=np.int16)
+    H, W = cur.shape
+    n = (W // 16) * (H // 16)
+    mv = np.zeros(2 * n, np.int16)
+    dist = np.zeros(n, np.int32)
+    avg = C.c_long(0)
+    pm = pi = inc = None
+    if prev_mv is not None:
+        prev_mv = np.ascontiguousarray(prev_mv, np.int16).reshape(-1)
+        prev_intra = np.ascontiguousarray(prev_intra if prev_intra is not None else np.zeros(n), np.uint8)
+        pm, pi = prev_mv.ctypes.data, prev_intra.ctypes.data
+    if include is not None:
+        include = np.ascontiguousarray(include, np.uint8)
+        inc = include.ctypes.data
+    rc = lib().orc_umhs_estimate(cur, ref, W, H, qrange, mode, C.c_void_p(pm), C.c_void_p(pi), C.c_void_p(inc),
+                                 mv, dist.ctypes.data, C.byref(avg))
+    if rc != 0:
+        raise ValueError(f"oracle umhs_estimate failed ({rc})")
+    return mv.reshape(n, 2), dist, avg.value
+
+
+def umhs_stats():
+    """Branch counts of the UMHS oracle since the previous call (13 counters,
+    named in me_oracle.c)."""
+    st = np.zeros(13, np.int64)
+    lib().orc_umhs_stats(st.ctypes.data)
+    return st
+
+
+def ref_umhs_estimate(R_lib, cur, ref, qrange, mode=0, prev_mv=None, prev_intra=None, include=None, reps=1):
+    """The compiled reference class (test infrastructure).  Returns (mv, dist,
+    avg, seconds)."""
+    cur = np.ascontiguousarray(cur, dtype=np.int16)
+    ref = np.ascontiguousarray(ref, dtype=np.int16)
+    H, W = cur.shape
+    n = (W // 16) * (H // 16)
+    mv = np.zeros(2 * n, np.int16)
+    dist = np.zeros(n, np.int32)
+    avg = C.c_long(0)
+    sec = C.c_double(0)
+    pm = pi = inc = None
+    if prev_mv is not None:
+        prev_mv = np.ascontiguousarray(prev_mv, np.int16).reshape(-1)
+        prev_intra = np.ascontiguousarray(prev_intra if prev_intra is not None else np.zeros(n), np.uint8)
+        pm, pi = prev_mv.ctypes.data, prev_intra.ctypes.data
+    if include is not None:
+        include = np.ascontiguousarray(include, np.uint8)
+        inc = include.ctypes.data
+    rc = R_lib.ref_umhs_estimate(W, H, qrange, mode, cur, ref, pm, pi, inc, mv.ctypes.data, dist.ctypes.data,
+                                 C.byref(avg), reps, C.byref(sec))
+    if rc != 0:
+        raise ValueError(f"reference umhs failed ({rc})")
+    return mv.reshape(n, 2), dist, avg.value, sec.value
 
 
 def vbs_pred(refs, mv8, ref8):

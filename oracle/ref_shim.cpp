@@ -13,6 +13,7 @@
 //   YuvRawFileHandler                    src/CodecUtils/YuvRawFileHandler.cpp:78-184
 //   VectorStructList::Dump               src/CodecUtils/VectorStructList.cpp:242-270
 //   CAVLCH264Impl + its VLC coders       src/CodecUtils/CAVLCH264Impl.cpp:134-577
+//   MotionEstimatorH264ImplUMHS          src/CodecUtils/MotionEstimatorH264ImplUMHS.cpp:231-940
 #include <math.h>  // the estimator header calls sqrt without including it
 
 #include <chrono>
@@ -29,6 +30,7 @@
 #include "MacroBlockH264.h"
 #include "MotionCompensatorH264ImplStd.h"
 #include "MotionEstimatorH264ImplFull.h"
+#include "MotionEstimatorH264ImplUMHS.h"
 #include "OverlayExtMem2Dv2.h"
 #include "OverlayMem2Dv2.h"
 #include "VectorStructList.h"
@@ -123,6 +125,46 @@ int ref_me_estimate(int W, int H, int motionRange, int pred_mode, const short* c
   }
   if (avg_out) *avg_out = avg;
   delete r;
+  return 0;
+}
+
+// One Estimate() of MotionEstimatorH264ImplUMHS (the constructor with the
+// previous frame's macroblocks; the shorter ones leave that pointer NULL,
+// which Estimate dereferences) over an H264MotionVectorPredictorImpl1 on a
+// fresh macroblock array.  prev_mv / prev_intra NULL: every previous MB
+// intra.  dist_out: the distortion the predictor keeps per MB.
+int ref_umhs_estimate(int W, int H, int motionRange, int mode, const short* cur, const short* ref,
+                      const short* prev_mv, const unsigned char* prev_intra, const unsigned char* include,
+                      short* mv_out, int* dist_out, long* avg_out, int reps, double* seconds) {
+  const int mbw = W / 16, mbh = H / 16, nmb = mbw * mbh;
+  std::vector<short> c(cur, cur + (long)W * H), r(ref, ref + (long)W * H);
+  std::vector<MacroBlockH264> mbs(nmb), prev(nmb);
+  std::vector<MacroBlockH264*> rows(mbh);
+  for (int i = 0; i < mbh; i++) rows[i] = &mbs[(size_t)i * mbw];
+  MacroBlockH264::Initialise(mbh, mbw, 0, nmb - 1, 0, rows.data());
+  for (int k = 0; k < nmb; k++) {
+    prev[k]._intraFlag = prev_mv ? (prev_intra[k] ? 1 : 0) : 1;
+    prev[k]._mvX[MacroBlockH264::_16x16] = prev_mv ? prev_mv[2 * k] : 0;
+    prev[k]._mvY[MacroBlockH264::_16x16] = prev_mv ? prev_mv[2 * k + 1] : 0;
+  }
+  H264MotionVectorPredictorImpl1 pred(mbs.data());
+  MotionEstimatorH264ImplUMHS me(c.data(), r.data(), W, H, motionRange, &pred, (void*)include, prev.data());
+  if (!me.Create()) return -1;
+  me.SetMode(mode);
+  long avg = 0;
+  VectorStructList* l = nullptr;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < (reps < 1 ? 1 : reps); i++) l = (VectorStructList*)me.Estimate(&avg);
+  auto t1 = std::chrono::steady_clock::now();
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  for (int i = 0; i < nmb; i++) {
+    if (mv_out) {
+      mv_out[2 * i] = (short)l->GetSimpleElement(i, 0);
+      mv_out[2 * i + 1] = (short)l->GetSimpleElement(i, 1);
+    }
+    if (dist_out) dist_out[i] = mbs[i]._distortion[0];
+  }
+  if (avg_out) *avg_out = avg;
   return 0;
 }
 

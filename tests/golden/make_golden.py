@@ -314,6 +314,68 @@ def make_cavlc(R_lib):
     np.savez_compressed(OUT / "cavlc.npz", **out)
 
 
+def umhs_scenes():
+    """Frame pairs for the UMHS fixtures: moving rectangles at several
+    sizes and ranges, and whole-picture pans that push vectors past the
+    picture edge (the predictor truncation)."""
+    sc = []
+    for (W, H, seed, nr, mv, sig) in [(176, 144, 1, 6, 8, 2.0), (352, 288, 2, 10, 16, 3.0),
+                                      (352, 288, 3, 12, 30, 4.0), (48, 32, 5, 3, 40, 6.0), (320, 240, 6, 8, 6, 2.0)]:
+        fp = synth.rect_motion_pair(W, H, seed, n_rects=nr, max_mv=mv, sigma_cur=sig)
+        sc.append((fp.cur.astype(np.uint8), fp.ref.astype(np.uint8)))
+    rng = np.random.default_rng(9)
+    for (W, H, sx, sy) in [(64, 48, 40, 0), (64, 48, 0, 30), (80, 64, -37, 25), (96, 64, 45, -33)]:
+        ref = rng.integers(0, 256, (H, W)).astype(np.uint8)
+        ys = np.clip(np.arange(H) + sy, 0, H - 1)
+        xs = np.clip(np.arange(W) + sx, 0, W - 1)
+        sc.append((ref[ys][:, xs].copy(), ref))
+    return sc
+
+
+# (scene, motionRange in quarter pels, mode, previous case or -1)
+UMHS_CASES = [(0, 64, 0, -1), (0, 64, 0, 0), (0, 64, 1, 1), (0, 64, 2, 1),
+              (1, 64, 0, -1), (1, 64, 0, 4), (2, 128, 0, -1), (2, 128, 1, 6),
+              (3, 256, 0, -1), (3, 256, 1, 8), (3, 256, 2, 8), (4, 32, 0, -1), (4, 32, 2, 11),
+              (5, 256, 0, -1), (6, 256, 0, -1), (7, 256, 1, -1), (8, 256, 0, -1), (8, 256, 2, 16),
+              (6, 64, 0, -1)]
+
+
+def make_umhs(R_lib):
+    """MotionEstimatorH264ImplUMHS from the compiled reference: vectors, the
+    per-MB distortion its predictor keeps and the average over an include
+    map.  A case with a previous case feeds that case's vectors as the
+    previous frame's, with about one MB in five marked intra, and codes the
+    current frame shifted 3 pels right (a second frame of the scene)."""
+    out = {}
+    scenes = umhs_scenes()
+    for i, (cur, ref) in enumerate(scenes):
+        out[f"s{i}_cur"], out[f"s{i}_ref"] = cur, ref
+    results = []
+    for k, (si, qr, mode, prev) in enumerate(UMHS_CASES):
+        cur, ref = scenes[si]
+        H, W = cur.shape
+        n = (W // 16) * (H // 16)
+        rng = np.random.default_rng(500 + k)
+        if prev >= 0:
+            cur = np.roll(cur, 3, axis=1)
+            pm = results[prev]
+            pi = (rng.random(n) < 0.2).astype(np.uint8)
+        else:
+            pm = pi = None
+        inc = (rng.random(n) < 0.7).astype(np.uint8)
+        mv, dist, avg, _ = orc.ref_umhs_estimate(R_lib, cur.astype(np.int16), ref.astype(np.int16), qr, mode,
+                                                 pm, pi, inc)
+        results.append(mv)
+        out[f"c{k}_case"] = np.array([si, qr, mode, prev], np.int32)
+        if pm is not None:
+            out[f"c{k}_prev_mv"], out[f"c{k}_prev_intra"] = pm, pi
+        out[f"c{k}_include"] = inc
+        out[f"c{k}_mv"], out[f"c{k}_dist"], out[f"c{k}_avg"] = mv, dist, np.int64(avg)
+    out["ncases"] = np.int32(len(UMHS_CASES))
+    np.savez_compressed(OUT / "umhs.npz", **out)
+    print("umhs", len(UMHS_CASES), "cases")
+
+
 def main():
     if not orc.build_ref():
         sys.exit("reference sources not available; fixtures are committed")
@@ -325,6 +387,7 @@ def main():
     make_mc_yuv(R_lib)
     make_io(R_lib)
     make_cavlc(R_lib)
+    make_umhs(R_lib)
 
 
 if __name__ == "__main__":
