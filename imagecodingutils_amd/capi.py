@@ -110,6 +110,11 @@ _SIGS = {
     "icu_umhs_estimate": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_long)]),
     "icu_umhs_estimate_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "icu_umhs_check": (C.c_int, [_vp]),
+    "icu_cross_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "icu_cross_destroy": (C.c_int, [_vp]),
+    "icu_cross_estimate": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_long)]),
+    "icu_cross_estimate_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "icu_cross_check": (C.c_int, [_vp]),
     "icu_last_error": (C.c_char_p, []),
     "icu_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_malloc": (_vp, [C.c_size_t]),
@@ -559,6 +564,43 @@ class UmhsEstimator:
     def close(self):
         if getattr(self, "h", None):
             lib.icu_umhs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class CrossEstimator:
+    """MotionEstimatorH264ImplCross on the device (icu_cross_*).  predictor
+    0 zero, 1 H.264 median."""
+
+    def __init__(self, width, height, motion_range, mode=0, predictor=0):
+        h = _vp()
+        _check(lib.icu_cross_create(width, height, motion_range, mode, predictor, C.byref(h)))
+        self.h, self.W, self.H = h, width, height
+        self.nmb = (width // 16) * (height // 16)
+
+    def estimate(self, cur, ref, include=None):
+        c = np.ascontiguousarray(cur, np.int16)
+        r = np.ascontiguousarray(ref, np.int16)
+        mv = np.zeros((self.nmb, 2), np.int16)
+        dist = np.zeros(self.nmb, np.int32)
+        avg = C.c_long(0)
+        inc = None if include is None else np.ascontiguousarray(include, np.uint8)
+        _check(lib.icu_cross_estimate(self.h, c.ctypes.data, r.ctypes.data, None if inc is None else inc.ctypes.data,
+                                      mv.ctypes.data, dist.ctypes.data, C.byref(avg)))
+        return mv, dist, avg.value
+
+    def estimate_device(self, d_cur, d_ref, d_mv, d_dist, stream: Stream | None = None):
+        _check(lib.icu_cross_estimate_device(self.h, _p(d_cur), _p(d_ref), _p(d_mv), _p(d_dist),
+                                             stream.handle if stream else None))
+
+    def check(self):
+        _check(lib.icu_cross_check(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.icu_cross_destroy(self.h)
             self.h = None
 
     def __del__(self):

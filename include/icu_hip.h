@@ -255,6 +255,28 @@ int icu_umhs_estimate_device(icu_umhs* h, const uint8_t* d_cur, const uint8_t* d
                              const uint8_t* d_prev_intra, int16_t* d_mv, int32_t* d_dist, void* stream);
 int icu_umhs_check(icu_umhs* h);
 
+/* --------------------------------------------------------- Cross estimator
+ * MotionEstimatorH264ImplCross (include/CodecUtils/MotionEstimatorH264ImplCross.h:78-190,
+ * src/CodecUtils/MotionEstimatorH264ImplCross.cpp:354-660): a cross search
+ * whose arm halves from range/2 to 1, half / quarter refinement and
+ * GetBestVector (lambda 0.9), every distortion the squared error over the
+ * estimator's 64-point sampling path.  motion_range and mode as
+ * icu_umhs_create; predictor 0 zero vector (MBs independent), 1 the H.264
+ * median of the MBs decided before (H264MotionVectorPredictorImpl1; rows
+ * run as a wavefront). */
+typedef struct icu_cross icu_cross;
+int icu_cross_create(int width, int height, int motion_range, int mode, int predictor, icu_cross** out);
+int icu_cross_destroy(icu_cross* h);
+/* Host planes (int16 0..255).  dist (nullable): per MB the path distortion
+ * of the chosen vector; *avg over include (nullable) as the reference. */
+int icu_cross_estimate(icu_cross* h, const int16_t* cur, const int16_t* ref, const uint8_t* include, int16_t* mv,
+                       int32_t* dist, long* avg);
+/* Device form (8-bit planes, stride W), asynchronous on stream; calls that
+ * overlap on one handle race.  icu_cross_check as icu_umhs_check. */
+int icu_cross_estimate_device(icu_cross* h, const uint8_t* d_cur, const uint8_t* d_ref, int16_t* d_mv,
+                              int32_t* d_dist, void* stream);
+int icu_cross_check(icu_cross* h);
+
 /* --------------------------------------------------- raw YUV and MV files
  * YuvRawFileHandler (include/CodecUtils/YuvRawFileHandler.h:53-98,
  * src/CodecUtils/YuvRawFileHandler.cpp:78-184 over RawFileHandlerBase,

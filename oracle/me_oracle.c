@@ -614,3 +614,165 @@ int orc_umhs_estimate(const int16_t *cur, const int16_t *refp, int W, int H, int
   if (avg_out) *avg_out = included ? total / included : 0;
   return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * MotionEstimatorH264ImplCross::Estimate (src/CodecUtils/MotionEstimatorH264ImplCross.cpp:354-660):
+ * a cross search whose arm halves from range/2 down to 1 around the running
+ * best, the full-search estimator's sub-pel stages, and GetBestVector
+ * (MotionEstimatorH264ImplCross.h:140-168) with lambda 0.9 over distortions
+ * normalised by the path length.  Every distortion is the squared error
+ * over the first 64 points of the estimator's sampling path
+ * (Tsd16x16PartialPath, OverlayMem2Dv2.cpp:2495-2518); the search's
+ * comparisons use Tsd16x16PartialPathLessThan (:2449-2493), which stops
+ * after any batch of 16 points whose extrapolation to the whole path
+ * exceeds the current minimum and returns that extrapolation.
+ * ---------------------------------------------------------------------- */
+
+/* The first 64 points of the sampling path (Cross.cpp:89-99), as y*16+x. */
+static const uint8_t CROSS_PATH[64] = {
+    165, 93,  18,  237, 225, 25,  97,  159, 216, 46,  71,  138, 244, 160, 48,  5,
+    188, 116, 12,  234, 195, 90,  223, 52,  135, 126, 214, 59,  146, 169, 31,  39,
+    83,  248, 193, 104, 16,  156, 254, 79,  203, 102, 183, 112, 190, 7,   242, 124,
+    50,  148, 44,  57,  220, 3,   229, 69,  208, 152, 178, 251, 26,  114, 111, 201};
+
+/* Squared error of each batch of 16 path points, block blk[256]. */
+static void cross_batches(const int16_t *cur, int W, int n, int m, const int *blk, int b[4]) {
+  for (int g = 0; g < 4; g++) {
+    b[g] = 0;
+    for (int k = 0; k < 16; k++) {
+      const int p = CROSS_PATH[16 * g + k], y = p >> 4, x = p & 15;
+      const int e = cur[(long)(m + y) * W + n + x] - blk[p];
+      b[g] += e * e;
+    }
+  }
+}
+
+static int cross_full(const int b[4]) { return b[0] + b[1] + b[2] + b[3]; }
+
+/* Branch counts since the last orc_cross_stats call: 0 a batch
+ * extrapolation rejected a candidate whose whole-path sum would have been
+ * accepted, 1 a tie taken by distance, 2 a tie kept, 3 the predictor
+ * chosen, 4 the zero vector chosen, 5 a cross arm outside the window. */
+static long cross_stat[6];
+void orc_cross_stats(long *out) {
+  for (int i = 0; i < 6; i++) { out[i] = cross_stat[i]; cross_stat[i] = 0; }
+}
+
+/* The LessThan variant's return value (4 batches of 16). */
+static int cross_lt(const int b[4], int min) {
+  int s = 0;
+  for (int g = 0; g < 4; g++) {
+    s += b[g];
+    const int e = (s * 4) / (g + 1);
+    if (g < 3 && e > min) return e;
+  }
+  return s;
+}
+
+int orc_cross_estimate(const int16_t *cur, const int16_t *refp, int W, int H, int qrange, int mode, int pred,
+                       const uint8_t *include, int16_t *mv_out, int32_t *dist_out, long *avg_out) {
+  if (W <= 0 || H <= 0 || (W & 15) || (H & 15) || qrange < 4) return -1;
+  const int mbw = W / 16, mbh = H / 16, mRng = qrange / 4;
+  plane_t ref = {refp, W, H};
+  qwin_t *win = (qwin_t *)malloc(sizeof(qwin_t));
+  if (!win) return -2;
+  memset(win, 0, sizeof(*win));
+  int blk[256], bt[4];
+  long total = 0;
+  int included = 0;
+  for (int row = 0; row < mbh; row++)
+    for (int col = 0; col < mbw; col++) {
+      const int n = col * 16, m = row * 16, vec = row * mbw + col;
+      int predX = 0, predY = 0;
+      if (pred) orc_median_pred(mv_out, mbw, col, row, &predX, &predY);
+      const int predXQ = predX % 4, predYQ = predY % 4;
+      int predX0 = predX / 4, predY0 = predY / 4;
+      const int pXr = predX < 0 ? (predX - 2) / 4 : (predX + 2) / 4;
+      const int pYr = predY < 0 ? (predY - 2) / 4 : (predY + 2) / 4;
+      read_full(&ref, n, m, blk);
+      cross_batches(cur, W, n, m, blk, bt);
+      const int zeroD = cross_full(bt);
+      int minD = zeroD, mx = 0, my = 0;
+      int xl, xr, yu, yd;
+      motion_range(n, m, W, H, mRng, 0, &xl, &xr, &yu, &yd);
+      for (int w = mRng / 2; w > 0; w /= 2) {
+        int rmx = 0, rmy = 0;
+        for (int x = 0; x < 4; x++) {
+          const int i = w * U_CROSS[x][1], j = w * U_CROSS[x][0];
+          if ((i + my) < yu || (i + my) > yd || (j + mx) < xl || (j + mx) > xr) { cross_stat[5]++; continue; }
+          read_full(&ref, n + j + mx, m + i + my, blk);
+          cross_batches(cur, W, n, m, blk, bt);
+          const int d = cross_lt(bt, minD);
+          if (d > minD && cross_full(bt) <= minD) cross_stat[0]++;
+          if (d <= minD) {
+            if (d == minD) {
+              const int cx = mx + rmx - pXr, cy = my + rmy - pYr, nx = mx + j - pXr, ny = my + i - pYr;
+              if ((cy * cy + cx * cx) - (ny * ny + nx * nx) < 0) { cross_stat[2]++; continue; }
+              cross_stat[1]++;
+            }
+            minD = d; rmx = j; rmy = i;
+          }
+        }
+        mx += rmx; my += rmy;
+      }
+      int mvx = mx * 4, mvy = my * 4;
+      if (mode != 2) {
+        int hmx = 0, hmy = 0;
+        load_half_window(win, &ref, n + mx, m + my);
+        for (int k = 0; k < 8; k++) {
+          const int ox = 2 * SUBPOS[k][0], oy = 2 * SUBPOS[k][1];
+          window_read(win, ox, oy, blk);
+          cross_batches(cur, W, n, m, blk, bt);
+          const int d = cross_lt(bt, minD);
+          if (d > minD && cross_full(bt) < minD) cross_stat[0]++;
+          if (d < minD) { minD = d; hmx = ox; hmy = oy; }
+        }
+        int qmx = hmx, qmy = hmy;
+        if (mode != 1) {
+          load_quarter_window(win, hmx, hmy);
+          for (int k = 0; k < 8; k++) {
+            const int ox = hmx + SUBPOS[k][0], oy = hmy + SUBPOS[k][1];
+            window_read(win, ox, oy, blk);
+            cross_batches(cur, W, n, m, blk, bt);
+            const int d = cross_lt(bt, minD);
+            if (d < minD) { minD = d; qmx = ox; qmy = oy; }
+          }
+        }
+        mvx += qmx;
+        mvy += qmy;
+      }
+      if (predX0 + n > W) predX0 = 16;
+      if (predX0 + n < -16) predX0 = -16;
+      if (predY0 + m > H) predY0 = 16;
+      if (predY0 + m < -16) predY0 = -16;
+      predX = predX0 * 4 + predXQ;
+      predY = predY0 * 4 + predYQ;
+      if (predXQ || predYQ)
+        orc_quarter_read(refp, W, H, predX0 + n, predY0 + m, predXQ, predYQ, blk);
+      else
+        read_full(&ref, predX0 + n, predY0 + m, blk);
+      cross_batches(cur, W, n, m, blk, bt);
+      const int predD = cross_full(bt);
+      /* GetBestVector over [best, pred, zero, zero, zero], lambda 0.9,
+         norm 64 (the path length) */
+      const int xs[5] = {mvx, predX, 0, 0, 0}, ys[5] = {mvy, predY, 0, 0, 0};
+      const int ds[5] = {minD, predD, zeroD, zeroD, zeroD};
+      int pos = 0;
+      double cc = ((double)ds[0] / 64.0) +
+                  (0.9 * sqrt((double)((xs[0] - predX) * (xs[0] - predX) + (ys[0] - predY) * (ys[0] - predY))));
+      for (int k = 1; k < 5; k++) {
+        const double nc = ((double)ds[k] / 64.0) +
+                          (0.9 * sqrt((double)((xs[k] - predX) * (xs[k] - predX) + (ys[k] - predY) * (ys[k] - predY))));
+        if (nc < cc) { cc = nc; pos = k; }
+      }
+      cross_stat[3] += pos == 1;
+      cross_stat[4] += pos >= 2;
+      if (include && include[vec]) { included++; total += ds[pos]; }
+      mv_out[2 * vec] = (int16_t)xs[pos];
+      mv_out[2 * vec + 1] = (int16_t)ys[pos];
+      if (dist_out) dist_out[vec] = ds[pos];
+    }
+  free(win);
+  if (avg_out) *avg_out = included ? total / included : 0;
+  return 0;
+}

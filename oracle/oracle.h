@@ -45,6 +45,11 @@ int orc_umhs_estimate(const int16_t *cur, const int16_t *ref, int W, int H, int 
                       const int16_t *prev_mv, const uint8_t *prev_intra, const uint8_t *include,
                       int16_t *mv_out, int32_t *dist_out, long *avg_out);
 void orc_umhs_stats(long *out13);
+/* MotionEstimatorH264ImplCross (me_oracle.c): pred 0 zero predictor, 1 the
+ * H.264 median; distortions are the estimator's 64-point path sums. */
+void orc_cross_stats(long *out6);
+int orc_cross_estimate(const int16_t *cur, const int16_t *ref, int W, int H, int qrange, int mode, int pred,
+                       const uint8_t *include, int16_t *mv_out, int32_t *dist_out, long *avg_out);
 void orc_umhs_pred(const int16_t *mv, const int32_t *dist, int mbw, int col, int row, int *px, int *py,
                    int *pd);
 

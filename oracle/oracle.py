@@ -74,6 +74,9 @@ def lib():
         L.orc_vbs_estimate.restype = C.c_int
         L.orc_vbs_pred.argtypes = [_u8p, C.c_int, C.c_int, _i16p, _u8p, _i16p]
         L.orc_umhs_stats.argtypes = [C.c_void_p]
+        L.orc_cross_stats.argtypes = [C.c_void_p]
+        L.orc_cross_estimate.argtypes = [_i16p, _i16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
+                                         _i16p, C.c_void_p, C.POINTER(C.c_long)]
         L.orc_umhs_estimate.argtypes = [_i16p, _i16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                         C.c_void_p, _i16p, C.c_void_p, C.POINTER(C.c_long)]
         L.orc_mc_chroma.argtypes = [_u8p, C.c_int, C.c_int, _i16p, _u8p]
@@ -101,6 +104,8 @@ def ref_lib():
         L.ref_yuv_read.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _i32p]
         L.ref_mv_dump.argtypes = [C.c_char_p, C.c_char_p, _i16p, C.c_int]
         L.ref_cavlc.argtypes = [_i16p, C.c_int, C.c_int, _i32p, C.c_int, _u8p, C.c_int, _i32p, _i16p, _i32p]
+        L.ref_cross_estimate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i16p, _i16p, C.c_void_p,
+                                         C.c_void_p, C.POINTER(C.c_long), C.c_int, C.POINTER(C.c_double)]
         L.ref_umhs_estimate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i16p, _i16p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_long), C.c_int,
                                         C.POINTER(C.c_double)]
@@ -229,6 +234,56 @@ def umhs_estimate(cur, ref, qrange, mode=0, prev_mv=None, prev_intra=None, inclu
     if rc != 0:
         raise ValueError(f"oracle umhs_estimate failed ({rc})")
     return mv.reshape(n, 2), dist, avg.value
+
+
+def cross_estimate(cur, ref, qrange, mode=0, pred=0, include=None):
+    """MotionEstimatorH264ImplCross through the oracle.  Returns (mv (nMB, 2)
+    int16, dist (nMB,) int32, avg)."""
+    cur = np.ascontiguousarray(cur, dtype=np.int16)
+    ref = np.ascontiguousarray(ref, dtype=np.int16)
+    H, W = cur.shape
+    n = (W // 16) * (H // 16)
+    mv = np.zeros(2 * n, np.int16)
+    dist = np.zeros(n, np.int32)
+    avg = C.c_long(0)
+    inc = None
+    if include is not None:
+        include = np.ascontiguousarray(include, np.uint8)
+        inc = include.ctypes.data
+    rc = lib().orc_cross_estimate(cur, ref, W, H, qrange, mode, pred, C.c_void_p(inc), mv, dist.ctypes.data,
+                                  C.byref(avg))
+    if rc != 0:
+        raise ValueError(f"oracle cross_estimate failed ({rc})")
+    return mv.reshape(n, 2), dist, avg.value
+
+
+def cross_stats():
+    """Branch counts of the Cross oracle since the previous call (6
+    counters, named in me_oracle.c)."""
+    st = np.zeros(6, np.int64)
+    lib().orc_cross_stats(st.ctypes.data)
+    return st
+
+
+def ref_cross_estimate(R_lib, cur, ref, qrange, mode=0, pred=0, include=None, reps=1):
+    """The compiled reference class (test infrastructure).  Returns (mv, avg,
+    seconds)."""
+    cur = np.ascontiguousarray(cur, dtype=np.int16)
+    ref = np.ascontiguousarray(ref, dtype=np.int16)
+    H, W = cur.shape
+    n = (W // 16) * (H // 16)
+    mv = np.zeros(2 * n, np.int16)
+    avg = C.c_long(0)
+    sec = C.c_double(0)
+    inc = None
+    if include is not None:
+        include = np.ascontiguousarray(include, np.uint8)
+        inc = include.ctypes.data
+    rc = R_lib.ref_cross_estimate(W, H, qrange, mode, pred, cur, ref, inc, mv.ctypes.data, C.byref(avg), reps,
+                                  C.byref(sec))
+    if rc != 0:
+        raise ValueError(f"reference cross failed ({rc})")
+    return mv.reshape(n, 2), avg.value, sec.value
 
 
 def umhs_stats():

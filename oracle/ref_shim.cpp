@@ -14,6 +14,7 @@
 //   VectorStructList::Dump               src/CodecUtils/VectorStructList.cpp:242-270
 //   CAVLCH264Impl + its VLC coders       src/CodecUtils/CAVLCH264Impl.cpp:134-577
 //   MotionEstimatorH264ImplUMHS          src/CodecUtils/MotionEstimatorH264ImplUMHS.cpp:231-940
+//   MotionEstimatorH264ImplCross         src/CodecUtils/MotionEstimatorH264ImplCross.cpp:137-660
 #include <math.h>  // the estimator header calls sqrt without including it
 
 #include <chrono>
@@ -31,6 +32,7 @@
 #include "MotionCompensatorH264ImplStd.h"
 #include "MotionEstimatorH264ImplFull.h"
 #include "MotionEstimatorH264ImplUMHS.h"
+#include "MotionEstimatorH264ImplCross.h"
 #include "OverlayExtMem2Dv2.h"
 #include "OverlayMem2Dv2.h"
 #include "VectorStructList.h"
@@ -165,6 +167,51 @@ int ref_umhs_estimate(int W, int H, int motionRange, int mode, const short* cur,
     if (dist_out) dist_out[i] = mbs[i]._distortion[0];
   }
   if (avg_out) *avg_out = avg;
+  return 0;
+}
+
+// One Estimate() of MotionEstimatorH264ImplCross with the zero predictor
+// (pred_mode 0) or H264MotionVectorPredictorImpl1 on a fresh macroblock
+// array (1).  dist_out: the distortion GetBestVector chose, per MB (read
+// back through a one-MB include map each, since the class reports only the
+// average).
+int ref_cross_estimate(int W, int H, int motionRange, int mode, int pred_mode, const short* cur, const short* ref,
+                       const unsigned char* include, short* mv_out, long* avg_out, int reps, double* seconds) {
+  const int mbw = W / 16, mbh = H / 16, nmb = mbw * mbh;
+  std::vector<short> c(cur, cur + (long)W * H), r(ref, ref + (long)W * H);
+  std::vector<MacroBlockH264> mbs;
+  std::vector<MacroBlockH264*> rows(mbh);
+  IMotionVectorPredictor* pred;
+  if (pred_mode) {
+    mbs.resize(nmb);
+    for (int i = 0; i < mbh; i++) rows[i] = &mbs[(size_t)i * mbw];
+    MacroBlockH264::Initialise(mbh, mbw, 0, nmb - 1, 0, rows.data());
+    pred = new H264MotionVectorPredictorImpl1(mbs.data());
+  } else {
+    pred = new ZeroPredictor();
+  }
+  MotionEstimatorH264ImplCross* me =
+      include ? new MotionEstimatorH264ImplCross(c.data(), r.data(), W, H, motionRange, pred, (void*)include)
+              : new MotionEstimatorH264ImplCross(c.data(), r.data(), W, H, motionRange, pred);
+  if (!me->Create()) {
+    delete me;
+    delete pred;
+    return -1;
+  }
+  me->SetMode(mode);
+  long avg = 0;
+  VectorStructList* l = nullptr;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < (reps < 1 ? 1 : reps); i++) l = (VectorStructList*)me->Estimate(&avg);
+  auto t1 = std::chrono::steady_clock::now();
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  for (int i = 0; mv_out && i < nmb; i++) {
+    mv_out[2 * i] = (short)l->GetSimpleElement(i, 0);
+    mv_out[2 * i + 1] = (short)l->GetSimpleElement(i, 1);
+  }
+  if (avg_out) *avg_out = avg;
+  delete me;
+  delete pred;
   return 0;
 }
 

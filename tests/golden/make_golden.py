@@ -376,6 +376,33 @@ def make_umhs(R_lib):
     print("umhs", len(UMHS_CASES), "cases")
 
 
+# (scene of umhs_scenes, motionRange in quarter pels, mode, predictor: 0
+# zero, 1 H.264 median)
+CROSS_CASES = [(0, 64, 0, 0), (0, 64, 0, 1), (0, 128, 1, 1), (0, 128, 2, 1), (1, 64, 0, 1), (1, 128, 1, 0),
+               (2, 256, 0, 1), (2, 256, 2, 0), (3, 256, 0, 1), (4, 32, 0, 1), (4, 32, 2, 0), (5, 256, 0, 1),
+               (6, 128, 0, 1), (6, 256, 1, 1), (7, 256, 0, 0), (8, 256, 2, 1)]
+
+
+def make_cross(R_lib):
+    """MotionEstimatorH264ImplCross from the compiled reference: vectors and
+    the average distortion over an include map, on the UMHS scenes (whose
+    planes umhs.npz holds)."""
+    out = {}
+    scenes = umhs_scenes()
+    for k, (si, qr, mode, pred) in enumerate(CROSS_CASES):
+        cur, ref = scenes[si]
+        H, W = cur.shape
+        n = (W // 16) * (H // 16)
+        inc = (np.random.default_rng(700 + k).random(n) < 0.7).astype(np.uint8)
+        mv, avg, _ = orc.ref_cross_estimate(R_lib, cur.astype(np.int16), ref.astype(np.int16), qr, mode, pred, inc)
+        out[f"c{k}_case"] = np.array([si, qr, mode, pred], np.int32)
+        out[f"c{k}_include"] = inc
+        out[f"c{k}_mv"], out[f"c{k}_avg"] = mv, np.int64(avg)
+    out["ncases"] = np.int32(len(CROSS_CASES))
+    np.savez_compressed(OUT / "cross.npz", **out)
+    print("cross", len(CROSS_CASES), "cases")
+
+
 def main():
     if not orc.build_ref():
         sys.exit("reference sources not available; fixtures are committed")
@@ -388,6 +415,7 @@ def main():
     make_io(R_lib)
     make_cavlc(R_lib)
     make_umhs(R_lib)
+    make_cross(R_lib)
 
 
 if __name__ == "__main__":
