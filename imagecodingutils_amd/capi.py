@@ -115,6 +115,11 @@ _SIGS = {
     "icu_cross_estimate": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_long)]),
     "icu_cross_estimate_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "icu_cross_check": (C.c_int, [_vp]),
+    "icu_fhs_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "icu_fhs_destroy": (C.c_int, [_vp]),
+    "icu_fhs_estimate": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_long)]),
+    "icu_fhs_estimate_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "icu_fhs_check": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "icu_last_error": (C.c_char_p, []),
     "icu_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_malloc": (_vp, [C.c_size_t]),
@@ -601,6 +606,45 @@ class CrossEstimator:
     def close(self):
         if getattr(self, "h", None):
             lib.icu_cross_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class FhsEstimator:
+    """MotionEstimatorH264ImplFHS on the device (icu_fhs_*); one handle per
+    stream, frames in order (the half-sample cache carries over)."""
+
+    def __init__(self, width, height, motion_range, mode=0):
+        h = _vp()
+        _check(lib.icu_fhs_create(width, height, motion_range, mode, C.byref(h)))
+        self.h, self.W, self.H = h, width, height
+        self.nmb = (width // 16) * (height // 16)
+
+    def estimate(self, cur, ref, include=None):
+        c = np.ascontiguousarray(cur, np.int16)
+        r = np.ascontiguousarray(ref, np.int16)
+        mv = np.zeros((self.nmb, 2), np.int16)
+        dist = np.zeros(self.nmb, np.int32)
+        avg = C.c_long(0)
+        inc = None if include is None else np.ascontiguousarray(include, np.uint8)
+        _check(lib.icu_fhs_estimate(self.h, c.ctypes.data, r.ctypes.data, None if inc is None else inc.ctypes.data,
+                                    mv.ctypes.data, dist.ctypes.data, C.byref(avg)))
+        return mv, dist, avg.value
+
+    def estimate_device(self, d_cur, d_ref, d_mv, d_dist, stream: Stream | None = None):
+        _check(lib.icu_fhs_estimate_device(self.h, _p(d_cur), _p(d_ref), _p(d_mv), _p(d_dist),
+                                           stream.handle if stream else None))
+
+    def check(self) -> int:
+        u = C.c_int(0)
+        _check(lib.icu_fhs_check(self.h, C.byref(u)))
+        return u.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.icu_fhs_destroy(self.h)
             self.h = None
 
     def __del__(self):

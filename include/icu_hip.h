@@ -277,6 +277,29 @@ int icu_cross_estimate_device(icu_cross* h, const uint8_t* d_cur, const uint8_t*
                               int32_t* d_dist, void* stream);
 int icu_cross_check(icu_cross* h);
 
+/* ----------------------------------------------------------- FHS estimator
+ * MotionEstimatorH264ImplFHS (include/CodecUtils/MotionEstimatorH264ImplFHS.h:78-214,
+ * src/CodecUtils/MotionEstimatorH264ImplFHS.cpp:497-1790): from the H.264
+ * median predictor with distortion, early exits, an uneven multi-hexagon,
+ * one extended hexagon and one diamond over 80-point path distortions,
+ * then HalfPelEstimate / QuarterPelEstimate (H.264 half and quarter
+ * samples) and a final test against the predictor.  motion_range and mode
+ * as icu_umhs_create.  The handle keeps the reference's half-sample cache
+ * from call to call, so a handle codes one stream's frames in order, like
+ * one reference instance; calls that overlap on one handle race. */
+typedef struct icu_fhs icu_fhs;
+int icu_fhs_create(int width, int height, int motion_range, int mode, icu_fhs** out);
+int icu_fhs_destroy(icu_fhs* h);
+int icu_fhs_estimate(icu_fhs* h, const int16_t* cur, const int16_t* ref, const uint8_t* include, int16_t* mv,
+                     int32_t* dist, long* avg);
+int icu_fhs_estimate_device(icu_fhs* h, const uint8_t* d_cur, const uint8_t* d_ref, int16_t* d_mv, int32_t* d_dist,
+                            void* stream);
+/* Waits for the device; ICU_E_STATE if a wavefront timed out (not
+ * expected).  *unwritten_reads (nullable): MBs whose quarter candidates read
+ * cache entries no call had written (the reference reads uninitialised
+ * memory there; here they read 0). */
+int icu_fhs_check(icu_fhs* h, int* unwritten_reads);
+
 /* --------------------------------------------------- raw YUV and MV files
  * YuvRawFileHandler (include/CodecUtils/YuvRawFileHandler.h:53-98,
  * src/CodecUtils/YuvRawFileHandler.cpp:78-184 over RawFileHandlerBase,

@@ -628,12 +628,14 @@ int orc_umhs_estimate(const int16_t *cur, const int16_t *refp, int W, int H, int
  * exceeds the current minimum and returns that extrapolation.
  * ---------------------------------------------------------------------- */
 
-/* The first 64 points of the sampling path (Cross.cpp:89-99), as y*16+x. */
-static const uint8_t CROSS_PATH[64] = {
+/* The first 80 points of the sampling path (Cross.cpp:89-99; the FHS
+ * estimator's unrolled path, FHS.cpp:1685-1790, is the same 80), as y*16+x. */
+static const uint8_t CROSS_PATH[80] = {
     165, 93,  18,  237, 225, 25,  97,  159, 216, 46,  71,  138, 244, 160, 48,  5,
     188, 116, 12,  234, 195, 90,  223, 52,  135, 126, 214, 59,  146, 169, 31,  39,
     83,  248, 193, 104, 16,  156, 254, 79,  203, 102, 183, 112, 190, 7,   242, 124,
-    50,  148, 44,  57,  220, 3,   229, 69,  208, 152, 178, 251, 26,  114, 111, 201};
+    50,  148, 44,  57,  220, 3,   229, 69,  208, 152, 178, 251, 26,  114, 111, 201,
+    37,  197, 65,  91,  141, 121, 61,  133, 186, 246, 173, 72,  163, 29,  129, 14};
 
 /* Squared error of each batch of 16 path points, block blk[256]. */
 static void cross_batches(const int16_t *cur, int W, int n, int m, const int *blk, int b[4]) {
@@ -773,6 +775,302 @@ int orc_cross_estimate(const int16_t *cur, const int16_t *refp, int W, int H, in
       if (dist_out) dist_out[vec] = ds[pos];
     }
   free(win);
+  if (avg_out) *avg_out = included ? total / included : 0;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * MotionEstimatorH264ImplFHS::Estimate (src/CodecUtils/MotionEstimatorH264ImplFHS.cpp:497-905)
+ * with HalfPelEstimate (:1109-1211), QuarterPelEstimate (:1213-1489), its
+ * QuarterRead (:1491-1683, the H.264 quarter samples, as OverlayMem2Dv2's)
+ * and Td16x16OptimalPathLessThan (:1685-1790): the path distortion over 80
+ * points, returned as the extrapolation sum * 256 / points at the first of
+ * the checkpoints 32, 40, ..., 72 where it exceeds the minimum, else at 80.
+ * The integer cost is (d + 269 floor(sqrt(dist^2))) >> 8 (:222).
+ *
+ * The half-pel stage fills a cache of b / h / j samples that persists in
+ * the estimator (and across Estimate calls) in five parts: h (rows -1..15,
+ * cols 0..15), then, unless the minimum fell below 1000, h's side columns
+ * and b (rows 0..15, cols -1..15), then, unless it fell below 1000 again,
+ * b's side rows and j.  The quarter stage reads the cache whatever was
+ * filled, so it can read samples an earlier MB left.  The cache is kept here
+ * as the reference keeps it; entries never written are 0 here (the
+ * reference's are uninitialised memory) and such reads are counted.
+ * ---------------------------------------------------------------------- */
+#define FHS_H 0
+#define FHS_B 1
+#define FHS_J 2
+
+typedef struct {
+  int c[3][18][18];
+  unsigned char w[3][18][18]; /* written at least once */
+} fhs_cache_t;
+
+static long fhs_stat[8];
+/* 0 uninitialised cache reads, 1 stale cache reads (written by an earlier
+ * MB), 2 predictor early exit, 3 zero-vector early exit, 4 hexagon stage
+ * run, 5 half-pel early exit after h, 6 after b, 7 quarter early exit. */
+void orc_fhs_stats(long *out) {
+  for (int i = 0; i < 8; i++) { out[i] = fhs_stat[i]; fhs_stat[i] = 0; }
+}
+
+static int fhs_path_lt(const int16_t *cur, int W, int n, int m, const int *blk, int min) {
+  int dp = 0;
+  for (int p = 0; p < 80; p++) {
+    const int q = CROSS_PATH[p], e = cur[(long)(m + (q >> 4)) * W + n + (q & 15)] - blk[q];
+    dp += e * e;
+    const int k = p + 1;
+    if (k >= 32 && k <= 72 && (k & 7) == 0) {
+      const int pd = (dp * 256) / k;
+      if (pd > min) return pd;
+    }
+  }
+  return (dp * 256) / 80;
+}
+
+static int fhs_cost(int d, int x, int y, int xr, int yr) {
+  const int mag = (int)sqrt((double)((x - xr) * (x - xr) + (y - yr) * (y - yr)));
+  return (d + 269 * mag) >> 8;
+}
+
+/* The sample a quarter candidate reads from the cache; marks stale /
+ * uninitialised reads.  fresh: this MB's fill generation of each part. */
+typedef struct {
+  fhs_cache_t *k;
+  unsigned char mine[3][18][18]; /* written by the current MB */
+} fhs_ctx_t;
+
+static int fhs_rd(fhs_ctx_t *x, int t, int r, int c) {
+  if (!x->k->w[t][r][c]) fhs_stat[0]++;
+  else if (!x->mine[t][r][c]) fhs_stat[1]++;
+  return x->k->c[t][r][c];
+}
+static void fhs_wr(fhs_ctx_t *x, int t, int r, int c, int v) {
+  x->k->c[t][r][c] = v;
+  x->k->w[t][r][c] = 1;
+  x->mine[t][r][c] = 1;
+}
+
+/* HalfPelEstimate around full pel (x0, y0) (the MB at (n, m)). */
+static int fhs_half(fhs_ctx_t *cx, const int16_t *cur, int W, const plane_t *ref, int n, int m, int x0, int y0,
+                    int min, int *ohx, int *ohy) {
+  int lmin = min, hx = 0, hy = 0;
+  int dm = 0, dp = 0;
+  for (int r = -1; r < 16; r++)
+    for (int c = 0; c < 16; c++) {
+      const int h = get_h(ref, x0 + c, y0 + r);
+      fhs_wr(cx, FHS_H, r + 1, c + 1, h);
+      if (dm <= lmin && r <= 14) { const int e = cur[(long)(m + r + 1) * W + n + c] - h; dm += e * e; }
+      if (dp <= lmin && r >= 0) { const int e = cur[(long)(m + r) * W + n + c] - h; dp += e * e; }
+    }
+  if (dm < lmin) { lmin = dm; hx = 0; hy = -1; }
+  if (dp < lmin) { lmin = dp; hx = 0; hy = 1; }
+  if (lmin < 1000) { fhs_stat[5]++; goto done; }
+  for (int r = -1; r < 16; r++) {
+    fhs_wr(cx, FHS_H, r + 1, 0, get_h(ref, x0 - 1, y0 + r));
+    fhs_wr(cx, FHS_H, r + 1, 17, get_h(ref, x0 + 16, y0 + r));
+  }
+  dm = dp = 0;
+  for (int r = 0; r < 16; r++)
+    for (int c = -1; c < 16; c++) {
+      const int b = get_b(ref, x0 + c, y0 + r);
+      fhs_wr(cx, FHS_B, r + 1, c + 1, b);
+      if (dm <= lmin && c <= 14) { const int e = cur[(long)(m + r) * W + n + c + 1] - b; dm += e * e; }
+      if (dp <= lmin && c >= 0) { const int e = cur[(long)(m + r) * W + n + c] - b; dp += e * e; }
+    }
+  if (dm < lmin) { lmin = dm; hx = -1; hy = 0; }
+  if (dp < lmin) { lmin = dp; hx = 1; hy = 0; }
+  if (lmin < 1000) { fhs_stat[6]++; goto done; }
+  for (int c = -1; c < 16; c++) {
+    fhs_wr(cx, FHS_B, 0, c + 1, get_b(ref, x0 + c, y0 - 1));
+    fhs_wr(cx, FHS_B, 17, c + 1, get_b(ref, x0 + c, y0 + 16));
+  }
+  {
+    int d[4] = {0, 0, 0, 0};
+    for (int r = -1; r < 16; r++)
+      for (int c = -1; c < 16; c++) {
+        const int j = get_j(ref, x0 + c, y0 + r);
+        fhs_wr(cx, FHS_J, r + 1, c + 1, j);
+        if (d[0] <= lmin && r <= 14 && c <= 14) { const int e = cur[(long)(m + r + 1) * W + n + c + 1] - j; d[0] += e * e; }
+        if (d[1] <= lmin && r <= 14 && c >= 0) { const int e = cur[(long)(m + r + 1) * W + n + c] - j; d[1] += e * e; }
+        if (d[2] <= lmin && r >= 0 && c <= 14) { const int e = cur[(long)(m + r) * W + n + c + 1] - j; d[2] += e * e; }
+        if (d[3] <= lmin && r >= 0 && c >= 0) { const int e = cur[(long)(m + r) * W + n + c] - j; d[3] += e * e; }
+      }
+    if (d[0] < lmin) { lmin = d[0]; hx = -1; hy = -1; }
+    if (d[1] < lmin) { lmin = d[1]; hx = 1; hy = -1; }
+    if (d[2] < lmin) { lmin = d[2]; hx = -1; hy = 1; }
+    if (d[3] < lmin) { lmin = d[3]; hx = 1; hy = 1; }
+  }
+done:
+  *ohx = hx;
+  *ohy = hy;
+  return lmin;
+}
+
+/* One quarter candidate's sample at block pel (r, c) (QuarterPelEstimate's
+ * switch, :1258-1349): qx, qy in -3..3 around full pel (x0, y0). */
+static int fhs_qsample(fhs_ctx_t *cx, const plane_t *ref, int x0, int y0, int qx, int qy, int r, int c) {
+  const int mx = qx < 0 ? -1 : 0, my = qy < 0 ? -1 : 0;
+  const int fx = qx - 4 * mx, fy = qy - 4 * my; /* 0..3 */
+  const int cy = my + 1 + r, cc = mx + 1 + c;     /* cache index of the sample's own b/h/j */
+  const int G = pix(ref, x0 + mx + c, y0 + my + r);
+  switch (fy * 4 + fx) {
+    case 1: return (G + fhs_rd(cx, FHS_B, cy, cc) + 1) >> 1;                                   /* a */
+    case 3: return (pix(ref, x0 + mx + c + 1, y0 + my + r) + fhs_rd(cx, FHS_B, cy, cc) + 1) >> 1; /* c */
+    case 4: return (G + fhs_rd(cx, FHS_H, cy, cc) + 1) >> 1;                                   /* d */
+    case 12: return (pix(ref, x0 + mx + c, y0 + my + r + 1) + fhs_rd(cx, FHS_H, cy, cc) + 1) >> 1; /* n */
+    case 5: return (fhs_rd(cx, FHS_B, cy, cc) + fhs_rd(cx, FHS_H, cy, cc) + 1) >> 1;           /* e */
+    case 7: return (fhs_rd(cx, FHS_B, cy, cc) + fhs_rd(cx, FHS_H, cy, cc + 1) + 1) >> 1;       /* g */
+    case 6: return (fhs_rd(cx, FHS_B, cy, cc) + fhs_rd(cx, FHS_J, cy, cc) + 1) >> 1;           /* f */
+    case 14: return (fhs_rd(cx, FHS_B, cy + 1, cc) + fhs_rd(cx, FHS_J, cy, cc) + 1) >> 1;      /* q */
+    case 9: return (fhs_rd(cx, FHS_H, cy, cc) + fhs_rd(cx, FHS_J, cy, cc) + 1) >> 1;           /* i */
+    case 11: return (fhs_rd(cx, FHS_H, cy, cc + 1) + fhs_rd(cx, FHS_J, cy, cc) + 1) >> 1;      /* k */
+    case 13: return (fhs_rd(cx, FHS_H, cy, cc) + fhs_rd(cx, FHS_B, cy + 1, cc) + 1) >> 1;      /* p */
+    case 15: return (fhs_rd(cx, FHS_H, cy, cc + 1) + fhs_rd(cx, FHS_B, cy + 1, cc) + 1) >> 1;  /* r */
+  }
+  abort(); /* whole and half positions are not quarter candidates */
+}
+
+static int fhs_quarter(fhs_ctx_t *cx, const int16_t *cur, int W, const plane_t *ref, int n, int m, int x0, int y0,
+                       int min, int *oqx, int *oqy) {
+  int lmin = min, hx = 0, hy = 0, qx = 0, qy = 0;
+  const int half = fhs_half(cx, cur, W, ref, n, m, x0, y0, min, &hx, &hy);
+  if (half < min) {
+    lmin = half;
+    if (half < 1000) goto done;
+  }
+  for (int k = 0; k < 8; k++) {
+    const int ox = 2 * hx + SUBPOS[k][0], oy = 2 * hy + SUBPOS[k][1];
+    int q = 0;
+    for (int r = 0; r < 16; r++)
+      for (int c = 0; c < 16; c++) {
+        const int e = cur[(long)(m + r) * W + n + c] - fhs_qsample(cx, ref, x0, y0, ox, oy, r, c);
+        q += e * e;
+      }
+    if (q < lmin) {
+      lmin = q; qx = ox; qy = oy;
+      if (q < 1000) { fhs_stat[7]++; break; }
+    }
+  }
+done:
+  *oqx = qx;
+  *oqy = qy;
+  return lmin;
+}
+
+/* cache: 3*18*18 ints and 3*18*18 written flags carried from call to call
+ * (both zero before the first; the reference allocates the cache in
+ * Create()).  dist_out: the distortion the predictor keeps per MB. */
+int orc_fhs_estimate(const int16_t *cur, const int16_t *refp, int W, int H, int qrange, int mode,
+                     const uint8_t *include, int32_t *cache, uint8_t *cache_written, int16_t *mv_out,
+                     int32_t *dist_out, long *avg_out) {
+  if (W <= 0 || H <= 0 || (W & 15) || (H & 15) || qrange < 4) return -1;
+  const int mbw = W / 16, mbh = H / 16, mRng = qrange / 4;
+  plane_t ref = {refp, W, H};
+  fhs_ctx_t *cx = (fhs_ctx_t *)calloc(1, sizeof(fhs_ctx_t));
+  fhs_cache_t *kc = (fhs_cache_t *)calloc(1, sizeof(fhs_cache_t));
+  int32_t *pdist = (int32_t *)calloc((size_t)mbw * mbh, sizeof(int32_t));
+  if (!cx || !kc || !pdist) { free(cx); free(kc); free(pdist); return -2; }
+  memcpy(kc->c, cache, sizeof(kc->c));
+  memcpy(kc->w, cache_written, sizeof(kc->w));
+  cx->k = kc;
+  int blk[256];
+  long total = 0;
+  int included = 0;
+  for (int row = 0; row < mbh; row++)
+    for (int col = 0; col < mbw; col++) {
+      const int n = col * 16, m = row * 16, vec = row * mbw + col;
+      memset(cx->mine, 0, sizeof(cx->mine));
+      int predX, predY, predD;
+      orc_umhs_pred(mv_out, pdist, mbw, col, row, &predX, &predY, &predD);
+      int pXQ = predX % 4, pYQ = predY % 4;
+      int predX0 = predX / 4, predY0 = predY / 4;
+      const int oX0 = predX0, oY0 = predY0;
+      if (predX0 + n > W) predX0 = 16;
+      if (predX0 + n < -16) predX0 = -16;
+      if (predY0 + m > H) predY0 = 16;
+      if (predY0 + m < -16) predY0 = -16;
+      if (oX0 != predX0 || oY0 != predY0) { pXQ = 0; pYQ = 0; }
+      const int rX = predX0 * 4 + pXQ, rY = predY0 * 4 + pYQ;
+      int xl, xr, yu, yd;
+      motion_range(n, m, W, H, mRng, 0, &xl, &xr, &yu, &yd);
+      if (pXQ || pYQ) orc_quarter_read(refp, W, H, n + predX0, m + predY0, pXQ, pYQ, blk);
+      else read_full(&ref, n + predX0, m + predY0, blk);
+      const int predVecD = blk_dist(cur, W, n, m, blk, 0);
+      int minD = predVecD, mx = predX0, my = predY0, minC = predVecD / 256;
+      int mvx, mvy;
+      if (predVecD < 1000) { mvx = rX; mvy = rY; fhs_stat[2]++; goto all_done; }
+      if (rX || rY) {
+        read_full(&ref, n, m, blk);
+        const int zd = fhs_path_lt(cur, W, n, m, blk, minD);
+        if (zd < minD) {
+          const int zc = fhs_cost(zd, 0, 0, predX0, predY0);
+          if (zc < minC) { minD = zd; minC = zc; mx = 0; my = 0; }
+          if (zd < 1000) { mvx = 0; mvy = 0; fhs_stat[3]++; goto all_done; }
+        }
+      }
+      {
+        int stage; /* 0 hexagon, 1 extended hexagon, 2 diamond */
+        if (minD < 1000) stage = 2;
+        else if ((minD < 4000) || ((minD == predVecD) && (minD < (predD * 12 / 10)) && (minD > (predD * 8 / 10))))
+          stage = 1;
+        else stage = 0;
+        int rmx = 0, rmy = 0;
+#define FHS_TRY(J, I)                                                                                   \
+  do {                                                                                                  \
+    const int j_ = (J), i_ = (I);                                                                       \
+    if ((i_ + my) >= yu && (i_ + my) <= yd && (j_ + mx) >= xl && (j_ + mx) <= xr) {                    \
+      read_full(&ref, n + mx + j_, m + my + i_, blk);                                                   \
+      const int d_ = fhs_path_lt(cur, W, n, m, blk, minD);                                              \
+      if (d_ <= minD) {                                                                                 \
+        const int c_ = fhs_cost(d_, mx + j_, my + i_, predX0, predY0);                                  \
+        if (c_ < minC) { minD = d_; minC = c_; rmx = j_; rmy = i_; }                                    \
+      }                                                                                                 \
+    }                                                                                                   \
+  } while (0)
+        if (stage == 0) {
+          fhs_stat[4]++;
+          rmx = rmy = 0;
+          for (int w = 1; w <= 4; w++)
+            for (int x = 0; x < 16; x++) FHS_TRY(w * U_HEX16[x][0], w * U_HEX16[x][1]);
+          mx += rmx; my += rmy;
+        }
+        if (stage <= 1) {
+          rmx = rmy = 0;
+          for (int x = 0; x < 6; x++) FHS_TRY(U_EXTHEX[x][0], U_EXTHEX[x][1]);
+          mx += rmx; my += rmy;
+        }
+        rmx = rmy = 0;
+        for (int x = 0; x < 4; x++) FHS_TRY(U_CROSS[x][0], U_CROSS[x][1]);
+        mx += rmx; my += rmy;
+#undef FHS_TRY
+      }
+      mvx = mx * 4;
+      mvy = my * 4;
+      if (mode == 0) {
+        int qx = 0, qy = 0;
+        const int nm = fhs_quarter(cx, cur, W, &ref, n, m, n + mx, m + my, minD, &qx, &qy);
+        if (nm < minD) { minD = nm; mvx += qx; mvy += qy; }
+      } else if (mode == 1) {
+        int hx = 0, hy = 0;
+        const int nm = fhs_half(cx, cur, W, &ref, n, m, n + mx, m + my, minD, &hx, &hy);
+        if (nm < minD) { minD = nm; mvx += 2 * hx; mvy += 2 * hy; }
+      }
+      if (fhs_cost(predVecD, rX, rY, predX0, predY0) <= fhs_cost(minD, mvx, mvy, predX0, predY0)) {
+        minD = predVecD; mvx = rX; mvy = rY;
+      }
+    all_done:
+      if (include && include[vec]) { included++; total += minD; }
+      mv_out[2 * vec] = (int16_t)mvx;
+      mv_out[2 * vec + 1] = (int16_t)mvy;
+      pdist[vec] = minD;
+      if (dist_out) dist_out[vec] = minD;
+    }
+  memcpy(cache, kc->c, sizeof(kc->c));
+  memcpy(cache_written, kc->w, sizeof(kc->w));
+  free(cx);
+  free(kc);
+  free(pdist);
   if (avg_out) *avg_out = included ? total / included : 0;
   return 0;
 }

@@ -48,6 +48,13 @@ void orc_umhs_stats(long *out13);
 /* MotionEstimatorH264ImplCross (me_oracle.c): pred 0 zero predictor, 1 the
  * H.264 median; distortions are the estimator's 64-point path sums. */
 void orc_cross_stats(long *out6);
+/* MotionEstimatorH264ImplFHS (me_oracle.c): median predictor with
+ * distortion; cache / cache_written (3*18*18 each) carry the estimator's
+ * half-sample cache from call to call (zeros before the first). */
+int orc_fhs_estimate(const int16_t *cur, const int16_t *ref, int W, int H, int qrange, int mode,
+                     const uint8_t *include, int32_t *cache, uint8_t *cache_written, int16_t *mv_out,
+                     int32_t *dist_out, long *avg_out);
+void orc_fhs_stats(long *out8);
 int orc_cross_estimate(const int16_t *cur, const int16_t *ref, int W, int H, int qrange, int mode, int pred,
                        const uint8_t *include, int16_t *mv_out, int32_t *dist_out, long *avg_out);
 void orc_umhs_pred(const int16_t *mv, const int32_t *dist, int mbw, int col, int row, int *px, int *py,

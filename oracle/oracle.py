@@ -75,6 +75,9 @@ def lib():
         L.orc_vbs_pred.argtypes = [_u8p, C.c_int, C.c_int, _i16p, _u8p, _i16p]
         L.orc_umhs_stats.argtypes = [C.c_void_p]
         L.orc_cross_stats.argtypes = [C.c_void_p]
+        L.orc_fhs_stats.argtypes = [C.c_void_p]
+        L.orc_fhs_estimate.argtypes = [_i16p, _i16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, _i32p, _u8p,
+                                       _i16p, C.c_void_p, C.POINTER(C.c_long)]
         L.orc_cross_estimate.argtypes = [_i16p, _i16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                          _i16p, C.c_void_p, C.POINTER(C.c_long)]
         L.orc_umhs_estimate.argtypes = [_i16p, _i16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
@@ -106,6 +109,8 @@ def ref_lib():
         L.ref_cavlc.argtypes = [_i16p, C.c_int, C.c_int, _i32p, C.c_int, _u8p, C.c_int, _i32p, _i16p, _i32p]
         L.ref_cross_estimate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i16p, _i16p, C.c_void_p,
                                          C.c_void_p, C.POINTER(C.c_long), C.c_int, C.POINTER(C.c_double)]
+        L.ref_fhs_chain.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i16p, _i16p, C.c_void_p,
+                                    _i16p, _i32p, C.POINTER(C.c_long), C.POINTER(C.c_double)]
         L.ref_umhs_estimate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i16p, _i16p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_long), C.c_int,
                                         C.POINTER(C.c_double)]
@@ -255,6 +260,66 @@ def cross_estimate(cur, ref, qrange, mode=0, pred=0, include=None):
     if rc != 0:
         raise ValueError(f"oracle cross_estimate failed ({rc})")
     return mv.reshape(n, 2), dist, avg.value
+
+
+class FhsCache:
+    """The FHS estimator's half-sample cache, carried from call to call."""
+
+    def __init__(self):
+        self.c = np.zeros(3 * 18 * 18, np.int32)
+        self.w = np.zeros(3 * 18 * 18, np.uint8)
+
+
+def fhs_estimate(cur, ref, qrange, mode=0, include=None, cache: FhsCache | None = None):
+    """MotionEstimatorH264ImplFHS through the oracle (one call; pass the same
+    cache for a chain of frames).  Returns (mv, dist, avg)."""
+    cur = np.ascontiguousarray(cur, dtype=np.int16)
+    ref = np.ascontiguousarray(ref, dtype=np.int16)
+    H, W = cur.shape
+    n = (W // 16) * (H // 16)
+    cache = cache or FhsCache()
+    mv = np.zeros(2 * n, np.int16)
+    dist = np.zeros(n, np.int32)
+    avg = C.c_long(0)
+    inc = None
+    if include is not None:
+        include = np.ascontiguousarray(include, np.uint8)
+        inc = include.ctypes.data
+    rc = lib().orc_fhs_estimate(cur, ref, W, H, qrange, mode, C.c_void_p(inc), cache.c, cache.w, mv,
+                                dist.ctypes.data, C.byref(avg))
+    if rc != 0:
+        raise ValueError(f"oracle fhs_estimate failed ({rc})")
+    return mv.reshape(n, 2), dist, avg.value
+
+
+def fhs_stats():
+    """Branch counts of the FHS oracle since the previous call (8 counters,
+    named in me_oracle.c; 0 = reads of never-written cache entries)."""
+    st = np.zeros(8, np.int64)
+    lib().orc_fhs_stats(st.ctypes.data)
+    return st
+
+
+def ref_fhs_chain(R_lib, curs, refs, qrange, mode=0, include=None):
+    """Frames through one compiled reference instance.  Returns (mv (F, nMB,
+    2), dist (F, nMB), avg (F,), seconds)."""
+    curs = np.ascontiguousarray(np.stack(curs), dtype=np.int16)
+    refs = np.ascontiguousarray(np.stack(refs), dtype=np.int16)
+    F, H, W = curs.shape
+    n = (W // 16) * (H // 16)
+    mv = np.zeros(F * 2 * n, np.int16)
+    dist = np.zeros(F * n, np.int32)
+    avg = (C.c_long * F)()
+    sec = C.c_double(0)
+    inc = None
+    if include is not None:
+        include = np.ascontiguousarray(include, np.uint8)
+        inc = include.ctypes.data
+    rc = R_lib.ref_fhs_chain(W, H, qrange, mode, F, curs.reshape(-1), refs.reshape(-1), inc, mv, dist, avg,
+                             C.byref(sec))
+    if rc != 0:
+        raise ValueError(f"reference fhs failed ({rc})")
+    return mv.reshape(F, n, 2), dist.reshape(F, n), np.array(avg[:], np.int64), sec.value
 
 
 def cross_stats():

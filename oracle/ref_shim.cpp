@@ -15,6 +15,7 @@
 //   CAVLCH264Impl + its VLC coders       src/CodecUtils/CAVLCH264Impl.cpp:134-577
 //   MotionEstimatorH264ImplUMHS          src/CodecUtils/MotionEstimatorH264ImplUMHS.cpp:231-940
 //   MotionEstimatorH264ImplCross         src/CodecUtils/MotionEstimatorH264ImplCross.cpp:137-660
+//   MotionEstimatorH264ImplFHS           src/CodecUtils/MotionEstimatorH264ImplFHS.cpp:231-905
 #include <math.h>  // the estimator header calls sqrt without including it
 
 #include <chrono>
@@ -33,6 +34,7 @@
 #include "MotionEstimatorH264ImplFull.h"
 #include "MotionEstimatorH264ImplUMHS.h"
 #include "MotionEstimatorH264ImplCross.h"
+#include "MotionEstimatorH264ImplFHS.h"
 #include "OverlayExtMem2Dv2.h"
 #include "OverlayMem2Dv2.h"
 #include "VectorStructList.h"
@@ -212,6 +214,44 @@ int ref_cross_estimate(int W, int H, int motionRange, int mode, int pred_mode, c
   if (avg_out) *avg_out = avg;
   delete me;
   delete pred;
+  return 0;
+}
+
+// nframes Estimate() calls of one MotionEstimatorH264ImplFHS instance (its
+// half-sample cache carries from call to call, as in a codec), frame f
+// coding curs[f] against refs[f] (copied into the planes the class was
+// built on).  Per frame: vectors, the distortion the predictor keeps per MB
+// and the average over include.
+int ref_fhs_chain(int W, int H, int motionRange, int mode, int nframes, const short* curs, const short* refs,
+                  const unsigned char* include, short* mv_out, int* dist_out, long* avg_out, double* seconds) {
+  const int mbw = W / 16, mbh = H / 16, nmb = mbw * mbh;
+  const long np = (long)W * H;
+  std::vector<short> c(np), r(np);
+  std::vector<MacroBlockH264> mbs(nmb);
+  std::vector<MacroBlockH264*> rows(mbh);
+  for (int i = 0; i < mbh; i++) rows[i] = &mbs[(size_t)i * mbw];
+  MacroBlockH264::Initialise(mbh, mbw, 0, nmb - 1, 0, rows.data());
+  H264MotionVectorPredictorImpl1 pred(mbs.data());
+  MotionEstimatorH264ImplFHS me(c.data(), r.data(), W, H, motionRange, &pred, (void*)include);
+  if (!me.Create()) return -1;
+  me.SetMode(mode);
+  double t = 0;
+  for (int f = 0; f < nframes; f++) {
+    std::memcpy(c.data(), curs + f * np, sizeof(short) * np);
+    std::memcpy(r.data(), refs + f * np, sizeof(short) * np);
+    long avg = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    VectorStructList* l = (VectorStructList*)me.Estimate(&avg);
+    auto t1 = std::chrono::steady_clock::now();
+    t += std::chrono::duration<double>(t1 - t0).count();
+    for (int i = 0; i < nmb; i++) {
+      mv_out[(long)f * 2 * nmb + 2 * i] = (short)l->GetSimpleElement(i, 0);
+      mv_out[(long)f * 2 * nmb + 2 * i + 1] = (short)l->GetSimpleElement(i, 1);
+      dist_out[(long)f * nmb + i] = mbs[i]._distortion[0];
+    }
+    avg_out[f] = avg;
+  }
+  if (seconds) *seconds = t;
   return 0;
 }
 

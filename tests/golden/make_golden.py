@@ -403,6 +403,67 @@ def make_cross(R_lib):
     print("cross", len(CROSS_CASES), "cases")
 
 
+def fhs_extra_scenes():
+    """Vertical stripes and, as the current picture, their horizontal half
+    sample b (scene 9, 11) or quarter sample a = (G + b + 1) >> 1 (10, 12):
+    a sub-pel shift matches almost exactly while whole pels do not, which
+    reaches the half- and quarter-pel stages' early exits."""
+    out = []
+    for (W, H, period, kind) in [(64, 48, 5, "b"), (64, 48, 5, "a"), (80, 48, 7, "b"), (80, 48, 6, "a")]:
+        rng = np.random.default_rng(1)
+        x = np.arange(W)
+        row = 128 + 100 * np.sin(2 * np.pi * x / period + rng.random() * 6)
+        ref = np.clip(np.round(np.tile(row, (H, 1))), 0, 255).astype(np.int32)
+        p = np.pad(ref, ((0, 0), (2, 3)), mode="edge")
+        t = p[:, 0:W] - 5 * p[:, 1:W + 1] + 20 * p[:, 2:W + 2] + 20 * p[:, 3:W + 3] - 5 * p[:, 4:W + 4] + p[:, 5:W + 5]
+        b = np.clip((t + 16) >> 5, 0, 255)
+        cur = b if kind == "b" else (ref + b + 1) >> 1
+        out.append((cur.astype(np.uint8), ref.astype(np.uint8)))
+    return out
+
+
+def fhs_chain(cur, ref):
+    """The three frames of a case: the scene, the scene shifted 3 pels right
+    against the same reference, then shifted 2 pels up against a reference
+    shifted 1 pel right."""
+    return ([cur, np.roll(cur, 3, axis=1), np.roll(cur, -2, axis=0)], [ref, ref, np.roll(ref, 1, axis=1)])
+
+
+# (scene: 0..8 of umhs_scenes, 9..12 of fhs_extra_scenes; motionRange; mode)
+FHS_CASES = [(0, 64, 0), (0, 64, 1), (0, 64, 2), (1, 64, 0), (2, 128, 0), (3, 256, 0), (4, 32, 0), (5, 64, 0),
+             (5, 128, 0), (6, 128, 0), (7, 256, 1), (8, 256, 0), (8, 128, 0), (9, 32, 0), (10, 32, 0), (11, 64, 1),
+             (12, 64, 0)]
+
+
+def make_fhs(R_lib):
+    """MotionEstimatorH264ImplFHS from the compiled reference: each case is
+    three frames through one instance (its half-sample cache carries over),
+    with vectors, per-MB distortions and averages per frame.  No case reads a
+    cache entry the reference never wrote (its value is undefined there)."""
+    out = {}
+    scenes = umhs_scenes() + fhs_extra_scenes()
+    for i, (cur, ref) in enumerate(scenes[9:], start=9):
+        out[f"s{i}_cur"], out[f"s{i}_ref"] = cur, ref
+    orc.fhs_stats()
+    for k, (si, qr, mode) in enumerate(FHS_CASES):
+        cur, ref = scenes[si]
+        curs, refs = fhs_chain(cur, ref)
+        H, W = cur.shape
+        n = (W // 16) * (H // 16)
+        inc = (np.random.default_rng(900 + k).random(n) < 0.7).astype(np.uint8)
+        mv, dist, avg, _ = orc.ref_fhs_chain(R_lib, curs, refs, qr, mode, inc)
+        cache = orc.FhsCache()
+        for f in range(3):
+            orc.fhs_estimate(curs[f], refs[f], qr, mode, inc, cache)
+        assert orc.fhs_stats()[0] == 0, f"case {k} reads the reference's uninitialised cache"
+        out[f"c{k}_case"] = np.array([si, qr, mode], np.int32)
+        out[f"c{k}_include"] = inc
+        out[f"c{k}_mv"], out[f"c{k}_dist"], out[f"c{k}_avg"] = mv, dist, avg
+    out["ncases"] = np.int32(len(FHS_CASES))
+    np.savez_compressed(OUT / "fhs.npz", **out)
+    print("fhs", len(FHS_CASES), "cases")
+
+
 def main():
     if not orc.build_ref():
         sys.exit("reference sources not available; fixtures are committed")
@@ -416,6 +477,7 @@ def main():
     make_cavlc(R_lib)
     make_umhs(R_lib)
     make_cross(R_lib)
+    make_fhs(R_lib)
 
 
 if __name__ == "__main__":
