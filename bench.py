@@ -309,6 +309,63 @@ def run_cavlc(capi, synth, steps, warmup):
             "stream_bits": int(bits), "nonzero_blocks": int(np.count_nonzero(tc)), "round_trip_exact": ok}
 
 
+FAST_RANGE = 128  # +-32 in the reference's quarter-pel motionRange
+
+
+def run_fast(capi, synth, steps, warmup):
+    """The fast estimators (UMHS, Cross with the median and the zero
+    predictor, FHS) on 1080p frames of a moving-rectangle stream, quarter-pel
+    mode, inputs resident: device time per frame.  UMHS also with 16 streams
+    of 1080p in flight on 4 HIP streams (each stream its own handle), for the
+    throughput the wavefront leaves room for."""
+    src = synth.StreamSource(W, H, 4040, n_rects=24, max_mv=16, sigma=2.0)
+    f0, f1 = src.frame(0), src.frame(1)
+    n_mb = (W // 16) * (H // 16)
+    d_cur, d_ref = capi.DeviceBuffer.from_array(f1), capi.DeviceBuffer.from_array(f0)
+    d_mv, d_dist = capi.DeviceBuffer(4 * n_mb), capi.DeviceBuffer(4 * n_mb)
+    s = capi.Stream()
+    out = {"workload": "1080p 1920x1088, +-32 (motionRange 128), quarter pel, moving rectangles"}
+    kinds = [("umhs", lambda: capi.UmhsEstimator(W, H, FAST_RANGE, 0)),
+             ("cross_median", lambda: capi.CrossEstimator(W, H, FAST_RANGE, 0, 1)),
+             ("cross_zero", lambda: capi.CrossEstimator(W, H, FAST_RANGE, 0, 0)),
+             ("fhs", lambda: capi.FhsEstimator(W, H, FAST_RANGE, 0))]
+    for name, make in kinds:
+        est = make()
+        for _ in range(warmup):
+            est.estimate_device(d_cur, d_ref, d_mv, d_dist, stream=s)
+        s.sync()
+        a, b = capi.Event(), capi.Event()
+        a.record(s)
+        for _ in range(steps):
+            est.estimate_device(d_cur, d_ref, d_mv, d_dist, stream=s)
+        b.record(s)
+        s.sync()
+        ms = a.elapsed_ms(b) / steps
+        out[name] = {"ms_per_frame": round(ms, 4), "mbs_per_s": round(n_mb / (ms / 1e3), 1)}
+        est.close()
+    # 16 streams in flight
+    nstream, lanes = 16, 4
+    ests = [capi.UmhsEstimator(W, H, FAST_RANGE, 0) for _ in range(nstream)]
+    mvs = [capi.DeviceBuffer(4 * n_mb) for _ in range(nstream)]
+    dss = [capi.DeviceBuffer(4 * n_mb) for _ in range(nstream)]
+    sts = [capi.Stream() for _ in range(lanes)]
+    for e, mv, ds, k in zip(ests, mvs, dss, range(nstream)):
+        e.estimate_device(d_cur, d_ref, mv, ds, stream=sts[k % lanes])
+    for st in sts:
+        st.sync()
+    reps = max(2, steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for e, mv, ds, k in zip(ests, mvs, dss, range(nstream)):
+            e.estimate_device(d_cur, d_ref, mv, ds, stream=sts[k % lanes])
+    for st in sts:
+        st.sync()
+    wall = time.perf_counter() - t0
+    out["umhs_16_streams"] = {"frames": nstream * reps, "mbs_per_s": round(nstream * reps * n_mb / wall, 1),
+                              "ms_per_frame_aggregate": round(wall * 1e3 / (nstream * reps), 4)}
+    return out
+
+
 SERVE_STREAMS, SERVE_RATE, SERVE_QP, SERVE_BUDGET = 32, 2000.0, 28, 64 << 30
 SERVE_RING = 3  # distinct frames generated per stream, submitted in turn
 
@@ -425,6 +482,17 @@ def cpu_baseline(synth, budget_s=12.0):
     per_frame = t_me + t_it
     out = {"value": nmb / per_frame, "unit": "MB/s", "cores": 1, "kind": kind, "sample": sample,
            "s_per_frame": per_frame}
+    # the fast estimators' reference classes on the same 1080p stream as
+    # the fast-estimator leg, one Estimate each
+    if ref_lib is not None:
+        src = synth.StreamSource(W, H, 4040, n_rects=24, max_mv=16, sigma=2.0)
+        c1, c0 = src.frame(1), src.frame(0)
+        _, _, _, t_u = orc.ref_umhs_estimate(ref_lib, c1, c0, 128, 0)
+        _, _, t_c = orc.ref_cross_estimate(ref_lib, c1, c0, 128, 0, 1)
+        _, _, _, t_f = orc.ref_fhs_chain(ref_lib, [c1], [c0], 128, 0)
+        out["fast_estimators_1080p"] = {"kind": "reference", "cores": 1, "unit": "ms per frame",
+                                        "umhs": round(t_u * 1e3, 2), "cross_median": round(t_c * 1e3, 2),
+                                        "fhs": round(t_f * 1e3, 2)}
     model, nproc, avail = _cpu_info()
     out.update(cpu_model=model, nproc=nproc, cpus_available=avail)
     # the same mode as the GPU (SAD, exhaustive +-32, filter off, zero
@@ -551,6 +619,7 @@ def main():
                           "achieved_T_sad_ops": round(bt["sad_ops"] / (bt["ms"] / 1e3) / 1e12, 2)}
     out["mc_yuv_1080p"] = run_mc_yuv(capi, synth, 20, 3)
     out["cavlc_1080p"] = run_cavlc(capi, synth, 20, 3)
+    out["fast_estimators_1080p"] = run_fast(capi, synth, 10, 2)
     if not args.no_4k:
         out["multiref_4k"] = run_multiref_4k(capi, synth, max(5, args.steps // 5), 2)
     if args.serve_seconds > 0:

@@ -17,6 +17,8 @@
 //   YuvRawFileHandler   include/CodecUtils/YuvRawFileHandler.h:53-98
 //   IMotionCompensator / MotionCompensatorH264ImplStd
 //       include/CodecUtils/IMotionCompensator.h, MotionCompensatorH264ImplStd.h:55-110
+//   MotionEstimatorH264ImplUMHS / ImplCross / ImplFHS
+//       include/CodecUtils/MotionEstimatorH264ImplUMHS.h:78-126, ...Cross.h:78-190, ...FHS.h:78-214
 //   IContextAwareRunLevelCodec / CAVLCH264Impl, BitStreamWriterMSB / BitStreamReaderMSB
 //       include/CodecUtils/IContextAwareRunLevelCodec.h:49-103, CAVLCH264Impl.h:59-150,
 //       BitStreamBaseMSB.h:62-87
@@ -66,6 +68,8 @@ class IMotionVectorPredictor {
   virtual int Create(void) = 0;
   virtual int Get16x16Prediction(void* pList, int blk, int* predX, int* predY) = 0;
   virtual void Set16x16MotionVector(int blk, int mvX, int mvY) {}
+  virtual void Set16x16MotionVector(int blk, int mvX, int mvY, int distortion) { Set16x16MotionVector(blk, mvX, mvY); }
+  virtual int Get16x16Prediction(void* pList, int blk, int* predX, int* predY, int* distortion) { return 0; }
   // 1 when the predictor is the H.264 16x16 median of A/B/C(D), which the
   // device evaluates itself as a wavefront; 0 for the zero vector.
   virtual int IsH264Median(void) const = 0;
@@ -77,12 +81,41 @@ class IMotionVectorPredictor {
 // Get16x16Prediction answers for later users.
 class H264MotionVectorPredictorImpl1 : public IMotionVectorPredictor {
  public:
-  H264MotionVectorPredictorImpl1(int mbCols, int mbRows) : _w(mbCols), _h(mbRows), _mv(2 * (size_t)mbCols * mbRows, 0) {}
+  H264MotionVectorPredictorImpl1(int mbCols, int mbRows)
+      : _w(mbCols), _h(mbRows), _mv(2 * (size_t)mbCols * mbRows, 0), _d((size_t)mbCols * mbRows, 0) {}
   int Create(void) override { return 1; }
   int IsH264Median(void) const override { return 1; }
   void Set16x16MotionVector(int blk, int mvX, int mvY) override {
     _mv[2 * (size_t)blk] = mvX;
     _mv[2 * (size_t)blk + 1] = mvY;
+  }
+  void Set16x16MotionVector(int blk, int mvX, int mvY, int distortion) override {
+    Set16x16MotionVector(blk, mvX, mvY);
+    _d[blk] = distortion;
+  }
+  // The median with its distortion (MacroBlockH264.cpp:818-915): each
+  // component takes the least distortion among the neighbours it equals;
+  // the y median is compared with C's x component, as the reference does.
+  int Get16x16Prediction(void*, int blk, int* predX, int* predY, int* distortion) override {
+    const int col = blk % _w, row = blk / _w;
+    int A[3] = {0, 0, 0}, B[3] = {0, 0, 0}, Cc[3] = {0, 0, 0};
+    const bool hasB = row > 0, hasC = row > 0 && col + 1 < _w;
+    if (col > 0) getd(blk - 1, A);
+    if (hasB) getd(blk - _w, B);
+    if (hasC) getd(blk - _w + 1, Cc);
+    else if (col > 0 && row > 0) getd(blk - _w - 1, Cc);
+    if (!hasB && !hasC)
+      for (int i = 0; i < 3; i++) B[i] = Cc[i] = A[i];
+    const int x = median(A[0], B[0], Cc[0]), y = median(A[1], B[1], Cc[1]);
+    int xd = A[2], yd = A[2];
+    if (x == B[0] && B[2] < xd) xd = B[2];
+    if (x == Cc[0] && Cc[2] < xd) xd = Cc[2];
+    if (y == B[1] && B[2] < yd) yd = B[2];
+    if (y == Cc[0] && Cc[2] < yd) yd = Cc[2];
+    *predX = x;
+    *predY = y;
+    *distortion = yd < xd ? yd : xd;
+    return 1;
   }
   int Get16x16Prediction(void*, int blk, int* predX, int* predY) override {
     const int col = blk % _w, row = blk / _w;
@@ -106,6 +139,10 @@ class H264MotionVectorPredictorImpl1 : public IMotionVectorPredictor {
     v[0] = _mv[2 * (size_t)blk];
     v[1] = _mv[2 * (size_t)blk + 1];
   }
+  void getd(int blk, int* v) const {
+    get(blk, v);
+    v[2] = _d[blk];
+  }
   static int median(int x, int y, int z) {
     int mn = y < z ? y : z, mx = y < z ? z : y;
     if (x < mn) mn = x;
@@ -113,7 +150,7 @@ class H264MotionVectorPredictorImpl1 : public IMotionVectorPredictor {
     return x + y + z - mn - mx;
   }
   int _w, _h;
-  std::vector<int> _mv;
+  std::vector<int> _mv, _d;
 };
 
 // Always predicts the zero vector.
@@ -216,6 +253,147 @@ class MotionEstimatorH264ImplFull : public IMotionEstimator {
   icu_me* _h_me = nullptr;
   VectorStructList _list;
   int _ready = 0, _mode = 1;
+};
+
+// ---------------------------------------------------------- fast estimators
+// MotionEstimatorH264ImplUMHS / ImplCross / ImplFHS
+// (include/CodecUtils/MotionEstimatorH264ImplUMHS.h:78-126, ...Cross.h:78-190,
+// ...FHS.h:78-214): the reference's constructors, Create(), SetMode (0
+// quarter, 1 half, 2 full pel; default quarter) and Estimate().  UMHS and
+// FHS take their starting point from the H.264 median predictor with
+// distortion, which the device evaluates itself (pass an
+// H264MotionVectorPredictorImpl1; the decided vectors and distortions are
+// written back to it, as the reference does); Cross also takes
+// ZeroMotionVectorPredictor.  UMHS's previous-frame macroblocks are given
+// as their 16x16 vectors and intra flags instead of MacroBlockH264s.  A
+// mode change takes effect at the next Estimate() with a new device handle
+// (for FHS, a fresh half-sample cache).
+class FastEstimatorBase : public IMotionEstimator {
+ public:
+  FastEstimatorBase(const void* pSrc, const void* pRef, int w, int h, int motionRange, IMotionVectorPredictor* pred,
+                    void* include)
+      : _src((const int16_t*)pSrc), _ref((const int16_t*)pRef), _w(w), _h(h), _range(motionRange), _pred(pred),
+        _include((const bool*)include) {}
+  int Create(void) override {
+    Release();
+    _list.SetLength((_w / 16) * (_h / 16));
+    _dist.assign((size_t)(_w / 16) * (_h / 16), 0);
+    _ready = (_w % 16 == 0 && _h % 16 == 0 && _w > 0 && _h > 0) ? 1 : 0;
+    return _ready;
+  }
+  void Reset(void) override {}
+  int Ready(void) override { return _ready; }
+  void SetMode(int mode) override { _mode = mode; }
+  int GetMode(void) override { return _mode; }
+  void* Estimate(const void*, const void*, long* avgDistortion) override { return Estimate(avgDistortion); }
+  void* Estimate(long* avgDistortion) override {
+    if (!_ready) return nullptr;
+    if (_made_mode != _mode) {
+      Release();
+      if (Make(_mode) != ICU_OK) return nullptr;
+      _made_mode = _mode;
+    }
+    const int n = _list.GetLength();
+    std::vector<uint8_t> inc;
+    if (_include) {
+      inc.resize(n);
+      for (int i = 0; i < n; i++) inc[i] = _include[i] ? 1 : 0;
+    }
+    long avg = 0;
+    if (Run(_include ? inc.data() : nullptr, &avg) != ICU_OK) return nullptr;
+    if (_pred)
+      for (int i = 0; i < n; i++)
+        _pred->Set16x16MotionVector(i, _list.GetSimpleElement(i, 0), _list.GetSimpleElement(i, 1), _dist[i]);
+    if (avgDistortion) *avgDistortion = avg;
+    return (void*)&_list;
+  }
+  const char* LastError(void) const { return icu_last_error(); }
+
+ protected:
+  virtual int Make(int mode) = 0;
+  virtual int Run(const uint8_t* include, long* avg) = 0;
+  virtual void Release(void) = 0;
+  const int16_t* _src;
+  const int16_t* _ref;
+  int _w, _h, _range;
+  IMotionVectorPredictor* _pred;
+  const bool* _include;
+  VectorStructList _list;
+  std::vector<int32_t> _dist;
+  int _ready = 0, _mode = 0, _made_mode = -1;
+};
+
+class MotionEstimatorH264ImplUMHS : public FastEstimatorBase {
+ public:
+  MotionEstimatorH264ImplUMHS(const void* pSrc, const void* pRef, int imgWidth, int imgHeight, int motionRange,
+                              IMotionVectorPredictor* pMVPred, void* pDistortionIncluded = nullptr,
+                              const int16_t* pPrevMv = nullptr, const uint8_t* pPrevIntra = nullptr)
+      : FastEstimatorBase(pSrc, pRef, imgWidth, imgHeight, motionRange, pMVPred, pDistortionIncluded),
+        _prev(pPrevMv), _prev_intra(pPrevIntra) {}
+  ~MotionEstimatorH264ImplUMHS() override { Release(); }
+
+ protected:
+  int Make(int mode) override { return icu_umhs_create(_w, _h, _range, mode, &_hd); }
+  int Run(const uint8_t* include, long* avg) override {
+    return icu_umhs_estimate(_hd, _src, _ref, _prev, _prev ? _prev_intra : nullptr, include, _list.Data(),
+                             _dist.data(), avg);
+  }
+  void Release(void) override {
+    if (_hd) icu_umhs_destroy(_hd);
+    _hd = nullptr;
+    _made_mode = -1;
+  }
+
+ private:
+  const int16_t* _prev;
+  const uint8_t* _prev_intra;
+  icu_umhs* _hd = nullptr;
+};
+
+class MotionEstimatorH264ImplCross : public FastEstimatorBase {
+ public:
+  MotionEstimatorH264ImplCross(const void* pSrc, const void* pRef, int imgWidth, int imgHeight, int motionRange,
+                               IMotionVectorPredictor* pMVPred, void* pDistortionIncluded = nullptr)
+      : FastEstimatorBase(pSrc, pRef, imgWidth, imgHeight, motionRange, pMVPred, pDistortionIncluded) {}
+  ~MotionEstimatorH264ImplCross() override { Release(); }
+
+ protected:
+  int Make(int mode) override {
+    return icu_cross_create(_w, _h, _range, mode, (_pred && _pred->IsH264Median()) ? 1 : 0, &_hd);
+  }
+  int Run(const uint8_t* include, long* avg) override {
+    return icu_cross_estimate(_hd, _src, _ref, include, _list.Data(), _dist.data(), avg);
+  }
+  void Release(void) override {
+    if (_hd) icu_cross_destroy(_hd);
+    _hd = nullptr;
+    _made_mode = -1;
+  }
+
+ private:
+  icu_cross* _hd = nullptr;
+};
+
+class MotionEstimatorH264ImplFHS : public FastEstimatorBase {
+ public:
+  MotionEstimatorH264ImplFHS(const void* pSrc, const void* pRef, int imgWidth, int imgHeight, int motionRange,
+                             IMotionVectorPredictor* pMVPred, void* pDistortionIncluded = nullptr)
+      : FastEstimatorBase(pSrc, pRef, imgWidth, imgHeight, motionRange, pMVPred, pDistortionIncluded) {}
+  ~MotionEstimatorH264ImplFHS() override { Release(); }
+
+ protected:
+  int Make(int mode) override { return icu_fhs_create(_w, _h, _range, mode, &_hd); }
+  int Run(const uint8_t* include, long* avg) override {
+    return icu_fhs_estimate(_hd, _src, _ref, include, _list.Data(), _dist.data(), avg);
+  }
+  void Release(void) override {
+    if (_hd) icu_fhs_destroy(_hd);
+    _hd = nullptr;
+    _made_mode = -1;
+  }
+
+ private:
+  icu_fhs* _hd = nullptr;
 };
 
 // --------------------------------------------------------------- transforms

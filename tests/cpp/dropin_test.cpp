@@ -173,6 +173,72 @@ int main(int argc, char** argv) {
     save(argv[8], out.data(), out.size());
     return 0;
   }
-  std::fprintf(stderr, "usage: dropin_test me|fwd|inv|mc|cavlc ...\n");
+  if (cmd == "fast" && argc >= 15) {
+    // fast umhs|cross|fhs W H range mode pred nframes curs.raw refs.raw
+    //      include.raw prev_mv.raw|- prev_intra.raw|- out.raw:
+    // frames through one estimator object built on two planes (each frame
+    // copied in before Estimate, as a codec refills its buffers); out.raw
+    // holds per frame the vectors, then every frame's average distortion.
+    const std::string kind = argv[2];
+    const int W = std::atoi(argv[3]), H = std::atoi(argv[4]), range = std::atoi(argv[5]);
+    const int mode = std::atoi(argv[6]), pred_median = std::atoi(argv[7]), nf = std::atoi(argv[8]);
+    const size_t np = (size_t)W * H;
+    const int nmb = (W / 16) * (H / 16);
+    auto curs = load<short>(argv[9], np * nf);
+    auto refs = load<short>(argv[10], np * nf);
+    auto inc8 = load<unsigned char>(argv[11], nmb);
+    std::vector<short> pmv;
+    std::vector<unsigned char> pin;
+    if (std::string(argv[12]) != "-") {
+      pmv = load<short>(argv[12], 2 * (size_t)nmb);
+      pin = load<unsigned char>(argv[13], nmb);
+    }
+    // the include map as the reference's bool array (one byte, 0 or 1, each)
+    std::vector<unsigned char> incb(nmb);
+    for (int i = 0; i < nmb; i++) incb[i] = inc8[i] != 0;
+    void* inc = incb.data();
+    std::vector<short> cur(np), ref(np);
+    H264MotionVectorPredictorImpl1 med(W / 16, H / 16);
+    ZeroMotionVectorPredictor zero;
+    IMotionVectorPredictor* pred = pred_median ? (IMotionVectorPredictor*)&med : (IMotionVectorPredictor*)&zero;
+    IMotionEstimator* me;
+    if (kind == "umhs")
+      me = new MotionEstimatorH264ImplUMHS(cur.data(), ref.data(), W, H, range, pred, inc,
+                                           pmv.empty() ? nullptr : pmv.data(), pin.empty() ? nullptr : pin.data());
+    else if (kind == "cross")
+      me = new MotionEstimatorH264ImplCross(cur.data(), ref.data(), W, H, range, pred, inc);
+    else
+      me = new MotionEstimatorH264ImplFHS(cur.data(), ref.data(), W, H, range, pred, inc);
+    if (!me->Create()) return 4;
+    me->SetMode(mode);
+    std::vector<short> out;
+    std::vector<long> avgs;
+    for (int f = 0; f < nf; f++) {
+      std::copy(curs.begin() + f * np, curs.begin() + (f + 1) * np, cur.begin());
+      std::copy(refs.begin() + f * np, refs.begin() + (f + 1) * np, ref.begin());
+      long avg = 0;
+      VectorStructList* l = (VectorStructList*)me->Estimate(&avg);
+      if (!l) {
+        std::fprintf(stderr, "Estimate failed: %s\n", icu_last_error());
+        return 5;
+      }
+      for (int i = 0; i < nmb; i++) {
+        out.push_back((short)l->GetSimpleElement(i, 0));
+        out.push_back((short)l->GetSimpleElement(i, 1));
+      }
+      avgs.push_back(avg);
+    }
+    delete me;
+    FILE* fo = std::fopen(argv[14], "wb");
+    if (!fo) return 3;
+    std::fwrite(out.data(), sizeof(short), out.size(), fo);
+    for (long a : avgs) {
+      const long long v = a;
+      std::fwrite(&v, sizeof(v), 1, fo);
+    }
+    std::fclose(fo);
+    return 0;
+  }
+  std::fprintf(stderr, "usage: dropin_test me|fwd|inv|mc|cavlc|fast ...\n");
   return 1;
 }

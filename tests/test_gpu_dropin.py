@@ -72,3 +72,63 @@ def test_dropin_compensator_matches_reference(tmp_path):
     out = np.fromfile(tmp_path / "yuv.raw", np.int16)
     want = np.concatenate([fx[k].astype(np.int16).ravel() for k in ("out_y", "out_u", "out_v")])
     np.testing.assert_array_equal(out, want)
+
+
+def _fast_scenes():
+    d = {}
+    for name in ("umhs.npz", "fhs.npz"):
+        f = np.load(GOLDEN / name)
+        d.update({k: f[k] for k in f.files if k.startswith("s")})
+    return d
+
+
+def _run_fast(tmp_path, kind, curs, refs, qr, mode, pred, inc, pm=None, pi=None):
+    H, W = curs[0].shape
+    np.stack(curs).astype(np.int16).tofile(tmp_path / "c.raw")
+    np.stack(refs).astype(np.int16).tofile(tmp_path / "r.raw")
+    np.asarray(inc, np.uint8).tofile(tmp_path / "i.raw")
+    pmf = pif = "-"
+    if pm is not None:
+        np.asarray(pm, np.int16).tofile(tmp_path / "pm.raw")
+        np.asarray(pi, np.uint8).tofile(tmp_path / "pi.raw")
+        pmf, pif = tmp_path / "pm.raw", tmp_path / "pi.raw"
+    _run("fast", kind, W, H, qr, mode, pred, len(curs), tmp_path / "c.raw", tmp_path / "r.raw", tmp_path / "i.raw",
+         pmf, pif, tmp_path / "o.raw")
+    raw = (tmp_path / "o.raw").read_bytes()
+    n = (W // 16) * (H // 16)
+    mv = np.frombuffer(raw[:4 * n * len(curs)], np.int16).reshape(len(curs), n, 2)
+    avg = np.frombuffer(raw[4 * n * len(curs):], np.int64)
+    return mv, avg
+
+
+def test_dropin_fast_estimators_match_reference(tmp_path):
+    """MotionEstimatorH264ImplUMHS / Cross / FHS from the C++ header, driven
+    like the reference classes, on the reference's fixtures."""
+    sc = _fast_scenes()
+    fu = np.load(GOLDEN / "umhs.npz")
+    for k in (1, 3, 8):
+        si, qr, mode, prev = (int(v) for v in fu[f"c{k}_case"])
+        cur, ref = sc[f"s{si}_cur"], sc[f"s{si}_ref"]
+        if prev >= 0:
+            cur = np.roll(cur, 3, axis=1)
+        pm = fu[f"c{k}_prev_mv"] if prev >= 0 else None
+        pi = fu[f"c{k}_prev_intra"] if prev >= 0 else None
+        mv, avg = _run_fast(tmp_path, "umhs", [cur], [ref], qr, mode, 1, fu[f"c{k}_include"], pm, pi)
+        np.testing.assert_array_equal(mv[0], fu[f"c{k}_mv"])
+        assert avg[0] == int(fu[f"c{k}_avg"])
+    fc = np.load(GOLDEN / "cross.npz")
+    for k in (1, 7, 12):
+        si, qr, mode, pred = (int(v) for v in fc[f"c{k}_case"])
+        mv, avg = _run_fast(tmp_path, "cross", [sc[f"s{si}_cur"]], [sc[f"s{si}_ref"]], qr, mode, pred,
+                            fc[f"c{k}_include"])
+        np.testing.assert_array_equal(mv[0], fc[f"c{k}_mv"])
+        assert avg[0] == int(fc[f"c{k}_avg"])
+    ff = np.load(GOLDEN / "fhs.npz")
+    for k in (0, 5, 13):
+        si, qr, mode = (int(v) for v in ff[f"c{k}_case"])
+        cur, ref = sc[f"s{si}_cur"], sc[f"s{si}_ref"]
+        curs = [cur, np.roll(cur, 3, axis=1), np.roll(cur, -2, axis=0)]
+        refs = [ref, ref, np.roll(ref, 1, axis=1)]
+        mv, avg = _run_fast(tmp_path, "fhs", curs, refs, qr, mode, 1, ff[f"c{k}_include"])
+        np.testing.assert_array_equal(mv, ff[f"c{k}_mv"])
+        np.testing.assert_array_equal(avg, ff[f"c{k}_avg"])
