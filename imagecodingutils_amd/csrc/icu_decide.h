@@ -68,18 +68,43 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-__device__ __forceinline__ int wave_sum(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+// Wave sums by DPP, within the VALU: four row shifts (1, 2, 4, 8; lanes
+// shifted in from outside the 16-lane row read 0) leave each lane the sum
+// of its row up to itself, so lane 16 r + 15 holds row r's sum; row_bcast:15
+// then adds row 0's into row 1 and row 2's into row 3, and row_bcast:31 adds
+// lane 31 into rows 2 and 3, which leaves the total in lane 63.  Every
+// caller runs these with all 64 lanes active.
+__device__ __forceinline__ int row_scan(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);  // row_shr:8
   return v;
 }
-// Eight wave sums at once: the shuffles of one step are independent, so
-// their latencies overlap.
+__device__ __forceinline__ int rows_to_total(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+  return __builtin_amdgcn_readlane(v, 63);
+}
+// The sum over the wave, the same in every lane.
+__device__ __forceinline__ int wave_sum(int v) { return rows_to_total(row_scan(v)); }
+// Eight wave sums at once: the DPP steps of one value are independent of
+// the others', so they interleave.
 __device__ __forceinline__ void wave_sum8(int (&a)[8]) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1)
+  for (int k = 0; k < 8; k++) a[k] += __builtin_amdgcn_update_dpp(0, a[k], 0x111, 0xf, 0xf, true);
 #pragma unroll
-    for (int k = 0; k < 8; k++) a[k] += __shfl_xor(a[k], o, 64);
+  for (int k = 0; k < 8; k++) a[k] += __builtin_amdgcn_update_dpp(0, a[k], 0x112, 0xf, 0xf, true);
+#pragma unroll
+  for (int k = 0; k < 8; k++) a[k] += __builtin_amdgcn_update_dpp(0, a[k], 0x114, 0xf, 0xf, true);
+#pragma unroll
+  for (int k = 0; k < 8; k++) a[k] += __builtin_amdgcn_update_dpp(0, a[k], 0x118, 0xf, 0xf, true);
+#pragma unroll
+  for (int k = 0; k < 8; k++) a[k] += __builtin_amdgcn_update_dpp(0, a[k], 0x142, 0xa, 0xf, false);
+#pragma unroll
+  for (int k = 0; k < 8; k++) a[k] += __builtin_amdgcn_update_dpp(0, a[k], 0x143, 0xc, 0xf, false);
+#pragma unroll
+  for (int k = 0; k < 8; k++) a[k] = __builtin_amdgcn_readlane(a[k], 63);
 }
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
 #pragma unroll
@@ -307,8 +332,9 @@ __device__ __forceinline__ void half_pel_direct(const DecideDesc& d, int x0, int
 // (Full.cpp:452-517; UMHS.cpp:795-862 is the same stage).  cw: this lane's
 // dword of the input block (row lane / 4, columns 4 (lane % 4)), also
 // staged in s.cur4.  Updates minD; returns the vector in quarter pels.
+// patch_ready: s.P already holds the 22 x 22 patch around (n + mx, m + my).
 __device__ inline void refine_subpel(const DecideDesc& d, WaveLds& s, int n, int m, int mx, int my, uint32_t cw,
-                                     int lane, int& minD, int& mvx, int& mvy) {
+                                     int lane, int& minD, int& mvx, int& mvy, bool patch_ready = false) {
   const uint8_t* cur8 = (const uint8_t*)s.cur4;
   mvx = mx * 4;
   mvy = my * 4;
@@ -328,7 +354,7 @@ __device__ inline void refine_subpel(const DecideDesc& d, WaveLds& s, int n, int
     mvx += 2 * hxh;
     mvy += 2 * hyh;
   } else if (d.subpel >= 2) {
-    load_patch(d, n + mx, m + my, s.P, lane);
+    if (!patch_ready) load_patch(d, n + mx, m + my, s.P, lane);
     wave_sync();
     build_half_grid(s, lane);
     // half-pel candidates, all eight summed at once, then taken in the
@@ -514,6 +540,23 @@ __device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, in
     }
   }
   return out;
+}
+
+// Hand-off between the waves of a wavefront kernel.  Everything one wave
+// reads of another's results in the same launch (vectors, distortions, the
+// FHS fill records, the progress counters) goes through relaxed agent-scope
+// atomics, which the hardware keeps coherent across the XCDs by themselves.
+// So publishing needs only the producer's own stores drained (s_waitcnt)
+// before its progress store, and the consumer only a relaxed poll.  An
+// acquire / release at agent scope would add an L2 invalidate to every poll
+// and an L2 write-back to every MB, for data that no plain load reads.
+__device__ __forceinline__ int poll_progress(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void publish_progress(int* p, int v) {
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have landed
+  asm volatile("" ::: "memory");
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ int median3(int x, int y, int z) {

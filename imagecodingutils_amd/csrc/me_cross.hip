@@ -24,6 +24,7 @@
 #include "../../include/icu_hip.h"
 #include "icu_decide.h"
 #include "icu_device.h"
+#include "icu_fastwin.h"
 #include "icu_internal.h"
 
 using namespace icu;
@@ -67,16 +68,16 @@ struct Batches {
 };
 
 // Reduce K candidates' per-lane squared errors to their batch sums.
+// (each batch is one 16-lane row: a DPP row scan leaves its sum in the
+// row's last lane)
 template <int K>
 __device__ __forceinline__ void batch_sums(int (&e)[K], Batches (&out)[K]) {
 #pragma unroll
-  for (int o = 1; o < 16; o <<= 1)
-#pragma unroll
-    for (int k = 0; k < K; k++) e[k] += __shfl_xor(e[k], o, 64);
+  for (int k = 0; k < K; k++) e[k] = row_scan(e[k]);
 #pragma unroll
   for (int k = 0; k < K; k++)
 #pragma unroll
-    for (int g = 0; g < 4; g++) out[k].b[g] = __shfl(e[k], 16 * g, 64);
+    for (int g = 0; g < 4; g++) out[k].b[g] = __builtin_amdgcn_readlane(e[k], 16 * g + 15);
 }
 
 __device__ __forceinline__ int sq(int a, int b) { return (a - b) * (a - b); }
@@ -86,14 +87,29 @@ __device__ __forceinline__ double cross_cost(int dist, int mag) {
   return __dadd_rn(__ddiv_rn((double)dist, 64.0), __dmul_rn(0.9, sqrt_int_rn(mag)));
 }
 
-__device__ void cross_mb(const CrossDesc& c, WaveLds& s, int mb, int predX, int predY, int lane) {
+// The MB's search window, staged by stage_window before cross_mb.
+__device__ __forceinline__ fw::Win stage_window(const CrossDesc& c, uint8_t* win, int mb, int lane) {
+  const DecideDesc& d = c.d;
+  const int n = (mb % d.mbw) * 16, m = (mb / d.mbw) * 16;
+  const fw::Win w = fw::shape(n, m, d.W, d.H, c.mrng - 1);
+  fw::issue(d, w, win, lane);
+  return w;
+}
+
+__device__ void cross_mb(const CrossDesc& c, WaveLds& s, const uint8_t* win, const fw::Win& w, int mb, int predX,
+                         int predY, int lane) {
   const DecideDesc& d = c.d;
   const int col = mb % d.mbw, row = mb / d.mbw;
   const int n = col * 16, m = row * 16;
   const int pp = kPath[lane], py = pp >> 4, px = pp & 15;
   const int cv = d.cur[(long)(m + py) * d.cur_stride + n + px];
+  fw::wait();
   const uint8_t* rp = d.ref + (long)(m + py) * d.ref_stride + n + px;  // this lane's pel at the zero vector
-  auto at = [&](int x, int y) { return (int)rp[(long)y * d.ref_stride + x]; };
+  // this lane's pel of the block at vector (x, y): from the window when staged
+  auto at = [&](int x, int y) {
+    return w.has(n + x, m + y, 16, 16, 0) ? fw::byte_at(win, w, n + x + px, m + y + py)
+                                          : (int)rp[(long)y * d.ref_stride + x];
+  };
 
   const int predXQ = predX % 4, predYQ = predY % 4;
   int predX0 = predX / 4, predY0 = predY / 4;
@@ -111,12 +127,12 @@ __device__ void cross_mb(const CrossDesc& c, WaveLds& s, int mb, int predX, int 
   int xl, xr, yu, yd;
   motion_window(n, m, d.W, d.H, c.mrng - 1, &xl, &xr, &yu, &yd);
   constexpr int kCx[4] = {-1, 1, 0, 0}, kCy[4] = {0, 0, -1, 1};
-  for (int w = c.mrng / 2; w > 0; w /= 2) {
+  for (int arm = c.mrng / 2; arm > 0; arm /= 2) {
     bool ok[4];
     int e[4];
 #pragma unroll
     for (int x = 0; x < 4; x++) {
-      const int i = w * kCy[x], j = w * kCx[x];
+      const int i = arm * kCy[x], j = arm * kCx[x];
       ok[x] = !((i + my) < yu || (i + my) > yd || (j + mx) < xl || (j + mx) > xr);
       e[x] = ok[x] ? sq(cv, at(mx + j, my + i)) : 0;
     }
@@ -126,7 +142,7 @@ __device__ void cross_mb(const CrossDesc& c, WaveLds& s, int mb, int predX, int 
 #pragma unroll
     for (int x = 0; x < 4; x++) {
       if (!ok[x]) continue;
-      const int i = w * kCy[x], j = w * kCx[x];
+      const int i = arm * kCy[x], j = arm * kCx[x];
       const int v = b[x].lt(minD);
       if (v <= minD) {
         if (v == minD) {
@@ -147,7 +163,7 @@ __device__ void cross_mb(const CrossDesc& c, WaveLds& s, int mb, int predX, int 
     // the half-pel lattice around (mx, my), then the 8 half candidates and
     // (quarter mode) the 8 quarter averages around the winner, sampled at
     // this lane's path point
-    load_patch(d, n + mx, m + my, s.P, lane);
+    fw::patch(d, win, w, w.has(n + mx, m + my, 16, 16, 3), n + mx, m + my, s.P, lane);
     wave_sync();
     build_half_grid(s, lane);
     const int16_t* h = s.Hg + (2 + 2 * py) * kHalf + 2 + 2 * px;
@@ -206,7 +222,8 @@ __device__ void cross_mb(const CrossDesc& c, WaveLds& s, int mb, int predX, int 
       int fox = 0, fx = predXQ, foy = 0, fy = predYQ;
       if (fx < 0) { fox = -1; fx += 4; }
       if (fy < 0) { foy = -1; fy += 4; }
-      load_patch(d, n + predX0 + fox, m + predY0 + foy, s.P, lane);
+      const int qx = n + predX0 + fox, qy = m + predY0 + foy;
+      fw::patch(d, win, w, w.has(qx, qy, 16, 16, 3), qx, qy, s.P, lane);
       wave_sync();
       e[0] = sq(cv, qpel_p(s.P, 3 + px, 3 + py, fx, fy));
     } else {
@@ -241,9 +258,13 @@ __device__ void cross_mb(const CrossDesc& c, WaveLds& s, int mb, int predX, int 
 // Zero predictor: MBs are independent.
 __global__ __launch_bounds__(64) void cross_parallel_kernel(CrossDesc c) {
   __shared__ WaveLds s;
+  __shared__ __align__(16) uint8_t s_win[16];
   const int nmb = c.d.mbw * c.d.mbh;
   for (int mb = blockIdx.x; mb < nmb; mb += gridDim.x) {
-    cross_mb(c, s, mb, 0, 0, threadIdx.x);
+    // no wait to hide the staging behind here: every read goes to the
+    // (L2-resident) padded plane
+    const fw::Win w{0, 0, 0, 0};
+    cross_mb(c, s, s_win, w, mb, 0, 0, threadIdx.x);
     __syncthreads();
   }
 }
@@ -252,6 +273,7 @@ __global__ __launch_bounds__(64) void cross_parallel_kernel(CrossDesc c) {
 // each waiting for the above-right MB of the row above.
 __global__ __launch_bounds__(64) void cross_wavefront_kernel(CrossDesc c) {
   __shared__ WaveLds s;
+  __shared__ __align__(16) uint8_t s_win[fw::kBytes];
   __shared__ int row_s;
   const DecideDesc& d = c.d;
   const int lane = threadIdx.x;
@@ -261,11 +283,13 @@ __global__ __launch_bounds__(64) void cross_wavefront_kernel(CrossDesc c) {
   if (row >= d.mbh) return;
   const uint32_t* mvw = (const uint32_t*)c.mv_out;
   for (int col = 0; col < d.mbw; col++) {
+    // the MB's window into LDS while it waits for its neighbours
+    const fw::Win w = stage_window(c, s_win, row * d.mbw + col, lane);
     if (row > 0) {
       const int need = min(col + 2, d.mbw);
       if (lane == 0) {
         int spins = 0;
-        while (__hip_atomic_load(&c.progress[row - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        while (poll_progress(&c.progress[row - 1]) < need) {
           if (++spins > kSpin) {
             atomicExch(&c.flags[1], 1);
             break;
@@ -291,11 +315,10 @@ __global__ __launch_bounds__(64) void cross_wavefront_kernel(CrossDesc c) {
       Bx = Cx = Ax;
       By = Cy = Ay;
     }
-    cross_mb(c, s, mb, median3(Ax, Bx, Cx), median3(Ay, By, Cy), lane);
+    cross_mb(c, s, s_win, w, mb, median3(Ax, Bx, Cx), median3(Ay, By, Cy), lane);
     __syncthreads();
     if (lane == 0) {
-      __threadfence();
-      __hip_atomic_store(&c.progress[row], col + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      publish_progress(&c.progress[row], col + 1);
     }
   }
 }

@@ -64,7 +64,7 @@ __global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* 
       const int need = min(col + 2, d.mbw);
       if (lane == 0) {
         int spins = 0;
-        while (__hip_atomic_load(&progress[row - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        while (poll_progress(&progress[row - 1]) < need) {
           if (++spins > kSpinLimit) {
             atomicExch(&ticket[1], 1);
             break;
@@ -94,8 +94,7 @@ __global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* 
     decide_mb(d, s, mb, px, py, lane);
     __syncthreads();
     if (lane == 0) {
-      __threadfence();
-      __hip_atomic_store(&progress[row], col + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      publish_progress(&progress[row], col + 1);
     }
   }
 }

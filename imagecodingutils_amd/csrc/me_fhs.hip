@@ -31,6 +31,7 @@
 #include "../../include/icu_hip.h"
 #include "icu_decide.h"
 #include "icu_device.h"
+#include "icu_fastwin.h"
 #include "icu_internal.h"
 
 using namespace icu;
@@ -97,21 +98,28 @@ struct Path80 {
   }
 };
 
+// (DPP row shifts by 1, 2 and 4 leave lane 8 g + 7 the sum of lanes
+// 8 g .. 8 g + 7, every group lying within one 16-lane row)
+__device__ __forceinline__ int scan8(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);  // row_shr:4
+  return v;
+}
+
 template <int K>
 __device__ __forceinline__ void path_sums(int (&e1)[K], int (&e2)[K], Path80 (&out)[K]) {
 #pragma unroll
-  for (int o = 1; o < 8; o <<= 1)
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      e1[k] += __shfl_xor(e1[k], o, 64);
-      e2[k] += __shfl_xor(e2[k], o, 64);
-    }
+  for (int k = 0; k < K; k++) {
+    e1[k] = scan8(e1[k]);
+    e2[k] = scan8(e2[k]);
+  }
 #pragma unroll
   for (int k = 0; k < K; k++) {
 #pragma unroll
-    for (int g = 0; g < 8; g++) out[k].g[g] = __shfl(e1[k], 8 * g, 64);
-    out[k].g[8] = __shfl(e2[k], 0, 64);
-    out[k].g[9] = __shfl(e2[k], 8, 64);
+    for (int g = 0; g < 8; g++) out[k].g[g] = __builtin_amdgcn_readlane(e1[k], 8 * g + 7);
+    out[k].g[8] = __builtin_amdgcn_readlane(e2[k], 7);
+    out[k].g[9] = __builtin_amdgcn_readlane(e2[k], 15);
   }
 }
 
@@ -124,6 +132,9 @@ struct Ctx {
   int cv1, cv2;              // input pels at this lane's path points
   const uint8_t* rp1;        // reference pel of path point lane at the zero vector
   const uint8_t* rp2;        // and of point 64 + lane
+  int p1x, p1y, p2x, p2y;    // the two path points in the block
+  const uint8_t* win;        // the MB's staged search window
+  fw::Win w;
   int xl, xr, yu, yd, pX0, pY0;
   int minD, minC, rmx, rmy;
 };
@@ -138,9 +149,10 @@ __device__ __forceinline__ void try8(Ctx& c, int mx, int my, const int (&ox)[8],
   for (int k = 0; k < 8; k++) {
     const int x = mx + ox[k], y = my + oy[k];
     ok[k] = k < cnt && y >= c.yu && y <= c.yd && x >= c.xl && x <= c.xr;
-    const long off = (long)y * c.d.ref_stride + x;
-    e1[k] = ok[k] ? sq(c.cv1, c.rp1[off]) : 0;
-    e2[k] = (ok[k] && c.lane < 16) ? sq(c.cv2, c.rp2[off]) : 0;
+    // window candidates are always staged
+    const int bx = c.n + x, by = c.m + y;
+    e1[k] = ok[k] ? sq(c.cv1, fw::byte_at(c.win, c.w, bx + c.p1x, by + c.p1y)) : 0;
+    e2[k] = (ok[k] && c.lane < 16) ? sq(c.cv2, fw::byte_at(c.win, c.w, bx + c.p2x, by + c.p2y)) : 0;
   }
   Path80 p[8];
   path_sums(e1, e2, p);
@@ -215,9 +227,11 @@ __device__ void carry_to_cache(const FhsDesc& f, FhsLds& s, int parts, int lane)
   }
 }
 
-__device__ __forceinline__ void build_lattice(const DecideDesc& d, FhsLds& s, int org, int lane) {
+__device__ __forceinline__ void build_lattice(const DecideDesc& d, FhsLds& s, int org, int lane,
+                                              const uint8_t* win = nullptr, const fw::Win* w = nullptr) {
   const int x0 = (int)(int16_t)(org & 0xFFFF), y0 = org >> 16;
-  load_patch(d, x0, y0, s.w.P, lane);
+  if (win) fw::patch(d, win, *w, w->has(x0, y0, 16, 16, 3), x0, y0, s.w.P, lane);
+  else load_patch(d, x0, y0, s.w.P, lane);
   wave_sync();
   build_half_grid(s.w, lane);
 }
@@ -226,6 +240,7 @@ __device__ __forceinline__ int pack_org(int x, int y) { return (int)(((uint32_t)
 
 __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
   __shared__ FhsLds s;
+  __shared__ __align__(16) uint8_t s_win[fw::kBytes];
   __shared__ int row_s;
   const DecideDesc& d = f.d;
   const int lane = threadIdx.x;
@@ -239,11 +254,20 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
   for (int i = 0; i < 5; i++) row_org[i] = 0;
   bool above_done = row == 0;
   for (int col = 0; col < d.mbw; col++) {
+    const int n = col * 16, m = row * 16;
+    // the MB's search window into LDS and its block into registers, while
+    // the MB waits for its neighbours
+    const fw::Win w = fw::shape(n, m, d.W, d.H, f.mrng - 1);
+    fw::issue(d, w, s_win, lane);
+    const uint32_t cw = *(const uint32_t*)(d.cur + (long)(m + (lane >> 2)) * d.cur_stride + n + 4 * (lane & 3));
+    const int p1 = kPath80[lane], p2 = kPath80[64 + (lane & 15)];
+    const int cv1 = d.cur[(long)(m + (p1 >> 4)) * d.cur_stride + n + (p1 & 15)];
+    const int cv2 = d.cur[(long)(m + (p2 >> 4)) * d.cur_stride + n + (p2 & 15)];
     if (row > 0) {
       const int need = min(col + 2, d.mbw);
       if (lane == 0) {
         int spins = 0;
-        while (__hip_atomic_load(&f.progress[row - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        while (poll_progress(&f.progress[row - 1]) < need) {
           if (++spins > kSpin) {
             atomicExch(&f.flags[1], 1);
             break;
@@ -253,8 +277,8 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
       }
       __syncthreads();
     }
+    fw::wait();
     const int mb = row * d.mbw + col;
-    const int n = col * 16, m = row * 16;
     // ---- predictor with distortion (MacroBlockH264.cpp:818-915)
     auto ld = [&](int k, int* x, int* y, int* dd) {
       const uint32_t w = __hip_atomic_load(&mvw[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -282,14 +306,18 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
     const int predD = yd < xd ? yd : xd;
 
     // ---- the block and this lane's path points
-    const uint32_t cw = *(const uint32_t*)(d.cur + (long)(m + (lane >> 2)) * d.cur_stride + n + 4 * (lane & 3));
     s.w.cur4[lane] = cw;
-    const int p1 = kPath80[lane], p2 = kPath80[64 + (lane & 15)];
     Ctx c{f, d, n, m, lane};
-    c.cv1 = d.cur[(long)(m + (p1 >> 4)) * d.cur_stride + n + (p1 & 15)];
-    c.cv2 = d.cur[(long)(m + (p2 >> 4)) * d.cur_stride + n + (p2 & 15)];
+    c.cv1 = cv1;
+    c.cv2 = cv2;
     c.rp1 = d.ref + (long)(m + (p1 >> 4)) * d.ref_stride + n + (p1 & 15);
     c.rp2 = d.ref + (long)(m + (p2 >> 4)) * d.ref_stride + n + (p2 & 15);
+    c.p1x = p1 & 15;
+    c.p1y = p1 >> 4;
+    c.p2x = p2 & 15;
+    c.p2y = p2 >> 4;
+    c.win = s_win;
+    c.w = w;
     wave_sync();
 
     int pXQ = predX % 4, pYQ = predY % 4;
@@ -306,11 +334,21 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
     c.pY0 = predY0;
     int predVecD;
     if (pXQ || pYQ) {
-      predVecD = quarter_dist(d, s.w, n + predX0, m + predY0, pXQ, pYQ, lane);
+      // QuarterRead at the predictor, its patch from the window when staged
+      int fx = pXQ, fy = pYQ, fox = 0, foy = 0;
+      if (fx < 0) { fox = -1; fx += 4; }
+      if (fy < 0) { foy = -1; fy += 4; }
+      const int px = n + predX0 + fox, py = m + predY0 + foy;
+      fw::patch(d, s_win, w, w.has(px, py, 16, 16, 3), px, py, s.w.P, lane);
+      wave_sync();
+      const uint8_t* cur8 = (const uint8_t*)s.w.cur4;
+      int a = 0;
+      for (int p = lane; p < 256; p += 64) a += sq(cur8[p], qpel_p(s.w.P, 3 + (p & 15), 3 + (p >> 4), fx, fy));
+      predVecD = wave_sum(a);
+      wave_sync();
     } else {
-      const int xs = n + predX0 + 4 * (lane & 3), sh = xs & 3;
-      const uint32_t* pp = (const uint32_t*)(d.ref + (long)(m + predY0 + (lane >> 2)) * d.ref_stride + (xs - sh));
-      const uint32_t rw = __builtin_amdgcn_alignbyte(pp[1], pp[0], sh);
+      const int px = n + predX0, py = m + predY0;
+      const uint32_t rw = fw::row4(d, s_win, w, w.has(px, py, 16, 16, 0), px, py, lane);
       int a = 0;
 #pragma unroll
       for (int i = 0; i < 4; i++) a += sq((int)((cw >> (8 * i)) & 0xFFu), (int)((rw >> (8 * i)) & 0xFFu));
@@ -328,7 +366,8 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
       done = true;
     }
     if (!done && (rX || rY)) {
-      int e1[1] = {sq(c.cv1, c.rp1[0])}, e2[1] = {lane < 16 ? sq(c.cv2, c.rp2[0]) : 0};
+      int e1[1] = {sq(c.cv1, fw::byte_at(s_win, w, n + c.p1x, m + c.p1y))};
+      int e2[1] = {lane < 16 ? sq(c.cv2, fw::byte_at(s_win, w, n + c.p2x, m + c.p2y)) : 0};
       Path80 p[1];
       path_sums(e1, e2, p);
       const int zd = p[0].lt(c.minD);
@@ -374,7 +413,7 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
         // ---- HalfPelEstimate: the lattice at the full-pel winner, the 8
         // candidates summed at once, then taken in the reference's passes
         org = pack_org(n + mx, m + my);
-        build_lattice(d, s, org, lane);
+        build_lattice(d, s, org, lane, s_win, &w);
         const uint8_t* cur8 = (const uint8_t*)s.w.cur4;
         int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int p = lane; p < 256; p += 64) {
@@ -432,7 +471,7 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
               for (int q = 0; q < 5; q++)
                 if ((missing & (1 << q)) && (row_has & (1 << q)) && row_org[q] == o) parts |= 1 << q;
               wave_sync();
-              build_lattice(d, s, o, lane);
+              build_lattice(d, s, o, lane, s_win, &w);
               lattice_to_cache(s, parts, lane);
               missing &= ~parts;
             }
@@ -442,7 +481,7 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
               if (!above_done && row > 0) {
                 if (lane == 0) {
                   int spins = 0;
-                  while (__hip_atomic_load(&f.progress[row - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
+                  while (poll_progress(&f.progress[row - 1]) <
                          d.mbw) {
                     if (++spins > kSpin) {
                       atomicExch(&f.flags[1], 1);
@@ -460,7 +499,7 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
                 if (!parts) continue;
                 const int o = __hip_atomic_load(&f.fill_org[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 wave_sync();
-                build_lattice(d, s, o, lane);
+                build_lattice(d, s, o, lane, s_win, &w);
                 lattice_to_cache(s, parts, lane);
                 missing &= ~parts;
               }
@@ -472,6 +511,9 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
             int qa[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             int unw = 0;
             const int x0 = n + mx, y0 = m + my;
+            // full pels of the quarter samples: one pel left / above to one
+            // right / below of the block
+            const bool gst = w.has(x0, y0, 16, 16, 1);
             for (int p = lane; p < 256; p += 64) {
               const int r = p >> 4, cc = p & 15;
               const int cvv = cur8[p];
@@ -481,7 +523,8 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
                 const int sx = ox < 0 ? -1 : 0, sy = oy < 0 ? -1 : 0;
                 const int fx = ox - 4 * sx, fy = oy - 4 * sy;
                 const int ci = sy + 1 + r, cj = sx + 1 + cc;
-                const uint8_t* gp = d.ref + (long)(y0 + sy + r) * d.ref_stride + x0 + sx + cc;
+                const uint8_t* gp = gst ? s_win + (y0 + sy + r - w.y0) * w.cols + (x0 + sx + cc - w.x0)
+                                        : d.ref + (long)(y0 + sy + r) * d.ref_stride + x0 + sx + cc;
                 auto C = [&](int t, int a, int b) {
                   unw += !s.fw[t][a][b];
                   return (int)s.fc[t][a][b];
@@ -491,7 +534,7 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
                   case 1: v = (gp[0] + C(1, ci, cj) + 1) >> 1; break;
                   case 3: v = (gp[1] + C(1, ci, cj) + 1) >> 1; break;
                   case 4: v = (gp[0] + C(0, ci, cj) + 1) >> 1; break;
-                  case 12: v = (gp[d.ref_stride] + C(0, ci, cj) + 1) >> 1; break;
+                  case 12: v = (gp[gst ? w.cols : d.ref_stride] + C(0, ci, cj) + 1) >> 1; break;
                   case 5: v = (C(1, ci, cj) + C(0, ci, cj) + 1) >> 1; break;
                   case 7: v = (C(1, ci, cj) + C(0, ci, cj + 1) + 1) >> 1; break;
                   case 6: v = (C(1, ci, cj) + C(2, ci, cj) + 1) >> 1; break;
@@ -547,8 +590,7 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
       const uint32_t w = (uint32_t)(uint16_t)(int16_t)mvx | ((uint32_t)(uint16_t)(int16_t)mvy << 16);
       __hip_atomic_store((uint32_t*)(f.mv_out + 2 * mb), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&f.dist_out[mb], c.minD, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __threadfence();
-      __hip_atomic_store(&f.progress[row], col + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      publish_progress(&f.progress[row], col + 1);
     }
     __syncthreads();
   }

@@ -24,6 +24,7 @@
 #include "../../include/icu_hip.h"
 #include "icu_decide.h"
 #include "icu_device.h"
+#include "icu_fastwin.h"
 #include "icu_internal.h"
 
 using namespace icu;
@@ -61,11 +62,16 @@ __device__ __forceinline__ int umhs_cost(int dist, int dx, int dy) {
   return (int)best_vector_cost(dist, dx * dx + dy * dy);
 }
 
-// This lane's SSD over its 4 pixels for the block at full pel (x, y).
-__device__ __forceinline__ int ssd4(const DecideDesc& d, int x, int y, uint32_t cw, int lane) {
-  const int xs = x + 4 * (lane & 3), sh = xs & 3;
-  const uint32_t* p = (const uint32_t*)(d.ref + (long)(y + (lane >> 2)) * d.ref_stride + (xs - sh));
-  const uint32_t rw = __builtin_amdgcn_alignbyte(p[1], p[0], sh);
+// The MB's staged search window (LDS) and its shape.
+struct WinRef {
+  const uint8_t* lds;
+  fw::Win w;
+};
+
+// This lane's SSD over its 4 pixels for the block at full pel (x, y), read
+// from the staged window when it lies inside.
+__device__ __forceinline__ int ssd4(const DecideDesc& d, const WinRef& wr, int x, int y, uint32_t cw, int lane) {
+  const uint32_t rw = fw::row4(d, wr.lds, wr.w, wr.w.has(x, y, 16, 16, 0), x, y, lane);
   int acc = 0;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
@@ -75,8 +81,30 @@ __device__ __forceinline__ int ssd4(const DecideDesc& d, int x, int y, uint32_t 
   return acc;
 }
 
-__device__ __forceinline__ int block_ssd(const DecideDesc& d, int x, int y, uint32_t cw, int lane) {
-  return wave_sum(ssd4(d, x, y, cw, lane));
+__device__ __forceinline__ int block_ssd(const DecideDesc& d, const WinRef& wr, int x, int y, uint32_t cw, int lane) {
+  return wave_sum(ssd4(d, wr, x, y, cw, lane));
+}
+
+// Distortion against the quarter-pel read at full pel (x0, y0) + (fx, fy)
+// (fractions -3..3; a negative one reads from the pel to the left / above),
+// its patch from the staged window when it lies inside.
+__device__ __forceinline__ int qdist(const DecideDesc& d, WaveLds& s, const WinRef& wr, int x0, int y0, int fx, int fy,
+                                     int lane) {
+  const uint8_t* cur8 = (const uint8_t*)s.cur4;
+  int fox = 0, foy = 0;
+  if (fx < 0) { fox = -1; fx += 4; }
+  if (fy < 0) { foy = -1; fy += 4; }
+  const int x = x0 + fox, y = y0 + foy;
+  fw::patch(d, wr.lds, wr.w, wr.w.has(x, y, 16, 16, 3), x, y, s.P, lane);
+  wave_sync();
+  int acc = 0;
+  for (int p = lane; p < 256; p += 64) {
+    const int r = p >> 4, c = p & 15;
+    acc += pix_dist(cur8[p], qpel_p(s.P, 3 + c, 3 + r, fx, fy), d.metric);
+  }
+  const int v = wave_sum(acc);
+  wave_sync();
+  return v;
 }
 
 // The running minimum of the search (minDiff, minCost, offset (rmx, rmy)
@@ -89,12 +117,13 @@ struct Best {
 // ok[k] inside the window, summed at once, then taken in order as the
 // reference's loop body (UMHS.cpp:598-609): a candidate no worse in
 // distortion and strictly cheaper in cost replaces the best.
-__device__ __forceinline__ void try8(const DecideDesc& d, int n, int m, int mx, int my, const int (&ox)[8],
-                                     const int (&oy)[8], const bool (&ok)[8], int cnt, int pX0, int pY0,
-                                     uint32_t cw, int lane, Best& b) {
+__device__ __forceinline__ void try8(const DecideDesc& d, const WinRef& wr, int n, int m, int mx, int my,
+                                     const int (&ox)[8], const int (&oy)[8], const bool (&ok)[8], int cnt, int pX0,
+                                     int pY0, uint32_t cw, int lane, Best& b) {
   int acc[8];
 #pragma unroll
-  for (int k = 0; k < 8; k++) acc[k] = (k < cnt && ok[k]) ? ssd4(d, n + mx + ox[k], m + my + oy[k], cw, lane) : 0;
+  for (int k = 0; k < 8; k++)
+    acc[k] = (k < cnt && ok[k]) ? ssd4(d, wr, n + mx + ox[k], m + my + oy[k], cw, lane) : 0;
   wave_sum8(acc);
 #pragma unroll
   for (int k = 0; k < 8; k++) {
@@ -117,9 +146,9 @@ struct Window {
 
 // A table of cnt offsets (a multiple of 8 or fewer than 8) scaled by w,
 // in groups of 8.
-__device__ __forceinline__ void try_table(const DecideDesc& d, const int8_t (*tab)[2], int cnt, int w, int n, int m,
-                                          int mx, int my, const Window& win, int pX0, int pY0, uint32_t cw,
-                                          int lane, Best& b) {
+__device__ __forceinline__ void try_table(const DecideDesc& d, const WinRef& wr, const int8_t (*tab)[2], int cnt, int w,
+                                          int n, int m, int mx, int my, const Window& win, int pX0, int pY0,
+                                          uint32_t cw, int lane, Best& b) {
   for (int g = 0; g < cnt; g += 8) {
     int ox[8], oy[8];
     bool ok[8];
@@ -131,7 +160,7 @@ __device__ __forceinline__ void try_table(const DecideDesc& d, const int8_t (*ta
       oy[k] = w * tab[t][1];
       ok[k] = k < c && win.in(mx + ox[k], my + oy[k]);
     }
-    try8(d, n, m, mx, my, ox, oy, ok, c, pX0, pY0, cw, lane, b);
+    try8(d, wr, n, m, mx, my, ox, oy, ok, c, pX0, pY0, cw, lane, b);
   }
 }
 
@@ -139,6 +168,7 @@ __device__ __forceinline__ int median3i(int x, int y, int z) { return median3(x,
 
 __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
   __shared__ WaveLds s;
+  __shared__ __align__(16) uint8_t s_win[fw::kBytes];
   __shared__ int row_s;
   const DecideDesc& d = u.d;
   const int lane = threadIdx.x;
@@ -148,11 +178,17 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
   if (row >= d.mbh) return;
   const uint32_t* mvw = (const uint32_t*)u.mv_out;
   for (int col = 0; col < d.mbw; col++) {
+    const int n = col * 16, m = row * 16;
+    // the MB's search window into LDS and its block into registers, while
+    // the MB waits for its neighbours
+    WinRef wr{s_win, fw::shape(n, m, d.W, d.H, u.mrng - 1)};
+    fw::issue(d, wr.w, s_win, lane);
+    const uint32_t cw = *(const uint32_t*)(d.cur + (long)(m + (lane >> 2)) * d.cur_stride + n + 4 * (lane & 3));
     if (row > 0) {
       const int need = min(col + 2, d.mbw);
       if (lane == 0) {
         int spins = 0;
-        while (__hip_atomic_load(&u.progress[row - 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        while (poll_progress(&u.progress[row - 1]) < need) {
           if (++spins > kSpin) {
             atomicExch(&u.flags[1], 1);
             break;
@@ -162,8 +198,8 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
       }
       __syncthreads();
     }
+    fw::wait();
     const int mb = row * d.mbw + col;
-    const int n = col * 16, m = row * 16;
     // ---- predictor with distortion (MacroBlockH264.cpp:818-915)
     auto ld = [&](int k, int* x, int* y, int* dd) {
       const uint32_t w = __hip_atomic_load(&mvw[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -191,7 +227,6 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
     const int predD = yd < xd ? yd : xd;
 
     // ---- starting points (UMHS.cpp:493-565)
-    const uint32_t cw = *(const uint32_t*)(d.cur + (long)(m + (lane >> 2)) * d.cur_stride + n + 4 * (lane & 3));
     s.cur4[lane] = cw;
     const int predXQ = predX % 4, predYQ = predY % 4;
     int predX0 = predX / 4, predY0 = predY / 4;
@@ -203,11 +238,11 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
     const bool trunc = oX0 != predX0 || oY0 != predY0;
     Window win;
     motion_window(n, m, d.W, d.H, u.mrng - 1, &win.xl, &win.xr, &win.yu, &win.yd);
-    int predVecD = block_ssd(d, n + predX0, m + predY0, cw, lane);
+    int predVecD = block_ssd(d, wr, n + predX0, m + predY0, cw, lane);
     int mx = predX0, my = predY0;
     Best b{predVecD, predVecD / 256, 0, 0};
     if (predX0 || predY0) {
-      const int zd = block_ssd(d, n, m, cw, lane);
+      const int zd = block_ssd(d, wr, n, m, cw, lane);
       const int zc = umhs_cost(zd, -predX0, -predY0);
       if (zc < b.minC) {
         b.minD = zd;
@@ -224,7 +259,7 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
       const int qX0 = min(max((int)(int16_t)(w & 0xFFFF) / 4, -(n + 16)), d.W - n);
       const int qY0 = min(max((int)(int16_t)(w >> 16) / 4, -(m + 16)), d.H - m);
       if ((qX0 || qY0) && qX0 != predX0 && qY0 != predY0) {
-        const int qd = block_ssd(d, n + qX0, m + qY0, cw, lane);
+        const int qd = block_ssd(d, wr, n + qX0, m + qY0, cw, lane);
         const int qc = umhs_cost(qd, qX0 - predX0, qY0 - predY0);
         if (qc < b.minC) {
           b.minD = qd;
@@ -256,7 +291,7 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
           oy[k] = w * kCross[k & 3][1];
           ok[k] = oy[k] < vlim && oy[k] > -vlim && win.in(mx + ox[k], my + oy[k]);
         }
-        try8(d, n, m, mx, my, ox, oy, ok, 8, predX0, predY0, cw, lane, b);
+        try8(d, wr, n, m, mx, my, ox, oy, ok, 8, predX0, predY0, cw, lane, b);
       }
       mx += b.rmx;
       my += b.rmy;
@@ -280,7 +315,7 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
                                              (sx == 0 && (oy[k] == 2 || oy[k] == -2)));
             ok[k] = !done && win.in(mx + ox[k], my + oy[k]);
           }
-          try8(d, n, m, mx, my, ox, oy, ok, 8, predX0, predY0, cw, lane, b);
+          try8(d, wr, n, m, mx, my, ox, oy, ok, 8, predX0, predY0, cw, lane, b);
         }
         if (b.minD < 1000) {
           stage = 2;
@@ -289,7 +324,7 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
           // improved enough
           prior = b.minD;
           for (int w = 1; w < 4; w++) {
-            try_table(d, kHex16, 16, w, n, m, mx, my, win, predX0, predY0, cw, lane, b);
+            try_table(d, wr, kHex16, 16, w, n, m, mx, my, win, predX0, predY0, cw, lane, b);
             if ((b.rmx || b.rmy) && (b.minD < 2000 || b.minD < prior / 5)) break;
           }
           mx += b.rmx;
@@ -301,7 +336,7 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
     if (stage == 1) {
       for (int it = 0; it < kMaxMoves; it++) {
         b.rmx = b.rmy = 0;
-        try_table(d, kExtHex, 6, 1, n, m, mx, my, win, predX0, predY0, cw, lane, b);
+        try_table(d, wr, kExtHex, 6, 1, n, m, mx, my, win, predX0, predY0, cw, lane, b);
         mx += b.rmx;
         my += b.rmy;
         if (!(b.rmx || b.rmy)) break;
@@ -309,7 +344,7 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
     }
     for (int it = 0; it < kMaxMoves; it++) {
       b.rmx = b.rmy = 0;
-      try_table(d, kCross, 4, 1, n, m, mx, my, win, predX0, predY0, cw, lane, b);
+      try_table(d, wr, kCross, 4, 1, n, m, mx, my, win, predX0, predY0, cw, lane, b);
       mx += b.rmx;
       my += b.rmy;
       if (!(b.rmx || b.rmy)) break;
@@ -317,10 +352,15 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
 
     // ---- sub-pel stages and the predictor at quarter pel (UMHS.cpp:795-903)
     int minD = b.minD, mvx = mx * 4, mvy = my * 4;
-    if (d.subpel >= 1) refine_subpel(d, s, n, m, mx, my, cw, lane, minD, mvx, mvy);
+    if (d.subpel >= 2) {
+      fw::patch(d, wr.lds, wr.w, wr.w.has(n + mx, m + my, 16, 16, 3), n + mx, m + my, s.P, lane);
+      refine_subpel(d, s, n, m, mx, my, cw, lane, minD, mvx, mvy, true);
+    } else if (d.subpel == 1) {
+      refine_subpel(d, s, n, m, mx, my, cw, lane, minD, mvx, mvy);
+    }
     const int rX = predX0 * 4 + predXQ, rY = predY0 * 4 + predYQ;
-    if (predXQ || predYQ) predVecD = quarter_dist(d, s, n + predX0, m + predY0, predXQ, predYQ, lane);
-    else if (trunc) predVecD = block_ssd(d, n + predX0, m + predY0, cw, lane);
+    if (predXQ || predYQ) predVecD = qdist(d, s, wr, n + predX0, m + predY0, predXQ, predYQ, lane);
+    else if (trunc) predVecD = block_ssd(d, wr, n + predX0, m + predY0, cw, lane);
     if (umhs_cost(predVecD, rX - predX, rY - predY) <= umhs_cost(minD, mvx - predX, mvy - predY)) {
       minD = predVecD;
       mvx = rX;
@@ -330,8 +370,7 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
       const uint32_t w = (uint32_t)(uint16_t)(int16_t)mvx | ((uint32_t)(uint16_t)(int16_t)mvy << 16);
       __hip_atomic_store((uint32_t*)(u.mv_out + 2 * mb), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&u.dist_out[mb], minD, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __threadfence();
-      __hip_atomic_store(&u.progress[row], col + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      publish_progress(&u.progress[row], col + 1);
     }
     __syncthreads();
   }
