@@ -309,6 +309,32 @@ def run_cavlc(capi, synth, steps, warmup):
             "stream_bits": int(bits), "nonzero_blocks": int(np.count_nonzero(tc)), "round_trip_exact": ok}
 
 
+def run_as_shipped(capi, synth, steps, warmup):
+    """The full-search estimator in the reference class's shipped
+    configuration (SSD, running-best filter, +-31 window, H.264 median
+    predictor, half + quarter pel, GetBestVector; bit-exact with the class),
+    on the 1080p frame pair of the plain leg: device time per estimate, the
+    same workload as the CPU baseline's reference-class timing."""
+    fp = synth.rect_motion_pair(W, H, 1080, n_rects=24, max_mv=32, sigma_cur=3.0)
+    est = capi.MotionEstimator(capi.reference_config(W, H, 4 * R))
+    d_cur, d_ref = capi.DeviceBuffer.from_array(fp.cur), capi.DeviceBuffer.from_array(fp.ref)
+    n = est.nmb
+    d_mv, d_dist = capi.DeviceBuffer(4 * n), capi.DeviceBuffer(4 * n)
+    s = capi.Stream()
+    for _ in range(warmup):
+        est.estimate_device(d_cur, d_ref, d_mv, d_dist, stream=s)
+    s.sync()
+    a, b = capi.Event(), capi.Event()
+    a.record(s)
+    for _ in range(steps):
+        est.estimate_device(d_cur, d_ref, d_mv, d_dist, stream=s)
+    b.record(s)
+    s.sync()
+    ms = a.elapsed_ms(b) / steps
+    return {"workload": "1080p, reference class as shipped (SSD, filter, +-31, median predictor, quarter pel, "
+                        "GetBestVector)", "ms_per_frame": round(ms, 4), "mbs_per_s": round(n / (ms / 1e3), 1)}
+
+
 FAST_RANGE = 128  # +-32 in the reference's quarter-pel motionRange
 
 
@@ -619,6 +645,7 @@ def main():
                           "achieved_T_sad_ops": round(bt["sad_ops"] / (bt["ms"] / 1e3) / 1e12, 2)}
     out["mc_yuv_1080p"] = run_mc_yuv(capi, synth, 20, 3)
     out["cavlc_1080p"] = run_cavlc(capi, synth, 20, 3)
+    out["as_shipped_1080p"] = run_as_shipped(capi, synth, 10, 2)
     out["fast_estimators_1080p"] = run_fast(capi, synth, 10, 2)
     if not args.no_4k:
         out["multiref_4k"] = run_multiref_4k(capi, synth, max(5, args.steps // 5), 2)
