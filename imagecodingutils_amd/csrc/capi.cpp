@@ -465,7 +465,7 @@ struct Engine {
     for (size_t i = 0; i < sp.size(); i++) {
       if (!sp[i].cfg.predictor) continue;
       launch_fill_i32(d_wave + wave_off[i], 0, sp[i].mbh + 2, st);
-      launch_decide_wavefront(d_decide + i, nullptr, sp[i].mbh, st);
+      launch_decide_wavefront(d_decide + i, sp[i].table ? 2 * sp[i].L + 1 : 0, sp[i].mbh, st);
     }
     if (ev) HIP_TRY(hipEventRecord(ev[3], st));
     HIP_TRY(hipGetLastError());

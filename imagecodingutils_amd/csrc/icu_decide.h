@@ -442,8 +442,10 @@ struct Decision {
   int imx, imy, iminD;
 };
 
+// staged_tab (nullable): the MB's cost table already copied to LDS.
 __device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, int predX, int predY,
-                                     int lane, const IntResult* known = nullptr) {
+                                     int lane, const IntResult* known = nullptr,
+                                     const uint32_t* staged_tab = nullptr) {
   const int col = mb % d.mbw, row = mb / d.mbw;
   const int n = col * 16, m = row * 16;
   const int slot = d.mb_base + mb;
@@ -472,7 +474,7 @@ __device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, in
   motion_window(n, m, d.W, d.H, d.L, &xl, &xr, &yu, &yd);
   if (d.table) {
     TabView tab;
-    tab.t = d.table + d.table_base + (long long)mb * d.table_dim * d.table_dim;
+    tab.t = staged_tab ? staged_tab : d.table + d.table_base + (long long)mb * d.table_dim * d.table_dim;
     tab.dim = d.table_dim;
     tab.L = d.L;
     tab.xl = xl;

@@ -66,8 +66,8 @@ int search_max_group();
 
 void launch_decide_parallel(const DecideDesc* d_descs, const DecideItem* d_items, int nitems,
                             hipStream_t st);
-void launch_decide_wavefront(const DecideDesc* d_desc, int* d_progress_and_ticket, int mbh,
-                             hipStream_t st);
+// table_dim: the per-MB cost table side (0 when no table is kept).
+void launch_decide_wavefront(const DecideDesc* d_desc, int table_dim, int mbh, hipStream_t st);
 void launch_fill_i32(int* p, int v, int n, hipStream_t st);
 // Adds an order-free checksum of `bytes` bytes at p (4-byte aligned) to *out.
 void launch_checksum(const void* p, long bytes, unsigned long long* out, hipStream_t st);

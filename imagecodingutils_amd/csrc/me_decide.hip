@@ -47,9 +47,13 @@ __global__ __launch_bounds__(64) void decide_parallel_kernel(const DecideDesc* _
 
 // One wave per MB row; rows are handed out by ticket in start order, so the
 // row a wave waits on always belongs to a wave that is already running.
-__global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* __restrict__ desc) {
+// stage_tab: the MB's cost table (table_dim^2 dwords) is copied into the
+// dynamic LDS with global_load_lds before the MB waits for its neighbours,
+// so the filtered scan's rounds read LDS instead of HBM.
+__global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* __restrict__ desc, int stage_tab) {
   __shared__ WaveLds s;
   __shared__ int row_s;
+  extern __shared__ uint32_t s_tab[];
   const DecideDesc& d = *desc;
   const int lane = threadIdx.x;
   int* progress = d.progress;
@@ -58,7 +62,15 @@ __global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* 
   __syncthreads();
   const int row = row_s;
   if (row >= d.mbh) return;
+  const int n2 = d.table_dim * d.table_dim;
   for (int col = 0; col < d.mbw; col++) {
+    if (stage_tab) {
+      const uint32_t* src = d.table + d.table_base + (long long)(row * d.mbw + col) * n2;
+      for (int base = 0; base < n2; base += 64)
+        if (base + lane < n2)
+          __builtin_amdgcn_global_load_lds((const void*)(src + base + lane),
+                                           (__attribute__((address_space(3))) void*)(s_tab + base), 4, 0, 0);
+    }
     if (row > 0) {
       // above-right (or above for the last column) must be final
       const int need = min(col + 2, d.mbw);
@@ -91,7 +103,11 @@ __global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* 
     else if (col > 0 && row > 0) ld(mb - d.mbw - 1, &Cx, &Cy);
     if (!hasB && !hasC) { Bx = Ax; By = Ay; Cx = Ax; Cy = Ay; }
     const int px = median3(Ax, Bx, Cx), py = median3(Ay, By, Cy);
-    decide_mb(d, s, mb, px, py, lane);
+    if (stage_tab) {
+      __builtin_amdgcn_s_waitcnt(0);  // the table has landed
+      wave_sync();
+    }
+    decide_mb(d, s, mb, px, py, lane, nullptr, stage_tab ? s_tab : nullptr);
     __syncthreads();
     if (lane == 0) {
       publish_progress(&progress[row], col + 1);
@@ -117,8 +133,11 @@ void launch_decide_parallel(const DecideDesc* d_descs, const DecideItem* d_items
   hipLaunchKernelGGL(decide_parallel_kernel, dim3(nitems), dim3(64), 0, st, d_descs, d_items, nitems);
 }
 
-void launch_decide_wavefront(const DecideDesc* d_desc, int* /*unused*/, int mbh, hipStream_t st) {
-  hipLaunchKernelGGL(decide_wavefront_kernel, dim3(mbh), dim3(64), 0, st, d_desc);
+void launch_decide_wavefront(const DecideDesc* d_desc, int table_dim, int mbh, hipStream_t st) {
+  // stage the per-MB table in LDS when it fits 48 KB (ranges up to +-55)
+  const size_t bytes = (size_t)table_dim * table_dim * 4;
+  const int stage = table_dim > 0 && bytes <= 48 * 1024;
+  hipLaunchKernelGGL(decide_wavefront_kernel, dim3(mbh), dim3(64), stage ? bytes : 0, st, d_desc, stage);
 }
 
 void launch_fill_i32(int* p, int v, int n, hipStream_t st) {
