@@ -209,10 +209,11 @@ __device__ inline void table_replay(const TabView& tab, int xl, int xr, int yu, 
       sidx += 64;
       continue;
     }
+    // the accepted candidate's lane is wave-uniform: read it out directly
     const int f = __ffsll((long long)mask) - 1;
-    minD = __shfl(c, f, 64);
-    mx = __shfl(j, f, 64);
-    my = __shfl(i, f, 64);
+    minD = __builtin_amdgcn_readlane(c, f);
+    mx = __builtin_amdgcn_readlane(j, f);
+    my = __builtin_amdgcn_readlane(i, f);
     sidx += f + 1;
   }
   *pmx = mx;
