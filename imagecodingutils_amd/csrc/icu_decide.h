@@ -120,6 +120,18 @@ __device__ __forceinline__ int pix_dist(int a, int b, int metric) {
   return metric == kMetricSAD ? (d < 0 ? -d : d) : d * d;
 }
 
+// acc + the distance of pels a and b (0..255) under a metric known at
+// compile time: for SAD one v_sad_u16 (|a - b| + acc on the low halves).
+template <int METRIC>
+__device__ __forceinline__ int dist_add(int acc, int a, int b) {
+  if constexpr (METRIC == kMetricSAD) {
+    return (int)__builtin_amdgcn_sad_u16((uint32_t)a, (uint32_t)b, (uint32_t)acc);
+  } else {
+    const int e = a - b;
+    return acc + e * e;
+  }
+}
+
 // Stage the 22x22 patch whose (3,3) is reference pel (x0, y0): 22 rows of 7
 // dwords read from the dword boundary at or left of x0-3 (the padded plane's
 // origin and stride are multiples of 16), every load issued before the first
@@ -269,6 +281,7 @@ __device__ inline void build_half_grid(WaveLds& s, int lane) {
 // search); every G/b/h/j sample is then filtered in registers with the same
 // arithmetic as build_half_grid (vertical 6-tap sums first, j from them).
 // acc[k] receives this lane's distortion of candidate k (kSubX/kSubY order).
+template <int METRIC>
 __device__ __forceinline__ void half_pel_direct(const DecideDesc& d, int x0, int y0, uint32_t cw,
                                                 int lane, int (&acc)[8]) {
   const int r = lane >> 2, c0 = 4 * (lane & 3);
@@ -317,14 +330,14 @@ __device__ __forceinline__ void half_pel_direct(const DecideDesc& d, int x0, int
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     const int cv = (int)((cw >> (8 * j)) & 0xFFu);
-    acc[0] += pix_dist(cv, J[0][j], d.metric);      // (-1,-1)
-    acc[1] += pix_dist(cv, Hh[0][j], d.metric);     // ( 0,-1)
-    acc[2] += pix_dist(cv, J[0][j + 1], d.metric);  // ( 1,-1)
-    acc[3] += pix_dist(cv, B[j], d.metric);         // (-1, 0)
-    acc[4] += pix_dist(cv, B[j + 1], d.metric);     // ( 1, 0)
-    acc[5] += pix_dist(cv, J[1][j], d.metric);      // (-1, 1)
-    acc[6] += pix_dist(cv, Hh[1][j], d.metric);     // ( 0, 1)
-    acc[7] += pix_dist(cv, J[1][j + 1], d.metric);  // ( 1, 1)
+    acc[0] = dist_add<METRIC>(acc[0], cv, J[0][j]);      // (-1,-1)
+    acc[1] = dist_add<METRIC>(acc[1], cv, Hh[0][j]);     // ( 0,-1)
+    acc[2] = dist_add<METRIC>(acc[2], cv, J[0][j + 1]);  // ( 1,-1)
+    acc[3] = dist_add<METRIC>(acc[3], cv, B[j]);         // (-1, 0)
+    acc[4] = dist_add<METRIC>(acc[4], cv, B[j + 1]);     // ( 1, 0)
+    acc[5] = dist_add<METRIC>(acc[5], cv, J[1][j]);      // (-1, 1)
+    acc[6] = dist_add<METRIC>(acc[6], cv, Hh[1][j]);     // ( 0, 1)
+    acc[7] = dist_add<METRIC>(acc[7], cv, J[1][j + 1]);  // ( 1, 1)
   }
 }
 
@@ -342,7 +355,8 @@ __device__ inline void refine_subpel(const DecideDesc& d, WaveLds& s, int n, int
   if (d.subpel == 1) {
     // half pel only: the eight candidates filtered in registers
     int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    half_pel_direct(d, n + mx, m + my, cw, lane, acc);
+    if (d.metric == kMetricSAD) half_pel_direct<kMetricSAD>(d, n + mx, m + my, cw, lane, acc);
+    else half_pel_direct<kMetricSSD>(d, n + mx, m + my, cw, lane, acc);
     wave_sum8(acc);
     int hxh = 0, hyh = 0;
 #pragma unroll
@@ -460,8 +474,12 @@ __device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, in
   if (!d.table && d.bestvec) {
     const uint32_t rw = *(const uint32_t*)(d.ref + (long)(m + (lane >> 2)) * d.ref_stride + n + 4 * (lane & 3));
     int z = 0;
+    if (d.metric == kMetricSAD) {
+      z = (int)__builtin_amdgcn_sad_u8(cw, rw, 0u);  // the 4 byte differences at once
+    } else {
 #pragma unroll
-    for (int i = 0; i < 4; i++) z += pix_dist((int)((cw >> (8 * i)) & 0xFF), (int)((rw >> (8 * i)) & 0xFF), d.metric);
+      for (int i = 0; i < 4; i++) z += pix_dist((int)((cw >> (8 * i)) & 0xFF), (int)((rw >> (8 * i)) & 0xFF), d.metric);
+    }
     zero_direct = wave_sum(z);
   }
 
