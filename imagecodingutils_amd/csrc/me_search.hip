@@ -100,6 +100,16 @@ __global__ __launch_bounds__(256) void pad_set_kernel(const PadSet s) {
 
 // ------------------------------------------------------------- tile bodies
 
+// u / d for 0 <= u < 2^17 and 1 <= d <= 2048 without an integer division:
+// (u + 1/2) / d lies at least 1/(2d) from an integer, and the f32 product
+// with v_rcp_f32's reciprocal (within 1 ulp of 1/d), rounded once or twice,
+// stays closer than that; checked exhaustively over the whole domain with the
+// reciprocal at 1/d and at both neighbouring floats.  Tile units stay below
+// 16 MBs x 33 quads x 17 row tiles.
+__device__ __forceinline__ int udiv_small(int u, int d) {
+  return (int)(((float)u + 0.5f) * __builtin_amdgcn_rcpf((float)d));
+}
+
 // Rows are software-pipelined one ahead: the LDS loads of row rho+1 are
 // issued before the qsads of row rho.  The empty asm at the top of each row
 // takes the row's dwords as operands and clobbers memory: the wait for them
@@ -240,8 +250,8 @@ __device__ __forceinline__ void run_tile(const StreamDesc& s, const SearchTask& 
   // unit u: column quad fastest, then MB, then row tile.  Lanes of adjacent
   // MBs read overlapping window dwords (MBs are 4 dwords apart), which the
   // LDS serves as broadcasts instead of bank conflicts.
-  const int gq = u / s.Q, q = u - gq * s.Q;
-  const int bi = gq / t.count, g = gq - bi * t.count;
+  const int gq = udiv_small(u, s.Q), q = u - __mul24(gq, s.Q);
+  const int bi = udiv_small(gq, t.count), g = gq - __mul24(bi, t.count);
   const int b = b_lo + bi;
   uint32_t c[64];
   const lds_u32x4* cp = (const lds_u32x4*)(curs + g * 64);
@@ -255,7 +265,8 @@ __device__ __forceinline__ void run_tile(const StreamDesc& s, const SearchTask& 
   }
   const lds_u32* wp = win + (b * V) * row_dw + 4 * g + q;
   const int mb = t.mb + g;
-  const int col = mb % s.mbw, row = mb / s.mbw;
+  // a task's MBs lie in one MB row: the division is the task's, not the lane's
+  const int row = t.mb / s.mbw, col = t.mb - row * s.mbw + g;
   const int n = col * 16, m = row * 16;
   int xl, xr, yu, yd;
   motion_window(n, m, s.W, s.H, s.L, &xl, &xr, &yu, &yd);
@@ -311,12 +322,12 @@ __device__ __forceinline__ void run_tile(const StreamDesc& s, const SearchTask& 
     const int vlo = max(yu - dy0, 0), vhi = min(yd - dy0, V - 1);
     const uint32_t inv_rows = vlo > vhi ? 0xFFFFFFFFu : ~(((2u << vhi) - 1u) & ~((1u << vlo) - 1u));
     uint32_t lbest = 0xFFFFFFFFu, lbest0 = 0xFFFFFFFFu;
-    uint32_t Evr = EvA;  // one-key tiles: the row part, updated row by row
+    uint32_t Evr = EvA;  // the row part, updated row by row (restarting at the second half)
     sad_tile<V>(wp, row_dw, c, [&](int v, uint32_t lo, uint32_t hi) {
       const bool second = v >= V1;
       const uint32_t u = (uint32_t)(second ? v - V1 : v);
-      // (dy+u)^2 - dy^2 = 2 dy u + u^2
-      const uint32_t Ev = V1 < V ? (second ? EvB + u * DlB : EvA + u * Dl) + ((u * u) << IB) - 4u * u : Evr;
+      if (V1 < V && v == V1) Evr = EvB;
+      const uint32_t Ev = Evr;
       const uint32_t rmask = (uint32_t)__builtin_amdgcn_sbfe((int)inv_rows, v, 1);
       const uint32_t l2 = lo | cm_lo | rmask, h2 = hi | cm_hi | rmask;
       const uint32_t k0 = (l2 << 16) | Ck[0];
@@ -332,9 +343,10 @@ __device__ __forceinline__ void run_tile(const StreamDesc& s, const SearchTask& 
         asm volatile("" : "+v"(lbest), "+v"(lbest0));
       } else {
         lbest0 = min(lbest0, m);
-        // Ev of row v+1: ((dy+1)^2 - dy^2) << IB, minus 4 for the raster part
-        Evr += Dl + (((2u * v + 1u) << IB) - 4u);
       }
+      // Ev of the half's next row: ((dy+1)^2 - dy^2) << IB = (2 dy + 2 u + 1)
+      // << IB, minus 4 for the raster part
+      Evr += (second ? DlB : Dl) + (((2u * u + 1u) << IB) - 4u);
     });
     auto fold = [&](uint32_t lb, int vbase) {
       if (lb < 0xFFFF0000u) {
