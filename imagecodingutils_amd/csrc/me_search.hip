@@ -488,6 +488,7 @@ void me_search_kernel(const StreamDesc* __restrict__ streams, const SearchTask* 
     const lds_u32* curs = buf(it) + g.win_chunks * 4;
     __builtin_amdgcn_s_waitcnt(0);  // this wave's staging loads have landed
     __syncthreads();                // ... and every other wave's
+    if constexpr (!TABLE) __builtin_amdgcn_s_setprio(0);  // tiles at the base priority
     const int tnext = ti + (int)gridDim.x;
     if (tnext < ntasks) {
       const SearchTask tn = tasks[tnext];
@@ -553,6 +554,9 @@ void me_search_kernel(const StreamDesc* __restrict__ streams, const SearchTask* 
       // reset by the one wave that decides it.
       __shared__ dec::WaveLds ws[kThreads / 64];
       const int wave = tid >> 6, lane = tid & 63;
+      // the decision is latency-bound and holds the workgroup's slot: its
+      // waves issue ahead of other workgroups' tiles
+      __builtin_amdgcn_s_setprio(2);
       for (int gg = wave; gg < t.count; gg += kThreads / 64) {
         const uint64_t k = best[gg];
         int mx, my;
