@@ -94,20 +94,8 @@ def run_plain(capi, synth, steps, warmup, tune=False):
 
 
 def mixed_batch_configs(synth, capi):
-    rng = np.random.default_rng(2026)
-    plan = [("QCIF", 24), ("CIF", 20), ("720p", 12), ("1080p", 8)]
-    cfgs, frames = [], []
-    for name, count in plan:
-        w, h = synth.SIZES[name]
-        if h % 16:
-            h -= h % 16
-        for _ in range(count):
-            Rs = int(rng.choice([8, 16, 32, 64]))
-            seed = int(rng.integers(1 << 30))
-            fp = synth.rect_motion_pair(w, h, seed, n_rects=int(rng.integers(8, 25)), max_mv=min(Rs, 32))
-            cfgs.append(capi.make_config(w, h, Rs, **MODES))
-            frames.append(fp)
-    return cfgs, frames
+    streams = synth.mixed_batch(2026)
+    return [capi.make_config(w, h, R, **MODES) for (w, h, R, _) in streams], [fp for (*_, fp) in streams]
 
 
 def run_batched(capi, synth, steps, warmup):

@@ -54,6 +54,8 @@ _SIGS = {
     "icu_me_estimate": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_long)]),
     "icu_me_estimate_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "icu_me_estimate_timed": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_float)]),
+    "icu_me_check": (C.c_int, [_vp]),
+    "icu_me_debug_set_wave_error": (C.c_int, [_vp]),
     "icu_me_get_tiling": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_me_autotune": (C.c_int, [_vp, _vp, _vp, C.c_int]),
     "icu_code_frame_device": (C.c_int, [_vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
@@ -75,6 +77,7 @@ _SIGS = {
     "icu_server_open_stream": (C.c_int, [_vp, C.POINTER(MeConfig), C.POINTER(C.c_int)]),
     "icu_server_submit": (C.c_int, [_vp, C.c_int, _vp, C.c_longlong]),
     "icu_server_poll": (C.c_int, [_vp, C.POINTER(ServerResult), C.c_int, C.c_int]),
+    "icu_server_stream_estimator": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
     "icu_server_stats": (C.c_int, [_vp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                    C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_server_clock": (C.c_double, []),
@@ -285,9 +288,19 @@ class MotionEstimator:
                                    dist.ctypes.data, C.byref(avg)))
         return mv, dist, avg.value
 
-    def estimate_device(self, d_cur, d_ref, d_mv, d_dist, d_int=None, stream: Stream | None = None):
+    def estimate_device(self, d_cur, d_ref, d_mv, d_dist, d_int=None, stream: Stream | None = None,
+                        check: bool = False):
+        """Asynchronous on `stream`.  check=True waits for the device and
+        raises if a median-predictor wavefront timed out (icu_me_check)."""
         _check(lib.icu_me_estimate_device(self.h, _p(d_cur), _p(d_ref), _p(d_mv), _p(d_dist),
                                           _p(d_int), stream.handle if stream else None))
+        if check:
+            self.check()
+
+    def check(self):
+        """Waits for the device; IcuError(ICU_E_STATE) if a median-predictor
+        wavefront of this handle timed out since the last check."""
+        _check(lib.icu_me_check(self.h))
 
     def estimate_timed(self, d_cur, d_ref, d_mv, d_dist, stream: Stream | None = None, search_reps: int = 1):
         """Per-stage device times in ms: (pad, search, decide); the search time
@@ -735,6 +748,13 @@ class Server:
             out.append(dict(tag=r.tag, stream=r.stream, frame=r.frame, submit_s=r.submit_s,
                             done_s=r.done_s, levels_sum=r.levels_sum, recon_sum=r.recon_sum, mv=mv))
         return out
+
+    def debug_set_wave_error(self, stream: int) -> None:
+        """Test hook: sets the wavefront error flag of a median-predictor
+        stream's estimator (icu_me_debug_set_wave_error)."""
+        h = _vp()
+        _check(lib.icu_server_stream_estimator(self.h, stream, C.byref(h)))
+        _check(lib.icu_me_debug_set_wave_error(h))
 
     def stats(self) -> dict:
         b, pk, q, f = C.c_longlong(), C.c_longlong(), C.c_int(), C.c_int()

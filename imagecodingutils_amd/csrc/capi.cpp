@@ -11,6 +11,7 @@
 #include "../../include/icu_hip.h"
 #include "icu_device.h"
 #include "icu_internal.h"
+#include "icu_tx.h"
 
 using namespace icu;
 
@@ -464,7 +465,9 @@ struct Engine {
     if (any_table && !h_items.empty()) launch_decide_parallel(d_decide, d_items, (int)h_items.size(), st);
     for (size_t i = 0; i < sp.size(); i++) {
       if (!sp[i].cfg.predictor) continue;
-      launch_fill_i32(d_wave + wave_off[i], 0, sp[i].mbh + 2, st);
+      // progress counters and the row ticket; the error flag after them
+      // stays set until check_wave_errors reads it
+      launch_fill_i32(d_wave + wave_off[i], 0, sp[i].mbh + 1, st);
       launch_decide_wavefront(d_decide + i, sp[i].table ? 2 * sp[i].L + 1 : 0, sp[i].mbh, st);
     }
     if (ev) HIP_TRY(hipEventRecord(ev[3], st));
@@ -476,14 +479,21 @@ struct Engine {
     return ICU_OK;
   }
 
+  // Reads (and clears) the median-predictor wavefronts' error flags; the
+  // device work that may set them must have finished.
   int check_wave_errors() {
+    bool timed_out = false;
     for (size_t i = 0; i < sp.size(); i++) {
       if (!sp[i].cfg.predictor) continue;
-      int flags[2];
-      HIP_TRY(hipMemcpy(flags, d_wave + wave_off[i] + sp[i].mbh, sizeof(flags), hipMemcpyDeviceToHost));
-      if (flags[1]) return fail(ICU_E_STATE, "median-predictor wavefront timed out");
+      int* flag = d_wave + wave_off[i] + sp[i].mbh + 1;
+      int v = 0;
+      HIP_TRY(hipMemcpy(&v, flag, sizeof(v), hipMemcpyDeviceToHost));
+      if (v) {
+        timed_out = true;
+        HIP_TRY(hipMemset(flag, 0, sizeof(int)));
+      }
     }
-    return ICU_OK;
+    return timed_out ? fail(ICU_E_STATE, "median-predictor wavefront timed out") : ICU_OK;
   }
 
   int retile(int si, Tiling t) {
@@ -520,6 +530,14 @@ struct icu_me {
 struct icu_me_batch {
   Engine eng;
 };
+
+int* icu::me_wave_flag(icu_me* h) {
+  const Engine& e = h->eng;
+  if (!e.sp[0].cfg.predictor) return nullptr;
+  return e.d_wave + e.wave_off[0] + e.sp[0].mbh + 1;
+}
+
+int icu::me_check(icu_me* h) { return h->eng.check_wave_errors(); }
 
 long long icu::me_device_bytes(const icu_me* h) {
   const long long px = (long long)h->eng.sp[0].cfg.width * h->eng.sp[0].cfg.height;
@@ -650,6 +668,22 @@ int icu_me_estimate(icu_me* h, const int16_t* cur, const int16_t* ref, const uin
         }
     *avg_out = inc ? total / inc : 0;
   }
+  return ICU_OK;
+}
+
+int icu_me_check(icu_me* h) {
+  if (!h) return fail(ICU_E_ARG, "null handle");
+  HIP_TRY(hipDeviceSynchronize());
+  return h->eng.check_wave_errors();
+}
+
+int icu_me_debug_set_wave_error(icu_me* h) {
+  if (!h) return fail(ICU_E_ARG, "null handle");
+  int* flag = icu::me_wave_flag(h);
+  if (!flag) return fail(ICU_E_STATE, "the handle has no median-predictor wavefront");
+  const int one = 1;
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(flag, &one, sizeof(one), hipMemcpyHostToDevice));
   return ICU_OK;
 }
 
@@ -787,26 +821,56 @@ int icu_inv4x4_device(const int16_t* d_in, int16_t* d_out, long n, int qp, int m
   return ICU_OK;
 }
 
+// Host buffers.  Up to kHostTxBlocks blocks (a per-block codec call is
+// one) run on the CPU with the kernels' own arithmetic (icu_tx.h): a device
+// round trip would cost allocations, two copies and a launch per call.
+// Larger batches go through the device kernels on buffers freed on every
+// path out.
+constexpr long kHostTxBlocks = 16384;
+
+namespace {
+struct DeviceScratch {
+  void* p = nullptr;
+  ~DeviceScratch() {
+    if (p) hipFree(p);
+  }
+};
+
+void tx_cpu(const int16_t* in, int16_t* out, long n, int qp, int intra, int mode, int layout, int inverse,
+            int cols_first, const int* w) {
+  for (long blk = 0; blk < n; blk++) {
+    int16_t b[16];
+    for (int p = 0; p < 16; p++) b[p] = in[icu::tx::elem_off(blk, p, layout)];
+    if (inverse) icu::tx::inv4x4(b, qp, mode, cols_first, w);
+    else icu::tx::fwd4x4(b, qp, intra, mode);
+    for (int p = 0; p < 16; p++) out[icu::tx::elem_off(blk, p, layout)] = b[p];
+  }
+}
+}  // namespace
+
 static int tx_host(const int16_t* in, int16_t* out, long n, int qp, int intra, int mode, int layout,
                    int inverse, int cols_first, const int* w) {
   if (!in || !out) return fail(ICU_E_ARG, "null argument");
   int rc = check_tx(n, qp, mode, layout);
   if (rc || n == 0) return rc;
+  if (n <= kHostTxBlocks) {
+    tx_cpu(in, out, n, qp, intra, mode, layout, inverse, cols_first, w);
+    return ICU_OK;
+  }
   const size_t bytes = (size_t)n * 16 * sizeof(int16_t);
-  int16_t *di = nullptr, *dout = nullptr;
-  int* dw = nullptr;
-  HIP_TRY(hipMalloc(&di, bytes));
-  HIP_TRY(hipMalloc(&dout, bytes));
-  if (w) HIP_TRY(hipMalloc(&dw, 16 * sizeof(int)));
-  HIP_TRY(hipMemcpy(di, in, bytes, hipMemcpyHostToDevice));
-  if (w) HIP_TRY(hipMemcpy(dw, w, 16 * sizeof(int), hipMemcpyHostToDevice));
-  if (inverse) launch_inv4x4(di, dout, n, qp, mode, layout, cols_first ? 1 : 0, dw, nullptr);
-  else launch_fwd4x4(di, dout, n, qp, intra ? 1 : 0, mode, layout, nullptr);
+  DeviceScratch di, dout, dw;
+  HIP_TRY(hipMalloc(&di.p, bytes));
+  HIP_TRY(hipMalloc(&dout.p, bytes));
+  if (w) HIP_TRY(hipMalloc(&dw.p, 16 * sizeof(int)));
+  HIP_TRY(hipMemcpy(di.p, in, bytes, hipMemcpyHostToDevice));
+  if (w) HIP_TRY(hipMemcpy(dw.p, w, 16 * sizeof(int), hipMemcpyHostToDevice));
+  if (inverse)
+    launch_inv4x4((const int16_t*)di.p, (int16_t*)dout.p, n, qp, mode, layout, cols_first ? 1 : 0,
+                  (const int*)dw.p, nullptr);
+  else
+    launch_fwd4x4((const int16_t*)di.p, (int16_t*)dout.p, n, qp, intra ? 1 : 0, mode, layout, nullptr);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost));
-  hipFree(di);
-  hipFree(dout);
-  hipFree(dw);
+  HIP_TRY(hipMemcpy(out, dout.p, bytes, hipMemcpyDeviceToHost));
   return ICU_OK;
 }
 

@@ -13,6 +13,12 @@ namespace icu {
 
 // Device bytes an estimator handle holds.
 long long me_device_bytes(const ::icu_me* h);
+// Device address of the handle's median-predictor wavefront error flag (one
+// int, nonzero after a bounded wait ran out), or nullptr without a predictor.
+int* me_wave_flag(::icu_me* h);
+// Reads and clears that flag (the work that sets it must be done):
+// ICU_E_STATE if it was set.
+int me_check(::icu_me* h);
 
 // Records msg as the calling thread's last error and returns code.
 int icu_fail(int code, const std::string& msg);

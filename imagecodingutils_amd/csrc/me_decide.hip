@@ -74,13 +74,17 @@ __global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* 
     if (row > 0) {
       // above-right (or above for the last column) must be final
       const int need = min(col + 2, d.mbw);
+      // Bounded: a wait that runs out sets the error flag (ticket[1]), which
+      // stays set until the host reads it; every other wait then gives up at
+      // once instead of spinning its own full limit.
       if (lane == 0) {
         int spins = 0;
         while (poll_progress(&progress[row - 1]) < need) {
           if (++spins > kSpinLimit) {
-            atomicExch(&ticket[1], 1);
+            __hip_atomic_store(&ticket[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
           }
+          if ((spins & 63) == 0 && poll_progress(&ticket[1])) break;
           __builtin_amdgcn_s_sleep(1);
         }
       }

@@ -96,7 +96,7 @@ struct Flight {
   uint8_t* d_cur;
   int16_t* d_mv;
   int32_t* d_dist;
-  void* h_out;  // pinned: vectors (4 B per MB) then the two checksums
+  void* h_out;  // pinned: vectors (4 B per MB), the two checksums, the wavefront error flag
   long long transient;
 };
 
@@ -132,6 +132,8 @@ struct icu_server {
     for (hipStream_t s : lanes) hipStreamDestroy(s);
   }
 
+  static size_t out_bytes(const SrvStream& s) { return 4 * (size_t)s.nmb + 24; }
+
   static long long transient_bytes(const SrvStream& s) {
     return (long long)s.W * s.H + 8LL * s.nmb;  // current plane, vectors, distortions
   }
@@ -149,7 +151,7 @@ struct icu_server {
     s.last[0] = f.d_cur;
     s.last[1] = f.d_mv;
     s.last[2] = f.d_dist;
-    f.h_out = host.get(4 * (size_t)s.nmb + 16);
+    f.h_out = host.get(out_bytes(s));
     if (!f.d_cur || !f.d_mv || !f.d_dist || !f.h_out) return icu::icu_fail(ICU_E_DEVICE, "server buffer allocation failed");
     if (events.empty()) {
       hipEvent_t e;
@@ -173,8 +175,14 @@ struct icu_server {
     icu::launch_fill_i32((int*)s.d_sums, 0, 4, st);
     icu::launch_checksum(s.d_levels, 2 * (long)px, s.d_sums, st);
     icu::launch_checksum(s.d_recon, (long)px, s.d_sums + 1, st);
+    // median-predictor streams: the wavefront's error flag comes back with
+    // the results (it stays set on the device until read)
+    int* flag = icu::me_wave_flag(s.h);
+    int* h_flag = (int*)((char*)f.h_out + 4 * (size_t)s.nmb + 16);
+    *h_flag = 0;
     if (hipMemcpyAsync(f.h_out, f.d_mv, 4 * (size_t)s.nmb, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipMemcpyAsync((char*)f.h_out + 4 * (size_t)s.nmb, s.d_sums, 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        (flag && hipMemcpyAsync(h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess) ||
         hipEventRecord(f.ev, st) != hipSuccess)
       return icu::icu_fail(ICU_E_DEVICE, "result download failed");
     s.queue.pop_front();
@@ -215,6 +223,7 @@ struct icu_server {
       icu_server_result& r = out[(*n)++];
       out_mv.emplace_back((const int16_t*)f.h_out, (const int16_t*)f.h_out + 2 * (size_t)s.nmb);
       const unsigned long long* sums = (const unsigned long long*)((const char*)f.h_out + 4 * (size_t)s.nmb);
+      const int wave_error = *(const int*)((const char*)f.h_out + 4 * (size_t)s.nmb + 16);
       r.tag = f.p.tag;
       r.stream = f.stream;
       r.frame = f.p.frame;
@@ -230,11 +239,21 @@ struct icu_server {
       dev.put(f.d_cur, (size_t)s.W * s.H);
       dev.put(f.d_mv, 4 * (size_t)s.nmb);
       dev.put(f.d_dist, 4 * (size_t)s.nmb);
-      host.put(f.h_out, 4 * (size_t)s.nmb + 16);
+      host.put(f.h_out, out_bytes(s));
       host.put(f.p.h_frame, (size_t)s.W * s.H);
       bytes -= f.transient;
       events.push_back(f.ev);
+      const int stream = f.stream, frame = f.p.frame;
       inflight.erase(inflight.begin() + i);
+      if (wave_error) {
+        // the frame's vectors used neighbours a timed-out wait never saw:
+        // report it (me_check clears the device flag)
+        (*n)--;
+        out_mv.pop_back();
+        icu::me_check(s.h);
+        return icu::icu_fail(ICU_E_STATE, "median-predictor wavefront timed out (stream " +
+                                              std::to_string(stream) + ", frame " + std::to_string(frame) + ")");
+      }
     }
     return ICU_OK;
   }
@@ -332,6 +351,13 @@ int icu_server_poll(icu_server* s, icu_server_result* out, int max_out, int wait
     if (!rc) rc = s->admit();
   }
   return rc ? rc : n;
+}
+
+int icu_server_stream_estimator(icu_server* s, int stream_id, icu_me** out) {
+  if (!s || !out) return icu::icu_fail(ICU_E_ARG, "null argument");
+  if (stream_id < 0 || stream_id >= (int)s->streams.size()) return icu::icu_fail(ICU_E_ARG, "bad stream id");
+  *out = s->streams[stream_id].h;
+  return ICU_OK;
 }
 
 int icu_server_stats(icu_server* s, long long* bytes_in_use, long long* peak_bytes, int* queued,

@@ -164,3 +164,25 @@ class StreamSource:
             dy[py:py + rh, px:px + rw] = t * vy
         noise = np.random.default_rng([self.seed, t]).normal(0.0, self.sigma, (h, w))
         return _to_u8(self.tex.eval(xs - dx, ys - dy) + noise)
+
+
+# The mixed batched workload: 24 QCIF + 20 CIF + 12 720p + 8 1080p frame
+# pairs, each stream with its own texture and 8..24 moving rectangles, and a
+# search range drawn from {8, 16, 32, 64} (motion bounded by min(R, 32)).
+BATCH_PLAN = (("QCIF", 24), ("CIF", 20), ("720p", 12), ("1080p", 8))
+
+
+def mixed_batch(seed: int = 2026) -> list[tuple[int, int, int, FramePair]]:
+    """(width, height, R, frame pair) per stream, in the plan's order."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for name, count in BATCH_PLAN:
+        w, h = SIZES[name]
+        h -= h % 16
+        for _ in range(count):
+            R = int(rng.choice([8, 16, 32, 64]))
+            sd = int(rng.integers(1 << 30))
+            fp = rect_motion_pair(w, h, sd, n_rects=int(rng.integers(8, 25)), max_mv=min(R, 32))
+            out.append((w, h, R, fp))
+    return out
+

@@ -30,6 +30,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "icu_hip.h"
@@ -62,17 +63,16 @@ class VectorStructList {
 };
 
 // ------------------------------------------------------------- predictors
+// The reference interface, member for member
+// (include/CodecUtils/IMotionVectorPredictor.h:51-83).
 class IMotionVectorPredictor {
  public:
   virtual ~IMotionVectorPredictor() {}
   virtual int Create(void) = 0;
   virtual int Get16x16Prediction(void* pList, int blk, int* predX, int* predY) = 0;
-  virtual void Set16x16MotionVector(int blk, int mvX, int mvY) {}
-  virtual void Set16x16MotionVector(int blk, int mvX, int mvY, int distortion) { Set16x16MotionVector(blk, mvX, mvY); }
   virtual int Get16x16Prediction(void* pList, int blk, int* predX, int* predY, int* distortion) { return 0; }
-  // 1 when the predictor is the H.264 16x16 median of A/B/C(D), which the
-  // device evaluates itself as a wavefront; 0 for the zero vector.
-  virtual int IsH264Median(void) const = 0;
+  virtual void Set16x16MotionVector(int blk, int mvX, int mvY) {}
+  virtual void Set16x16MotionVector(int blk, int mvX, int mvY, int distortion) {}
 };
 
 // H.264 median predictor over the MBs of one picture (one slice).  The
@@ -84,7 +84,6 @@ class H264MotionVectorPredictorImpl1 : public IMotionVectorPredictor {
   H264MotionVectorPredictorImpl1(int mbCols, int mbRows)
       : _w(mbCols), _h(mbRows), _mv(2 * (size_t)mbCols * mbRows, 0), _d((size_t)mbCols * mbRows, 0) {}
   int Create(void) override { return 1; }
-  int IsH264Median(void) const override { return 1; }
   void Set16x16MotionVector(int blk, int mvX, int mvY) override {
     _mv[2 * (size_t)blk] = mvX;
     _mv[2 * (size_t)blk + 1] = mvY;
@@ -157,13 +156,22 @@ class H264MotionVectorPredictorImpl1 : public IMotionVectorPredictor {
 class ZeroMotionVectorPredictor : public IMotionVectorPredictor {
  public:
   int Create(void) override { return 1; }
-  int IsH264Median(void) const override { return 0; }
   int Get16x16Prediction(void*, int, int* predX, int* predY) override {
     *predX = 0;
     *predY = 0;
     return 1;
   }
 };
+
+// The predictors the device reproduces, by the predictor object's type: 1
+// the H.264 median (H264MotionVectorPredictorImpl1), 0 the zero vector
+// (ZeroMotionVectorPredictor), -1 anything else (the estimators' Create()
+// then fails rather than predict differently from the object it was given).
+inline int DevicePredictorKind(IMotionVectorPredictor* p) {
+  if (dynamic_cast<H264MotionVectorPredictorImpl1*>(p)) return 1;
+  if (dynamic_cast<ZeroMotionVectorPredictor*>(p)) return 0;
+  return -1;
+}
 
 // ---------------------------------------------------------------- estimator
 class IMotionEstimator {
@@ -176,6 +184,7 @@ class IMotionEstimator {
   virtual int GetMode(void) = 0;
   virtual void* Estimate(const void* pSrc, const void* pRef, long* avgDistortion) = 0;
   virtual void* Estimate(long* avgDistortion) = 0;
+  virtual void* Estimate(long* avgDistortion, void* param) { return Estimate(avgDistortion); }
 };
 
 // The full-search estimator.  pSrc / pRef are int16 planes of 8-bit
@@ -200,11 +209,20 @@ class MotionEstimatorH264ImplFull : public IMotionEstimator {
 
   int Create(void) override {
     Destroy();
+    _err.clear();
+    const int kind = DevicePredictorKind(_pred);
+    if (kind < 0) {
+      // the reference dereferences the predictor (Full.cpp:368)
+      _err = _pred ? "predictor type not reproduced on the device (H264MotionVectorPredictorImpl1 or "
+                     "ZeroMotionVectorPredictor)"
+                   : "null motion vector predictor";
+      return 0;
+    }
     if (!_have_modes) icu_me_config_reference(&_cfg, _w, _h, _range);
     _cfg.width = _w;
     _cfg.height = _h;
     _cfg.range = _range / 4;
-    _cfg.predictor = (_pred && _pred->IsH264Median()) ? 1 : 0;
+    _cfg.predictor = kind;
     if (icu_me_create(&_cfg, &_h_me) != ICU_OK) return 0;
     _list.SetLength((_w / 16) * (_h / 16));
     _ready = 1;
@@ -235,7 +253,7 @@ class MotionEstimatorH264ImplFull : public IMotionEstimator {
     if (avgDistortion) *avgDistortion = avg;
     return (void*)&_list;
   }
-  const char* LastError(void) const { return icu_last_error(); }
+  const char* LastError(void) const { return _err.empty() ? icu_last_error() : _err.c_str(); }
 
  private:
   void Destroy(void) {
@@ -253,6 +271,7 @@ class MotionEstimatorH264ImplFull : public IMotionEstimator {
   icu_me* _h_me = nullptr;
   VectorStructList _list;
   int _ready = 0, _mode = 1;
+  std::string _err;
 };
 
 // ---------------------------------------------------------- fast estimators
@@ -264,7 +283,8 @@ class MotionEstimatorH264ImplFull : public IMotionEstimator {
 // distortion, which the device evaluates itself (pass an
 // H264MotionVectorPredictorImpl1; the decided vectors and distortions are
 // written back to it, as the reference does); Cross also takes
-// ZeroMotionVectorPredictor.  UMHS's previous-frame macroblocks are given
+// ZeroMotionVectorPredictor.  Create() fails for any other predictor
+// object.  UMHS's previous-frame macroblocks are given
 // as their 16x16 vectors and intra flags instead of MacroBlockH264s.  A
 // mode change takes effect at the next Estimate() with a new device handle
 // (for FHS, a fresh half-sample cache).
@@ -278,7 +298,9 @@ class FastEstimatorBase : public IMotionEstimator {
     Release();
     _list.SetLength((_w / 16) * (_h / 16));
     _dist.assign((size_t)(_w / 16) * (_h / 16), 0);
-    _ready = (_w % 16 == 0 && _h % 16 == 0 && _w > 0 && _h > 0) ? 1 : 0;
+    const int kind = DevicePredictorKind(_pred);
+    _ready = (_w % 16 == 0 && _h % 16 == 0 && _w > 0 && _h > 0 && (kind == 1 || (kind == 0 && ZeroPredictorOk())))
+                 ? 1 : 0;
     return _ready;
   }
   void Reset(void) override {}
@@ -313,6 +335,8 @@ class FastEstimatorBase : public IMotionEstimator {
   virtual int Make(int mode) = 0;
   virtual int Run(const uint8_t* include, long* avg) = 0;
   virtual void Release(void) = 0;
+  // whether the estimator also runs from the zero predictor
+  virtual bool ZeroPredictorOk(void) const { return false; }
   const int16_t* _src;
   const int16_t* _ref;
   int _w, _h, _range;
@@ -358,9 +382,8 @@ class MotionEstimatorH264ImplCross : public FastEstimatorBase {
   ~MotionEstimatorH264ImplCross() override { Release(); }
 
  protected:
-  int Make(int mode) override {
-    return icu_cross_create(_w, _h, _range, mode, (_pred && _pred->IsH264Median()) ? 1 : 0, &_hd);
-  }
+  int Make(int mode) override { return icu_cross_create(_w, _h, _range, mode, DevicePredictorKind(_pred), &_hd); }
+  bool ZeroPredictorOk(void) const override { return true; }
   int Run(const uint8_t* include, long* avg) override {
     return icu_cross_estimate(_hd, _src, _ref, include, _list.Data(), _dist.data(), avg);
   }
@@ -436,7 +459,9 @@ class IInverseTransform {
 // Forward 4x4 IT + quant.  Layout 0 (one 4x4 block, 16 shorts per call) is
 // FastForward4x4ITImpl1; layout 1 (a 16x16 raster MB) is
 // FastForward4x4On16x16ITImpl1.  TransformBlocks(ptr, n) is the batched
-// form: n calls' worth in one launch.
+// form: n calls' worth in one call.  Per-block calls run on the CPU with the
+// device kernels' arithmetic (icu_fwd4x4's small-batch path); batches of
+// more than 16384 blocks go to the device.
 template <int LAYOUT>
 class FastForward4x4ITHip : public IForwardTransform {
  public:
@@ -465,6 +490,9 @@ class FastForward4x4ITHip : public IForwardTransform {
   int _mode = TransformAndQuant, _intra = 1, _q = 1;
 };
 typedef FastForward4x4ITHip<0> FastForward4x4ITImpl1;
+// The reference's unrolled variant (src/CodecUtils/FastForward4x4ITImpl2.cpp:110-450)
+// gives Impl1's output in every mode, so it is the same class here.
+typedef FastForward4x4ITHip<0> FastForward4x4ITImpl2;
 typedef FastForward4x4ITHip<1> FastForward4x4On16x16ITImpl1;
 
 // Dequant + inverse 4x4 IT.  Layout 0 / rows first is FastInverse4x4ITImpl1;

@@ -92,6 +92,20 @@ int icu_me_estimate_device(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref
 int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int16_t* d_mv,
                           int32_t* d_dist, void* stream, int search_reps, float* stage_ms);
 
+/* Waits for the device and reports whether a median-predictor wavefront of
+ * this handle ran out its bounded wait for a neighbour row since the last
+ * check (ICU_E_STATE; its vectors are then not the reference's).  The flag
+ * stays set on the device until read here (icu_me_estimate reads it too);
+ * while it is set, later wavefronts of the handle stop waiting as well.
+ * Asynchronous callers of icu_me_estimate_device with the median predictor
+ * call this after their results are complete.  No reference counterpart
+ * (the reference's raster loop cannot time out). */
+int icu_me_check(icu_me* h);
+/* Test hook: sets the handle's wavefront error flag from the host, so the
+ * reporting paths can be exercised without stalling a GPU.  ICU_E_STATE for
+ * a handle without the median predictor.  No reference counterpart. */
+int icu_me_debug_set_wave_error(icu_me* h);
+
 /* Tile choice in effect for this handle (after tuning): candidate rows per
  * lane, MBs per workgroup task, and the search grid mode: 0 one task per
  * workgroup, 1 persistent (as many workgroups as stay resident claim tasks
@@ -129,7 +143,10 @@ int icu_code_frame_qps_device(icu_me* h, const uint8_t* d_cur, const int16_t* d_
 /* Batched 4x4 transforms on host buffers.  layout 0: contiguous 4x4 blocks
  * (16 int16 each, raster within the block); layout 1: 16x16 raster MBs of
  * 16 blocks (the FastForward4x4On16x16ITImpl1 layout).  mode 0 IT+Q, 1 IT
- * only, 2 Q only (IForwardTransform::SetMode).
+ * only, 2 Q only (IForwardTransform::SetMode).  Up to 16384 blocks are
+ * transformed on the CPU with the device kernels' arithmetic (a per-block
+ * codec call must not pay a device round trip); larger batches run on the
+ * device.  in and out may be the same buffer.
  * icu_fwd4x4 replaces IForwardTransform::Transform + SetParameter(QUANT_ID,
  *   INTRA_FLAG_ID) (include/CodecUtils/IForwardTransform.h:56-126,
  *   src/CodecUtils/FastForward4x4ITImpl1.cpp:100-278).
@@ -383,6 +400,11 @@ int icu_server_submit(icu_server* s, int stream_id, const uint8_t* frame, long l
  * least one frame finishes when any is in flight.  Returns the number
  * collected (>= 0) or an error code. */
 int icu_server_poll(icu_server* s, icu_server_result* out, int max_out, int wait);
+/* The estimator handle of a stream (owned by the server, valid until
+ * icu_server_destroy); calls on it race with the stream's frames in flight.
+ * A poll that collects a median-predictor stream's frame whose wavefront
+ * timed out (icu_me_check) returns ICU_E_STATE and drops that result. */
+int icu_server_stream_estimator(icu_server* s, int stream_id, icu_me** out);
 int icu_server_stats(icu_server* s, long long* bytes_in_use, long long* peak_bytes, int* queued,
                      int* inflight);
 /* Seconds on the server's monotonic clock. */

@@ -252,7 +252,16 @@ int orc_me_estimate(const orc_me_cfg *cfg, const int16_t *cur, const int16_t *re
 int orc_me_estimate_subset(const orc_me_cfg *cfg, const int16_t *cur, const int16_t *refp,
                            const uint8_t *include, int step, int16_t *mv_out, int32_t *dist_out,
                            int32_t *int_out, long *avg_out) {
-  if (step < 1 || (step > 1 && cfg->pred)) return -3;
+  return orc_me_estimate_phase(cfg, cur, refp, include, step, 0, mv_out, dist_out, int_out, avg_out);
+}
+
+/* Only the MBs whose raster index is phase mod step (0 <= phase < step).
+ * Disjoint phases write disjoint outputs, so calls over the phases of one
+ * step may run on separate threads into the same arrays (avg_out NULL). */
+int orc_me_estimate_phase(const orc_me_cfg *cfg, const int16_t *cur, const int16_t *refp,
+                          const uint8_t *include, int step, int phase, int16_t *mv_out,
+                          int32_t *dist_out, int32_t *int_out, long *avg_out) {
+  if (step < 1 || (step > 1 && cfg->pred) || phase < 0 || phase >= step) return -3;
   const int W = cfg->width, H = cfg->height, R = cfg->range;
   if (W <= 0 || H <= 0 || (W & 15) || (H & 15) || R < 1) return -1;
   const int mbw = W / 16, mbh = H / 16;
@@ -267,7 +276,7 @@ int orc_me_estimate_subset(const orc_me_cfg *cfg, const int16_t *cur, const int1
   for (int row = 0; row < mbh; row++)
     for (int col = 0; col < mbw; col++) {
       const int n = col * 16, m = row * 16, vec = row * mbw + col;
-      if (vec % step) continue;
+      if (vec % step != phase) continue;
       int predX = 0, predY = 0;
       if (cfg->pred) orc_median_pred(mv_out, mbw, col, row, &predX, &predY);
       int predXQ = predX % 4, predYQ = predY % 4;

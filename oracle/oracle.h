@@ -30,6 +30,9 @@ int orc_me_estimate(const orc_me_cfg *cfg, const int16_t *cur, const int16_t *re
 int orc_me_estimate_subset(const orc_me_cfg *cfg, const int16_t *cur, const int16_t *ref,
                            const uint8_t *include, int step, int16_t *mv_out, int32_t *dist_out,
                            int32_t *int_out, long *avg_out);
+int orc_me_estimate_phase(const orc_me_cfg *cfg, const int16_t *cur, const int16_t *ref,
+                          const uint8_t *include, int step, int phase, int16_t *mv_out,
+                          int32_t *dist_out, int32_t *int_out, long *avg_out);
 int orc_qpel_sample(const int16_t *ref, int W, int H, int x, int y, int fx, int fy);
 int orc_block_dist(const int16_t *a256, const int16_t *b256, int metric);
 void orc_quarter_read(const int16_t *ref, int W, int H, int x, int y, int qx, int qy,

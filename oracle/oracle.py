@@ -14,7 +14,8 @@ from pathlib import Path
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "liboracle.so"
+# ICU_ORACLE_LIB: another build of the same sources (the sanitizer build)
+LIB = Path(os.environ.get("ICU_ORACLE_LIB", HERE / "liboracle.so")).resolve()
 REF_LIB = HERE / "_ref" / "libvppref.so"
 
 _i16p = np.ctypeslib.ndpointer(np.int16, flags="C_CONTIGUOUS")
@@ -31,6 +32,8 @@ AS_SHIPPED = dict(metric=0, window=0, filter=1, subpel=2, pred=1, bestvec=1)
 
 
 def build(force: bool = False) -> None:
+    if "ICU_ORACLE_LIB" in os.environ:
+        return
     if force or not LIB.exists():
         subprocess.run(["make", "-s", "-C", str(HERE), "all"], check=True)
 
@@ -58,6 +61,9 @@ def lib():
         L.orc_me_estimate.restype = C.c_int
         L.orc_me_estimate_subset.argtypes = [C.POINTER(MeCfg), _i16p, _i16p, C.c_void_p, C.c_int,
                                              _i16p, C.c_void_p, C.c_void_p, C.POINTER(C.c_long)]
+        L.orc_me_estimate_phase.argtypes = [C.POINTER(MeCfg), _i16p, _i16p, C.c_void_p, C.c_int, C.c_int,
+                                            _i16p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_me_estimate_phase.restype = C.c_int
         L.orc_quarter_read.argtypes = [_i16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_int, _i32p]
         L.orc_fwd4x4_n.argtypes = [_i16p, C.c_long, C.c_int, C.c_int, C.c_int]
@@ -142,6 +148,43 @@ def me_estimate(cur, ref, R, include=None, step=1, **modes):
     if rc != 0:
         raise ValueError(f"oracle me_estimate failed ({rc})")
     return mv.reshape(n, 2), dist, integ.reshape(n, 3), avg.value
+
+
+def me_estimate_par(cur, ref, R, step=1, threads=None, **modes):
+    """me_estimate over every step-th MB (zero predictor), the MBs split
+    over threads (the C calls release the GIL and write disjoint MBs).
+    Returns (mv, dist, integer stage) with the undecided MBs zero."""
+    import threading
+
+    cur = np.ascontiguousarray(cur, dtype=np.int16)
+    ref = np.ascontiguousarray(ref, dtype=np.int16)
+    H, W = cur.shape
+    m = dict(AS_SHIPPED)
+    m.update(modes)
+    if m["pred"]:
+        raise ValueError("parallel oracle needs the zero predictor")
+    cfg = MeCfg(W, H, R, m["metric"], m["window"], m["filter"], m["subpel"], m["pred"], m["bestvec"])
+    n = (W // 16) * (H // 16)
+    mv = np.zeros(2 * n, np.int16)
+    dist = np.zeros(n, np.int32)
+    integ = np.zeros(3 * n, np.int32)
+    T = threads or max(1, min(16, len(os.sched_getaffinity(0))))
+    T = max(1, min(T, (n + step - 1) // step))
+    rcs = [0] * T
+
+    def work(t):
+        rcs[t] = lib().orc_me_estimate_phase(C.byref(cfg), cur, ref, None, step * T, step * t, mv,
+                                             dist.ctypes.data, integ.ctypes.data, None)
+
+    lib()
+    ths = [threading.Thread(target=work, args=(t,)) for t in range(T)]
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    if any(rcs):
+        raise ValueError(f"oracle me_estimate failed ({rcs})")
+    return mv.reshape(n, 2), dist, integ.reshape(n, 3)
 
 
 def fwd4x4(blocks, qp, intra, mode):

@@ -2,6 +2,7 @@
 // classes (construct, Create, Estimate, read the VectorStructList; transform
 // objects with SetParameter / SetMode / Transform in place).  Inputs and
 // outputs are raw files; tests/test_gpu_dropin.py compares them.
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -239,6 +240,93 @@ int main(int argc, char** argv) {
     std::fclose(fo);
     return 0;
   }
-  std::fprintf(stderr, "usage: dropin_test me|fwd|inv|mc|cavlc|fast ...\n");
+  if (cmd == "api") {
+    // The interface points a codec relies on beyond the fixtures: which
+    // predictors Create() accepts, the Estimate(avg, param) overload, the
+    // Impl2 transform name.
+    const int W = 64, H = 48;
+    std::vector<short> cur((size_t)W * H), ref((size_t)W * H);
+    for (int y = 0; y < H; y++)
+      for (int x = 0; x < W; x++) {
+        ref[(size_t)y * W + x] = (short)((x * 7 + y * 13 + ((x * y) % 11)) & 255);
+        cur[(size_t)y * W + x] = ref[(size_t)y * W + ((x + 3) % W)];
+      }
+    struct OtherPredictor : IMotionVectorPredictor {
+      int Create(void) override { return 1; }
+      int Get16x16Prediction(void*, int, int* x, int* y) override {
+        *x = 4;
+        *y = 4;
+        return 1;
+      }
+    } other;
+    MotionEstimatorH264ImplFull none(cur.data(), ref.data(), W, H, 32, nullptr);
+    if (none.Create()) return 10;
+    MotionEstimatorH264ImplFull odd(cur.data(), ref.data(), W, H, 32, &other);
+    if (odd.Create() || std::string(odd.LastError()).find("predictor") == std::string::npos) return 11;
+    ZeroMotionVectorPredictor zero;
+    MotionEstimatorH264ImplFull me(cur.data(), ref.data(), W, H, 32, &zero);
+    if (!me.Create()) return 12;
+    IMotionEstimator* ime = &me;
+    long a1 = -1, a2 = -1;
+    VectorStructList* l1 = (VectorStructList*)ime->Estimate(&a1);
+    std::vector<int> v1;
+    for (int i = 0; l1 && i < l1->GetLength(); i++) v1.push_back(l1->GetSimpleElement(i, 0) * 65536 + l1->GetSimpleElement(i, 1));
+    VectorStructList* l2 = (VectorStructList*)ime->Estimate(&a2, (void*)&other);
+    if (!l1 || !l2 || a1 != a2) return 13;
+    for (int i = 0; i < l2->GetLength(); i++)
+      if (v1[i] != l2->GetSimpleElement(i, 0) * 65536 + l2->GetSimpleElement(i, 1)) return 14;
+    // UMHS / FHS start from the median with distortion; Cross also from zero
+    MotionEstimatorH264ImplUMHS um(cur.data(), ref.data(), W, H, 32, &zero);
+    MotionEstimatorH264ImplFHS fh(cur.data(), ref.data(), W, H, 32, &other);
+    MotionEstimatorH264ImplCross cr(cur.data(), ref.data(), W, H, 32, &zero);
+    MotionEstimatorH264ImplCross cr2(cur.data(), ref.data(), W, H, 32, &other);
+    if (um.Create() || fh.Create() || !cr.Create() || cr2.Create()) return 15;
+    // FastForward4x4ITImpl2 is Impl1's transform in every mode
+    for (int mode = 0; mode < 3; mode++)
+      for (int qp = 0; qp < 52; qp += 3) {
+        short b1[16], b2[16];
+        for (int k = 0; k < 16; k++) b1[k] = b2[k] = (short)(((k * 37 + qp * 11 + mode * 5) % 511) - 255);
+        FastForward4x4ITImpl1 t1;
+        FastForward4x4ITImpl2 t2;
+        IForwardTransform* ft = &t2;
+        t1.SetParameter(IForwardTransform::QUANT_ID, qp);
+        ft->SetParameter(IForwardTransform::QUANT_ID, qp);
+        t1.SetMode(mode);
+        ft->SetMode(mode);
+        t1.Transform(b1);
+        ft->Transform(b2);
+        if (std::memcmp(b1, b2, sizeof(b1))) return 16;
+      }
+    std::printf("ok\n");
+    return 0;
+  }
+  if (cmd == "txtime" && argc >= 3) {
+    // txtime n: n single-block Transform() calls, then n InverseTransform()
+    // calls, as the reference's BlockH264 makes them (one per 4x4 block),
+    // checked against the batched call; prints ns per call.
+    const long n = std::atol(argv[2]);
+    std::vector<short> x((size_t)n * 16), y;
+    for (size_t i = 0; i < x.size(); i++) x[i] = (short)(((i * 2654435761u) >> 7) % 511) - 255;
+    y = x;
+    std::vector<short> z = x;
+    FastForward4x4ITImpl1 f;
+    FastInverse4x4ITImpl1 inv;
+    f.SetParameter(IForwardTransform::QUANT_ID, 28);
+    f.SetParameter(IForwardTransform::INTRA_FLAG_ID, 0);
+    inv.SetParameter(IInverseTransform::QUANT_ID, 28);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (long i = 0; i < n; i++) f.Transform(y.data() + 16 * i);
+    const auto t1 = std::chrono::steady_clock::now();
+    for (long i = 0; i < n; i++) inv.InverseTransform(y.data() + 16 * i);
+    const auto t2 = std::chrono::steady_clock::now();
+    f.TransformBlocks(z.data(), n);
+    inv.InverseTransformBlocks(z.data(), n);
+    if (y != z) return 20;
+    const double fns = std::chrono::duration<double, std::nano>(t1 - t0).count() / n;
+    const double ins = std::chrono::duration<double, std::nano>(t2 - t1).count() / n;
+    std::printf("fwd_ns_per_call %.1f inv_ns_per_call %.1f\n", fns, ins);
+    return 0;
+  }
+  std::fprintf(stderr, "usage: dropin_test me|fwd|inv|mc|cavlc|fast|api|txtime ...\n");
   return 1;
 }

@@ -132,3 +132,21 @@ def test_dropin_fast_estimators_match_reference(tmp_path):
         mv, avg = _run_fast(tmp_path, "fhs", curs, refs, qr, mode, 1, ff[f"c{k}_include"])
         np.testing.assert_array_equal(mv, ff[f"c{k}_mv"])
         np.testing.assert_array_equal(avg, ff[f"c{k}_avg"])
+
+
+def test_dropin_interface_points():
+    """Create() accepts the predictors the device reproduces and refuses
+    others (and a null one), Estimate(avg, param) equals Estimate(avg), and
+    FastForward4x4ITImpl2 is Impl1's transform."""
+    assert _run("api").strip() == "ok"
+
+
+def test_dropin_per_block_transforms_stay_on_the_host():
+    """130,560 single-block Transform / InverseTransform calls (one 1080p
+    frame's luma blocks, one call each as BlockH264 makes them) equal the
+    batched call and cost well under a microsecond each: no device round
+    trip per call."""
+    out = _run("txtime", 130560).split()
+    fwd_ns, inv_ns = float(out[1]), float(out[3])
+    print(f"per-block calls: forward {fwd_ns:.1f} ns, inverse {inv_ns:.1f} ns")
+    assert fwd_ns < 1000 and inv_ns < 1000

@@ -46,6 +46,42 @@ def test_inverse_transform_matches_reference_fixtures():
                                       fx["inv_weighted"][qp])
 
 
+def test_device_transform_kernels_match_reference_fixtures():
+    """The device kernels themselves (icu_*4x4_device; host calls of a few
+    blocks run on the CPU), both layouts, both inverse orders, SetScale
+    weights, against the compiled reference's outputs."""
+    fx = np.load(GOLDEN / "transforms.npz")
+    s = capi.Stream()
+
+    def run(fn, x, n, *args, dw=None):
+        x = np.ascontiguousarray(x, np.int16)
+        d_in, d_out = capi.DeviceBuffer.from_array(x), capi.DeviceBuffer(x.nbytes)
+        if fn == "fwd":
+            capi._check(capi.lib.icu_fwd4x4_device(d_in.ptr, d_out.ptr, n, *args, s.handle))
+        else:
+            capi._check(capi.lib.icu_inv4x4_device(d_in.ptr, d_out.ptr, n, *args,
+                                                    dw.ptr if dw is not None else None, s.handle))
+        s.sync()
+        return d_out.download(np.int16, x.shape)
+
+    for qp in range(0, 52, 3):
+        for intra in (0, 1):
+            for mode in (0, 1, 2):
+                np.testing.assert_array_equal(run("fwd", fx["fwd_in"], len(fx["fwd_in"]), qp, intra, mode, 0),
+                                              fx["fwd"][qp, intra, mode])
+                np.testing.assert_array_equal(
+                    run("fwd", fx["fwd_mb_in"], 16 * len(fx["fwd_mb_in"]), qp, intra, mode, 1),
+                    fx["fwd_mb"][qp, intra, mode])
+    dw = capi.DeviceBuffer.from_array(np.ascontiguousarray(fx["weights"], np.int32))
+    for qp in range(52):
+        x = fx["inv_in"][qp]
+        for mode in (0, 1, 2):
+            np.testing.assert_array_equal(run("inv", x, len(x), qp, mode, 0, 0), fx["inv"][qp, mode])
+            np.testing.assert_array_equal(run("inv", x.reshape(4, 256), len(x), qp, mode, 1, 1),
+                                          fx["inv_mb"][qp, mode])
+        np.testing.assert_array_equal(run("inv", x, len(x), qp, 0, 0, 0, dw=dw), fx["inv_weighted"][qp])
+
+
 def test_transforms_match_oracle_on_large_batch():
     rng = np.random.default_rng(11)
     x = rng.integers(-255, 256, (20000, 16)).astype(np.int16)
@@ -120,20 +156,40 @@ def test_estimator_modes_match_oracle(modes, size, R, seed):
     np.testing.assert_array_equal(dist, odist)
 
 
-def test_bench_workload_1080p_subset_matches_oracle():
-    """1080p, +-32 exhaustive SAD, half-pel: every 16th MB checked against the
-    oracle, and the whole frame recovers the painted motion."""
+def test_bench_workload_1080p_whole_frame_matches_oracle():
+    """The benchmark's step at its full size: 1080p, +-32 exhaustive SAD,
+    half-pel, then MC and residual coding at QP 22/28/34 in one pass.  All
+    8160 MBs' vectors, distortions and integer-stage results equal the
+    oracle's, the prediction equals the oracle's compensation, and every QP's
+    levels and reconstruction equal the oracle's coding of the whole frame."""
     W, H, R = 1920, 1088, 32
+    qps = [22, 28, 34]
     fp = synth.rect_motion_pair(W, H, 1080, n_rects=24, max_mv=32, sigma_cur=3.0)
     modes = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=0)
     est = capi.MotionEstimator(_cfg(W, H, R, **modes))
-    mv, dist, _ = est.estimate(fp.cur.astype(np.int16), fp.ref.astype(np.int16))
-    omv, odist, _, _ = orc.me_estimate(fp.cur, fp.ref, R, step=16, **_oracle_modes(modes))
-    sel = np.arange(0, mv.shape[0], 16)
-    np.testing.assert_array_equal(mv[sel], omv[sel])
-    np.testing.assert_array_equal(dist[sel], odist[sel])
+    n = est.nmb
+    d_cur, d_ref = capi.DeviceBuffer.from_array(fp.cur), capi.DeviceBuffer.from_array(fp.ref)
+    d_mv, d_dist, d_int = capi.DeviceBuffer(4 * n), capi.DeviceBuffer(4 * n), capi.DeviceBuffer(12 * n)
+    d_lv = [capi.DeviceBuffer(2 * W * H) for _ in qps]
+    d_rc = [capi.DeviceBuffer(W * H) for _ in qps]
+    d_pr = capi.DeviceBuffer(2 * W * H)
+    s = capi.Stream()
+    est.estimate_device(d_cur, d_ref, d_mv, d_dist, d_int, stream=s)
+    est.code_frame_qps_device(d_cur, d_mv, qps, 0, 0, d_lv, d_rc, d_pr, stream=s)
+    s.sync()
+    mv = d_mv.download(np.int16, (n, 2))
+    omv, odist, ointeg = orc.me_estimate_par(fp.cur, fp.ref, R, **_oracle_modes(modes))
+    np.testing.assert_array_equal(d_int.download(np.int32, (n, 3)), ointeg)
+    np.testing.assert_array_equal(mv, omv)
+    np.testing.assert_array_equal(d_dist.download(np.int32, n), odist)
     truth = (4 * fp.true_mv.reshape(-1, 2))
     assert (np.abs(mv - truth) <= 2).all(axis=1).mean() > 0.8
+    pred = orc.mc_luma(fp.ref, omv)
+    np.testing.assert_array_equal(d_pr.download(np.int16, (H, W)), pred)
+    for i, qp in enumerate(qps):
+        lv, rc = orc.code_residual(fp.cur, pred, qp, 0, 0)
+        np.testing.assert_array_equal(d_lv[i].download(np.int16, (H, W)), lv, err_msg=f"QP {qp}")
+        np.testing.assert_array_equal(d_rc[i].download(np.uint8, (H, W)).astype(np.int16), rc, err_msg=f"QP {qp}")
 
 
 def test_estimate_is_deterministic_and_rerunnable():

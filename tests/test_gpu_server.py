@@ -182,3 +182,46 @@ def test_server_rejects_bad_requests():
     assert e.value.code == capi.ICU_E_ARG
     assert srv.poll() == []
     srv.close()
+
+
+def test_wavefront_timeout_is_reported_by_check_and_by_the_server():
+    """The median-predictor wavefront's error flag, set from the host (no
+    GPU stall), is reported by icu_me_check and by the server's poll, then
+    cleared; results afterwards are the oracle's again."""
+    w, h, R = 176, 144, 8
+    m = dict(metric=0, window=0, filt=1, subpel=2, predictor=1, best_vector=1)
+    f0, f1 = _frames(2)[0]
+    est = capi.MotionEstimator(_cfg(w, h, R, m))
+    n = est.nmb
+    d_cur, d_ref = capi.DeviceBuffer.from_array(f1), capi.DeviceBuffer.from_array(f0)
+    d_mv, d_dist = capi.DeviceBuffer(4 * n), capi.DeviceBuffer(4 * n)
+    est.estimate_device(d_cur, d_ref, d_mv, d_dist, check=True)
+    capi._check(capi.lib.icu_me_debug_set_wave_error(est.h))
+    with pytest.raises(capi.IcuError) as e:
+        est.check()
+    assert e.value.code == capi.ICU_E_STATE
+    est.check()  # read once, cleared
+    capi._check(capi.lib.icu_me_debug_set_wave_error(est.h))
+    with pytest.raises(capi.IcuError) as e:
+        est.estimate(f1.astype(np.int16), f0.astype(np.int16))
+    assert e.value.code == capi.ICU_E_STATE
+    mv, _, _ = est.estimate(f1.astype(np.int16), f0.astype(np.int16))
+    omv, _, _, _ = orc.me_estimate(f1, f0, R, metric=0, window=0, filter=1, subpel=2, pred=1, bestvec=1)
+    np.testing.assert_array_equal(mv, omv)
+    # zero-predictor handles have no wavefront
+    z = capi.MotionEstimator(_cfg(w, h, R, SERVE))
+    with pytest.raises(capi.IcuError) as e:
+        capi._check(capi.lib.icu_me_debug_set_wave_error(z.h))
+    assert e.value.code == capi.ICU_E_STATE
+    z.check()
+    # the server drops the frame whose wavefront timed out and says so
+    srv = capi.Server(lanes=1, max_inflight=1, qp=QP)
+    sid = srv.open_stream(_cfg(w, h, R, m))
+    srv.submit(sid, f0, tag=0)
+    assert len(_drain(srv, 1)) == 1
+    srv.debug_set_wave_error(sid)
+    srv.submit(sid, f1, tag=1)
+    with pytest.raises(capi.IcuError) as e:
+        _drain(srv, 1)
+    assert e.value.code == capi.ICU_E_STATE
+    srv.close()
