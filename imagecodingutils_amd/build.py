@@ -18,7 +18,7 @@ PKG = Path(__file__).resolve().parent
 SRC = PKG / "csrc"
 OUT = PKG / "libicu_hip.so"
 BUILD = PKG / "_build"
-SOURCES = ["me_search.hip", "me_decide.hip", "me_vbs.hip", "it_kernels.hip", "capi.cpp", "server.cpp", "vbs.cpp", "mc.cpp", "io.cpp", "cavlc.hip", "me_umhs.hip", "me_cross.hip", "me_fhs.hip"]
+SOURCES = ["me_search.hip", "me_decide.hip", "me_vbs.hip", "it_kernels.hip", "capi.cpp", "server.cpp", "vbs.cpp", "mc.cpp", "io.cpp", "cavlc.hip", "me_umhs.hip", "me_cross.hip", "me_fhs.hip", "noalloc.cpp"]
 
 ARCH = os.environ.get("ICU_OFFLOAD_ARCH", "gfx950")
 FLAGS = [

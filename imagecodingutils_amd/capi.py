@@ -54,6 +54,7 @@ _SIGS = {
     "icu_me_estimate": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(C.c_long)]),
     "icu_me_estimate_device": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "icu_me_estimate_timed": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_float)]),
+    "icu_me_estimate_device_noalloc": (C.c_int, [C.POINTER(MeConfig), _vp, _vp, _vp, _vp, _vp, _vp]),
     "icu_me_check": (C.c_int, [_vp]),
     "icu_me_debug_set_wave_error": (C.c_int, [_vp]),
     "icu_me_get_tiling": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),

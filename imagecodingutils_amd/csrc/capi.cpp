@@ -37,11 +37,11 @@ namespace {
       return fail(ICU_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_));        \
   } while (0)
 
-// ------------------------------------------------------------------ tiling
-struct Tiling {
-  int V, B, Q, G;
-};
+}  // namespace
 
+namespace icu {
+
+// ------------------------------------------------------------------ tiling
 int quads_for(int L) { return (L - tile_dx0(L)) / 4 + 1; }
 
 // Lane tiles: rows V per lane so that B*V covers the 2L+1 window rows with
@@ -129,6 +129,10 @@ bool valid_cfg(const icu_me_config& c, std::string* why) {
   }
   return true;
 }
+
+}  // namespace icu
+
+namespace {
 
 // ------------------------------------------------------------------ engine
 struct StreamPlan {

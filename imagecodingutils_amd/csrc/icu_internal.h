@@ -5,6 +5,7 @@
 
 #include <string>
 
+#include "../../include/icu_hip.h"
 #include "icu_device.h"
 
 struct icu_me;
@@ -22,6 +23,18 @@ int me_check(::icu_me* h);
 
 // Records msg as the calling thread's last error and returns code.
 int icu_fail(int code, const std::string& msg);
+
+// Search tiling of one stream: V candidate rows per lane tile, B row tiles,
+// Q column quads, G MBs per workgroup task.
+struct Tiling {
+  int V, B, Q, G;
+};
+// The planner's tiling for a picture mbw MBs wide and window half-width L;
+// argmin: no cost table (exact lane argmin keys needed).  V_req / G_req
+// nonzero force those; V = 0 when a forced V is not allowed.
+Tiling plan_tiling(int mbw, int L, int metric, bool argmin, int V_req, int G_req);
+// Configuration checks of icu_me_create; *why receives the reason.
+bool valid_cfg(const icu_me_config& c, std::string* why);
 
 // Per-stream parameters of the decision / sub-pel stage.
 struct DecideDesc {

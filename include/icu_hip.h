@@ -81,6 +81,21 @@ int icu_me_estimate(icu_me* h, const int16_t* cur, const int16_t* ref, const uin
 int icu_me_estimate_device(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int16_t* d_mv,
                            int32_t* d_dist, int32_t* d_int, void* stream);
 
+/* icu_me_estimate_device without a handle and without allocating: no
+ * hipMalloc / hipFree or other allocating call on any path.  No counterpart
+ * in the reference; it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, hosts that forbid allocation).  The
+ * padded reference, descriptors and task list live in a static device
+ * scratch array of 20 MiB per GPU, held until the process ends; a picture
+ * whose padded plane and tasks exceed it (beyond about 3840x2160) gets
+ * ICU_E_RANGE.  cfg as icu_me_create, with filter 0 and predictor 0 (the
+ * modes that keep no per-MB cost table; ICU_E_ARG otherwise).  Outputs as
+ * icu_me_estimate_device, asynchronous on stream (NULL: the null stream).
+ * All calls on one GPU share its scratch: calls that overlap on one GPU
+ * (other streams, other threads) race. */
+int icu_me_estimate_device_noalloc(const icu_me_config* cfg, const uint8_t* d_cur, const uint8_t* d_ref,
+                                   int16_t* d_mv, int32_t* d_dist, int32_t* d_int, void* stream);
+
 /* icu_me_estimate_device with HIP events between its stages, recorded on
  * the stream the kernels run on; waits for completion and writes the
  * duration of each stage in ms: [0] reference padding, [1] integer search
