@@ -119,8 +119,8 @@ bool valid_cfg(const icu_me_config& c, std::string* why) {
     *why = "mbs_per_task must be 0..16";
     return false;
   }
-  if (c.grid_mode < 0 || c.grid_mode > 3) {
-    *why = "grid_mode must be 0..3";
+  if (c.grid_mode < 0 || c.grid_mode > 4) {
+    *why = "grid_mode must be 0..4";
     return false;
   }
   if (2 * c.range + 1 > 129) {
@@ -152,7 +152,9 @@ struct Engine {
   std::vector<StreamPlan> sp;
   int total_mb = 0;
   long long table_total = 0;
-  std::vector<SearchTask> h_tasks;
+  std::vector<SearchTask> h_tasks;   // workgroup tasks (me_search_kernel)
+  std::vector<SearchTask> h_wtasks;  // one MB per wave (me_wave_kernel), after h_tasks on the device
+  bool wave_on = false;              // the wave kernel runs the streams it takes
   std::vector<DecideItem> h_items;
   // Descriptor slots.  kEager slots serve eager calls: a call uses the slot
   // already bound to its buffers, or refills the least recently used one
@@ -193,7 +195,8 @@ struct Engine {
   // (one LDS buffer, so the most workgroups fit); 1 persistent workgroups, as
   // many as stay resident, run every grid-th task with the next task's window
   // streaming into a second buffer; 2 the same with the grid reduced so every
-  // workgroup runs equally many tasks
+  // workgroup runs equally many tasks; 3 one MB per wave for the streams
+  // whose window the wave kernel takes (wave_rows), mode 0 for the rest
   int grid_mode = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -226,10 +229,28 @@ struct Engine {
     own = nullptr;
   }
 
+  // The wave kernel takes argmin (no cost table) SAD streams with windows
+  // 63 or 65 wide.
+  static bool wave_fits(const StreamPlan& s) {
+    return !s.table && s.cfg.metric == ICU_METRIC_SAD && (s.L == 32 || s.L == 31);
+  }
+  bool any_wave_stream() const {
+    for (const auto& s : sp)
+      if (wave_fits(s)) return true;
+    return false;
+  }
+  bool wave_stream(const StreamPlan& s) const { return wave_on && wave_fits(s); }
+
   void build_tasks() {
     h_tasks.clear();
+    h_wtasks.clear();
+    wave_on = grid_mode == 3;
     for (size_t si = 0; si < sp.size(); si++) {
       const StreamPlan& s = sp[si];
+      if (wave_stream(s)) {
+        for (int mb = 0; mb < s.nmb; mb++) h_wtasks.push_back(SearchTask{(int)si, mb, 1});
+        continue;
+      }
       for (int row = 0; row < s.mbh; row++)
         for (int c = 0; c < s.mbw; c += s.t.G)
           h_tasks.push_back(SearchTask{(int)si, row * s.mbw + c, std::min(s.t.G, s.mbw - c)});
@@ -244,7 +265,7 @@ struct Engine {
     // one staging buffer fits the largest task; the kernel holds two
     stage_dwords = 0;
     for (const auto& s : sp)
-      stage_dwords = std::max(stage_dwords, search_stage_dwords(s.t.B, s.t.V, s.t.Q, s.t.G));
+      if (!wave_stream(s)) stage_dwords = std::max(stage_dwords, search_stage_dwords(s.t.B, s.t.V, s.t.Q, s.t.G));
     stage_dwords = (stage_dwords + 63) & ~63;
   }
 
@@ -288,6 +309,9 @@ struct Engine {
       pad_base.push_back(pad_total);
       pad_total += pad_plane_workgroups(s.cfg.width, s.cfg.height);
     }
+    // unless a grid mode is asked for, streams the wave kernel takes run one
+    // MB per wave (measured faster than the workgroup tiles at +-31 / +-32)
+    if (!cfgs[0].grid_mode && any_wave_stream()) grid_mode = 3;
     build_tasks();
     // descriptor block: three arrays, each starting on a 256-byte boundary
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -312,7 +336,7 @@ struct Engine {
     HIP_TRY(hipMemset(d_wave, 0, sizeof(int) * wave_total));
     if (table_total) HIP_TRY(hipMalloc(&d_table, sizeof(uint32_t) * table_total));
     dev_bytes += sizeof(uint32_t) * table_total;
-    HIP_TRY(hipMemcpy(d_tasks, h_tasks.data(), sizeof(SearchTask) * h_tasks.size(), hipMemcpyHostToDevice));
+    HIP_TRY(upload_tasks());
     h_items.clear();
     for (int i = 0; i < n; i++)
       if (!sp[i].cfg.predictor)
@@ -455,15 +479,20 @@ struct Engine {
     if (ev) HIP_TRY(hipEventRecord(ev[0], st));
     launch_pad_planes(ds[set].d_pad, (int)sp.size(), pad_total, st);
     if (ev) HIP_TRY(hipEventRecord(ev[1], st));
-    int vfix = sp[0].t.V;
+    int vfix = -1;
     bool any_table = false;
     for (const auto& x : sp) {
-      if (x.t.V != vfix) vfix = 0;
+      if (wave_stream(x)) continue;
+      vfix = vfix < 0 || vfix == x.t.V ? x.t.V : 0;
       any_table |= x.table;
     }
-    for (int r = 0; r < search_reps; r++)
-      launch_search(d_streams, d_tasks, (int)h_tasks.size(), grid_mode, sp[0].cfg.metric, vfix,
-                    any_table, stage_dwords, d_ires, d_table, d_decide, sp.size() == 1, st);
+    for (int r = 0; r < search_reps; r++) {
+      if (!h_tasks.empty())
+        launch_search(d_streams, d_tasks, (int)h_tasks.size(), grid_mode == 3 ? 0 : grid_mode, sp[0].cfg.metric,
+                      std::max(vfix, 0), any_table, stage_dwords, d_ires, d_table, d_decide, sp.size() == 1, st);
+      if (!h_wtasks.empty())
+        launch_wave_search(d_streams, d_tasks + h_tasks.size(), (int)h_wtasks.size(), d_decide, true, st);
+    }
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
     // without cost tables the search kernel decides every MB itself
     if (any_table && !h_items.empty()) launch_decide_parallel(d_decide, d_items, (int)h_items.size(), st);
@@ -485,6 +514,18 @@ struct Engine {
 
   // Reads (and clears) the median-predictor wavefronts' error flags; the
   // device work that may set them must have finished.
+  // Workgroup tasks, then wave tasks, into d_tasks (total_mb entries hold
+  // both: every task covers at least one MB).
+  hipError_t upload_tasks() {
+    hipError_t e = hipSuccess;
+    if (!h_tasks.empty())
+      e = hipMemcpy(d_tasks, h_tasks.data(), sizeof(SearchTask) * h_tasks.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !h_wtasks.empty())
+      e = hipMemcpy(d_tasks + h_tasks.size(), h_wtasks.data(), sizeof(SearchTask) * h_wtasks.size(),
+                    hipMemcpyHostToDevice);
+    return e;
+  }
+
   int check_wave_errors() {
     bool timed_out = false;
     for (size_t i = 0; i < sp.size(); i++) {
@@ -503,10 +544,10 @@ struct Engine {
   int retile(int si, Tiling t) {
     sp[si].t = t;
     build_tasks();
-    if ((int)h_tasks.size() > total_mb) return fail(ICU_E_STATE, "task list exceeds its allocation");
+    if ((int)(h_tasks.size() + h_wtasks.size()) > total_mb) return fail(ICU_E_STATE, "task list exceeds its allocation");
     // kernels of an earlier call may still read the task list
     HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(d_tasks, h_tasks.data(), sizeof(SearchTask) * h_tasks.size(), hipMemcpyHostToDevice));
+    HIP_TRY(upload_tasks());
     for (auto& d : ds) d.valid = false;
     return ICU_OK;
   }
@@ -754,6 +795,17 @@ int icu_me_autotune(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int r
           best_mode = mode;
         }
       }
+    }
+  }
+  // one MB per wave, where the stream's window takes it
+  if (rc == ICU_OK && e.any_wave_stream()) {
+    e.grid_mode = 3;
+    float ms = 0;
+    if ((rc = e.retile(0, s.t)) == ICU_OK && (rc = time_estimates(h, d_cur, d_ref, reps, a, b, &ms)) == ICU_OK &&
+        ms < best_ms) {
+      best_ms = ms;
+      best = s.t;
+      best_mode = 3;
     }
   }
   hipEventDestroy(a);

@@ -106,6 +106,32 @@ __device__ __forceinline__ void wave_sum8(int (&a)[8]) {
 #pragma unroll
   for (int k = 0; k < 8; k++) a[k] = __builtin_amdgcn_readlane(a[k], 63);
 }
+// Eight wave sums whose totals are below 2^16 (a 16x16 SAD is at most
+// 65280): packed in pairs into 32-bit words, no partial sum carries into
+// its neighbour, so four sums of words do the work of eight.
+__device__ __forceinline__ void wave_sum8_small(int (&a)[8]) {
+  int w[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) w[i] = a[2 * i] | (a[2 * i + 1] << 16);
+#pragma unroll
+  for (int i = 0; i < 4; i++) w[i] += __builtin_amdgcn_update_dpp(0, w[i], 0x111, 0xf, 0xf, true);
+#pragma unroll
+  for (int i = 0; i < 4; i++) w[i] += __builtin_amdgcn_update_dpp(0, w[i], 0x112, 0xf, 0xf, true);
+#pragma unroll
+  for (int i = 0; i < 4; i++) w[i] += __builtin_amdgcn_update_dpp(0, w[i], 0x114, 0xf, 0xf, true);
+#pragma unroll
+  for (int i = 0; i < 4; i++) w[i] += __builtin_amdgcn_update_dpp(0, w[i], 0x118, 0xf, 0xf, true);
+#pragma unroll
+  for (int i = 0; i < 4; i++) w[i] += __builtin_amdgcn_update_dpp(0, w[i], 0x142, 0xa, 0xf, false);
+#pragma unroll
+  for (int i = 0; i < 4; i++) w[i] += __builtin_amdgcn_update_dpp(0, w[i], 0x143, 0xc, 0xf, false);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const uint32_t t = (uint32_t)__builtin_amdgcn_readlane(w[i], 63);
+    a[2 * i] = (int)(t & 0xFFFFu);
+    a[2 * i + 1] = (int)(t >> 16);
+  }
+}
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -355,9 +381,13 @@ __device__ inline void refine_subpel(const DecideDesc& d, WaveLds& s, int n, int
   if (d.subpel == 1) {
     // half pel only: the eight candidates filtered in registers
     int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (d.metric == kMetricSAD) half_pel_direct<kMetricSAD>(d, n + mx, m + my, cw, lane, acc);
-    else half_pel_direct<kMetricSSD>(d, n + mx, m + my, cw, lane, acc);
-    wave_sum8(acc);
+    if (d.metric == kMetricSAD) {
+      half_pel_direct<kMetricSAD>(d, n + mx, m + my, cw, lane, acc);
+      wave_sum8_small(acc);
+    } else {
+      half_pel_direct<kMetricSSD>(d, n + mx, m + my, cw, lane, acc);
+      wave_sum8(acc);
+    }
     int hxh = 0, hyh = 0;
 #pragma unroll
     for (int k = 0; k < 8; k++)

@@ -573,6 +573,194 @@ void me_search_kernel(const StreamDesc* __restrict__ streams, const SearchTask* 
   }
 }
 
+// ------------------------------------------------------- one MB per wave
+// For windows 63 or 65 candidates wide (L = 31 or 32) a workgroup is a
+// single wave that searches one MB.  The MB's block lives in SGPRs (every
+// lane of the wave uses the same 64 dwords), which leaves the VGPRs to the
+// accumulators, and the wave needs no barrier with any other: its own
+// window (80 rows x 80 bytes) is staged with global_load_lds and waited for
+// with s_waitcnt.  Lane l takes column quad l % 16 (candidate columns
+// -32 + 4 (l % 16) .. +3) and row band l / 16 (16 candidate rows from
+// -L + 16 (l / 16)), run as two tiles of 8 rows: 64 columns x 64 rows.
+// L = 32 leaves column +32 and row +32: lane l costs candidates (32, l - 32)
+// and (l - 32, 32) with v_sad_u8 against LDS rows (the row's reference bytes
+// start at byte l, realigned with alignbyte), and candidate (32, 32) is one
+// block dword per lane summed over the wave.  Each tile folds its candidates
+// into one lane-local key as run_tile does; the wave's minimum key is then
+// the reference's decision, and the same wave decides the MB (sub-pel,
+// outputs) with the window's LDS reused.
+constexpr int kWaveQ = 16;       // column quads (64 candidate columns)
+constexpr int kWaveV = 16;       // candidate rows per band (4 bands: 64 rows)
+constexpr int kWaveRows = 80;    // staged rows: 64 candidate rows + 15, and row +32's block
+constexpr int kWaveRowDw = 20;   // 80 staged bytes per row: quad q reads bytes 4q .. 4q + 19
+constexpr int kWaveLdsBytes = kWaveRows * kWaveRowDw * 4;
+
+// Per-wave LDS: the staged window, reused by the decision afterwards.
+static_assert(sizeof(dec::WaveLds) <= (size_t)kWaveLdsBytes, "the decision's LDS must fit the window's");
+
+// One lane tile of R candidate rows x 4 columns from (dx0, dy0): the least
+// lane key (cost << 16) | ((dist2 - least dist2 of the tile) << IB) |
+// (2^IB - 1 - raster index), folded into the 64-bit key.  The codes fit 16
+// bits for every 8-row tile of windows up to +-32 (lane_keys_fit(32, 8)).
+template <int R, int IB>
+__device__ __forceinline__ void wave_tile(const lds_u32* wp, const uint32_t (&c)[64], int dx0, int dy0, int xl,
+                                          int xr, int yu, int yd, uint64_t& kbest) {
+  const uint32_t cm_lo = (dx0 >= xl && dx0 <= xr ? 0u : 0xFFFFu) |
+                         (dx0 + 1 >= xl && dx0 + 1 <= xr ? 0u : 0xFFFF0000u);
+  const uint32_t cm_hi = (dx0 + 2 >= xl && dx0 + 2 <= xr ? 0u : 0xFFFFu) |
+                         (dx0 + 3 >= xl && dx0 + 3 <= xr ? 0u : 0xFFFF0000u);
+  const int ax = dx0 > 0 ? dx0 : (dx0 + 3 < 0 ? -(dx0 + 3) : 0);
+  const int ay = dy0 > 0 ? dy0 : (dy0 + R - 1 < 0 ? -(dy0 + R - 1) : 0);
+  uint32_t Ck[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    Ck[k] = ((uint32_t)((dx0 + k) * (dx0 + k) - ax * ax) << IB) + (uint32_t)(3 - k) + ((1u << IB) - 4u * R);
+  uint32_t Ev = ((uint32_t)(dy0 * dy0 - ay * ay) << IB) + 4u * (R - 1);
+  const uint32_t Dl = (uint32_t)(2 * dy0) << IB;
+  const int vlo = max(yu - dy0, 0), vhi = min(yd - dy0, R - 1);
+  const uint32_t inv_rows = vlo > vhi ? 0xFFFFFFFFu : ~(((2u << vhi) - 1u) & ~((1u << vlo) - 1u));
+  uint32_t lbest = 0xFFFFFFFFu;
+  sad_tile<R>(wp, kWaveRowDw, c, [&](int v, uint32_t lo, uint32_t hi) {
+    const uint32_t rmask = (uint32_t)__builtin_amdgcn_sbfe((int)inv_rows, v, 1);
+    const uint32_t l2 = lo | cm_lo | rmask, h2 = hi | cm_hi | rmask;
+    const uint32_t k0 = (l2 << 16) | Ck[0];
+    const uint32_t k1 = (l2 & 0xFFFF0000u) | Ck[1];
+    const uint32_t k2 = (h2 << 16) | Ck[2];
+    const uint32_t k3 = (h2 & 0xFFFF0000u) | Ck[3];
+    lbest = min(lbest, min(min(k0, k1), min(k2, k3)) + Ev);
+    // Ev of the next row: ((dy+1)^2 - dy^2) << IB, minus 4 for the raster part
+    Ev += Dl + (((2u * (uint32_t)v + 1u) << IB) - 4u);
+  });
+  if (lbest < 0xFFFF0000u) {
+    const int idx = (1 << IB) - 1 - (int)(lbest & ((1u << IB) - 1u));
+    const int dx = dx0 + (idx & 3), dy = dy0 + (idx >> 2);
+    const uint32_t cost = lbest >> 16;
+    const uint64_t key = (dx == 0 && dy == 0) ? pack_key(cost, 0, kZeroRRank)
+                                              : pack_key(cost, (uint32_t)(dx * dx + dy * dy), raster_rrank(dy, dx));
+    kbest = key < kbest ? key : kbest;
+  }
+}
+
+__device__ __forceinline__ void fold_key(uint64_t& kbest, uint32_t cost, int dx, int dy, int xl, int xr, int yu,
+                                         int yd) {
+  if (dx >= xl && dx <= xr && dy >= yu && dy <= yd) {
+    const uint64_t key = pack_key(cost, (uint32_t)(dx * dx + dy * dy), raster_rrank(dy, dx));
+    kbest = key < kbest ? key : kbest;
+  }
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
+void me_wave_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __restrict__ tasks, int ntasks,
+                    const DecideDesc* __restrict__ decide, int band) {
+  __shared__ __attribute__((aligned(16))) uint32_t win_raw[kWaveLdsBytes / 4];
+  lds_u32* const win = (lds_u32*)win_raw;
+  const int lane = threadIdx.x;
+  int ti = blockIdx.x;
+  if (band) {
+    // XCD bands, as in me_search_kernel
+    const int x = (int)blockIdx.x & 7, k = (int)blockIdx.x >> 3;
+    const int q = ntasks >> 3, r = ntasks & 7;
+    ti = x * q + min(x, r) + k;
+  }
+  const SearchTask t = tasks[ti];
+  const StreamDesc s = streams[t.stream];
+  const int row = t.mb / s.mbw, col = t.mb - row * s.mbw;
+  const int n = 16 * col, m = 16 * row;
+  const int L = s.L;
+  // the block: uniform addresses, so scalar loads straight into SGPRs; cw
+  // is lane l's dword (l / 4, l % 4) of it
+  uint32_t c[64];
+  {
+    // through the constant address space: uniform loads there become s_load
+    typedef __attribute__((address_space(4))) const uint32_t const_u32;
+    const const_u32* cb = (const const_u32*)(s.cur + (long)m * s.cur_stride + n);
+    const int sdw = s.cur_stride >> 2;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) c[4 * i + k] = cb[i * sdw + k];
+  }
+  const uint32_t cw = *(const uint32_t*)(s.cur + (long)(m + (lane >> 2)) * s.cur_stride + n + 4 * (lane & 3));
+  // staging: rows m - L .. m - L + 79, bytes n - 32 .. n + 47 (16-byte
+  // aligned: n is a multiple of 16)
+  {
+    constexpr int kChunks = kWaveRows * kWaveRowDw / 4;
+    const uint8_t* wsrc = s.ref + (long)(m - L) * s.ref_stride + n - 32;
+#pragma unroll
+    for (int base = 0; base < kChunks; base += 64) {
+      const int ch = base + lane;
+      const int r = ch / 5, c4 = ch - 5 * r;
+      if (ch < kChunks)
+        __builtin_amdgcn_global_load_lds((const void*)(wsrc + (long)r * s.ref_stride + 16 * c4),
+                                         (__attribute__((address_space(3))) void*)(win + 4 * base), 16, 0, 0);
+    }
+  }
+  int xl, xr, yu, yd;
+  motion_window(n, m, s.W, s.H, L, &xl, &xr, &yu, &yd);
+  __builtin_amdgcn_s_waitcnt(0);  // the block and the window have landed (one wave: no barrier)
+  dec::wave_sync();
+
+  const int q = lane & (kWaveQ - 1), b = lane >> 4;
+  const int dx0 = -32 + 4 * q;
+  const int dy0 = -L + b * kWaveV;
+  uint64_t kbest = ~0ull;
+  wave_tile<8, 5>(win + (b * kWaveV) * kWaveRowDw + q, c, dx0, dy0, xl, xr, yu, yd, kbest);
+  wave_tile<8, 5>(win + (b * kWaveV + 8) * kWaveRowDw + q, c, dx0, dy0 + 8, xl, xr, yu, yd, kbest);
+  if (L == 32) {
+    // column +32: candidate (32, lane - 32) from window rows lane .. lane + 15,
+    // bytes 64 .. 79
+    {
+      uint32_t acc = 0;
+      const lds_u32x4* rp = (const lds_u32x4*)(win + lane * kWaveRowDw + 16);
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const u32x4 w = rp[r * (kWaveRowDw / 4)];
+        acc = __builtin_amdgcn_sad_u8(w.x, c[4 * r + 0], acc);
+        acc = __builtin_amdgcn_sad_u8(w.y, c[4 * r + 1], acc);
+        acc = __builtin_amdgcn_sad_u8(w.z, c[4 * r + 2], acc);
+        acc = __builtin_amdgcn_sad_u8(w.w, c[4 * r + 3], acc);
+      }
+      fold_key(kbest, acc, 32, lane - 32, xl, xr, yu, yd);
+    }
+    // row +32: candidate (lane - 32, 32) from window rows 64 .. 79, bytes
+    // lane .. lane + 15
+    {
+      uint32_t acc = 0;
+      const lds_u32* rp = win + 64 * kWaveRowDw + (lane >> 2);
+      const uint32_t sh = (uint32_t)(lane & 3);
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        uint32_t w[5];
+#pragma unroll
+        for (int j = 0; j < 5; j++) w[j] = rp[r * kWaveRowDw + j];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(w[k + 1], w[k], sh), c[4 * r + k], acc);
+      }
+      fold_key(kbest, acc, lane - 32, 32, xl, xr, yu, yd);
+    }
+    // candidate (32, 32): lane l the block dword (l / 4, l % 4)
+    {
+      const uint32_t rw = win[(64 + (lane >> 2)) * kWaveRowDw + 16 + (lane & 3)];
+      const int tot = dec::wave_sum((int)__builtin_amdgcn_sad_u8(cw, rw, 0u));
+      fold_key(kbest, (uint32_t)tot, 32, 32, xl, xr, yu, yd);
+    }
+  }
+  kbest = dec::wave_min_u64(kbest);
+  int mx, my;
+  key_vector(kbest, &mx, &my);
+  dec::wave_sync();  // every lane's window reads are done: the decision reuses the LDS
+  __builtin_amdgcn_s_setprio(2);
+  decide_in_search(decide + t.stream, (dec::WaveLds*)win_raw, t.mb, lane, mx, my, (int)key_cost(kbest));
+}
+
+void launch_wave_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks,
+                        const DecideDesc* d_decide, bool band, hipStream_t st) {
+  if (ntasks <= 0) return;
+  hipLaunchKernelGGL(me_wave_kernel, dim3(ntasks), dim3(64), 0, st, d_streams, d_tasks, ntasks, d_decide,
+                     band ? 1 : 0);
+}
+
 // ----------------------------------------------------------------- host side
 int pad_plane_workgroups(int W, int H) {
   const long chunks = (long)(W + 2 * kPad) / 16 * (H + 2 * kPad);

@@ -45,9 +45,10 @@ typedef struct icu_me_config {
   int best_vector;   /* 1: GetBestVector over {best, predictor, zero} (reference); 0: keep best */
   int tile_rows;     /* 0: chosen by the tuner; else candidate rows per lane: 8, 11, 13 or 16 */
   int mbs_per_task;  /* 0: chosen by the tuner; else MBs staged per workgroup task (1..16) */
-  int grid_mode;     /* 0: chosen by the tuner (one task per workgroup until tuned); else 1 + the mode
-                        icu_me_get_tiling reports (1 one task per workgroup, 2 persistent,
-                        3 persistent balanced) */
+  int grid_mode;     /* 0: the planner's choice (one MB per wave where the window takes it, else one
+                        task per workgroup) until tuned; else 1 + the mode icu_me_get_tiling
+                        reports (1 one task per workgroup, 2 persistent, 3 persistent balanced,
+                        4 one MB per wave) */
 } icu_me_config;
 
 typedef struct icu_me icu_me;
@@ -125,7 +126,10 @@ int icu_me_debug_set_wave_error(icu_me* h);
  * lane, MBs per workgroup task, and the search grid mode: 0 one task per
  * workgroup, 1 persistent (as many workgroups as stay resident claim tasks
  * until none are left), 2 persistent with the grid reduced so every
- * workgroup runs equally many tasks. */
+ * workgroup runs equally many tasks, 3 one MB per wave (SAD without a cost
+ * table at +-31 / +-32: the block in scalar registers, 64 x 64 candidates
+ * in lane tiles plus the last column and row; the rows per lane and MBs
+ * per task reported then apply to no stream of a single-stream handle). */
 int icu_me_get_tiling(icu_me* h, int* tile_rows, int* mbs_per_task, int* grid_mode);
 
 /* Time each candidate tiling (rows per lane x MBs per task x grid mode) on

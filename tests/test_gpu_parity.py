@@ -255,6 +255,51 @@ def test_split_argmin_keys_tie_break_at_large_ranges(R):
         np.testing.assert_array_equal(dist, odist)
 
 
+# One MB per wave (grid mode 3, config grid_mode 4): the default for SAD
+# argmin streams at +-31 / +-32.  Pictures small enough that every window
+# clamps, the reference and exhaustive windows, all sub-pel depths and the
+# final vector choice; the workgroup kernel (grid mode 0) must agree too.
+@pytest.mark.parametrize("size,seed", [((64, 48), 1), ((176, 144), 2), ((352, 288), 3), ((96, 160), 4)])
+@pytest.mark.parametrize("window,subpel,bestvec", [(1, 1, 0), (0, 2, 1), (1, 0, 0), (0, 1, 1)])
+def test_wave_kernel_matches_oracle(size, seed, window, subpel, bestvec):
+    W, H = size
+    R = 32
+    fp = synth.rect_motion_pair(W, H, 60 + seed, n_rects=8, max_mv=min(W, H) // 3)
+    modes = dict(metric=1, window=window, filt=0, subpel=subpel, predictor=0, best_vector=bestvec)
+    omv, odist, ointeg, _ = orc.me_estimate(fp.cur, fp.ref, R, **_oracle_modes(modes))
+    for gm in (4, 1, 0):
+        est = capi.MotionEstimator(_cfg(W, H, R, grid_mode=gm, **modes))
+        if gm != 1:
+            assert est.tiling()[2] == 3
+        d_cur, d_ref = capi.DeviceBuffer.from_array(fp.cur), capi.DeviceBuffer.from_array(fp.ref)
+        n = est.nmb
+        d_mv, d_dist, d_int = capi.DeviceBuffer(4 * n), capi.DeviceBuffer(4 * n), capi.DeviceBuffer(12 * n)
+        est.estimate_device(d_cur, d_ref, d_mv, d_dist, d_int)
+        capi.lib.icu_stream_sync(None)
+        np.testing.assert_array_equal(d_int.download(np.int32, (n, 3)), ointeg, err_msg=f"grid {gm}")
+        np.testing.assert_array_equal(d_mv.download(np.int16, (n, 2)), omv, err_msg=f"grid {gm}")
+        np.testing.assert_array_equal(d_dist.download(np.int32, n), odist, err_msg=f"grid {gm}")
+
+
+@pytest.mark.parametrize("window", [0, 1])
+def test_wave_kernel_tie_break_on_flat_and_periodic_content(window):
+    """Many equal costs across the wave's tiles, its leftover column and row
+    and the (32, 32) corner: the wave's key reduction must pick the
+    reference's candidate."""
+    W, H, R = 352, 288, 32
+    yy, xx = np.mgrid[0:H, 0:W]
+    cur = np.where((xx // 48 + yy // 48) % 2 == 0, 90, ((xx % 4) * 30 + (yy % 8) * 9)).astype(np.uint8)
+    ref = np.roll(cur, (-29, 27), axis=(0, 1))
+    ref[:, 300:] = 90
+    modes = dict(metric=1, window=window, filt=0, subpel=0, predictor=0, best_vector=0)
+    omv, odist, _, _ = orc.me_estimate(cur, ref, R, **_oracle_modes(modes))
+    est = capi.MotionEstimator(_cfg(W, H, R, **modes))
+    assert est.tiling()[2] == 3
+    mv, dist, _ = est.estimate(cur.astype(np.int16), ref.astype(np.int16))
+    np.testing.assert_array_equal(mv, omv)
+    np.testing.assert_array_equal(dist, odist)
+
+
 @pytest.mark.parametrize("tile_rows,R", [(16, 32), (16, 8), (16, 64)])
 def test_too_tall_tile_for_exact_keys_is_rejected(tile_rows, R):
     with pytest.raises(capi.IcuError) as e:
@@ -272,7 +317,7 @@ def test_tuned_estimator_matches_oracle():
     d_cur, d_ref = capi.DeviceBuffer.from_array(fp.cur), capi.DeviceBuffer.from_array(fp.ref)
     est.autotune(d_cur, d_ref, reps=1)
     v, g, p = est.tiling()
-    assert v in (8, 11, 13, 16) and 1 <= g <= 16 and p in (0, 1, 2)
+    assert v in (8, 11, 13, 16) and 1 <= g <= 16 and p in (0, 1, 2, 3)
     mv, dist, _ = est.estimate(fp.cur.astype(np.int16), fp.ref.astype(np.int16))
     omv, odist, _, _ = orc.me_estimate(fp.cur, fp.ref, R, **_oracle_modes(modes))
     np.testing.assert_array_equal(mv, omv)
