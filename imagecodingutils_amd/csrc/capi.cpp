@@ -153,7 +153,10 @@ struct Engine {
   int total_mb = 0;
   long long table_total = 0;
   std::vector<SearchTask> h_tasks;   // workgroup tasks (me_search_kernel)
-  std::vector<SearchTask> h_wtasks;  // one MB per wave (me_wave_kernel), after h_tasks on the device
+  std::vector<SearchTask> h_wtasks;  // one MB per wave (me_wave_kernel), after h_tasks on the device,
+                                     // grouped by window class: kWaveClasses[k] has wave_n[k] tasks
+  static constexpr int kWaveClasses[3] = {32, 16, 8};
+  int wave_n[3] = {0, 0, 0};
   bool wave_on = false;              // the wave kernel runs the streams it takes
   std::vector<DecideItem> h_items;
   // Descriptor slots.  kEager slots serve eager calls: a call uses the slot
@@ -229,10 +232,10 @@ struct Engine {
     own = nullptr;
   }
 
-  // The wave kernel takes argmin (no cost table) SAD streams with windows
-  // 63 or 65 wide.
+  // The wave kernel takes argmin (no cost table) SAD streams whose window
+  // half-width has a class (wave_class).
   static bool wave_fits(const StreamPlan& s) {
-    return !s.table && s.cfg.metric == ICU_METRIC_SAD && (s.L == 32 || s.L == 31);
+    return !s.table && s.cfg.metric == ICU_METRIC_SAD && wave_class(s.L) != 0;
   }
   bool any_wave_stream() const {
     for (const auto& s : sp)
@@ -245,12 +248,18 @@ struct Engine {
     h_tasks.clear();
     h_wtasks.clear();
     wave_on = grid_mode == 3;
+    for (int k = 0; k < 3; k++) {
+      wave_n[k] = 0;
+      for (size_t si = 0; si < sp.size(); si++) {
+        const StreamPlan& s = sp[si];
+        if (!wave_stream(s) || wave_class(s.L) != kWaveClasses[k]) continue;
+        for (int mb = 0; mb < s.nmb; mb++) h_wtasks.push_back(SearchTask{(int)si, mb, 1});
+        wave_n[k] += s.nmb;
+      }
+    }
     for (size_t si = 0; si < sp.size(); si++) {
       const StreamPlan& s = sp[si];
-      if (wave_stream(s)) {
-        for (int mb = 0; mb < s.nmb; mb++) h_wtasks.push_back(SearchTask{(int)si, mb, 1});
-        continue;
-      }
+      if (wave_stream(s)) continue;
       for (int row = 0; row < s.mbh; row++)
         for (int c = 0; c < s.mbw; c += s.t.G)
           h_tasks.push_back(SearchTask{(int)si, row * s.mbw + c, std::min(s.t.G, s.mbw - c)});
@@ -490,8 +499,8 @@ struct Engine {
       if (!h_tasks.empty())
         launch_search(d_streams, d_tasks, (int)h_tasks.size(), grid_mode == 3 ? 0 : grid_mode, sp[0].cfg.metric,
                       std::max(vfix, 0), any_table, stage_dwords, d_ires, d_table, d_decide, sp.size() == 1, st);
-      if (!h_wtasks.empty())
-        launch_wave_search(d_streams, d_tasks + h_tasks.size(), (int)h_wtasks.size(), d_decide, true, st);
+      for (int k = 0, off = (int)h_tasks.size(); k < 3; off += wave_n[k], k++)
+        launch_wave_search(d_streams, d_tasks + off, wave_n[k], kWaveClasses[k], d_decide, true, st);
     }
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
     // without cost tables the search kernel decides every MB itself

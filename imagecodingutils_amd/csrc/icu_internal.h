@@ -78,9 +78,11 @@ void launch_pad_planes(const PadDesc* d_pads, int n, int total_wgs, hipStream_t 
 void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int grid_mode,
                    int metric, int vfix, bool table, int buf_dwords, IntResult* d_out,
                    uint32_t* d_table, const DecideDesc* d_decide, bool band, hipStream_t st);
-// One MB per wave (me_wave_kernel) for argmin SAD streams with L = 31 or
-// 32; decides each MB in place.
-void launch_wave_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks,
+// One MB per wave (me_wave_kernel) for argmin SAD streams whose window
+// half-width L is L0 or L0 - 1, L0 = wave_class(L) in {8, 16, 32} (0: no
+// class); decides each MB in place.  All tasks of one launch share L0.
+int wave_class(int L);
+void launch_wave_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int L0,
                         const DecideDesc* d_decide, bool band, hipStream_t st);
 // Dwords of one staging buffer (window rows padded to 16 bytes + blocks).
 int search_stage_dwords(int B, int V, int Q, int G);

@@ -47,10 +47,13 @@ typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 // Edge-replicated copy of an 8-bit plane into the padded plane
 // (OverlayExtMem2Dv2::ExtendBoundary/FillBoundary,
 // src/ImageUtils/OverlayExtMem2Dv2.cpp:120-217, as a coordinate clamp).
-// 16 bytes per thread: chunks inside the picture copy one 16-byte load
-// (kPad, W and the source stride are multiples of 16), border chunks gather
-// clamped bytes.  Every plane of a batch pads in one launch: a workgroup finds
-// its plane by binary search over the planes' first workgroups.
+// 16 bytes per thread.  kPad is a multiple of 16, so a chunk lies wholly
+// inside the picture's columns (one 16-byte load of the clamped row), wholly
+// left / right of it (the row's first / last pel, one byte load,
+// replicated), or, only where W is not a multiple of 16 (4:2:0 chroma
+// planes), across the right edge (bytes gathered with the column clamped).
+// Every plane of a batch pads in one launch: a workgroup finds its plane by
+// binary search over the planes' first workgroups.
 __device__ __forceinline__ void pad_chunk(const PadDesc& p) {
   const int cpr = (p.W + 2 * kPad) / 16;  // 16-byte chunks per padded row
   const int idx = ((int)blockIdx.x - p.wg_base) * (int)blockDim.x + (int)threadIdx.x;
@@ -60,18 +63,34 @@ __device__ __forceinline__ void pad_chunk(const PadDesc& p) {
   const uint8_t* row = p.src + (long)yy * p.src_stride;
   const int x = x16 - kPad;
   uint4 v;
-  if (x >= 0 && x + 16 <= p.W && (((uintptr_t)row & 15) == 0)) {
-    v = *(const uint4*)(row + x);
+  if (x >= 0 && x + 16 <= p.W) {
+    if (((uintptr_t)row & 15) == 0) {
+      v = *(const uint4*)(row + x);
+    } else {
+      // a caller's plane without 16-byte aligned rows
+      uint32_t w[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (((uintptr_t)row & 3) == 0) {
+          w[q] = *(const uint32_t*)(row + x + 4 * q);
+        } else {
+          w[q] = 0;
+#pragma unroll
+          for (int i = 0; i < 4; i++) w[q] |= (uint32_t)row[x + 4 * q + i] << (8 * i);
+        }
+      }
+      v = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+  } else if (x < 0 || x >= p.W) {
+    const uint32_t e = 0x01010101u * row[x < 0 ? 0 : p.W - 1];
+    v = make_uint4(e, e, e, e);
   } else {
     uint32_t w[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       w[q] = 0;
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int xx = min(max(x + 4 * q + i, 0), p.W - 1);
-        w[q] |= (uint32_t)row[xx] << (8 * i);
-      }
+      for (int i = 0; i < 4; i++) w[q] |= (uint32_t)row[min(x + 4 * q + i, p.W - 1)] << (8 * i);
     }
     v = make_uint4(w[0], w[1], w[2], w[3]);
   }
@@ -574,35 +593,27 @@ void me_search_kernel(const StreamDesc* __restrict__ streams, const SearchTask* 
 }
 
 // ------------------------------------------------------- one MB per wave
-// For windows 63 or 65 candidates wide (L = 31 or 32) a workgroup is a
-// single wave that searches one MB.  The MB's block lives in SGPRs (every
-// lane of the wave uses the same 64 dwords), which leaves the VGPRs to the
-// accumulators, and the wave needs no barrier with any other: its own
-// window (80 rows x 80 bytes) is staged with global_load_lds and waited for
-// with s_waitcnt.  Lane l takes column quad l % 16 (candidate columns
-// -32 + 4 (l % 16) .. +3) and row band l / 16 (16 candidate rows from
-// -L + 16 (l / 16)), run as two tiles of 8 rows: 64 columns x 64 rows.
-// L = 32 leaves column +32 and row +32: lane l costs candidates (32, l - 32)
-// and (l - 32, 32) with v_sad_u8 against LDS rows (the row's reference bytes
-// start at byte l, realigned with alignbyte), and candidate (32, 32) is one
-// block dword per lane summed over the wave.  Each tile folds its candidates
-// into one lane-local key as run_tile does; the wave's minimum key is then
-// the reference's decision, and the same wave decides the MB (sub-pel,
-// outputs) with the window's LDS reused.
-constexpr int kWaveQ = 16;       // column quads (64 candidate columns)
-constexpr int kWaveV = 16;       // candidate rows per band (4 bands: 64 rows)
-constexpr int kWaveRows = 80;    // staged rows: 64 candidate rows + 15, and row +32's block
-constexpr int kWaveRowDw = 20;   // 80 staged bytes per row: quad q reads bytes 4q .. 4q + 19
-constexpr int kWaveLdsBytes = kWaveRows * kWaveRowDw * 4;
-
-// Per-wave LDS: the staged window, reused by the decision afterwards.
-static_assert(sizeof(dec::WaveLds) <= (size_t)kWaveLdsBytes, "the decision's LDS must fit the window's");
+// For windows of half-width L = L0 or L0 - 1 with L0 = 8, 16 or 32 a
+// workgroup is a single wave that searches one MB.  The MB's block lives in
+// SGPRs (every lane of the wave uses the same 64 dwords), which leaves the
+// VGPRs to the accumulators, and the wave needs no barrier with any other:
+// its own window (2 L0 + 16 rows) is staged with global_load_lds and waited
+// for with s_waitcnt.  The 64 lanes tile the 2 L0 x 2 L0 candidates from
+// (-L0, -L) as QW column quads x BB row bands of VV rows (16 x 4 x 16 at
+// L0 = 32, 8 x 8 x 4 at 16, 4 x 16 x 1 at 8), each band run as tiles of at
+// most 8 rows.  L = L0 leaves column +L0 and row +L0: lane l costs
+// candidates (L0, l - L0) and (l - L0, L0) with v_sad_u8 against LDS rows
+// (the row's reference bytes start at byte l, realigned with alignbyte),
+// and candidate (L0, L0) is one block dword per lane summed over the wave.
+// Each tile folds its candidates into one lane-local key as run_tile does;
+// the wave's minimum key is then the reference's decision, and the same wave
+// decides the MB (sub-pel, outputs) with the window's LDS reused.
 
 // One lane tile of R candidate rows x 4 columns from (dx0, dy0): the least
 // lane key (cost << 16) | ((dist2 - least dist2 of the tile) << IB) |
 // (2^IB - 1 - raster index), folded into the 64-bit key.  The codes fit 16
-// bits for every 8-row tile of windows up to +-32 (lane_keys_fit(32, 8)).
-template <int R, int IB>
+// bits for tiles of up to 8 rows at every range up to 64 (IB = 5).
+template <int R, int IB, int ROW_DW>
 __device__ __forceinline__ void wave_tile(const lds_u32* wp, const uint32_t (&c)[64], int dx0, int dy0, int xl,
                                           int xr, int yu, int yd, uint64_t& kbest) {
   const uint32_t cm_lo = (dx0 >= xl && dx0 <= xr ? 0u : 0xFFFFu) |
@@ -620,7 +631,7 @@ __device__ __forceinline__ void wave_tile(const lds_u32* wp, const uint32_t (&c)
   const int vlo = max(yu - dy0, 0), vhi = min(yd - dy0, R - 1);
   const uint32_t inv_rows = vlo > vhi ? 0xFFFFFFFFu : ~(((2u << vhi) - 1u) & ~((1u << vlo) - 1u));
   uint32_t lbest = 0xFFFFFFFFu;
-  sad_tile<R>(wp, kWaveRowDw, c, [&](int v, uint32_t lo, uint32_t hi) {
+  sad_tile<R>(wp, ROW_DW, c, [&](int v, uint32_t lo, uint32_t hi) {
     const uint32_t rmask = (uint32_t)__builtin_amdgcn_sbfe((int)inv_rows, v, 1);
     const uint32_t l2 = lo | cm_lo | rmask, h2 = hi | cm_hi | rmask;
     const uint32_t k0 = (l2 << 16) | Ck[0];
@@ -641,18 +652,36 @@ __device__ __forceinline__ void wave_tile(const lds_u32* wp, const uint32_t (&c)
   }
 }
 
-__device__ __forceinline__ void fold_key(uint64_t& kbest, uint32_t cost, int dx, int dy, int xl, int xr, int yu,
-                                         int yd) {
-  if (dx >= xl && dx <= xr && dy >= yu && dy <= yd) {
+__device__ __forceinline__ void fold_key(uint64_t& kbest, bool live, uint32_t cost, int dx, int dy, int xl, int xr,
+                                         int yu, int yd) {
+  if (live && dx >= xl && dx <= xr && dy >= yu && dy <= yd) {
     const uint64_t key = pack_key(cost, (uint32_t)(dx * dx + dy * dy), raster_rrank(dy, dx));
     kbest = key < kbest ? key : kbest;
   }
 }
 
+// Geometry of one window class.
+template <int L0>
+struct WaveGeom {
+  static constexpr int QW = L0 / 2;                        // column quads: 2 L0 columns
+  static constexpr int BB = 64 / QW;                       // row bands
+  static constexpr int VV = 2 * L0 / BB;                   // rows per band
+  static constexpr int TR = VV < 8 ? VV : 8;               // rows per tile
+  static constexpr int SHIFT = ((16 - L0 % 16) % 16) / 4;  // dwords from the 16-byte boundary to column -L0
+  static constexpr int ROW_DW = ((2 * L0 + 16 + 4 * SHIFT + 15) / 16) * 4;
+  static constexpr int ROWS = 2 * L0 + 16;
+  static constexpr int WIN_BYTES = ROWS * ROW_DW * 4;
+  static constexpr int LDS_BYTES =
+      WIN_BYTES > (int)sizeof(dec::WaveLds) ? WIN_BYTES : (int)((sizeof(dec::WaveLds) + 15) / 16 * 16);
+  static_assert(QW * BB == 64 && BB * VV == 2 * L0 && VV % TR == 0, "lane tiling");
+};
+
+template <int L0>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
 void me_wave_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __restrict__ tasks, int ntasks,
                     const DecideDesc* __restrict__ decide, int band) {
-  __shared__ __attribute__((aligned(16))) uint32_t win_raw[kWaveLdsBytes / 4];
+  using G = WaveGeom<L0>;
+  __shared__ __attribute__((aligned(16))) uint32_t win_raw[G::LDS_BYTES / 4];
   lds_u32* const win = (lds_u32*)win_raw;
   const int lane = threadIdx.x;
   int ti = blockIdx.x;
@@ -681,15 +710,15 @@ void me_wave_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __
       for (int k = 0; k < 4; k++) c[4 * i + k] = cb[i * sdw + k];
   }
   const uint32_t cw = *(const uint32_t*)(s.cur + (long)(m + (lane >> 2)) * s.cur_stride + n + 4 * (lane & 3));
-  // staging: rows m - L .. m - L + 79, bytes n - 32 .. n + 47 (16-byte
-  // aligned: n is a multiple of 16)
+  // staging: rows m - L .. m - L + 2 L0 + 15, bytes from the 16-byte boundary
+  // at or left of n - L0 (n is a multiple of 16)
   {
-    constexpr int kChunks = kWaveRows * kWaveRowDw / 4;
-    const uint8_t* wsrc = s.ref + (long)(m - L) * s.ref_stride + n - 32;
+    constexpr int kCpr = G::ROW_DW / 4, kChunks = G::ROWS * kCpr;
+    const uint8_t* wsrc = s.ref + (long)(m - L) * s.ref_stride + n - L0 - 4 * G::SHIFT;
 #pragma unroll
     for (int base = 0; base < kChunks; base += 64) {
       const int ch = base + lane;
-      const int r = ch / 5, c4 = ch - 5 * r;
+      const int r = ch / kCpr, c4 = ch - kCpr * r;
       if (ch < kChunks)
         __builtin_amdgcn_global_load_lds((const void*)(wsrc + (long)r * s.ref_stride + 16 * c4),
                                          (__attribute__((address_space(3))) void*)(win + 4 * base), 16, 0, 0);
@@ -700,50 +729,51 @@ void me_wave_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __
   __builtin_amdgcn_s_waitcnt(0);  // the block and the window have landed (one wave: no barrier)
   dec::wave_sync();
 
-  const int q = lane & (kWaveQ - 1), b = lane >> 4;
-  const int dx0 = -32 + 4 * q;
-  const int dy0 = -L + b * kWaveV;
+  const lds_u32* w0 = win + G::SHIFT;  // column -L0
+  const int q = lane % G::QW, b = lane / G::QW;
+  const int dx0 = -L0 + 4 * q;
+  const int dy0 = -L + b * G::VV;
   uint64_t kbest = ~0ull;
-  wave_tile<8, 5>(win + (b * kWaveV) * kWaveRowDw + q, c, dx0, dy0, xl, xr, yu, yd, kbest);
-  wave_tile<8, 5>(win + (b * kWaveV + 8) * kWaveRowDw + q, c, dx0, dy0 + 8, xl, xr, yu, yd, kbest);
-  if (L == 32) {
-    // column +32: candidate (32, lane - 32) from window rows lane .. lane + 15,
-    // bytes 64 .. 79
-    {
-      uint32_t acc = 0;
-      const lds_u32x4* rp = (const lds_u32x4*)(win + lane * kWaveRowDw + 16);
 #pragma unroll
-      for (int r = 0; r < 16; r++) {
-        const u32x4 w = rp[r * (kWaveRowDw / 4)];
-        acc = __builtin_amdgcn_sad_u8(w.x, c[4 * r + 0], acc);
-        acc = __builtin_amdgcn_sad_u8(w.y, c[4 * r + 1], acc);
-        acc = __builtin_amdgcn_sad_u8(w.z, c[4 * r + 2], acc);
-        acc = __builtin_amdgcn_sad_u8(w.w, c[4 * r + 3], acc);
-      }
-      fold_key(kbest, acc, 32, lane - 32, xl, xr, yu, yd);
-    }
-    // row +32: candidate (lane - 32, 32) from window rows 64 .. 79, bytes
-    // lane .. lane + 15
+  for (int h = 0; h < G::VV / G::TR; h++)
+    wave_tile<G::TR, 5, G::ROW_DW>(w0 + (b * G::VV + h * G::TR) * G::ROW_DW + q, c, dx0, dy0 + h * G::TR, xl, xr, yu,
+                                   yd, kbest);
+  if (L == L0) {
+    const bool live = lane < 2 * L0;
+    const int l = live ? lane : 2 * L0 - 1;  // lanes past the line read in bounds, fold nothing
+    // column +L0: candidate (L0, l - L0) from window rows l .. l + 15, bytes
+    // 2 L0 .. 2 L0 + 15
     {
       uint32_t acc = 0;
-      const lds_u32* rp = win + 64 * kWaveRowDw + (lane >> 2);
-      const uint32_t sh = (uint32_t)(lane & 3);
+      const lds_u32* rp = w0 + l * G::ROW_DW + L0 / 2;
+#pragma unroll
+      for (int r = 0; r < 16; r++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc = __builtin_amdgcn_sad_u8(rp[r * G::ROW_DW + k], c[4 * r + k], acc);
+      fold_key(kbest, live, acc, L0, l - L0, xl, xr, yu, yd);
+    }
+    // row +L0: candidate (l - L0, L0) from window rows 2 L0 .. 2 L0 + 15,
+    // bytes l .. l + 15
+    {
+      uint32_t acc = 0;
+      const lds_u32* rp = w0 + 2 * L0 * G::ROW_DW + (l >> 2);
+      const uint32_t sh = (uint32_t)(l & 3);
 #pragma unroll
       for (int r = 0; r < 16; r++) {
         uint32_t w[5];
 #pragma unroll
-        for (int j = 0; j < 5; j++) w[j] = rp[r * kWaveRowDw + j];
+        for (int j = 0; j < 5; j++) w[j] = rp[r * G::ROW_DW + j];
 #pragma unroll
         for (int k = 0; k < 4; k++)
           acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(w[k + 1], w[k], sh), c[4 * r + k], acc);
       }
-      fold_key(kbest, acc, lane - 32, 32, xl, xr, yu, yd);
+      fold_key(kbest, live, acc, l - L0, L0, xl, xr, yu, yd);
     }
-    // candidate (32, 32): lane l the block dword (l / 4, l % 4)
+    // candidate (L0, L0): lane l the block dword (l / 4, l % 4)
     {
-      const uint32_t rw = win[(64 + (lane >> 2)) * kWaveRowDw + 16 + (lane & 3)];
+      const uint32_t rw = w0[(2 * L0 + (lane >> 2)) * G::ROW_DW + L0 / 2 + (lane & 3)];
       const int tot = dec::wave_sum((int)__builtin_amdgcn_sad_u8(cw, rw, 0u));
-      fold_key(kbest, (uint32_t)tot, 32, 32, xl, xr, yu, yd);
+      fold_key(kbest, true, (uint32_t)tot, L0, L0, xl, xr, yu, yd);
     }
   }
   kbest = dec::wave_min_u64(kbest);
@@ -754,11 +784,21 @@ void me_wave_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __
   decide_in_search(decide + t.stream, (dec::WaveLds*)win_raw, t.mb, lane, mx, my, (int)key_cost(kbest));
 }
 
-void launch_wave_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks,
+void launch_wave_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int L0,
                         const DecideDesc* d_decide, bool band, hipStream_t st) {
   if (ntasks <= 0) return;
-  hipLaunchKernelGGL(me_wave_kernel, dim3(ntasks), dim3(64), 0, st, d_streams, d_tasks, ntasks, d_decide,
-                     band ? 1 : 0);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(ntasks), dim3(64), 0, st, d_streams, d_tasks, ntasks, d_decide, band ? 1 : 0);
+  };
+  if (L0 == 32) go(me_wave_kernel<32>);
+  else if (L0 == 16) go(me_wave_kernel<16>);
+  else go(me_wave_kernel<8>);
+}
+
+int wave_class(int L) {
+  for (int L0 : {8, 16, 32})
+    if (L == L0 || L == L0 - 1) return L0;
+  return 0;
 }
 
 // ----------------------------------------------------------------- host side

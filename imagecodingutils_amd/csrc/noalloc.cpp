@@ -54,8 +54,8 @@ int device_scratch(uint8_t** out) {
 size_t up256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Descriptors into the scratch, and the task list in the order the handle
-// path sorts it: the full tasks (G MBs) row by row, then each row's shorter
-// last task.
+// path sorts it: one MB per task for the wave kernel (G = 0), else the full
+// tasks (G MBs) row by row, then each row's shorter last task.
 __global__ __launch_bounds__(256) void setup_kernel(const StreamDesc sd, const DecideDesc dd, const PadDesc pd,
                                                     StreamDesc* o_sd, DecideDesc* o_dd, PadDesc* o_pd,
                                                     SearchTask* tasks, int ntasks, int mbw, int G) {
@@ -66,9 +66,15 @@ __global__ __launch_bounds__(256) void setup_kernel(const StreamDesc sd, const D
     *o_pd = pd;
   }
   if (i >= ntasks) return;
-  const int per = mbw / G, nfull = per * sd.mbh;
   SearchTask t;
   t.stream = 0;
+  if (G == 0) {
+    t.mb = i;
+    t.count = 1;
+    tasks[i] = t;
+    return;
+  }
+  const int per = mbw / G, nfull = per * sd.mbh;
   if (i < nfull) {
     const int row = i / per;
     t.mb = row * mbw + (i - row * per) * G;
@@ -98,7 +104,9 @@ extern "C" int icu_me_estimate_device_noalloc(const icu_me_config* cfg, const ui
   const int L = cfg->window == ICU_WINDOW_FULL ? cfg->range : cfg->range - 1;
   const Tiling t = plan_tiling(mbw, L, cfg->metric, true, cfg->tile_rows, cfg->mbs_per_task);
   if (t.V == 0) return icu_fail(ICU_E_ARG, "tile_rows too tall for exact SAD argmin keys at this range");
-  const int per = mbw / t.G, ntasks = per * mbh + (mbw % t.G ? mbh : 0);
+  // one MB per wave where the window takes it (the handle path's default)
+  const int L0 = cfg->metric == ICU_METRIC_SAD && (cfg->grid_mode == 0 || cfg->grid_mode == 4) ? wave_class(L) : 0;
+  const int per = mbw / t.G, ntasks = L0 ? mbw * mbh : per * mbh + (mbw % t.G ? mbh : 0);
   // scratch layout: descriptors, tasks, padded reference plane
   const int ref_stride = (W + 2 * kPad + 255) & ~255;
   const size_t o_dec = up256(sizeof(StreamDesc)), o_pad = o_dec + up256(sizeof(DecideDesc));
@@ -164,11 +172,15 @@ extern "C" int icu_me_estimate_device_noalloc(const icu_me_config* cfg, const ui
   PadDesc* p_pd = (PadDesc*)(base + o_pad);
   SearchTask* p_task = (SearchTask*)(base + o_task);
   hipLaunchKernelGGL(setup_kernel, dim3((ntasks + 255) / 256), dim3(256), 0, st, sd, dd, pd, p_sd, p_dd, p_pd,
-                     p_task, ntasks, mbw, t.G);
+                     p_task, ntasks, mbw, L0 ? 0 : t.G);
   launch_pad_planes(p_pd, 1, pad_plane_workgroups(W, H), st);
-  const int stage = (search_stage_dwords(t.B, t.V, t.Q, t.G) + 63) & ~63;
-  launch_search(p_sd, p_task, ntasks, cfg->grid_mode ? cfg->grid_mode - 1 : 0, cfg->metric, t.V, false, stage,
-                nullptr, nullptr, p_dd, true, st);
+  if (L0) {
+    launch_wave_search(p_sd, p_task, ntasks, L0, p_dd, true, st);
+  } else {
+    const int stage = (search_stage_dwords(t.B, t.V, t.Q, t.G) + 63) & ~63;
+    launch_search(p_sd, p_task, ntasks, cfg->grid_mode ? cfg->grid_mode - 1 : 0, cfg->metric, t.V, false, stage,
+                  nullptr, nullptr, p_dd, true, st);
+  }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return icu_fail(ICU_E_DEVICE, std::string("launch: ") + hipGetErrorString(e));
   return ICU_OK;

@@ -256,15 +256,16 @@ def test_split_argmin_keys_tie_break_at_large_ranges(R):
 
 
 # One MB per wave (grid mode 3, config grid_mode 4): the default for SAD
-# argmin streams at +-31 / +-32.  Pictures small enough that every window
-# clamps, the reference and exhaustive windows, all sub-pel depths and the
-# final vector choice; the workgroup kernel (grid mode 0) must agree too.
+# argmin streams at +-7/8, +-15/16 and +-31/32.  Pictures small enough that
+# every window clamps, the reference and exhaustive windows, all sub-pel
+# depths and the final vector choice; the workgroup kernel (grid mode 0)
+# must agree too.
+@pytest.mark.parametrize("R", [8, 16, 32])
 @pytest.mark.parametrize("size,seed", [((64, 48), 1), ((176, 144), 2), ((352, 288), 3), ((96, 160), 4)])
 @pytest.mark.parametrize("window,subpel,bestvec", [(1, 1, 0), (0, 2, 1), (1, 0, 0), (0, 1, 1)])
-def test_wave_kernel_matches_oracle(size, seed, window, subpel, bestvec):
+def test_wave_kernel_matches_oracle(R, size, seed, window, subpel, bestvec):
     W, H = size
-    R = 32
-    fp = synth.rect_motion_pair(W, H, 60 + seed, n_rects=8, max_mv=min(W, H) // 3)
+    fp = synth.rect_motion_pair(W, H, 60 + seed + R, n_rects=8, max_mv=min(W // 3, H // 3, R))
     modes = dict(metric=1, window=window, filt=0, subpel=subpel, predictor=0, best_vector=bestvec)
     omv, odist, ointeg, _ = orc.me_estimate(fp.cur, fp.ref, R, **_oracle_modes(modes))
     for gm in (4, 1, 0):
@@ -281,15 +282,16 @@ def test_wave_kernel_matches_oracle(size, seed, window, subpel, bestvec):
         np.testing.assert_array_equal(d_dist.download(np.int32, n), odist, err_msg=f"grid {gm}")
 
 
+@pytest.mark.parametrize("R", [8, 16, 32])
 @pytest.mark.parametrize("window", [0, 1])
-def test_wave_kernel_tie_break_on_flat_and_periodic_content(window):
+def test_wave_kernel_tie_break_on_flat_and_periodic_content(window, R):
     """Many equal costs across the wave's tiles, its leftover column and row
-    and the (32, 32) corner: the wave's key reduction must pick the
+    and the (R, R) corner: the wave's key reduction must pick the
     reference's candidate."""
-    W, H, R = 352, 288, 32
+    W, H = 352, 288
     yy, xx = np.mgrid[0:H, 0:W]
     cur = np.where((xx // 48 + yy // 48) % 2 == 0, 90, ((xx % 4) * 30 + (yy % 8) * 9)).astype(np.uint8)
-    ref = np.roll(cur, (-29, 27), axis=(0, 1))
+    ref = np.roll(cur, (-R + 3, R - 5), axis=(0, 1))
     ref[:, 300:] = 90
     modes = dict(metric=1, window=window, filt=0, subpel=0, predictor=0, best_vector=0)
     omv, odist, _, _ = orc.me_estimate(cur, ref, R, **_oracle_modes(modes))
