@@ -62,83 +62,116 @@ __global__ __launch_bounds__(256) void inv_kernel(const int16_t* __restrict__ in
 // Fused coder: one workgroup of 256 lanes codes 16 MBs, one lane per 4x4.
 constexpr int kMbPerWg = 16;
 
-// The 22 x 22 luma patch of one MB for quarter-pel vector (mvx, mvy), staged
-// by the MB's 16 lanes: full / quarter split with C truncation, negative
-// fractions reflected to the full pel to the left / above; the patch origin
-// is clamped into the padding, where rows and columns past the picture
-// repeat its edge, so a clamped patch predicts what the far vector would.
-// The dword reads span [ox-6, ox+24] x [oy-3, oy+18]: 22 rows x 7 dwords from
-// the dword boundary at or left of ox-3 (the padded plane's origin and
-// stride are multiples of 16), ten loads per lane, issued by load() and
-// written to LDS by store() so that other loads can go in flight between.
-struct LumaPatch {
-  uint32_t w[10];
-  int sh, qx, qy;
-  __device__ __forceinline__ void load(const uint8_t* ref, int ref_stride, int W, int H, int n, int m,
-                                       int mvx, int mvy, int blk) {
-    const int fx = mvx / 4, fy = mvy / 4;
-    qx = mvx % 4;
-    qy = mvy % 4;
-    int fox = 0, foy = 0;
-    if (qx < 0) { fox = -1; qx += 4; }
-    if (qy < 0) { foy = -1; qy += 4; }
-    const int ox = min(max(n + fx + fox, -kPad + 3), W + kPad - 25);
-    const int oy = min(max(m + fy + foy, -kPad + 3), H + kPad - 19);
-    const int xa = ox - 3;
-    sh = xa & 3;
-    const uint32_t* base = (const uint32_t*)(ref + (long)(oy - 3) * ref_stride + (xa - sh));
-    const int sdw = ref_stride >> 2;
+// The 4x4 luma prediction of block (bi, bj) of the MB at (n, m) for the
+// quarter-pel vector (mvx, mvy), filtered in registers from the 9 x 9
+// reference samples it needs (rows and columns -2 .. +6 of the block's
+// full-pel origin), read as 9 x 3 aligned dwords from the padded plane: no
+// LDS patch and no barrier.  The MB's full-pel origin is split and clamped
+// into the padding (where rows and columns past the picture repeat its edge),
+// so a clamped origin predicts what the far vector would.  Each of the 16 fraction cases computes only
+// the half samples it needs (b, h, j of H.264 8.4.2.2.1, with the raw
+// vertical sums shared across j's columns); the MB's 16 lanes take the same
+// case.  pr[4 r + c]: the prediction at block row r, column c.
+__device__ __forceinline__ void predict_block(const uint8_t* __restrict__ ref, int ref_stride, int W, int H, int n,
+                                              int m, int mvx, int mvy, int bi, int bj, int (&pr)[16]) {
+  const int fx = mvx / 4, fy = mvy / 4;
+  int qx = mvx % 4, qy = mvy % 4, fox = 0, foy = 0;
+  if (qx < 0) { fox = -1; qx += 4; }
+  if (qy < 0) { foy = -1; qy += 4; }
+  const int ox = min(max(n + fx + fox, -kPad + 3), W + kPad - 25);
+  const int oy = min(max(m + fy + foy, -kPad + 3), H + kPad - 19);
+  const int x0 = ox + 4 * bj - 2, y0 = oy + 4 * bi - 2;
+  const int xa = x0 & ~3, sh = x0 - xa;
+  const uint32_t* base = (const uint32_t*)(ref + (long)y0 * ref_stride + xa);
+  const int sdw = ref_stride >> 2;
+  uint32_t d[9][3];
 #pragma unroll
-    for (int j = 0; j < 10; j++) {
-      const int i = blk + 16 * j, r = i / 7, k = i - 7 * r;
-      w[j] = i < 7 * kPatch ? base[r * sdw + k] : 0u;
+  for (int k = 0; k < 9; k++)
+#pragma unroll
+    for (int q = 0; q < 3; q++) d[k][q] = base[k * sdw + q];
+  // p[k][c]: reference sample (x0 + c, y0 + k); the block's full pels are p[2 + r][2 + c]
+  int p[9][9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const uint32_t a0 = __builtin_amdgcn_alignbyte(d[k][1], d[k][0], sh);
+    const uint32_t a1 = __builtin_amdgcn_alignbyte(d[k][2], d[k][1], sh);
+    const uint32_t a2 = d[k][2] >> (8 * sh);
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      const uint32_t a = c < 4 ? a0 : (c < 8 ? a1 : a2);
+      p[k][c] = (int)((a >> (8 * (c & 3))) & 0xFFu);
     }
   }
-  __device__ __forceinline__ void store(uint8_t* patch, int blk) const {
+  // half samples: b right of (r, c), h below it, j diagonal
+  auto G = [&](int r, int c) { return p[2 + r][2 + c]; };
+  auto B = [&](int r, int c) {
+    const int* q = p[2 + r];
+    return clip255((ICU_TAP6(q[c], q[c + 1], q[c + 2], q[c + 3], q[c + 4], q[c + 5]) + 16) >> 5);
+  };
+  auto Hh = [&](int r, int c) {
+    return clip255((ICU_TAP6(p[r][2 + c], p[r + 1][2 + c], p[r + 2][2 + c], p[r + 3][2 + c], p[r + 4][2 + c],
+                             p[r + 5][2 + c]) + 16) >> 5);
+  };
+  // j needs the raw vertical sums of rows r .. r + 5 at columns c .. c + 5
+  int J[16];
+  auto jgrid = [&]() {
 #pragma unroll
-    for (int j = 0; j < 10; j++) {
-      const int i = blk + 16 * j, r = i / 7, k = i - 7 * r;
-      if (i < 7 * kPatch) {
+    for (int r = 0; r < 4; r++) {
+      int v[9];
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-          const int c = 4 * k + b - sh;
-          if (c >= 0 && c < kPatch) patch[r * kPatch + c] = (uint8_t)(w[j] >> (8 * b));
-        }
-      }
+      for (int cc = 0; cc < 9; cc++)
+        v[cc] = ICU_TAP6(p[r][cc], p[r + 1][cc], p[r + 2][cc], p[r + 3][cc], p[r + 4][cc], p[r + 5][cc]);
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        J[4 * r + c] = clip255((ICU_TAP6(v[c], v[c + 1], v[c + 2], v[c + 3], v[c + 4], v[c + 5]) + 512) >> 10);
     }
+  };
+#define ICU_PRED(expr)                  \
+  _Pragma("unroll") for (int r = 0; r < 4; r++) \
+    _Pragma("unroll") for (int c = 0; c < 4; c++) pr[4 * r + c] = (expr)
+  switch (qx | (qy << 2)) {
+    case 0: ICU_PRED(G(r, c)); break;
+    case 2: ICU_PRED(B(r, c)); break;
+    case 8: ICU_PRED(Hh(r, c)); break;
+    case 10: jgrid(); ICU_PRED(J[4 * r + c]); break;
+    case 1: ICU_PRED((G(r, c) + B(r, c) + 1) >> 1); break;
+    case 3: ICU_PRED((G(r, c + 1) + B(r, c) + 1) >> 1); break;
+    case 4: ICU_PRED((G(r, c) + Hh(r, c) + 1) >> 1); break;
+    case 12: ICU_PRED((G(r + 1, c) + Hh(r, c) + 1) >> 1); break;
+    case 6: jgrid(); ICU_PRED((J[4 * r + c] + B(r, c) + 1) >> 1); break;
+    case 14: jgrid(); ICU_PRED((J[4 * r + c] + B(r + 1, c) + 1) >> 1); break;
+    case 9: jgrid(); ICU_PRED((J[4 * r + c] + Hh(r, c) + 1) >> 1); break;
+    case 11: jgrid(); ICU_PRED((J[4 * r + c] + Hh(r, c + 1) + 1) >> 1); break;
+    case 5: ICU_PRED((B(r, c) + Hh(r, c) + 1) >> 1); break;
+    case 7: ICU_PRED((B(r, c) + Hh(r, c + 1) + 1) >> 1); break;
+    case 13: ICU_PRED((Hh(r, c) + B(r + 1, c) + 1) >> 1); break;
+    default: ICU_PRED((Hh(r, c + 1) + B(r + 1, c) + 1) >> 1); break;
   }
-};
+#undef ICU_PRED
+}
 
 __global__ __launch_bounds__(256) void code_frame_kernel(const uint8_t* __restrict__ cur, int cur_stride,
                                                          const uint8_t* __restrict__ ref, int ref_stride,
                                                          int W, int H, const int16_t* __restrict__ mv,
                                                          const QpSet qs, int intra, int cols_first,
                                                          int16_t* __restrict__ pred_out) {
-  __shared__ uint8_t patch[kMbPerWg][kPatch * kPatch];
+  // one lane per 4x4 block; the 16 lanes of an MB are consecutive
   const int mbw = W / 16, nmb = mbw * (H / 16);
-  const int lmb = threadIdx.x >> 4, blk = threadIdx.x & 15;
-  const int mb = blockIdx.x * kMbPerWg + lmb;
-  const bool live = mb < nmb;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int mb = gid >> 4, blk = gid & 15;
+  if (mb >= nmb) return;
   const int n = (mb % mbw) * 16, m = (mb / mbw) * 16;
   const int bi = blk >> 2, bj = blk & 3;
-  LumaPatch lp;
-  lp.qx = lp.qy = 0;
   uint32_t cw[4];
-  if (live) {
-    lp.load(ref, ref_stride, W, H, n, m, mv[2 * mb], mv[2 * mb + 1], blk);
-    // this lane's 4x4 input block, in flight with the patch loads
 #pragma unroll
-    for (int rr = 0; rr < 4; rr++)
-      cw[rr] = *(const uint32_t*)(cur + (long)(m + 4 * bi + rr) * cur_stride + n + 4 * bj);
-    lp.store(patch[lmb], blk);
-  }
-  __syncthreads();
-  if (!live) return;
+  for (int rr = 0; rr < 4; rr++)
+    cw[rr] = *(const uint32_t*)(cur + (long)(m + 4 * bi + rr) * cur_stride + n + 4 * bj);
+  int prd[16];
+  predict_block(ref, ref_stride, W, H, n, m, mv[2 * mb], mv[2 * mb + 1], bi, bj, prd);
   int16_t b[16], pr[16];
 #pragma unroll
   for (int p = 0; p < 16; p++) {
-    const int r = 4 * bi + (p >> 2), c = 4 * bj + (p & 3);
-    pr[p] = (int16_t)qpel_p(patch[lmb], 3 + c, 3 + r, lp.qx, lp.qy);
+    pr[p] = (int16_t)prd[p];
     b[p] = (int16_t)((int)((cw[p >> 2] >> (8 * (p & 3))) & 0xFF) - pr[p]);
   }
   // 4 samples per row: 8-byte level / prediction stores, 4-byte recon stores
@@ -292,19 +325,15 @@ __global__ __launch_bounds__(256) void mc_yuv_kernel(const uint8_t* __restrict__
                                                      const uint8_t* __restrict__ ref_v, int cs, int W, int H,
                                                      const int16_t* __restrict__ mv, uint8_t* out_y,
                                                      uint8_t* out_u, uint8_t* out_v) {
-  __shared__ uint8_t patch[kMbPerWg][kPatch * kPatch];
   const int mbw = W / 16, nmb = mbw * (H / 16);
   const int lmb = threadIdx.x >> 4, blk = threadIdx.x & 15;
   const int mb = blockIdx.x * kMbPerWg + lmb;
-  const bool live = mb < nmb;
+  if (mb >= nmb) return;
   const int n = (mb % mbw) * 16, m = (mb / mbw) * 16;
-  LumaPatch lp;
-  lp.qx = lp.qy = 0;
+  const int mvx = mv[2 * mb], mvy = mv[2 * mb + 1];
   uint32_t cu[4], cv[4];
   int cfx = 0, cfy = 0, csh = 0;
-  if (live) {
-    const int mvx = mv[2 * mb], mvy = mv[2 * mb + 1];
-    lp.load(ref_y, ys, W, H, n, m, mvx, mvy, blk);
+  {
     // chroma: rows oy + r, oy + r + 1, columns ox + c0 .. + 4
     const int Wc = W >> 1, Hc = H >> 1;
     cfx = mvx % 8;
@@ -325,18 +354,16 @@ __global__ __launch_bounds__(256) void mc_yuv_kernel(const uint8_t* __restrict__
       cv[2 * k] = *(const uint32_t*)(ref_v + o);
       cv[2 * k + 1] = *(const uint32_t*)(ref_v + o + 4);
     }
-    lp.store(patch[lmb], blk);
   }
-  __syncthreads();
-  if (!live) return;
   // luma 4x4 block
   const int bi = blk >> 2, bj = blk & 3;
+  int prd[16];
+  predict_block(ref_y, ys, W, H, n, m, mvx, mvy, bi, bj, prd);
 #pragma unroll
   for (int rr = 0; rr < 4; rr++) {
     uint32_t o = 0;
 #pragma unroll
-    for (int cc = 0; cc < 4; cc++)
-      o |= (uint32_t)qpel_p(patch[lmb], 3 + 4 * bj + cc, 3 + 4 * bi + rr, lp.qx, lp.qy) << (8 * cc);
+    for (int cc = 0; cc < 4; cc++) o |= (uint32_t)prd[4 * rr + cc] << (8 * cc);
     *(uint32_t*)(out_y + (long)(m + 4 * bi + rr) * W + n + 4 * bj) = o;
   }
   // chroma: 4 samples of one row per plane
