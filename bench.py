@@ -328,7 +328,8 @@ def run_as_shipped(capi, synth, steps, warmup):
     s.sync()
     ms = a.elapsed_ms(b) / steps
     return {"workload": "1080p, reference class as shipped (SSD, filter, +-31, median predictor, quarter pel, "
-                        "GetBestVector)", "ms_per_frame": round(ms, 4), "mbs_per_s": round(n / (ms / 1e3), 1)}
+                        "GetBestVector)", "ms_per_frame": round(ms, 4), "mbs_per_s": round(n / (ms / 1e3), 1),
+            "tie_mbs": est.tie_count(), "mbs": n}
 
 
 FAST_RANGE = 128  # +-32 in the reference's quarter-pel motionRange

@@ -56,6 +56,7 @@ _SIGS = {
     "icu_me_estimate_timed": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, C.POINTER(C.c_float)]),
     "icu_me_estimate_device_noalloc": (C.c_int, [C.POINTER(MeConfig), _vp, _vp, _vp, _vp, _vp, _vp]),
     "icu_me_check": (C.c_int, [_vp]),
+    "icu_me_tie_count": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "icu_me_debug_set_wave_error": (C.c_int, [_vp]),
     "icu_me_get_tiling": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "icu_me_autotune": (C.c_int, [_vp, _vp, _vp, C.c_int]),
@@ -302,6 +303,13 @@ class MotionEstimator:
         """Waits for the device; IcuError(ICU_E_STATE) if a median-predictor
         wavefront of this handle timed out since the last check."""
         _check(lib.icu_me_check(self.h))
+
+    def tie_count(self) -> int:
+        """MBs of the last median-predictor estimate decided again in the
+        wavefront because their integer scan met an equal cost."""
+        t = C.c_int(0)
+        _check(lib.icu_me_tie_count(self.h, C.byref(t)))
+        return t.value
 
     def estimate_timed(self, d_cur, d_ref, d_mv, d_dist, stream: Stream | None = None, search_reps: int = 1):
         """Per-stage device times in ms: (pad, search, decide); the search time

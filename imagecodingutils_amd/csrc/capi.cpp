@@ -189,6 +189,7 @@ struct Engine {
   IntResult* d_ires = nullptr;
   uint32_t* d_table = nullptr;
   int* d_wave = nullptr;  // per stream: mbh progress counters + 2 (ticket, error)
+  uint8_t* d_pf = nullptr;  // predictor-free passes of the median-predictor streams' MBs
   std::vector<int> wave_off;
   std::vector<int> pad_base;  // first padding workgroup of each stream
   int pad_total = 0;          // padding workgroups of all streams
@@ -226,8 +227,9 @@ struct Engine {
     hipFree(d_ires);
     hipFree(d_table);
     hipFree(d_wave);
+    hipFree(d_pf);
     d_tasks = nullptr; d_items = nullptr;
-    d_ires = nullptr; d_table = nullptr; d_wave = nullptr;
+    d_ires = nullptr; d_table = nullptr; d_wave = nullptr; d_pf = nullptr;
     if (own) hipStreamDestroy(own);
     own = nullptr;
   }
@@ -345,6 +347,12 @@ struct Engine {
     HIP_TRY(hipMemset(d_wave, 0, sizeof(int) * wave_total));
     if (table_total) HIP_TRY(hipMalloc(&d_table, sizeof(uint32_t) * table_total));
     dev_bytes += sizeof(uint32_t) * table_total;
+    bool any_pred = false;
+    for (const auto& s : sp) any_pred |= s.cfg.predictor != 0;
+    if (any_pred) {
+      HIP_TRY(hipMalloc(&d_pf, (size_t)predfree_bytes() * total_mb));
+      dev_bytes += (long long)predfree_bytes() * total_mb;
+    }
     HIP_TRY(upload_tasks());
     h_items.clear();
     for (int i = 0; i < n; i++)
@@ -509,8 +517,10 @@ struct Engine {
       if (!sp[i].cfg.predictor) continue;
       // progress counters and the row ticket; the error flag after them
       // stays set until check_wave_errors reads it
+      uint8_t* pf = d_pf + (size_t)predfree_bytes() * sp[i].mb_base;
+      launch_decide_predfree(d_decide + i, pf, sp[i].nmb, st);
       launch_fill_i32(d_wave + wave_off[i], 0, sp[i].mbh + 1, st);
-      launch_decide_wavefront(d_decide + i, sp[i].table ? 2 * sp[i].L + 1 : 0, sp[i].mbh, st);
+      launch_decide_wavefront(d_decide + i, sp[i].table ? 2 * sp[i].L + 1 : 0, sp[i].mbh, pf, st);
     }
     if (ev) HIP_TRY(hipEventRecord(ev[3], st));
     HIP_TRY(hipGetLastError());
@@ -729,6 +739,20 @@ int icu_me_check(icu_me* h) {
   if (!h) return fail(ICU_E_ARG, "null handle");
   HIP_TRY(hipDeviceSynchronize());
   return h->eng.check_wave_errors();
+}
+
+int icu_me_tie_count(icu_me* h, int* tie_mbs) {
+  if (!h || !tie_mbs) return fail(ICU_E_ARG, "null argument");
+  Engine& e = h->eng;
+  *tie_mbs = 0;
+  if (!e.d_pf) return ICU_OK;
+  HIP_TRY(hipDeviceSynchronize());
+  const int pfb = predfree_bytes();
+  std::vector<int32_t> pf((size_t)pfb / 4 * e.total_mb);
+  HIP_TRY(hipMemcpy(pf.data(), e.d_pf, pf.size() * 4, hipMemcpyDeviceToHost));
+  // the tie flag is the last int of each record
+  for (int i = 0; i < e.total_mb; i++) *tie_mbs += pf[(size_t)(i + 1) * (pfb / 4) - 1] != 0;
+  return ICU_OK;
 }
 
 int icu_me_debug_set_wave_error(icu_me* h) {

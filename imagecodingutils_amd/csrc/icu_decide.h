@@ -218,6 +218,81 @@ __device__ inline void table_argmin(const TabView& tab, int xl, int xr, int yu, 
   *minD = (int)(best >> 40);
 }
 
+// The scan of table_replay / table_argmin without a predictor, for the
+// median-predictor mode's parallel pass.  The predictor enters the integer
+// stage only through the tie rule (Full.cpp:424-436): a candidate whose cost
+// equals the running minimum is taken unless the incumbent lies strictly
+// closer to the rounded predictor.  Here only strictly smaller costs are
+// taken, and *tie reports whether the scan ever met an equal cost among the
+// candidates the reference would have examined.  Without one, the result is
+// the reference's for every predictor; with one, the MB is decided again
+// once its predictor is known.
+__device__ inline void table_scan_nopred(const TabView& tab, bool filter, int xl, int xr, int yu, int yd,
+                                         int lane, int* pmx, int* pmy, int* pminD, int* ptie) {
+  const int w = xr - xl + 1, n = w * (yd - yu + 1);
+  int mx = 0, my = 0;
+  int minD = (int)tab[(0 - yu) * w + (0 - xl)];
+  bool tie = false;
+  if (!filter) {
+    // argmin: the least cost, and whether it is reached more than once
+    // (the zero vector's entry included, as the reference's incumbent)
+    unsigned long long best = ~0ull;
+    int cnt = 0;
+    for (int idx = lane; idx < n; idx += 64) {
+      const int i = yu + idx / w, j = xl + idx % w;
+      const uint32_t c = tab[idx];
+      const unsigned long long rr = (i == 0 && j == 0) ? 0x1FFFFull
+                                                       : 0xFFFFull - (unsigned long long)((i + 64) * 256 + (j + 64));
+      const unsigned long long key = ((unsigned long long)c << 40) | rr;
+      best = key < best ? key : best;
+    }
+    best = wave_min_u64(best);
+    const uint32_t cmin = (uint32_t)(best >> 40);
+    for (int idx = lane; idx < n; idx += 64) cnt += tab[idx] == cmin;
+    tie = wave_sum(cnt) > 1;
+    const uint32_t rr = (uint32_t)(best & 0x1FFFF);
+    if (rr != 0x1FFFF) {
+      const uint32_t rank = 0xFFFFu - rr;
+      my = (int)(rank >> 8) - 64;
+      mx = (int)(rank & 255) - 64;
+    }
+    minD = (int)cmin;
+  } else {
+    int sidx = 0;
+    while (sidx < n) {
+      const int idx = sidx + lane;
+      bool acc = false, eq = false;
+      int i = 0, j = 0, c = 0;
+      if (idx < n) {
+        i = yu + idx / w;
+        j = xl + idx % w;
+        if ((i | j) != 0 && !((i + my) < yu || (i + my) > yd || (j + mx) < xl || (j + mx) > xr)) {
+          c = (int)tab[idx];
+          acc = c < minD;
+          eq = c == minD;
+        }
+      }
+      const unsigned long long mask = __ballot(acc), eqm = __ballot(eq);
+      if (mask == 0) {
+        tie |= eqm != 0;
+        sidx += 64;
+        continue;
+      }
+      const int f = __ffsll((long long)mask) - 1;
+      // equal costs before the accepted one were examined at this minimum
+      tie |= (eqm & ((1ull << f) - 1ull)) != 0;
+      minD = __builtin_amdgcn_readlane(c, f);
+      mx = __builtin_amdgcn_readlane(j, f);
+      my = __builtin_amdgcn_readlane(i, f);
+      sidx += f + 1;
+    }
+  }
+  *pmx = mx;
+  *pmy = my;
+  *pminD = minD;
+  *ptie = tie ? 1 : 0;
+}
+
 // The reference's raster scan with its running-best filter (Full.cpp:401-446).
 __device__ inline void table_replay(const TabView& tab, int xl, int xr, int yu, int yd, int pXr, int pYr,
                              int lane, int* pmx, int* pmy, int* pminD) {
@@ -461,30 +536,103 @@ __device__ inline void refine_subpel(const DecideDesc& d, WaveLds& s, int n, int
 
 }
 
+// Four quarter-pel samples (OverlayMem2Dv2::QuarterRead's 16 cases, H.264
+// 8.4.2.2.1) at full pels (x .. x + 3, y) with fractions fx, fy in 0..3,
+// filtered in registers from reference rows y - 2 .. y + 3 and columns
+// x - 2 .. x + 6 of the padded plane, read as 6 x 3 aligned dwords.
+__device__ __forceinline__ void qpel_row4(const uint8_t* ref, int stride, int x, int y, int fx, int fy,
+                                          int (&out)[4]) {
+  const int xs = x - 2, xa = xs & ~3, sh = xs - xa;
+  const uint32_t* base = (const uint32_t*)(ref + (long)(y - 2) * stride + xa);
+  const int sdw = stride >> 2;
+  uint32_t d[6][3];
+#pragma unroll
+  for (int k = 0; k < 6; k++)
+#pragma unroll
+    for (int q = 0; q < 3; q++) d[k][q] = base[k * sdw + q];
+  // p[k][c]: reference (x - 2 + c, y - 2 + k); the full pels are p[2][2 + c]
+  int p[6][9];
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const uint32_t a0 = __builtin_amdgcn_alignbyte(d[k][1], d[k][0], sh);
+    const uint32_t a1 = __builtin_amdgcn_alignbyte(d[k][2], d[k][1], sh);
+    const uint32_t a2 = d[k][2] >> (8 * sh);
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      const uint32_t a = c < 4 ? a0 : (c < 8 ? a1 : a2);
+      p[k][c] = (int)((a >> (8 * (c & 3))) & 0xFFu);
+    }
+  }
+  auto G = [&](int r, int c) { return p[2 + r][2 + c]; };
+  auto B = [&](int r, int c) {  // right of (c, r), r = 0 or 1
+    const int* q = p[2 + r];
+    return clip255((ICU_TAP6(q[c], q[c + 1], q[c + 2], q[c + 3], q[c + 4], q[c + 5]) + 16) >> 5);
+  };
+  auto V = [&](int cc) {  // raw vertical sum below row 0 at column cc - 2
+    return ICU_TAP6(p[0][cc], p[1][cc], p[2][cc], p[3][cc], p[4][cc], p[5][cc]);
+  };
+  auto Hh = [&](int c) { return clip255((V(2 + c) + 16) >> 5); };
+  auto J = [&](int c) {
+    return clip255((ICU_TAP6(V(c), V(c + 1), V(c + 2), V(c + 3), V(c + 4), V(c + 5)) + 512) >> 10);
+  };
+#define ICU_ROW4(expr) _Pragma("unroll") for (int c = 0; c < 4; c++) out[c] = (expr)
+  switch (fx | (fy << 2)) {
+    case 0: ICU_ROW4(G(0, c)); break;
+    case 2: ICU_ROW4(B(0, c)); break;
+    case 8: ICU_ROW4(Hh(c)); break;
+    case 10: ICU_ROW4(J(c)); break;
+    case 1: ICU_ROW4((G(0, c) + B(0, c) + 1) >> 1); break;
+    case 3: ICU_ROW4((G(0, c + 1) + B(0, c) + 1) >> 1); break;
+    case 4: ICU_ROW4((G(0, c) + Hh(c) + 1) >> 1); break;
+    case 12: ICU_ROW4((G(1, c) + Hh(c) + 1) >> 1); break;
+    case 6: ICU_ROW4((J(c) + B(0, c) + 1) >> 1); break;
+    case 14: ICU_ROW4((J(c) + B(1, c) + 1) >> 1); break;
+    case 9: ICU_ROW4((J(c) + Hh(c) + 1) >> 1); break;
+    case 11: ICU_ROW4((J(c) + Hh(c + 1) + 1) >> 1); break;
+    case 5: ICU_ROW4((B(0, c) + Hh(c) + 1) >> 1); break;
+    case 7: ICU_ROW4((B(0, c) + Hh(c + 1) + 1) >> 1); break;
+    case 13: ICU_ROW4((Hh(c) + B(1, c) + 1) >> 1); break;
+    default: ICU_ROW4((Hh(c + 1) + B(1, c) + 1) >> 1); break;
+  }
+#undef ICU_ROW4
+}
+
 // Distortion of the input block against the quarter-pel read at full pel
 // (x0, y0) + fractions (fx, fy) in -3..3 (OverlayMem2Dv2::QuarterRead; a
 // negative fraction reads from the pel to the left / above with 4 + f,
 // OverlayMem2Dv2.cpp:709-724).
 __device__ inline int quarter_dist(const DecideDesc& d, WaveLds& s, int x0, int y0, int fx, int fy, int lane) {
-  const uint8_t* cur8 = (const uint8_t*)s.cur4;
   int fox = 0, foy = 0;
   if (fx < 0) { fox = -1; fx = 4 + fx; }
   if (fy < 0) { foy = -1; fy = 4 + fy; }
-  load_patch(d, x0 + fox, y0 + foy, s.P, lane);
-  wave_sync();
+  // lane l: block row l / 4, columns 4 (l % 4) .. + 3
+  const int r = lane >> 2, c0 = 4 * (lane & 3);
+  int q[4];
+  qpel_row4(d.ref, d.ref_stride, x0 + fox + c0, y0 + foy + r, fx, fy, q);
+  const uint32_t cw = s.cur4[lane];
   int acc = 0;
-  for (int p = lane; p < 256; p += 64) {
-    const int r = p >> 4, c = p & 15;
-    acc += pix_dist(cur8[p], qpel_p(s.P, 3 + c, 3 + r, fx, fy), d.metric);
-  }
-  const int v = wave_sum(acc);
-  wave_sync();
-  return v;
+#pragma unroll
+  for (int c = 0; c < 4; c++) acc += pix_dist((int)((cw >> (8 * c)) & 0xFF), q[c], d.metric);
+  return wave_sum(acc);
 }
 
 struct Decision {
   int mvx, mvy, dist;
   int imx, imy, iminD;
+};
+
+__device__ inline Decision finish_mb(const DecideDesc& d, WaveLds& s, int mb, int predX, int predY, int lane,
+                                     int mvx, int mvy, int minD, int zeroD, Decision out);
+
+// The median-predictor mode's predictor-free pass over one MB: the integer
+// stage without a predictor (table_scan_nopred) and the sub-pel refinement,
+// which depends on the integer vector only.
+struct PredFree {
+  int mvx, mvy, dist;   // refined vector (quarter pel) and its distortion
+  int imx, imy, iminD;  // integer stage
+  int zeroD;            // the zero vector's cost
+  int tie;              // the scan met an equal cost: decide again with the predictor (last field:
+                        // icu_me_tie_count reads it there)
 };
 
 // staged_tab (nullable): the MB's cost table already copied to LDS.
@@ -546,7 +694,19 @@ __device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, in
 
   int mvx, mvy;
   refine_subpel(d, s, n, m, mx, my, cw, lane, minD, mvx, mvy);
+  return finish_mb(d, s, mb, predX, predY, lane, mvx, mvy, minD, zeroD, out);
+}
 
+// The predictor-dependent end of an MB's decision: the predictor's
+// distortion and GetBestVector over {refined best, predictor, zero}
+// (Full.cpp:529-579), then the outputs.  out carries the integer stage.
+__device__ inline Decision finish_mb(const DecideDesc& d, WaveLds& s, int mb, int predX, int predY, int lane,
+                                     int mvx, int mvy, int minD, int zeroD, Decision out) {
+  const int col = mb % d.mbw, row = mb / d.mbw;
+  const int n = col * 16, m = row * 16;
+  const int slot = d.mb_base + mb;
+  const int predXQ = predX % 4, predYQ = predY % 4;
+  int predX0 = predX / 4, predY0 = predY / 4;
   int outx = mvx, outy = mvy, outd = minD;
   if (d.bestvec) {
     if (predX0 + n > d.W) predX0 = 16;
@@ -591,6 +751,52 @@ __device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, in
     }
   }
   return out;
+}
+
+// The predictor-free pass of one MB (cost-table modes).
+__device__ inline void predfree_mb(const DecideDesc& d, WaveLds& s, int mb, int lane, PredFree* out) {
+  const int col = mb % d.mbw, row = mb / d.mbw;
+  const int n = col * 16, m = row * 16;
+  const uint32_t cw = *(const uint32_t*)(d.cur + (long)(m + (lane >> 2)) * d.cur_stride + n + 4 * (lane & 3));
+  s.cur4[lane] = cw;
+  int xl, xr, yu, yd;
+  motion_window(n, m, d.W, d.H, d.L, &xl, &xr, &yu, &yd);
+  TabView tab;
+  tab.t = d.table + d.table_base + (long long)mb * d.table_dim * d.table_dim;
+  tab.dim = d.table_dim;
+  tab.L = d.L;
+  tab.xl = xl;
+  tab.yu = yu;
+  tab.w = xr - xl + 1;
+  const int zeroD = (int)tab[(0 - yu) * tab.w + (0 - xl)];
+  int mx, my, minD, tie;
+  table_scan_nopred(tab, d.filter != 0, xl, xr, yu, yd, lane, &mx, &my, &minD, &tie);
+  PredFree pf;
+  pf.imx = mx;
+  pf.imy = my;
+  pf.iminD = minD;
+  pf.zeroD = zeroD;
+  pf.tie = tie;
+  int mvx, mvy;
+  refine_subpel(d, s, n, m, mx, my, cw, lane, minD, mvx, mvy);
+  pf.mvx = mvx;
+  pf.mvy = mvy;
+  pf.dist = minD;
+  if (lane == 0) *out = pf;
+}
+
+// An MB's decision once its predictor is known, from its predictor-free
+// pass: without a tie only the predictor-dependent end remains.
+// cw: this lane's dword of the input block (row lane / 4, columns 4 (lane % 4)).
+__device__ inline Decision decide_mb_after(const DecideDesc& d, WaveLds& s, int mb, int predX, int predY, int lane,
+                                           const PredFree& pf, uint32_t cw, const uint32_t* staged_tab = nullptr) {
+  if (pf.tie) return decide_mb(d, s, mb, predX, predY, lane, nullptr, staged_tab);
+  s.cur4[lane] = cw;
+  Decision out;
+  out.imx = pf.imx;
+  out.imy = pf.imy;
+  out.iminD = pf.iminD;
+  return finish_mb(d, s, mb, predX, predY, lane, pf.mvx, pf.mvy, pf.dist, pf.zeroD, out);
 }
 
 // Hand-off between the waves of a wavefront kernel.  Everything one wave

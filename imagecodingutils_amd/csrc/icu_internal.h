@@ -91,8 +91,14 @@ int search_max_group();
 
 void launch_decide_parallel(const DecideDesc* d_descs, const DecideItem* d_items, int nitems,
                             hipStream_t st);
-// table_dim: the per-MB cost table side (0 when no table is kept).
-void launch_decide_wavefront(const DecideDesc* d_desc, int table_dim, int mbh, hipStream_t st);
+// Median-predictor streams: the predictor-free pass of the stream's nmb MBs
+// into d_pf (predfree_bytes() each), then the wavefront that finishes each
+// MB once its neighbours are decided.  table_dim: the per-MB cost table side
+// (0 when no table is kept).
+int predfree_bytes();
+void launch_decide_predfree(const DecideDesc* d_desc, void* d_pf, int nmb, hipStream_t st);
+void launch_decide_wavefront(const DecideDesc* d_desc, int table_dim, int mbh, const void* d_pf,
+                             hipStream_t st);
 void launch_fill_i32(int* p, int v, int n, hipStream_t st);
 // Adds an order-free checksum of `bytes` bytes at p (4-byte aligned) to *out.
 void launch_checksum(const void* p, long bytes, unsigned long long* out, hipStream_t st);

@@ -45,12 +45,28 @@ __global__ __launch_bounds__(64) void decide_parallel_kernel(const DecideDesc* _
   }
 }
 
+// The median-predictor mode's predictor-free pass, one wave per MB, all MBs
+// in parallel: the integer stage without a predictor and the sub-pel
+// refinement (predfree_mb).
+__global__ __launch_bounds__(64) void decide_predfree_kernel(const DecideDesc* __restrict__ desc,
+                                                             PredFree* __restrict__ pf, int nmb) {
+  __shared__ WaveLds s;
+  const int lane = threadIdx.x;
+  for (int mb = blockIdx.x; mb < nmb; mb += gridDim.x) {
+    predfree_mb(*desc, s, mb, lane, pf + mb);
+    __syncthreads();
+  }
+}
+
 // One wave per MB row; rows are handed out by ticket in start order, so the
 // row a wave waits on always belongs to a wave that is already running.
-// stage_tab: the MB's cost table (table_dim^2 dwords) is copied into the
-// dynamic LDS with global_load_lds before the MB waits for its neighbours,
-// so the filtered scan's rounds read LDS instead of HBM.
-__global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* __restrict__ desc, int stage_tab) {
+// pf: the MBs' predictor-free passes; an MB whose scan met no tie only adds
+// its predictor's distortion and the final choice, one that did is decided
+// in full.  stage_tab: such an MB's cost table (table_dim^2 dwords) is
+// copied into the dynamic LDS with global_load_lds before the MB waits for
+// its neighbours, so the filtered scan's rounds read LDS instead of HBM.
+__global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* __restrict__ desc, int stage_tab,
+                                                              const PredFree* __restrict__ pf) {
   __shared__ WaveLds s;
   __shared__ int row_s;
   extern __shared__ uint32_t s_tab[];
@@ -63,8 +79,14 @@ __global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* 
   const int row = row_s;
   if (row >= d.mbh) return;
   const int n2 = d.table_dim * d.table_dim;
+  int Ax = 0, Ay = 0;  // the left neighbour: this wave's previous MB
   for (int col = 0; col < d.mbw; col++) {
-    if (stage_tab) {
+    // what does not depend on the neighbours is read before waiting for them
+    const PredFree mine = pf[row * d.mbw + col];
+    const uint32_t cw =
+        *(const uint32_t*)(d.cur + (long)(16 * row + (lane >> 2)) * d.cur_stride + 16 * col + 4 * (lane & 3));
+    const bool stage = stage_tab && mine.tie;
+    if (stage) {
       const uint32_t* src = d.table + d.table_base + (long long)(row * d.mbw + col) * n2;
       for (int base = 0; base < n2; base += 64)
         if (base + lane < n2)
@@ -99,19 +121,20 @@ __global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* 
       *x = (int)(int16_t)(w & 0xFFFF);
       *y = (int)(int16_t)(w >> 16);
     };
-    int Ax = 0, Ay = 0, Bx = 0, By = 0, Cx = 0, Cy = 0;
+    int Bx = 0, By = 0, Cx = 0, Cy = 0;
     const bool hasB = row > 0, hasC = row > 0 && col + 1 < d.mbw;
-    if (col > 0) ld(mb - 1, &Ax, &Ay);
     if (hasB) ld(mb - d.mbw, &Bx, &By);
     if (hasC) ld(mb - d.mbw + 1, &Cx, &Cy);
     else if (col > 0 && row > 0) ld(mb - d.mbw - 1, &Cx, &Cy);
     if (!hasB && !hasC) { Bx = Ax; By = Ay; Cx = Ax; Cy = Ay; }
     const int px = median3(Ax, Bx, Cx), py = median3(Ay, By, Cy);
-    if (stage_tab) {
+    if (stage) {
       __builtin_amdgcn_s_waitcnt(0);  // the table has landed
       wave_sync();
     }
-    decide_mb(d, s, mb, px, py, lane, nullptr, stage_tab ? s_tab : nullptr);
+    const Decision dec = decide_mb_after(d, s, mb, px, py, lane, mine, cw, stage ? s_tab : nullptr);
+    Ax = dec.mvx;
+    Ay = dec.mvy;
     __syncthreads();
     if (lane == 0) {
       publish_progress(&progress[row], col + 1);
@@ -137,12 +160,21 @@ void launch_decide_parallel(const DecideDesc* d_descs, const DecideItem* d_items
   hipLaunchKernelGGL(decide_parallel_kernel, dim3(nitems), dim3(64), 0, st, d_descs, d_items, nitems);
 }
 
-void launch_decide_wavefront(const DecideDesc* d_desc, int table_dim, int mbh, hipStream_t st) {
-  // stage the per-MB table in LDS when it fits 48 KB (ranges up to +-55)
+void launch_decide_predfree(const DecideDesc* d_desc, void* d_pf, int nmb, hipStream_t st) {
+  if (nmb <= 0) return;
+  hipLaunchKernelGGL(decide_predfree_kernel, dim3(nmb), dim3(64), 0, st, d_desc, (PredFree*)d_pf, nmb);
+}
+
+void launch_decide_wavefront(const DecideDesc* d_desc, int table_dim, int mbh, const void* d_pf,
+                             hipStream_t st) {
+  // stage a tie MB's table in LDS when it fits 48 KB (ranges up to +-55)
   const size_t bytes = (size_t)table_dim * table_dim * 4;
   const int stage = table_dim > 0 && bytes <= 48 * 1024;
-  hipLaunchKernelGGL(decide_wavefront_kernel, dim3(mbh), dim3(64), stage ? bytes : 0, st, d_desc, stage);
+  hipLaunchKernelGGL(decide_wavefront_kernel, dim3(mbh), dim3(64), stage ? bytes : 0, st, d_desc, stage,
+                     (const PredFree*)d_pf);
 }
+
+int predfree_bytes() { return (int)sizeof(PredFree); }
 
 void launch_fill_i32(int* p, int v, int n, hipStream_t st) {
   if (n <= 0) return;

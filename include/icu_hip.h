@@ -117,6 +117,12 @@ int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref,
  * call this after their results are complete.  No reference counterpart
  * (the reference's raster loop cannot time out). */
 int icu_me_check(icu_me* h);
+/* Median-predictor handles: waits for the device and counts the MBs of the
+ * last estimate whose integer scan met an equal cost, so the wavefront
+ * decided them again with their predictor (the others only add the
+ * predictor's distortion and the final choice there).  0 for handles
+ * without the predictor.  No reference counterpart; a diagnostic. */
+int icu_me_tie_count(icu_me* h, int* tie_mbs);
 /* Test hook: sets the handle's wavefront error flag from the host, so the
  * reporting paths can be exercised without stalling a GPU.  ICU_E_STATE for
  * a handle without the median predictor.  No reference counterpart. */
