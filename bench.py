@@ -412,9 +412,12 @@ def run_serving(capi, synth, seconds, lanes=4, max_inflight=32):
         nmb.append((w // 16) * (h // 16))
         ids.append(srv.open_stream(capi.make_config(w, h, R, **MODES)))
     sent = [0] * SERVE_STREAMS
+    call_max = [0.0]
 
     def submit(k, tag):
+        t = capi.Server.clock()
         srv.submit(ids[k], ring[k][sent[k] % SERVE_RING], tag=tag)
+        call_max[0] = max(call_max[0], capi.Server.clock() - t)
         sent[k] += 1
 
     # warm-up: a few frames per stream (first frames, kernels, buffer pools)
@@ -425,6 +428,7 @@ def run_serving(capi, synth, seconds, lanes=4, max_inflight=32):
     while done < 2 * SERVE_STREAMS:
         done += len(srv.poll(wait=True))
     # Poisson arrivals
+    call_max[0] = 0.0
     n = max(1, int(SERVE_RATE * seconds))
     arrive = np.cumsum(rng.exponential(1.0 / SERVE_RATE, n))
     who = rng.integers(0, SERVE_STREAMS, n)
@@ -466,6 +470,10 @@ def run_serving(capi, synth, seconds, lanes=4, max_inflight=32):
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
             "max_ms": round(float(lat.max()) * 1e3, 4),
+            # the longest a single submit call blocked the request loop (the
+            # runtime's pinned upload of a frame has stalled for ms at times)
+            "max_submit_call_ms": round(call_max[0] * 1e3, 4),
+            "frames_over_1ms": int((lat > 1e-3).sum()),
             "device_bytes_peak": st["peak_bytes"],
             "saturated_mbs_per_s": round(sat_mbs / sat_s, 1),
             "saturated_frames_per_s": round(sat_n / sat_s, 1)}
@@ -577,7 +585,7 @@ def main():
     ap.add_argument("--no-4k", action="store_true", help="skip the 4K multi-reference leg")
     ap.add_argument("--settle-ms", type=float, default=100.0,
                     help="untimed steps before the warm-up, so the clocks have left their idle state")
-    ap.add_argument("--serve-seconds", type=float, default=10.0,
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
                     help="length of the serving leg's Poisson arrival run (0: skip the leg)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB: a kernel of known traffic "

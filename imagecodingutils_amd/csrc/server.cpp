@@ -68,6 +68,9 @@ struct Pool {
   }
 };
 
+// Frames of a stream whose transient buffers are allocated when it opens.
+constexpr int kPoolFrames = 3;
+
 struct Pending {
   long long tag;
   void* h_frame;  // pinned copy of the submitted frame
@@ -312,6 +315,38 @@ int icu_server_open_stream(icu_server* s, const icu_me_config* cfg, int* stream_
     hipFree(st.d_levels);
     hipFree(st.d_sums);
     return icu::icu_fail(ICU_E_DEVICE, "stream allocation failed");
+  }
+  // fill the buffer pools for a few frames of this stream now: growing a
+  // pool mid-run calls hipMalloc / hipHostMalloc, which block the caller for
+  // milliseconds
+  {
+    const size_t ob = icu_server::out_bytes(st);
+    void* d[3 * kPoolFrames];
+    void* h[2 * kPoolFrames];
+    bool ok = true;
+    for (int k = 0; k < kPoolFrames; k++) {
+      d[3 * k] = s->dev.get((size_t)px);
+      d[3 * k + 1] = s->dev.get(4 * (size_t)st.nmb);
+      d[3 * k + 2] = s->dev.get(4 * (size_t)st.nmb);
+      h[2 * k] = s->host.get((size_t)px);
+      h[2 * k + 1] = s->host.get(ob);
+      ok = ok && d[3 * k] && d[3 * k + 1] && d[3 * k + 2] && h[2 * k] && h[2 * k + 1];
+    }
+    for (int k = 0; k < kPoolFrames; k++) {
+      if (d[3 * k]) s->dev.put(d[3 * k], (size_t)px);
+      if (d[3 * k + 1]) s->dev.put(d[3 * k + 1], 4 * (size_t)st.nmb);
+      if (d[3 * k + 2]) s->dev.put(d[3 * k + 2], 4 * (size_t)st.nmb);
+      if (h[2 * k]) s->host.put(h[2 * k], (size_t)px);
+      if (h[2 * k + 1]) s->host.put(h[2 * k + 1], ob);
+    }
+    if (!ok) {
+      icu_me_destroy(st.h);
+      hipFree(st.d_ref);
+      hipFree(st.d_recon);
+      hipFree(st.d_levels);
+      hipFree(st.d_sums);
+      return icu::icu_fail(ICU_E_DEVICE, "stream buffer pool allocation failed");
+    }
   }
   st.resident = resident;
   s->resident += resident;
