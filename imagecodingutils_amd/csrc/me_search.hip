@@ -677,7 +677,7 @@ struct WaveGeom {
 };
 
 template <int L0>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
 void me_wave_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __restrict__ tasks, int ntasks,
                     const DecideDesc* __restrict__ decide, int band) {
   using G = WaveGeom<L0>;
@@ -781,7 +781,10 @@ void me_wave_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __
   key_vector(kbest, &mx, &my);
   dec::wave_sync();  // every lane's window reads are done: the decision reuses the LDS
   __builtin_amdgcn_s_setprio(2);
-  decide_in_search(decide + t.stream, (dec::WaveLds*)win_raw, t.mb, lane, mx, my, (int)key_cost(kbest));
+  // inlined: with the block in SGPRs the tile and the decision fit 80 VGPRs
+  // together (6 waves per SIMD); called out of line the decision alone took 87
+  const IntResult r{mx, my, (int)key_cost(kbest), -1};
+  dec::decide_mb(decide[t.stream], *(dec::WaveLds*)win_raw, t.mb, 0, 0, lane, &r);
 }
 
 void launch_wave_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int L0,
