@@ -29,11 +29,12 @@ QPS = (22, 28, 34)
 
 # HBM traffic of one search launch at this workload, from two rocprofv3
 # --pmc runs of `bench.py --no-cpu --no-batched --no-4k --serve-seconds 0
-# --steps 5 --warmup 1 --copy-probe 256` (KiB per dispatch), each counter scaled by what it reads
+# --steps 20 --warmup 5 --copy-probe 256` (KiB per dispatch of
+# me_wave_kernel<32>), each counter scaled by what it reads
 # for the 256 MiB device copy / fill of --copy-probe: FETCH_SIZE reads half
 # the copy's 262144 KiB (x2.0), WRITE_SIZE reads it exactly (x1.0).  Measured
 # counters, not computed here: refresh them when the kernel changes.
-SEARCH_FETCH_KIB, FETCH_SCALE = 2836.1, 2.0
+SEARCH_FETCH_KIB, FETCH_SCALE = 2940.3, 2.0
 SEARCH_WRITE_KIB, WRITE_SCALE = 287.0, 1.0
 MODES = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=0)
 
@@ -572,6 +573,11 @@ def _cpu_info():
     return model, os.cpu_count(), len(os.sched_getaffinity(0))
 
 
+def note(msg):
+    """One progress line per finished leg, on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -606,6 +612,7 @@ def main():
         capi.lib.icu_stream_sync(None)
         del a, b
     plain = run_plain(capi, synth, args.steps, args.warmup, tune=args.tune, settle_ms=args.settle_ms)
+    note(f"plain 1080p leg: {plain['total_ms'] / args.steps:.4f} ms per step")
     ms_per_step = plain["total_ms"] / args.steps
     value = plain["nmb"] / (ms_per_step / 1e3)
     ops = sad_ops(W, H, R)
@@ -650,16 +657,23 @@ def main():
                           "value": round(bt["mbps"], 1), "plain_same_streams_mbps": round(bt["plain_mbps"], 1),
                           "batched_vs_plain": round(bt["mbps"] / bt["plain_mbps"], 3),
                           "achieved_T_sad_ops": round(bt["sad_ops"] / (bt["ms"] / 1e3) / 1e12, 2)}
+        note("batched leg done")
     out["mc_yuv_1080p"] = run_mc_yuv(capi, synth, 20, 3)
     out["cavlc_1080p"] = run_cavlc(capi, synth, 20, 3)
+    note("YUV compensation and CAVLC legs done")
     out["as_shipped_1080p"] = run_as_shipped(capi, synth, 10, 2)
+    note("as-shipped leg done")
     out["fast_estimators_1080p"] = run_fast(capi, synth, 10, 2)
+    note("fast estimator legs done")
     if not args.no_4k:
         out["multiref_4k"] = run_multiref_4k(capi, synth, max(5, args.steps // 5), 2)
+        note("4K multi-reference leg done")
     if args.serve_seconds > 0:
         out["serving"] = run_serving(capi, synth, args.serve_seconds)
+        note("serving leg done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(synth)
+        note("CPU baseline done")
     print(json.dumps(out), flush=True)
     return 0
 
