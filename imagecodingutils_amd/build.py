@@ -74,7 +74,7 @@ def build_dropin_test() -> Path:
     src = PKG.parent / "tests" / "cpp" / "dropin_test.cpp"
     exe = src.with_suffix("")
     cxx = shutil.which("g++") or "g++"
-    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-I", str(PKG.parent / "include"), str(src), "-o", str(exe),
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-pthread", "-I", str(PKG.parent / "include"), str(src), "-o", str(exe),
            "-L", str(PKG), "-licu_hip", "-Wl,-rpath,$ORIGIN/../../imagecodingutils_amd"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

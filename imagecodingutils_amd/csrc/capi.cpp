@@ -545,16 +545,20 @@ struct Engine {
     return e;
   }
 
+  // Only the handle's own stream is used (a null-stream copy would wait for
+  // every other handle's work too).
   int check_wave_errors() {
     bool timed_out = false;
     for (size_t i = 0; i < sp.size(); i++) {
       if (!sp[i].cfg.predictor) continue;
       int* flag = d_wave + wave_off[i] + sp[i].mbh + 1;
       int v = 0;
-      HIP_TRY(hipMemcpy(&v, flag, sizeof(v), hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpyAsync(&v, flag, sizeof(v), hipMemcpyDeviceToHost, own));
+      HIP_TRY(hipStreamSynchronize(own));
       if (v) {
         timed_out = true;
-        HIP_TRY(hipMemset(flag, 0, sizeof(int)));
+        launch_fill_i32(flag, 0, 1, own);
+        HIP_TRY(hipStreamSynchronize(own));
       }
     }
     return timed_out ? fail(ICU_E_STATE, "median-predictor wavefront timed out") : ICU_OK;

@@ -463,12 +463,13 @@ int icu_cross_estimate(icu_cross* h, const int16_t* cur, const int16_t* ref, con
   std::vector<int32_t> dd(nmb);
   CR_TRY(hipMemcpyAsync(mv, h->d_mv, 4 * nmb, hipMemcpyDeviceToHost, h->own));
   CR_TRY(hipMemcpyAsync(dd.data(), h->d_dist, 4 * nmb, hipMemcpyDeviceToHost, h->own));
+  // everything on the handle's own stream: objects driven from different
+  // threads overlap on the device (a null-stream copy would wait for them all)
+  int flags[2] = {0, 0};
+  if (h->median)
+    CR_TRY(hipMemcpyAsync(flags, h->d_wave + h->mbh, sizeof(flags), hipMemcpyDeviceToHost, h->own));
   CR_TRY(hipStreamSynchronize(h->own));
-  if (h->median) {
-    int flags[2] = {0, 0};
-    CR_TRY(hipMemcpy(flags, h->d_wave + h->mbh, sizeof(flags), hipMemcpyDeviceToHost));
-    if (flags[1]) return icu_fail(ICU_E_STATE, "Cross wavefront timed out");
-  }
+  if (flags[1]) return icu_fail(ICU_E_STATE, "Cross wavefront timed out");
   if (dist) std::copy(dd.begin(), dd.end(), dist);
   if (avg) {
     long long tot = 0;

@@ -524,9 +524,11 @@ int icu_umhs_estimate(icu_umhs* h, const int16_t* cur, const int16_t* ref, const
   std::vector<int32_t> dd(nmb);
   UM_TRY(hipMemcpyAsync(mv, h->d_mv, 4 * nmb, hipMemcpyDeviceToHost, h->own));
   UM_TRY(hipMemcpyAsync(dd.data(), h->d_dist, 4 * nmb, hipMemcpyDeviceToHost, h->own));
-  UM_TRY(hipStreamSynchronize(h->own));
+  // everything on the handle's own stream: objects driven from different
+  // threads overlap on the device (a null-stream copy would wait for them all)
   int flags[2] = {0, 0};
-  UM_TRY(hipMemcpy(flags, h->d_wave + h->mbh, sizeof(flags), hipMemcpyDeviceToHost));
+  UM_TRY(hipMemcpyAsync(flags, h->d_wave + h->mbh, sizeof(flags), hipMemcpyDeviceToHost, h->own));
+  UM_TRY(hipStreamSynchronize(h->own));
   if (flags[1]) return icu_fail(ICU_E_STATE, "UMHS wavefront timed out");
   if (dist) std::copy(dd.begin(), dd.end(), dist);
   if (avg) {

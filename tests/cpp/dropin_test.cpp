@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "codecutils_hip.hpp"
@@ -327,6 +328,75 @@ int main(int argc, char** argv) {
     std::printf("fwd_ns_per_call %.1f inv_ns_per_call %.1f\n", fns, ins);
     return 0;
   }
-  std::fprintf(stderr, "usage: dropin_test me|fwd|inv|mc|cavlc|fast|api|txtime ...\n");
+  if (cmd == "threads" && argc >= 7) {
+    // threads umhs|fhs W H nthreads nframes: one estimator object per
+    // thread (each its own stream of frames), as a codec coding several
+    // streams on several host threads drives the reference classes; first
+    // all streams one after another, then all threads at once.  Results
+    // must be the same; prints the wall time per frame of both.
+    const std::string kind = argv[2];
+    const int W = std::atoi(argv[3]), H = std::atoi(argv[4]), nt = std::atoi(argv[5]), nf = std::atoi(argv[6]);
+    const size_t np = (size_t)W * H;
+    const int nmb = (W / 16) * (H / 16);
+    // frame f of stream t: a texture panned by (t + f, 2 f) pels
+    auto frame = [&](int t, int f, std::vector<short>& out) {
+      for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+          const int u = x + t + f, v = y + 2 * f;
+          out[(size_t)y * W + x] = (short)((u * 7 + v * 13 + ((u * v) % 23) + ((u / 9 + v / 7) % 5) * 31) & 255);
+        }
+    };
+    auto run = [&](int t, std::vector<short>* out) {
+      std::vector<short> cur(np), ref(np);
+      H264MotionVectorPredictorImpl1 med(W / 16, H / 16);
+      IMotionEstimator* me;
+      if (kind == "umhs") me = new MotionEstimatorH264ImplUMHS(cur.data(), ref.data(), W, H, 128, &med);
+      else me = new MotionEstimatorH264ImplFHS(cur.data(), ref.data(), W, H, 128, &med);
+      if (!me->Create()) {
+        delete me;
+        return false;
+      }
+      me->SetMode(0);
+      bool ok = true;
+      for (int f = 0; f < nf && ok; f++) {
+        frame(t, f, ref);
+        frame(t, f + 1, cur);
+        long avg = 0;
+        VectorStructList* l = (VectorStructList*)me->Estimate(&avg);
+        if (!l) {
+          ok = false;
+          break;
+        }
+        for (int i = 0; i < nmb; i++) {
+          out->push_back((short)l->GetSimpleElement(i, 0));
+          out->push_back((short)l->GetSimpleElement(i, 1));
+        }
+      }
+      delete me;
+      return ok;
+    };
+    std::vector<std::vector<short>> serial(nt), par(nt);
+    run(0, &serial[0]);  // warm-up: device initialisation, first launches
+    serial[0].clear();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int t = 0; t < nt; t++)
+      if (!run(t, &serial[t])) return 30;
+    const auto t1 = std::chrono::steady_clock::now();
+    std::vector<std::thread> th;
+    std::vector<int> oks(nt, 0);
+    for (int t = 0; t < nt; t++) th.emplace_back([&, t] { oks[t] = run(t, &par[t]) ? 1 : 0; });
+    for (auto& x : th) x.join();
+    const auto t2 = std::chrono::steady_clock::now();
+    for (int t = 0; t < nt; t++) {
+      if (!oks[t]) return 31;
+      if (par[t] != serial[t]) return 32;
+    }
+    const double frames = (double)nt * nf;
+    std::printf("serial_ms_per_frame %.3f threaded_ms_per_frame %.3f\n",
+                std::chrono::duration<double, std::milli>(t1 - t0).count() / frames,
+                std::chrono::duration<double, std::milli>(t2 - t1).count() / frames);
+    return 0;
+  }
+  std::fprintf(stderr, "usage: dropin_test me|fwd|inv|mc|cavlc|fast|api|txtime|threads ...\n");
   return 1;
 }

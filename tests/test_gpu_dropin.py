@@ -150,3 +150,15 @@ def test_dropin_per_block_transforms_stay_on_the_host():
     fwd_ns, inv_ns = float(out[1]), float(out[3])
     print(f"per-block calls: forward {fwd_ns:.1f} ns, inverse {inv_ns:.1f} ns")
     assert fwd_ns < 1000 and inv_ns < 1000
+
+
+@pytest.mark.parametrize("kind", ["umhs", "fhs"])
+def test_dropin_fast_estimators_overlap_across_threads(kind):
+    """Four UMHS / FHS objects, one per host thread (four streams of 720p
+    frames, 3 frames each), give the same vectors as the four run one after
+    another, and overlap on the device: each object works on its own HIP
+    stream only, so the wavefronts of different objects run side by side."""
+    out = _run("threads", kind, 1280, 720, 4, 3).split()
+    serial, threaded = float(out[1]), float(out[3])
+    print(f"{kind}: serial {serial:.3f} ms per frame, 4 threads {threaded:.3f} ms per frame")
+    assert threaded < 0.8 * serial
