@@ -393,11 +393,12 @@ class MotionEstimatorBatch:
 
 
 # -------------------------------------------------------------------- CAVLC
-CAVLC_2X2, CAVLC_4X4 = 0, 1
+CAVLC_2X2, CAVLC_4X4, CAVLC_8X8 = 0, 1, 2
+CAVLC_SIZE = {CAVLC_2X2: 4, CAVLC_4X4: 16, CAVLC_8X8: 64}
 
 
 def cavlc_encode(blocks: np.ndarray, mode: int, ncs, dc_skip: int = 0):
-    """Host encode of blocks (n x 16 or n x 4 int16) one after another into
+    """Host encode of blocks (n x 4, 16 or 64 int16) one after another into
     one MSB-first stream.  Returns (stream bytes, bits per block)."""
     b = np.ascontiguousarray(blocks, np.int16)
     n = b.shape[0]
@@ -417,7 +418,7 @@ def cavlc_encode(blocks: np.ndarray, mode: int, ncs, dc_skip: int = 0):
 def cavlc_decode(stream: np.ndarray, nblocks: int, mode: int, ncs, dc_skip: int = 0):
     s = np.ascontiguousarray(stream, np.uint8)
     ncs = np.broadcast_to(np.asarray(ncs, np.int64), (nblocks,))
-    size = 4 if mode == CAVLC_2X2 else 16
+    size = CAVLC_SIZE[mode]
     out = np.zeros((nblocks, size), np.int16)
     bits = np.zeros(nblocks, np.int32)
     pos = C.c_long(0)

@@ -1,4 +1,4 @@
-// CAVLC entropy coding of quantised 4x4 / 2x2 blocks (icu_cavlc_*):
+// CAVLC entropy coding of quantised 4x4 / 2x2 / 8x8 blocks (icu_cavlc_*):
 // single blocks on the host, as CAVLCH264Impl::Encode / Decode
 // (src/CodecUtils/CAVLCH264Impl.cpp:134-577) over an MSB-first bit stream
 // (BitStreamWriterMSB / BitStreamReaderMSB), and whole batches on the device.
@@ -30,6 +30,7 @@ namespace {
 struct HostSink {
   uint8_t* s;
   long cap, pos;
+  bool fits(int n) const { return pos + n <= cap; }
   bool put(uint32_t v, int n) {
     if (pos + n > cap) return false;
     if (s)
@@ -60,6 +61,7 @@ struct HostSource {
 };
 
 struct CountSink {
+  __device__ bool fits(int) const { return true; }
   __device__ bool put(uint32_t, int) { return true; }
 };
 
@@ -69,6 +71,7 @@ struct AtomicSink {
   uint32_t* s;
   long long pos;
   __device__ static uint32_t bswap(uint32_t x) { return __builtin_bswap32(x); }
+  __device__ bool fits(int) const { return true; }  // the stream was sized by the counting pass
   __device__ bool put(uint32_t v, int n) {
     if (n == 0) return true;
     const uint32_t x = n == 32 ? v : (v & ((1u << n) - 1u));
@@ -116,7 +119,7 @@ __global__ __launch_bounds__(256) void cavlc_count_kernel(const int16_t* __restr
     bits[n] = 0;  // so that the exclusive scan's last entry is the total
     return;
   }
-  const int size = mode == cavlc::kMode2x2 ? 4 : 16;
+  const int size = cavlc::block_size(mode);
   CountSink sink;
   const int r = cavlc::encode_block(coef + b * size, mode, block_nc(d_nc, nc, b), dc_skip, sink);
   if (r < 0) {
@@ -133,11 +136,13 @@ __global__ __launch_bounds__(256) void cavlc_write_kernel(const int16_t* __restr
                                                           uint32_t* __restrict__ stream) {
   const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
-  const int size = mode == cavlc::kMode2x2 ? 4 : 16;
+  const int size = cavlc::block_size(mode);
   AtomicSink sink{stream, offsets[b]};
   cavlc::encode_block(coef + b * size, mode, block_nc(d_nc, nc, b), dc_skip, sink);
 }
 
+// MAXC: the largest block the launch decodes (16 for 2x2 / 4x4, 64 for 8x8)
+template <int MAXC>
 __global__ __launch_bounds__(256) void cavlc_decode_kernel(const uint32_t* __restrict__ stream, long long ndw,
                                                            long long cap_bits, const long long* __restrict__ offsets,
                                                            long n, int mode, const int8_t* __restrict__ d_nc, int nc,
@@ -145,9 +150,9 @@ __global__ __launch_bounds__(256) void cavlc_decode_kernel(const uint32_t* __res
                                                            int32_t* __restrict__ status) {
   const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
-  const int size = mode == cavlc::kMode2x2 ? 4 : 16;
+  const int size = cavlc::block_size(mode);
   DeviceSource src{stream, ndw, cap_bits, offsets[b]};
-  int16_t out[16];
+  int16_t out[MAXC];
   const int r = cavlc::decode_block(src, mode, block_nc(d_nc, nc, b), dc_skip, out);
   for (int i = 0; i < size; i++) coef[b * size + i] = out[i];
   if (status) status[b] = r;
@@ -159,10 +164,11 @@ __global__ __launch_bounds__(256) void zero_u32_kernel(uint32_t* p, long long n)
 }
 
 int check_mode(int mode, int nc, int dc_skip) {
-  if (mode != cavlc::kMode2x2 && mode != cavlc::kMode4x4) return icu_fail(ICU_E_ARG, "mode must be 0 (2x2) or 1 (4x4)");
+  if (mode != cavlc::kMode2x2 && mode != cavlc::kMode4x4 && mode != cavlc::kMode8x8)
+    return icu_fail(ICU_E_ARG, "mode must be 0 (2x2), 1 (4x4) or 2 (8x8)");
   if (dc_skip != 0 && dc_skip != 1) return icu_fail(ICU_E_ARG, "dc_skip must be 0 or 1");
-  if (nc < -2 || (mode == cavlc::kMode4x4 && nc < 0))
-    return icu_fail(ICU_E_ARG, "nC must be >= 0 (4x4 blocks) or -1 / -2 / >= 0 (2x2 blocks)");
+  if (nc < -2 || (mode != cavlc::kMode2x2 && nc < 0))
+    return icu_fail(ICU_E_ARG, "nC must be >= 0 (4x4 and 8x8 blocks) or -1 / -2 / >= 0 (2x2 blocks)");
   return ICU_OK;
 }
 
@@ -192,8 +198,10 @@ int icu_cavlc_encode(const int16_t* coef, int mode, int nc, int dc_skip, uint8_t
   if (rc) return rc;
   HostSink sink{stream, stream ? stream_bits : (1L << 62), *bit_pos};
   const int r = cavlc::encode_block(coef, mode, nc, dc_skip, sink);
-  if (r < 0) return err_code(r);
+  // on an error the symbols written before it stay, and so does the position
+  *nbits = (int)(sink.pos - *bit_pos);
   *bit_pos = sink.pos;
+  if (r < 0) return err_code(r);
   *nbits = r;
   return ICU_OK;
 }
@@ -284,10 +292,10 @@ int icu_cavlc_decode_device(icu_cavlc* h, const uint8_t* d_stream, long stream_b
   if (((uintptr_t)d_stream & 3) || (stream_bytes & 3))
     return icu_fail(ICU_E_ARG, "d_stream and stream_bytes must be multiples of 4");
   if (nblocks > 0)
-    hipLaunchKernelGGL(cavlc_decode_kernel, dim3((unsigned)((nblocks + 255) / 256)), dim3(256), 0,
-                       (hipStream_t)stream, (const uint32_t*)d_stream, (long long)stream_bytes / 4,
-                       (long long)stream_bytes * 8, (const long long*)d_offsets, nblocks, mode, d_nc, nc, dc_skip,
-                       d_coef, d_status);
+    hipLaunchKernelGGL(mode == cavlc::kMode8x8 ? cavlc_decode_kernel<64> : cavlc_decode_kernel<16>,
+                       dim3((unsigned)((nblocks + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       (const uint32_t*)d_stream, (long long)stream_bytes / 4, (long long)stream_bytes * 8,
+                       (const long long*)d_offsets, nblocks, mode, d_nc, nc, dc_skip, d_coef, d_status);
   if (hipGetLastError() != hipSuccess) return icu_fail(ICU_E_DEVICE, "CAVLC launch failed");
   return ICU_OK;
 }

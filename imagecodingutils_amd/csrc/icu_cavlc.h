@@ -153,8 +153,32 @@ constexpr Vlc kRunBefore[7][15] = {
 constexpr uint8_t kZigZag4x4[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};
 constexpr uint8_t kZigZag2x2[4] = {0, 1, 2, 3};
 
-enum { kMode2x2 = 0, kMode4x4 = 1 };
+// The 8x8 zig-zag (frame) scan: anti-diagonals in turn, odd ones walked
+// down-left (row rising), even ones up-right.
+struct Scan64 {
+  uint8_t v[64];
+};
+constexpr Scan64 make_zigzag8x8() {
+  Scan64 z{};
+  int k = 0;
+  for (int d = 0; d < 15; d++)
+    for (int j = 0; j < 8; j++) {
+      const int row = (d % 2) ? j : 7 - j;
+      const int col = d - row;
+      if (col >= 0 && col < 8) z.v[k++] = (uint8_t)(row * 8 + col);
+    }
+  return z;
+}
+constexpr Scan64 kZigZag8x8 = make_zigzag8x8();
+static_assert(kZigZag8x8.v[2] == 8 && kZigZag8x8.v[3] == 16 && kZigZag8x8.v[63] == 63, "8x8 zig-zag");
+
+enum { kMode2x2 = 0, kMode4x4 = 1, kMode8x8 = 2 };
 enum { kErrSymbol = -1, kErrStream = -2 };
+
+__host__ __device__ inline const uint8_t* scan_order(int mode) {
+  return mode == kMode2x2 ? kZigZag2x2 : (mode == kMode8x8 ? kZigZag8x8.v : kZigZag4x4);
+}
+__host__ __device__ inline int block_size(int mode) { return mode == kMode2x2 ? 4 : (mode == kMode8x8 ? 64 : 16); }
 
 // coeff_token code for (TotalCoeff, TrailingOnes) under nC.
 __host__ __device__ inline Vlc coeff_token(int tc, int t1, int nc) {
@@ -166,12 +190,25 @@ __host__ __device__ inline Vlc coeff_token(int tc, int t1, int nc) {
   return kCoeffToken[t][t1][tc];
 }
 
-// Writes bits MSB-first; Sink provides put(value, nbits) (nbits <= 32) and
-// returns false when the destination is full.
+// Writes bits MSB-first; Sink provides put(value, nbits) (nbits <= 32),
+// which returns false when the destination is full, and fits(nbits).  A
+// symbol is written whole or not at all; on an error the symbols written
+// before it stay, as with the reference's writer.
+//
+// 8x8 blocks (CAVLCH264Impl::Mode8x8, src/CodecUtils/CAVLCH264Impl.cpp:588-611)
+// scan 64 levels but keep the 4x4 tables: the reference's coder indexes its
+// coeff_token table with TotalCoeff and its total_zeros table with
+// (total_zeros, TotalCoeff) as they come (CoeffTokenH264VlcEncoder.cpp:85-126,
+// TotalZeros4x4H264VlcEncoder.cpp:83-92).  Within those tables (at most 15
+// levels, total_zeros at most 15) the code is the 4x4 one over the 8x8
+// scan, and a (total_zeros, TotalCoeff) pair the table has no code for fails
+// after coeff_token and the levels are written, as there.  16 levels or more,
+// or 16 zeros and more below the last level, index past the reference's
+// tables; such a block is refused here before anything is written.
 template <class Sink>
 __host__ __device__ inline int encode_block(const int16_t* coef, int mode, int nc, int dc_skip, Sink& out) {
-  const uint8_t* zz = mode == kMode2x2 ? kZigZag2x2 : kZigZag4x4;
-  const int maxc = mode == kMode2x2 ? 4 : 16;
+  const uint8_t* zz = scan_order(mode);
+  const int maxc = block_size(mode);
   // levels in reverse scan order and the zero runs between them
   int level[16], run[16];
   int tc = 0, t1 = 0, tz = 0, zeros = 0;
@@ -184,12 +221,14 @@ __host__ __device__ inline int encode_block(const int16_t* coef, int mode, int n
       }
       continue;
     }
+    if (tc == 16) return kErrSymbol;  // a 17th level: 8x8 blocks only
     if (tc > 0) run[tc - 1] = zeros;
     zeros = 0;
     level[tc++] = x;
     // trailing ones: +-1 levels at the high-frequency end, at most 3
     if (t1 == tc - 1 && t1 < 3 && (x == 1 || x == -1)) t1++;
   }
+  if (mode == kMode8x8 && (tc > 15 || tz > 15)) return kErrSymbol;
   int bits = 0;
   const Vlc ct = coeff_token(tc, t1, nc);
   if (ct.len == 0) return kErrSymbol;
@@ -223,7 +262,8 @@ __host__ __device__ inline int encode_block(const int16_t* coef, int mode, int n
       ssize = prefix - 3;
       suffix = ssize >= 32 ? lc : (lc & (int)((1u << ssize) - 1u));
     }
-    // level_prefix: prefix zeros then a one
+    // level_prefix (prefix zeros then a one) and the suffix go in together
+    if (!out.fits(prefix + 1 + ssize)) return kErrStream;
     for (int z = prefix; z > 0; z -= 32)
       if (!out.put(0u, z > 32 ? 32 : z)) return kErrStream;
     if (!out.put(1u, 1)) return kErrStream;
@@ -269,8 +309,8 @@ __host__ __device__ inline bool read_vlc(Source& in, const Vlc* tab, int n, int*
 
 template <class Source>
 __host__ __device__ inline int decode_block(Source& in, int mode, int nc, int dc_skip, int16_t* coef) {
-  const uint8_t* zz = mode == kMode2x2 ? kZigZag2x2 : kZigZag4x4;
-  const int maxc = mode == kMode2x2 ? 4 : 16;
+  const uint8_t* zz = scan_order(mode);
+  const int maxc = block_size(mode);
   for (int i = 0; i < maxc; i++) coef[i] = 0;
   if (in.left() <= 0) return kErrStream;
   int bits = 0, tc = -1, t1 = 0;
@@ -288,6 +328,8 @@ __host__ __device__ inline int decode_block(Source& in, int mode, int nc, int dc
       }
     }
   if (tc < 0 || tc > maxc - dc_skip) return kErrSymbol;
+  // 8x8: 16 levels would read total_zeros past the 4x4 table (see encode_block)
+  if (mode == kMode8x8 && tc > 15) return kErrSymbol;
   if (tc == 0) return bits;
   int level[16], run[16];
   int sl = (tc > 10 && t1 < 3) ? 1 : 0;

@@ -732,9 +732,12 @@ class CAVLCH264Impl : public IContextAwareRunLevelCodec {
  public:
   static const int VLC_SYMBOL_NOT_RECOGNISED = -1;
   static const int STREAM_ACCESS_DENIED = -2;
+  // Mode8x8: 64 levels in the 8x8 zig-zag scan, coded with the 4x4 tables
+  // as the reference does (icu_cavlc_encode says which blocks code).  On an
+  // error the writer keeps the symbols written before it, as the
+  // reference's does.
   int Encode(void* in, void* stream) override {
     BitStreamWriterMSB* w = (BitStreamWriterMSB*)stream;
-    if (_mode == Mode8x8) return VLC_SYMBOL_NOT_RECOGNISED;
     int nbits = 0;
     const int rc = icu_cavlc_encode((const int16_t*)in, _mode, _nc, _dc, w->Bytes(), w->GetStreamBitSize(),
                                     &w->Cursor(), &nbits);
@@ -744,7 +747,6 @@ class CAVLCH264Impl : public IContextAwareRunLevelCodec {
   }
   int Decode(void* stream, void* out) override {
     BitStreamReaderMSB* r = (BitStreamReaderMSB*)stream;
-    if (_mode == Mode8x8) return VLC_SYMBOL_NOT_RECOGNISED;
     int nbits = 0;
     const int rc = icu_cavlc_decode(r->Bytes(), r->GetStreamBitSize(), &r->Cursor(), _mode, _nc, _dc,
                                     (int16_t*)out, &nbits);
@@ -768,7 +770,7 @@ class CAVLCH264Impl : public IContextAwareRunLevelCodec {
   // levels coded in the block: every position but a skipped DC
   int count(const int16_t* b) const {
     int n = 0;
-    for (int i = _dc ? 1 : 0; i < (_mode == Mode2x2 ? 4 : 16); i++) n += b[i] != 0;
+    for (int i = _dc ? 1 : 0; i < (_mode == Mode2x2 ? 4 : (_mode == Mode8x8 ? 64 : 16)); i++) n += b[i] != 0;
     return n;
   }
   int _mode = Mode4x4, _nc = 0, _dc = 0, _num = 0;

@@ -232,17 +232,29 @@ int icu_mc_compensate_device(icu_mc* h, const uint8_t* d_ref_y, const uint8_t* d
 /* ------------------------------------------------------------------ CAVLC
  * Context-adaptive VLC of one block of quantised levels (H.264 9.2):
  * replaces CAVLCH264Impl::Encode / Decode with its VLC coders
- * (src/CodecUtils/CAVLCH264Impl.cpp:134-577; SetMode Mode2x2 / Mode4x4,
- * SetParameter NUM_TOT_NEIGHBOR_COEFF_ID = nc, DC_SKIP_FLAG_ID = dc_skip)
- * over an MSB-first bit stream (BitStreamWriterMSB / BitStreamReaderMSB).
- * Blocks are 16 int16 levels in raster order (mode ICU_CAVLC_4X4) or 4
- * (ICU_CAVLC_2X2, chroma DC; nc -1 selects the 4:2:0 chroma DC
- * coeff_token table, -2 the 4:2:2 one). */
+ * (src/CodecUtils/CAVLCH264Impl.cpp:134-611; SetMode Mode2x2 / Mode4x4 /
+ * Mode8x8, SetParameter NUM_TOT_NEIGHBOR_COEFF_ID = nc, DC_SKIP_FLAG_ID =
+ * dc_skip) over an MSB-first bit stream (BitStreamWriterMSB /
+ * BitStreamReaderMSB).  Blocks are 16 int16 levels in raster order (mode
+ * ICU_CAVLC_4X4), 4 (ICU_CAVLC_2X2, chroma DC; nc -1 selects the 4:2:0
+ * chroma DC coeff_token table, -2 the 4:2:2 one) or 64 (ICU_CAVLC_8X8,
+ * scanned in the 8x8 zig-zag order but coded with the 4x4 tables, as the
+ * reference's Mode8x8 does: a block codes when it has at most 15 levels
+ * and at most 15 zeros below its last one; a (total_zeros, TotalCoeff)
+ * pair without a 4x4 code fails with ICU_E_STATE after coeff_token and the
+ * levels are written, as there; 16 levels or 16 such zeros, where the
+ * reference reads past its tables, fail with ICU_E_STATE before anything
+ * is written). */
 #define ICU_CAVLC_2X2 0
 #define ICU_CAVLC_4X4 1
+#define ICU_CAVLC_8X8 2
 /* Writes the block's code at *bit_pos of stream (stream_bits long; bits are
  * set or cleared, others untouched) and advances *bit_pos; stream NULL only
- * counts.  *nbits: bits written.  ICU_E_RANGE when the stream is full. */
+ * counts.  *nbits: bits written.  ICU_E_RANGE when the stream is full,
+ * ICU_E_STATE when a symbol has no code; on either error the symbols written
+ * before it stay, *bit_pos is advanced past them and *nbits counts them (the
+ * reference's writer keeps them too).  A symbol is written whole or not at
+ * all. */
 int icu_cavlc_encode(const int16_t* coef, int mode, int nc, int dc_skip, uint8_t* stream, long stream_bits,
                      long* bit_pos, int* nbits);
 /* Reads one block from *bit_pos (all mode-size outputs written; zeros

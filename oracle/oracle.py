@@ -113,6 +113,8 @@ def ref_lib():
         L.ref_yuv_read.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _i32p]
         L.ref_mv_dump.argtypes = [C.c_char_p, C.c_char_p, _i16p, C.c_int]
         L.ref_cavlc.argtypes = [_i16p, C.c_int, C.c_int, _i32p, C.c_int, _u8p, C.c_int, _i32p, _i16p, _i32p]
+        L.ref_cavlc_each.argtypes = [_i16p, C.c_int, C.c_int, _i32p, C.c_int, _u8p, C.c_int, _i32p, _i32p,
+                                     _i16p, _i32p]
         L.ref_cross_estimate.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i16p, _i16p, C.c_void_p,
                                          C.c_void_p, C.POINTER(C.c_long), C.c_int, C.POINTER(C.c_double)]
         L.ref_fhs_chain.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _i16p, _i16p, C.c_void_p,

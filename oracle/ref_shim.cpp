@@ -441,4 +441,60 @@ int ref_cavlc(const short* blocks, int n, int mode, const int* nc, int dc_skip, 
   return 0;
 }
 
+// CAVLC of n blocks of `size` shorts (mode 0: 4, 1: 16, 2: 64), each into
+// its own stream of block_bytes bytes (out + b * block_bytes), so that a
+// block whose Encode fails leaves no trace in the next one.  bits[b]:
+// Encode's return value; used[b]: the bits the writer holds afterwards (a
+// failed Encode keeps the symbols it wrote before the failure).  Blocks
+// encoded without error are decoded back from their stream into dec
+// (dec_bits: Decode's return value).
+int ref_cavlc_each(const short* blocks, int n, int mode, const int* nc, int dc_skip, unsigned char* out,
+                   int block_bytes, int* bits, int* used, short* dec, int* dec_bits) {
+  PrefixH264VlcEncoderImpl1 pe;
+  PrefixH264VlcDecoderImpl1 pd;
+  CoeffTokenH264VlcEncoder ce;
+  CoeffTokenH264VlcDecoder cd;
+  TotalZeros4x4H264VlcEncoder t4e;
+  TotalZeros4x4H264VlcDecoder t4d;
+  TotalZeros2x2H264VlcEncoder t2e;
+  TotalZeros2x2H264VlcDecoder t2d;
+  RunBeforeH264VlcEncoder re;
+  RunBeforeH264VlcDecoder rd;
+  CAVLCH264Impl c;
+  c.SetTokenCoeffVlcEncoder(&ce);
+  c.SetTokenCoeffVlcDecoder(&cd);
+  c.SetPrefixVlcEncoder(&pe);
+  c.SetPrefixVlcDecoder(&pd);
+  c.SetRunBeforeVlcEncoder(&re);
+  c.SetRunBeforeVlcDecoder(&rd);
+  if (mode == CAVLCH264Impl::Mode2x2) {
+    c.SetTotalZerosVlcEncoder(&t2e);
+    c.SetTotalZerosVlcDecoder(&t2d);
+  } else {
+    c.SetTotalZerosVlcEncoder(&t4e);
+    c.SetTotalZerosVlcDecoder(&t4d);
+  }
+  c.SetMode(mode);
+  const int size = mode == CAVLCH264Impl::Mode2x2 ? 4 : (mode == CAVLCH264Impl::Mode8x8 ? 64 : 16);
+  for (int b = 0; b < n; b++) {
+    unsigned char* s = out + (long)b * block_bytes;
+    std::memset(s, 0, block_bytes);
+    c.SetParameter(CAVLCH264Impl::NUM_TOT_NEIGHBOR_COEFF_ID, nc[b]);
+    c.SetParameter(CAVLCH264Impl::DC_SKIP_FLAG_ID, dc_skip);
+    BitStreamWriterMSB w;
+    w.SetStream(s, block_bytes * 8);
+    bits[b] = c.Encode((void*)(blocks + (long)b * size), (void*)static_cast<IBitStreamWriter*>(&w));
+    used[b] = block_bytes * 8 - w.GetStreamBitsRemaining();
+    short* d = dec + (long)b * size;
+    std::memset(d, 0, sizeof(short) * size);
+    dec_bits[b] = 0;
+    if (bits[b] > 0) {
+      BitStreamReaderMSB r;
+      r.SetStream(s, block_bytes * 8);
+      dec_bits[b] = c.Decode((void*)static_cast<IBitStreamReader*>(&r), (void*)d);
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"

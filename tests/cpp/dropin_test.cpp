@@ -139,7 +139,7 @@ int main(int argc, char** argv) {
     // a BitStreamReaderMSB; out.raw holds per block the encoded bits, the
     // decoded bits, NUM_TOT_COEFF after each and the decoded levels.
     const int mode = std::atoi(argv[2]), dc = std::atoi(argv[3]), n = std::atoi(argv[4]);
-    const int size = mode == IContextAwareRunLevelCodec::Mode2x2 ? 4 : 16;
+    const int size = mode == IContextAwareRunLevelCodec::Mode2x2 ? 4 : (mode == IContextAwareRunLevelCodec::Mode8x8 ? 64 : 16);
     auto blocks = load<short>(argv[5], (size_t)n * size);
     auto nc = load<signed char>(argv[6], (size_t)n);
     std::vector<unsigned char> stream((size_t)n * 128 + 16, 0xA5);
@@ -173,6 +173,43 @@ int main(int argc, char** argv) {
     out.push_back(w.Seek(p));
     out.push_back(w.GetStreamBitsRemaining());
     save(argv[8], out.data(), out.size());
+    return 0;
+  }
+  if (cmd == "cavlc_each" && argc >= 8) {
+    // cavlc_each mode dc n blocks.raw nc.raw out.raw: every block into a
+    // writer of its own (256 zeroed bytes), so a failed Encode cannot shift
+    // the next block; out.raw holds per block Encode's return, the bits the
+    // writer holds afterwards, NUM_TOT_COEFF, the 256 stream bytes, then (a
+    // block that encoded) Decode's return and the decoded levels.
+    const int mode = std::atoi(argv[2]), dc = std::atoi(argv[3]), n = std::atoi(argv[4]);
+    const int size = mode == IContextAwareRunLevelCodec::Mode2x2 ? 4 : (mode == IContextAwareRunLevelCodec::Mode8x8 ? 64 : 16);
+    auto blocks = load<short>(argv[5], (size_t)n * size);
+    auto nc = load<signed char>(argv[6], (size_t)n);
+    CAVLCH264Impl codec;
+    codec.SetMode(mode);
+    codec.SetParameter(IContextAwareRunLevelCodec::DC_SKIP_FLAG_ID, dc);
+    std::vector<int> out;
+    for (int i = 0; i < n; i++) {
+      std::vector<unsigned char> stream(256, 0);
+      BitStreamWriterMSB w;
+      w.SetStream(stream.data(), 256 * 8);
+      codec.SetParameter(IContextAwareRunLevelCodec::NUM_TOT_NEIGHBOR_COEFF_ID, nc[i]);
+      const int e = codec.Encode(&blocks[(size_t)i * size], &w);
+      out.push_back(e);
+      out.push_back(256 * 8 - w.GetStreamBitsRemaining());
+      out.push_back(codec.GetParameter(IContextAwareRunLevelCodec::NUM_TOT_COEFF_ID));
+      for (unsigned char c : stream) out.push_back(c);
+      std::vector<short> dec(size, 0);
+      int d = 0;
+      if (e > 0) {
+        BitStreamReaderMSB r;
+        r.SetStream(stream.data(), 256 * 8);
+        d = codec.Decode(&r, dec.data());
+      }
+      out.push_back(d);
+      for (int k = 0; k < size; k++) out.push_back(dec[k]);
+    }
+    save(argv[7], out.data(), out.size());
     return 0;
   }
   if (cmd == "fast" && argc >= 15) {

@@ -314,6 +314,95 @@ def make_cavlc(R_lib):
     np.savez_compressed(OUT / "cavlc.npz", **out)
 
 
+def zigzag(n):
+    """Raster index of each scan position of the n x n zig-zag (frame) scan:
+    anti-diagonals in turn, odd ones walked down-left, even ones up-right."""
+    order = []
+    for d in range(2 * n - 1):
+        cells = [(i, d - i) for i in range(n) if 0 <= d - i < n]  # (row, col), row rising
+        order += cells if d % 2 else cells[::-1]
+    return [r * n + c for r, c in order]
+
+
+def cavlc8x8_blocks(dc_skip, rng):
+    """8x8 blocks (64 levels) for CAVLCH264Impl's Mode8x8, whose coder keeps
+    the 4x4 tables: the defined cases only.  Blocks whose levels all lie in
+    the first 16 scan positions (at most 15 levels), which code; and blocks
+    with at most 15 levels and total_zeros at most 15 that the 4x4
+    total_zeros table has no code for (total_zeros + TotalCoeff > 16), which
+    fail after writing coeff_token and the levels.  More levels, or 16 zeros
+    and more below the last level, index past the reference's tables, so no
+    such block is here."""
+    zz = zigzag(8)
+    out, kinds = [], []
+
+    def place(levels, positions):
+        blk = np.zeros(64, np.int16)
+        for lv, p in zip(levels, positions):
+            blk[zz[p]] = lv
+        return blk
+
+    def levels(tc, t1):
+        lv = [int(rng.choice([-1, 1])) for _ in range(t1)]
+        lv += [int(rng.choice([2, 3, 5, 9, 40, 300, 3000])) * int(rng.choice([-1, 1])) for _ in range(tc - t1)]
+        return lv[::-1]  # low frequency first
+
+    span = 16 - dc_skip
+    for tc in range(0, min(15, span) + 1):
+        for t1 in range(0, min(3, tc) + 1):
+            for rep in range(4):
+                pos = sorted(rng.choice(span, tc, replace=False) + dc_skip) if tc else []
+                out.append(place(levels(tc, t1), pos))
+                kinds.append(0)
+    # every (TotalCoeff, total_zeros) of the 4x4 table, top position last
+    for tc in range(1, min(15, span) + 1):
+        for tz in range(0, span - tc + 1):
+            top = dc_skip + tc + tz - 1
+            pos = sorted(list(rng.choice(top - dc_skip, tc - 1, replace=False) + dc_skip)) + [top]
+            out.append(place(levels(tc, min(3, tc) if rng.random() < 0.5 else 0), pos))
+            kinds.append(0)
+    # no total_zeros code: tz + tc > 16 with tz <= 15
+    for tc in range(1, 16):
+        for tz in range(17 - tc, 16):
+            top = dc_skip + tc + tz - 1
+            if top > 63:
+                continue
+            pos = sorted(list(rng.choice(top - dc_skip, tc - 1, replace=False) + dc_skip)) + [top]
+            out.append(place(levels(tc, int(rng.integers(0, min(3, tc) + 1))), pos))
+            kinds.append(1)
+    return np.stack(out), np.array(kinds, np.int8)
+
+
+def make_cavlc8x8(R_lib):
+    rng = np.random.default_rng(171)
+    out = {}
+    for dc_skip in (0, 1):
+        blocks, kinds = cavlc8x8_blocks(dc_skip, rng)
+        n = len(blocks)
+        nc = rng.choice([0, 1, 2, 3, 4, 7, 8, 16], size=n).astype(np.int32)
+        per = 256
+        stream = np.zeros(n * per, np.uint8)
+        bits = np.zeros(n, np.int32)
+        used = np.zeros(n, np.int32)
+        dec = np.zeros_like(blocks)
+        dbits = np.zeros(n, np.int32)
+        assert R_lib.ref_cavlc_each(np.ascontiguousarray(blocks).reshape(-1), n, 2, nc, dc_skip, stream, per,
+                                    bits, used, dec.reshape(-1), dbits) == 0
+        assert ((bits > 0) == (kinds == 0)).all(), "the reference coded a block outside its tables or failed one"
+        key = f"m2_s{dc_skip}"
+        out[key + "_blocks"] = blocks
+        out[key + "_nc"] = nc.astype(np.int8)
+        out[key + "_bits"] = bits
+        out[key + "_used"] = used
+        # each block's own stream, cut to the bits the writer holds
+        nbytes = (used.max() + 7) // 8
+        out[key + "_streams"] = stream.reshape(n, per)[:, :nbytes]
+        out[key + "_dec"] = dec
+        out[key + "_dec_bits"] = dbits
+        print("cavlc 8x8", key, n, "blocks,", int((kinds == 1).sum()), "without a total_zeros code")
+    np.savez_compressed(OUT / "cavlc8x8.npz", **out)
+
+
 def umhs_scenes():
     """Frame pairs for the UMHS fixtures: moving rectangles at several
     sizes and ranges, and whole-picture pans that push vectors past the
@@ -464,20 +553,19 @@ def make_fhs(R_lib):
     print("fhs", len(FHS_CASES), "cases")
 
 
+MAKERS = {"me": make_me, "transforms": make_transforms, "qpel": make_qpel_and_metrics, "mc": make_mc,
+          "mc_yuv": make_mc_yuv, "io": make_io, "cavlc": make_cavlc, "cavlc8x8": make_cavlc8x8,
+          "umhs": make_umhs, "cross": make_cross, "fhs": make_fhs}
+
+
 def main():
+    """All fixtures, or only those named on the command line."""
     if not orc.build_ref():
         sys.exit("reference sources not available; fixtures are committed")
     R_lib = orc.ref_lib()
-    make_me(R_lib)
-    make_transforms(R_lib)
-    make_qpel_and_metrics(R_lib)
-    make_mc(R_lib)
-    make_mc_yuv(R_lib)
-    make_io(R_lib)
-    make_cavlc(R_lib)
-    make_umhs(R_lib)
-    make_cross(R_lib)
-    make_fhs(R_lib)
+    names = sys.argv[1:] or list(MAKERS)
+    for name in names:
+        MAKERS[name](R_lib)
 
 
 if __name__ == "__main__":

@@ -54,7 +54,7 @@ def test_counting_without_a_stream_and_errors():
     rc = capi.lib.icu_cavlc_encode(blk.ctypes.data, 1, 0, 0, small.ctypes.data, 4, capi.C.byref(pos),
                                    capi.C.byref(nb))
     assert rc == capi.ICU_E_RANGE
-    for mode, nc, dc in ((1, -1, 0), (2, 0, 0), (1, 0, 2)):
+    for mode, nc, dc in ((1, -1, 0), (3, 0, 0), (2, -1, 0), (1, 0, 2)):
         rc = capi.lib.icu_cavlc_encode(blk.ctypes.data, mode, nc, dc, None, 0, capi.C.byref(pos),
                                        capi.C.byref(nb))
         assert rc == capi.ICU_E_ARG
@@ -110,3 +110,109 @@ def test_cpp_dropin_codec_matches_reference(fx, key, tmp_path):
     np.testing.assert_array_equal(dec[:, 2:], fx[key + "_dec"])
     # Seek(GetStreamBitPos()) after 13 bits: back to bit 13
     assert tail[0] == 1 and tail[1] == (n * 128 + 16) * 8 - 13
+
+
+# ------------------------------------------------------------ 8x8 mode
+@pytest.fixture(scope="module")
+def fx8():
+    return np.load(GOLDEN / "cavlc8x8.npz")
+
+
+def _encode_one(blk, mode, nc, dc):
+    out = np.zeros(256, np.uint8)
+    pos, nb = capi.C.c_long(0), capi.C.c_int(0)
+    rc = capi.lib.icu_cavlc_encode(np.ascontiguousarray(blk, np.int16).ctypes.data, mode, int(nc), dc,
+                                   out.ctypes.data, 256 * 8, capi.C.byref(pos), capi.C.byref(nb))
+    return rc, out, pos.value, nb.value
+
+
+@pytest.mark.parametrize("dc", [0, 1])
+def test_8x8_mode_matches_reference_block_by_block(fx8, dc):
+    """CAVLCH264Impl's Mode8x8 (64 levels, 8x8 zig-zag scan, the 4x4 tables)
+    against the compiled reference, each block into a stream of its own:
+    blocks that code give the same bits and decode to the same levels;
+    blocks whose (total_zeros, TotalCoeff) has no 4x4 code fail as there,
+    after the same coeff_token and level bits."""
+    key = f"m2_s{dc}"
+    blocks, nc, bits, used = fx8[key + "_blocks"], fx8[key + "_nc"], fx8[key + "_bits"], fx8[key + "_used"]
+    streams = fx8[key + "_streams"]
+    assert (bits > 0).sum() > 300 and (bits < 0).sum() > 100
+    for b in range(len(blocks)):
+        rc, out, pos, nb = _encode_one(blocks[b], capi.CAVLC_8X8, nc[b], dc)
+        if bits[b] > 0:
+            assert rc == 0 and nb == bits[b], b
+        else:
+            assert bits[b] == -1 and rc == capi.ICU_E_STATE, b
+        assert pos == used[b], b
+        np.testing.assert_array_equal(out[:streams.shape[1]], streams[b], err_msg=f"block {b}")
+        assert not out[streams.shape[1]:].any()
+        if bits[b] > 0:
+            dec, dbits = capi.cavlc_decode(out, 1, capi.CAVLC_8X8, [nc[b]], dc)
+            np.testing.assert_array_equal(dec[0], fx8[key + "_dec"][b], err_msg=f"block {b}")
+            assert dbits[0] == fx8[key + "_dec_bits"][b]
+
+
+def test_8x8_mode_is_the_4x4_code_of_its_first_16_scan_positions(fx8):
+    """Closed form of the 8x8 mode within the reference's tables: the code
+    of an 8x8 block equals the 4x4 code of the block that holds, at 4x4
+    zig-zag position k, the level at 8x8 zig-zag position k."""
+    zz8 = [0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5]  # the first 16 of the 8x8 scan
+    zz4 = [0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15]
+    blocks, nc, bits = fx8["m2_s0_blocks"], fx8["m2_s0_nc"], fx8["m2_s0_bits"]
+    for b in np.nonzero(bits > 0)[0][:200]:
+        b4 = np.zeros(16, np.int16)
+        for k in range(16):
+            b4[zz4[k]] = blocks[b][zz8[k]]
+        rc8, s8, p8, _ = _encode_one(blocks[b], capi.CAVLC_8X8, nc[b], 0)
+        rc4, s4, p4, _ = _encode_one(b4, capi.CAVLC_4X4, nc[b], 0)
+        assert rc8 == rc4 == 0 and p8 == p4
+        np.testing.assert_array_equal(s8, s4)
+
+
+def test_8x8_mode_refuses_blocks_past_the_reference_tables_without_writing():
+    """16 levels or more, or 16 zeros and more below the last level, index
+    past the reference's coeff_token / total_zeros tables; such a block is
+    refused with nothing written."""
+    zz8 = [0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12]
+    cases = []
+    b = np.zeros(64, np.int16)
+    b[zz8[:16]] = 3  # 16 levels, the first 16 scan positions
+    cases.append(b)
+    b = np.zeros(64, np.int16)
+    b[zz8[16]] = 1  # one level at scan position 16: 16 zeros below it
+    cases.append(b)
+    b = np.arange(1, 65, dtype=np.int16)  # 64 levels
+    cases.append(b)
+    for blk in cases:
+        rc, out, pos, nb = _encode_one(blk, capi.CAVLC_8X8, 0, 0)
+        assert rc == capi.ICU_E_STATE and pos == 0 and nb == 0 and not out.any()
+
+
+@pytest.mark.parametrize("dc", [0, 1])
+def test_cpp_dropin_codec_8x8_mode_matches_reference(fx8, dc, tmp_path):
+    """CAVLCH264Impl::SetMode(Mode8x8) of the C++ class, block by block as in
+    the fixture: Encode's return, the writer's position after it (also after
+    a failure), NUM_TOT_COEFF, the stream and the decode."""
+    import subprocess
+    if not EXE.exists():
+        pytest.fail(f"{EXE} not built (run __graft_entry__.build())")
+    key = f"m2_s{dc}"
+    blocks, nc = fx8[key + "_blocks"], fx8[key + "_nc"]
+    n = len(blocks)
+    blocks.astype(np.int16).tofile(tmp_path / "b.raw")
+    nc.astype(np.int8).tofile(tmp_path / "nc.raw")
+    r = subprocess.run([str(EXE), "cavlc_each", "2", str(dc), str(n), str(tmp_path / "b.raw"),
+                        str(tmp_path / "nc.raw"), str(tmp_path / "o.raw")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    rec = np.fromfile(tmp_path / "o.raw", np.int32).reshape(n, 3 + 256 + 1 + 64)
+    bits = fx8[key + "_bits"]
+    np.testing.assert_array_equal(rec[:, 0], bits)
+    np.testing.assert_array_equal(rec[:, 1], fx8[key + "_used"])
+    coded = np.count_nonzero(blocks[:, 1:] if dc else blocks, axis=1)
+    ok = bits > 0
+    np.testing.assert_array_equal(rec[ok, 2], coded[ok])
+    w = fx8[key + "_streams"].shape[1]
+    np.testing.assert_array_equal(rec[:, 3:3 + w].astype(np.uint8), fx8[key + "_streams"])
+    np.testing.assert_array_equal(rec[ok, 3 + 256], fx8[key + "_dec_bits"][ok])
+    np.testing.assert_array_equal(rec[ok, 4 + 256:], fx8[key + "_dec"][ok])

@@ -32,7 +32,7 @@ def _encode(dev, blocks, mode, nc, dc, cap_bytes=None):
 
 def _decode(dev, bufs, n, mode, nc, dc):
     d_stream, cap, d_off, d_nc = bufs
-    size = 4 if mode == 0 else 16
+    size = capi.CAVLC_SIZE[mode]
     d_coef = capi.DeviceBuffer(n * size * 2)
     d_status = capi.DeviceBuffer(4 * n)
     dev.decode(d_stream, cap, d_off, n, mode, d_nc, 0 if d_nc else int(nc), dc, d_coef, d_status)
@@ -131,3 +131,28 @@ def test_device_errors():
     tot = dev.encode(d_coef, 0, 1, None, 0, 0, d_stream, 64, d_off)
     capi.lib.icu_stream_sync(None)
     assert tot == 0 and d_off.download(np.int64, (1,))[0] == 0
+
+
+@pytest.mark.parametrize("dc", [0, 1])
+def test_device_8x8_mode_matches_reference(dc):
+    """Mode8x8 batches on the device: the fixture's blocks that the reference
+    codes (64 levels, 8x8 zig-zag, 4x4 tables), each block's bits equal to
+    the reference's, the stream equal to the host coder's, and the decode
+    back to the reference's levels; a batch holding a block the tables have
+    no code for is refused."""
+    fx = np.load(GOLDEN / "cavlc8x8.npz")
+    key = f"m2_s{dc}"
+    ok = fx[key + "_bits"] > 0
+    blocks, nc, bits = fx[key + "_blocks"][ok], fx[key + "_nc"][ok], fx[key + "_bits"][ok]
+    dev = capi.CavlcDevice()
+    stream, off, bufs = _encode(dev, blocks, 2, nc, dc)
+    np.testing.assert_array_equal(np.diff(off), bits)
+    want, _ = capi.cavlc_encode(blocks, 2, nc.astype(int), dc)
+    np.testing.assert_array_equal(stream, want)
+    dec, status = _decode(dev, bufs, len(blocks), 2, nc, dc)
+    np.testing.assert_array_equal(dec, fx[key + "_dec"][ok])
+    np.testing.assert_array_equal(status, fx[key + "_dec_bits"][ok])
+    bad = fx[key + "_blocks"][~ok][:4]
+    with pytest.raises(capi.IcuError) as e:
+        _encode(dev, np.concatenate([blocks[:8], bad]), 2, 0, dc)
+    assert e.value.code == capi.ICU_E_STATE
