@@ -85,6 +85,15 @@ __host__ __device__ inline void motion_window(int n, int m, int W, int H, int L,
 // multiple of 4 so every lane's 4-candidate quad starts on a dword.
 __host__ __device__ inline int tile_dx0(int L) { return -4 * ((L + 3) / 4); }
 
+// u / d for 0 <= u < 2^17 and 1 <= d <= 2048 without an integer division:
+// (u + 1/2) / d lies at least 1/(2d) from an integer, and the f32 product
+// with v_rcp_f32's reciprocal (within 1 ulp of 1/d), rounded once or twice,
+// stays closer than that; checked exhaustively over the whole domain with the
+// reciprocal at 1/d and at both neighbouring floats.
+__device__ __forceinline__ int udiv_small(int u, int d) {
+  return (int)(((float)u + 0.5f) * __builtin_amdgcn_rcpf((float)d));
+}
+
 // Lane tiles of the search kernel's argmin keys: tiles taller than 8 rows
 // keep one key per half, rows [0, V1) and [V1, V).
 __host__ __device__ constexpr int lane_key_split(int V) { return V > 8 ? (V + 1) / 2 : V; }

@@ -119,15 +119,8 @@ __global__ __launch_bounds__(256) void pad_set_kernel(const PadSet s) {
 
 // ------------------------------------------------------------- tile bodies
 
-// u / d for 0 <= u < 2^17 and 1 <= d <= 2048 without an integer division:
-// (u + 1/2) / d lies at least 1/(2d) from an integer, and the f32 product
-// with v_rcp_f32's reciprocal (within 1 ulp of 1/d), rounded once or twice,
-// stays closer than that; checked exhaustively over the whole domain with the
-// reciprocal at 1/d and at both neighbouring floats.  Tile units stay below
-// 16 MBs x 33 quads x 17 row tiles.
-__device__ __forceinline__ int udiv_small(int u, int d) {
-  return (int)(((float)u + 0.5f) * __builtin_amdgcn_rcpf((float)d));
-}
+// Tile units (udiv_small's numerators) stay below 16 MBs x 33 quads x 17 row
+// tiles.
 
 // Rows are software-pipelined one ahead: the LDS loads of row rho+1 are
 // issued before the qsads of row rho.  The empty asm at the top of each row

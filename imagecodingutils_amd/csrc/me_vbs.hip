@@ -43,15 +43,44 @@ typedef __attribute__((address_space(3))) u64_a4 lds_u64_a4;
 
 constexpr int kThreads = 256;
 
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// Two u16 SADs in one dword added lane-wise (v_pk_add_u16): partition sums
+// of up to four 8x8 SADs stay below 65536.
+__device__ __forceinline__ uint32_t add_u16x2(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) + __builtin_bit_cast(u16x2, b));
+}
+
+// A candidate's key, half * 2^15 + off, from one u16 half of a packed SAD
+// pair in a single v_mad_u32_u16 (op_sel picks the high half); k15 holds
+// 2^15 in a register (VOP3 takes no literal here; the SGPRs hold the block).
+__device__ __forceinline__ uint32_t key_lo16(uint32_t x, uint32_t k15, uint32_t off) {
+  uint32_t r;
+  asm("v_mad_u32_u16 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(k15), "v"(off));
+  return r;
+}
+__device__ __forceinline__ uint32_t key_hi16(uint32_t x, uint32_t k15, uint32_t off) {
+  uint32_t r;
+  asm("v_mad_u32_u16 %0, %1, %2, %3 op_sel:[1,0,0,0]" : "=v"(r) : "v"(x), "v"(k15), "v"(off));
+  return r;
+}
+
 // Partition order: 0 16x16; 1, 2 16x8 top / bottom; 3, 4 8x16 left / right;
 // 5..8 8x8 in raster order.  Quadrants: 0 top-left, 1 top-right,
 // 2 bottom-left, 3 bottom-right.
+__device__ __forceinline__ uint32_t min3_u32(uint32_t a, uint32_t b, uint32_t c) { return min(min(a, b), c); }
+
 __device__ __forceinline__ int part_of_quadrant(int mode, int k) {
   return mode == 0 ? 0 : (mode == 1 ? 1 + (k >> 1) : (mode == 2 ? 3 + (k & 1) : 5 + k));
 }
 
+// Waves per SIMD each tile height is compiled for: its accumulators (4 x V
+// packed pairs) and the epilogue fit without spilling.
+constexpr int vbs_max_waves(int V) { return V <= 4 ? 5 : (V <= 7 ? 4 : 3); }
+
 template <int V>
-__global__ __launch_bounds__(kThreads) void vbs_search_kernel(const VbsParams p) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, vbs_max_waves(V))))
+void vbs_search_kernel(const VbsParams p) {
   extern __shared__ __attribute__((aligned(16))) uint32_t dyn[];
   __shared__ uint32_t red[kThreads / 64][9];
   lds_u32* const win = (lds_u32*)dyn;
@@ -69,7 +98,8 @@ __global__ __launch_bounds__(kThreads) void vbs_search_kernel(const VbsParams p)
   const int n = 16 * mcol, m = 16 * mrow;
 
   // stage every reference's window: rows m-R .. m-R+rows-1, 16-byte chunks
-  // from the 16-byte boundary at or left of column n + dx0
+  // from the 16-byte boundary at or left of column n + dx0 (chunk counts
+  // stay within udiv_small's domain: vbs_plan checks them)
   {
     const int cpr = p.row_dw >> 2;
     const int per_ref = p.rows * cpr;
@@ -77,8 +107,8 @@ __global__ __launch_bounds__(kThreads) void vbs_search_kernel(const VbsParams p)
     for (int base = wave * 64; base < total; base += kThreads) {
       const int ch = base + lane;
       if (ch < total) {
-        const int r = ch / per_ref, rem = ch - r * per_ref;
-        const int row = rem / cpr, c4 = rem - row * cpr;
+        const int r = udiv_small(ch, per_ref), rem = ch - r * per_ref;
+        const int row = udiv_small(rem, cpr), c4 = rem - row * cpr;
         const uint8_t* rb = r == 0 ? p.ref[0] : (r == 1 ? p.ref[1] : (r == 2 ? p.ref[2] : p.ref[3]));
         const uint8_t* src = rb + (long)(m - p.R + row) * p.ref_stride + n + p.dx0 - 4 * p.shift + 16 * c4;
         __builtin_amdgcn_global_load_lds((const void*)src,
@@ -107,9 +137,10 @@ __global__ __launch_bounds__(kThreads) void vbs_search_kernel(const VbsParams p)
   const int QG = p.Q * p.Gr;
   const int T = p.nrefs * QG;
   const int NN = p.N * p.N;
+  const uint32_t k15 = 1u << 15;
   for (int t = tid; t < T; t += kThreads) {
-    const int r = t / QG, rem = t - r * QG;
-    const int g = rem / p.Q, q = rem - g * p.Q;
+    const int r = udiv_small(t, QG), rem = t - r * QG;
+    const int g = udiv_small(rem, p.Q), q = rem - g * p.Q;
     const lds_u32* rp = win + (r * p.rows + g * V) * p.row_dw + p.shift + q;
     const int dx0 = p.dx0 + 4 * q, dy0 = -p.R + g * V;
     // Key offsets (rate << 15) + tie-break index, split into a column part
@@ -133,7 +164,7 @@ __global__ __launch_bounds__(kThreads) void vbs_search_kernel(const VbsParams p)
       for (int v = 0; v < V; v++) acc[k][v] = 0;
     auto span = [&](int k) { return *(const lds_u64_a4*)(rp + k); };
     uint64_t n0 = span(0), n1 = span(1), n2 = span(2), n3 = span(3);
-#pragma unroll
+#pragma clang loop unroll(full)
     for (int rho = 0; rho < V + 15; rho++) {
       uint64_t p0 = n0, p1 = n1, p2 = n2, p3 = n3;
       // the wait for this row's spans lands here, a row of qsads after
@@ -165,28 +196,22 @@ __global__ __launch_bounds__(kThreads) void vbs_search_kernel(const VbsParams p)
         const bool row_ok = dy >= yu && dy <= yd;
         const uint32_t offr = (__umul24((uint32_t)p.lambda, (uint32_t)(dy < 0 ? -dy : dy)) << 15) +
                               rowbase + (uint32_t)(v * p.N);
-        uint32_t k[9][4];
+        uint32_t off[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const uint32_t off = (row_ok && colok[j]) ? offr + offc[j] : 0x80000000u;
-          const uint32_t s0 = (uint32_t)(acc[0][v] >> (16 * j)) & 0xFFFFu;
-          const uint32_t s1 = (uint32_t)(acc[1][v] >> (16 * j)) & 0xFFFFu;
-          const uint32_t s2 = (uint32_t)(acc[2][v] >> (16 * j)) & 0xFFFFu;
-          const uint32_t s3 = (uint32_t)(acc[3][v] >> (16 * j)) & 0xFFFFu;
-          const uint32_t top = s0 + s1, bot = s2 + s3;
-          k[0][j] = ((top + bot) << 15) + off;
-          k[1][j] = (top << 15) + off;
-          k[2][j] = (bot << 15) + off;
-          k[3][j] = ((s0 + s2) << 15) + off;
-          k[4][j] = ((s1 + s3) << 15) + off;
-          k[5][j] = (s0 << 15) + off;
-          k[6][j] = (s1 << 15) + off;
-          k[7][j] = (s2 << 15) + off;
-          k[8][j] = (s3 << 15) + off;
+        for (int j = 0; j < 4; j++) off[j] = (row_ok && colok[j]) ? offr + offc[j] : 0x80000000u;
+        // the nine partitions' SADs, two candidates (columns 2h, 2h + 1) per
+        // dword: the quadrants as accumulated, the sums lane-wise
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const uint32_t s0 = (uint32_t)(acc[0][v] >> (32 * h)), s1 = (uint32_t)(acc[1][v] >> (32 * h));
+          const uint32_t s2 = (uint32_t)(acc[2][v] >> (32 * h)), s3 = (uint32_t)(acc[3][v] >> (32 * h));
+          const uint32_t top = add_u16x2(s0, s1), bot = add_u16x2(s2, s3);
+          const uint32_t part[9] = {add_u16x2(top, bot), top, bot, add_u16x2(s0, s2), add_u16x2(s1, s3),
+                                    s0, s1, s2, s3};
+#pragma unroll
+          for (int i = 0; i < 9; i++)
+            best[i] = min3_u32(best[i], key_lo16(part[i], k15, off[2 * h]), key_hi16(part[i], k15, off[2 * h + 1]));
         }
-#pragma unroll
-        for (int i = 0; i < 9; i++)
-          best[i] = min(min(min(min(best[i], k[i][0]), k[i][1]), k[i][2]), k[i][3]);
       }
     }
   }
@@ -255,6 +280,9 @@ void vbs_plan(int R, int nrefs, VbsParams* p) {
   p->Gr = (N + bestV - 1) / bestV;
   p->rows = p->Gr * bestV + 15;
   p->row_dw = ((p->shift + p->Q + 4) + 3) & ~3;
+  // the kernel divides with udiv_small: at R <= 32 and 4 references the
+  // chunk and task counts stay far inside u < 2^17, d <= 2048 (at most
+  // 4 x 87 x 6 chunks, 4 x 17 x 17 tasks)
 }
 
 int vbs_lds_bytes(const VbsParams& p) { return p.nrefs * p.rows * p.row_dw * 4; }
