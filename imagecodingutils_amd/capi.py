@@ -74,6 +74,10 @@ _SIGS = {
     "icu_me_batch_estimate_device": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), _vp, _vp, _vp]),
     "icu_me_batch_capture": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), _vp, _vp]),
     "icu_me_batch_graph_launch": (C.c_int, [_vp, _vp]),
+    "icu_me_batch_autotune": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), _vp, _vp, C.c_int,
+                                        C.POINTER(C.c_float)]),
+    "icu_me_batch_get_plan": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "icu_me_batch_set_plan": (C.c_int, [_vp, C.c_int, C.c_int]),
     "icu_server_create": (C.c_int, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
     "icu_server_destroy": (C.c_int, [_vp]),
     "icu_server_open_stream": (C.c_int, [_vp, C.POINTER(MeConfig), C.POINTER(C.c_int)]),
@@ -382,6 +386,23 @@ class MotionEstimatorBatch:
 
     def graph_launch(self, stream: Stream | None = None):
         _check(lib.icu_me_batch_graph_launch(self.h, stream.handle if stream else None))
+
+    def autotune(self, curs, refs, d_mv, d_dist, reps: int = 10) -> float:
+        """Times the launch plans and keeps the fastest; returns its ms per
+        estimate.  Capture again afterwards."""
+        ms = C.c_float()
+        _check(lib.icu_me_batch_autotune(self.h, self._ptrs(curs), self._ptrs(refs), _p(d_mv), _p(d_dist), reps,
+                                         C.byref(ms)))
+        return ms.value
+
+    def set_plan(self, grid_mode: int, queue_waves: int = 6):
+        _check(lib.icu_me_batch_set_plan(self.h, grid_mode, queue_waves))
+
+    def plan(self) -> tuple[int, int]:
+        """(grid mode, queue waves per SIMD)."""
+        g, w = C.c_int(), C.c_int()
+        _check(lib.icu_me_batch_get_plan(self.h, C.byref(g), C.byref(w)))
+        return g.value, w.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -6,6 +6,8 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <functional>
+#include <queue>
 #include <vector>
 
 #include "../../include/icu_hip.h"
@@ -119,8 +121,8 @@ bool valid_cfg(const icu_me_config& c, std::string* why) {
     *why = "mbs_per_task must be 0..16";
     return false;
   }
-  if (c.grid_mode < 0 || c.grid_mode > 4) {
-    *why = "grid_mode must be 0..4";
+  if (c.grid_mode < 0 || c.grid_mode > 6) {
+    *why = "grid_mode must be 0..6";
     return false;
   }
   if (2 * c.range + 1 > 129) {
@@ -149,15 +151,22 @@ struct StreamPlan {
 struct Engine {
   int device = 0;
   hipStream_t own = nullptr;
+  hipStream_t side = nullptr;  // grid mode 5: window classes 16 and 8 run here, beside the others
   std::vector<StreamPlan> sp;
   int total_mb = 0;
   long long table_total = 0;
   std::vector<SearchTask> h_tasks;   // workgroup tasks (me_search_kernel)
-  std::vector<SearchTask> h_wtasks;  // one MB per wave (me_wave_kernel), after h_tasks on the device,
-                                     // grouped by window class: kWaveClasses[k] has wave_n[k] tasks
-  static constexpr int kWaveClasses[3] = {32, 16, 8};
-  int wave_n[3] = {0, 0, 0};
-  bool wave_on = false;              // the wave kernel runs the streams it takes
+  std::vector<SearchTask> h_wtasks;  // one MB per wave, after h_tasks on the device: grouped by window
+                                     // class (kWaveClasses[k] has wave_n[k] tasks) for per-class launches
+                                     // of me_wave_kernel, or wave by wave for me_queue_kernel (wave b
+                                     // runs [h_woff[b], h_woff[b + 1]))
+  static constexpr int kWaveClasses[4] = {64, 32, 16, 8};
+  static constexpr int kMaxQueueWaves = 8;
+  int wave_n[4] = {0, 0, 0, 0};
+  std::vector<int> h_woff;
+  int* d_woff = nullptr;             // kMaxQueueWaves x 4 x cus + 1 ints
+  bool wave_on = false;              // the wave kernels run the streams they take
+  int queue_waves = 6;               // grid of the queue kernel: waves per SIMD x 4 x CUs
   std::vector<DecideItem> h_items;
   // Descriptor slots.  kEager slots serve eager calls: a call uses the slot
   // already bound to its buffers, or refills the least recently used one
@@ -178,6 +187,7 @@ struct Engine {
     int32_t* integ = nullptr;
     bool valid = false;
     bool used = false;        // `done` records this slot's last launch
+    hipEvent_t fork = nullptr, join = nullptr;  // grid mode 5: the side stream's start and end
     hipEvent_t done = nullptr;
     unsigned long long stamp = 0;
   } ds[kEager + 1];
@@ -200,7 +210,12 @@ struct Engine {
   // many as stay resident, run every grid-th task with the next task's window
   // streaming into a second buffer; 2 the same with the grid reduced so every
   // workgroup runs equally many tasks; 3 one MB per wave for the streams
-  // whose window the wave kernel takes (wave_rows), mode 0 for the rest
+  // whose window the wave kernel takes (wave_class), one launch per window
+  // class, mode 0 for the rest; 4 the same streams in one persistent launch
+  // over per-XCD macroblock queues (me_queue_kernel), mode 0 for the rest;
+  // 5 as 3 with the +-8 and +-16 classes on a second stream, so their
+  // latency-bound waves run beside the +-32 / +-64 launches (measured
+  // better than a stream per class)
   int grid_mode = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -220,6 +235,8 @@ struct Engine {
       hipFree(d.d_block);
       hipHostFree(d.h_block);
       if (d.done) hipEventDestroy(d.done);
+      if (d.fork) hipEventDestroy(d.fork);
+      if (d.join) hipEventDestroy(d.join);
       d = DescSet{};
     }
     hipFree(d_tasks);
@@ -228,10 +245,14 @@ struct Engine {
     hipFree(d_table);
     hipFree(d_wave);
     hipFree(d_pf);
+    hipFree(d_woff);
+    d_woff = nullptr;
     d_tasks = nullptr; d_items = nullptr;
     d_ires = nullptr; d_table = nullptr; d_wave = nullptr; d_pf = nullptr;
     if (own) hipStreamDestroy(own);
+    if (side) hipStreamDestroy(side);
     own = nullptr;
+    side = nullptr;
   }
 
   // The wave kernel takes argmin (no cost table) SAD streams whose window
@@ -249,8 +270,8 @@ struct Engine {
   void build_tasks() {
     h_tasks.clear();
     h_wtasks.clear();
-    wave_on = grid_mode == 3;
-    for (int k = 0; k < 3; k++) {
+    wave_on = grid_mode >= 3;
+    for (int k = 0; k < 4; k++) {
       wave_n[k] = 0;
       for (size_t si = 0; si < sp.size(); si++) {
         const StreamPlan& s = sp[si];
@@ -259,6 +280,8 @@ struct Engine {
         wave_n[k] += s.nmb;
       }
     }
+    h_woff.clear();
+    if (grid_mode == 4) build_queue();
     for (size_t si = 0; si < sp.size(); si++) {
       const StreamPlan& s = sp[si];
       if (wave_stream(s)) continue;
@@ -280,11 +303,59 @@ struct Engine {
     stage_dwords = (stage_dwords + 63) & ~63;
   }
 
+  // The queue kernel's schedule: each window class split into 8 contiguous
+  // runs of MBs, run x to XCD x (so an XCD's L2 serves neighbouring windows);
+  // an XCD's MBs, costliest first, each to the XCD's wave with the least
+  // estimated work so far (workgroup b runs on XCD b mod 8); then the tasks
+  // wave by wave.
+  void build_queue() {
+    const int nw = queue_waves * 4 * cus;
+    std::vector<std::vector<SearchTask>> lists(nw);
+    for (int x = 0; x < 8; x++) {
+      std::vector<std::pair<long, SearchTask>> share;
+      for (int k = 0, off = 0; k < 4; off += wave_n[k], k++) {
+        const int nq = wave_n[k] / 8, rq = wave_n[k] % 8;
+        const int a = off + x * nq + std::min(x, rq), len = nq + (x < rq ? 1 : 0);
+        for (int i = a; i < a + len; i++) {
+          const SearchTask& t = h_wtasks[i];
+          const StreamPlan& s = sp[t.stream];
+          share.push_back({wave_task_cost(16 * (t.mb % s.mbw), 16 * (t.mb / s.mbw), s.cfg.width, s.cfg.height, s.L),
+                           t});
+        }
+      }
+      std::stable_sort(share.begin(), share.end(),
+                       [](const std::pair<long, SearchTask>& a, const std::pair<long, SearchTask>& b) {
+                         return a.first > b.first;
+                       });
+      // least-loaded wave first; ties to the lower wave index
+      typedef std::pair<long, int> Load;
+      std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+      for (int b = x; b < nw; b += 8) heap.push({0, b});
+      if (heap.empty()) continue;
+      for (const auto& ct : share) {
+        Load l = heap.top();
+        heap.pop();
+        lists[l.second].push_back(ct.second);
+        heap.push({l.first + ct.first, l.second});
+      }
+    }
+    std::vector<SearchTask> out;
+    out.reserve(h_wtasks.size());
+    h_woff.assign(nw + 1, 0);
+    for (int b = 0; b < nw; b++) {
+      h_woff[b] = (int)out.size();
+      out.insert(out.end(), lists[b].begin(), lists[b].end());
+    }
+    h_woff[nw] = (int)out.size();
+    h_wtasks.swap(out);
+  }
+
   int init(const icu_me_config* cfgs, int n) {
     HIP_TRY(hipGetDevice(&device));
     // a blocking stream: work on it is ordered with the null stream, so a
     // caller's plain hipMemcpy after an asynchronous call sees the results
     HIP_TRY(hipStreamCreate(&own));
+    HIP_TRY(hipStreamCreate(&side));
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     cus = prop.multiProcessorCount;
@@ -320,9 +391,11 @@ struct Engine {
       pad_base.push_back(pad_total);
       pad_total += pad_plane_workgroups(s.cfg.width, s.cfg.height);
     }
-    // unless a grid mode is asked for, streams the wave kernel takes run one
-    // MB per wave (measured faster than the workgroup tiles at +-31 / +-32)
-    if (!cfgs[0].grid_mode && any_wave_stream()) grid_mode = 3;
+    // unless a grid mode is asked for, streams the wave kernels take run one
+    // MB per wave (measured faster than the workgroup tiles at +-31 / +-32),
+    // one launch per window class; a batch's +-8 / +-16 classes on a second
+    // stream (the fastest untuned plan on the benchmark's mixed batch)
+    if (!cfgs[0].grid_mode && any_wave_stream()) grid_mode = n > 1 ? 5 : 3;
     build_tasks();
     // descriptor block: three arrays, each starting on a 256-byte boundary
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -333,6 +406,8 @@ struct Engine {
       dev_bytes += desc_bytes;
       HIP_TRY(hipHostMalloc(&d.h_block, desc_bytes, hipHostMallocDefault));
       HIP_TRY(hipEventCreateWithFlags(&d.done, hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&d.fork, hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&d.join, hipEventDisableTiming));
       d.d_streams = (StreamDesc*)d.d_block;
       d.d_decide = (DecideDesc*)((char*)d.d_block + o_dec);
       d.d_pad = (PadDesc*)((char*)d.d_block + o_pad);
@@ -343,6 +418,8 @@ struct Engine {
     HIP_TRY(hipMalloc(&d_tasks, sizeof(SearchTask) * total_mb));
     HIP_TRY(hipMalloc(&d_ires, sizeof(IntResult) * total_mb));
     HIP_TRY(hipMalloc(&d_wave, sizeof(int) * wave_total));
+    HIP_TRY(hipMalloc(&d_woff, sizeof(int) * (kMaxQueueWaves * 4 * cus + 1)));
+    dev_bytes += sizeof(int) * (kMaxQueueWaves * 4 * cus + 1);
     dev_bytes += (sizeof(SearchTask) + sizeof(IntResult)) * total_mb + sizeof(int) * wave_total;
     HIP_TRY(hipMemset(d_wave, 0, sizeof(int) * wave_total));
     if (table_total) HIP_TRY(hipMalloc(&d_table, sizeof(uint32_t) * table_total));
@@ -505,10 +582,26 @@ struct Engine {
     }
     for (int r = 0; r < search_reps; r++) {
       if (!h_tasks.empty())
-        launch_search(d_streams, d_tasks, (int)h_tasks.size(), grid_mode == 3 ? 0 : grid_mode, sp[0].cfg.metric,
+        launch_search(d_streams, d_tasks, (int)h_tasks.size(), grid_mode >= 3 ? 0 : grid_mode, sp[0].cfg.metric,
                       std::max(vfix, 0), any_table, stage_dwords, d_ires, d_table, d_decide, sp.size() == 1, st);
-      for (int k = 0, off = (int)h_tasks.size(); k < 3; off += wave_n[k], k++)
-        launch_wave_search(d_streams, d_tasks + off, wave_n[k], kWaveClasses[k], d_decide, true, st);
+      if (grid_mode == 4) {
+        launch_wave_queue(d_streams, d_tasks + h_tasks.size(), d_woff, queue_waves * 4 * cus, d_decide, st);
+      } else {
+        // mode 5: classes 64 and 32 on st, 16 and 8 on the side stream,
+        // which starts after the padding and which st waits for at the end
+        const bool split = grid_mode == 5 && (wave_n[0] + wave_n[1]) > 0 && (wave_n[2] + wave_n[3]) > 0;
+        if (split) {
+          HIP_TRY(hipEventRecord(ds[set].fork, st));
+          HIP_TRY(hipStreamWaitEvent(side, ds[set].fork, 0));
+        }
+        for (int k = 0, off = (int)h_tasks.size(); k < 4; off += wave_n[k], k++)
+          launch_wave_search(d_streams, d_tasks + off, wave_n[k], kWaveClasses[k], d_decide, true,
+                             split && k >= 2 ? side : st);
+        if (split) {
+          HIP_TRY(hipEventRecord(ds[set].join, side));
+          HIP_TRY(hipStreamWaitEvent(st, ds[set].join, 0));
+        }
+      }
     }
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
     // without cost tables the search kernel decides every MB itself
@@ -542,6 +635,8 @@ struct Engine {
     if (e == hipSuccess && !h_wtasks.empty())
       e = hipMemcpy(d_tasks + h_tasks.size(), h_wtasks.data(), sizeof(SearchTask) * h_wtasks.size(),
                     hipMemcpyHostToDevice);
+    if (e == hipSuccess && !h_woff.empty())
+      e = hipMemcpy(d_woff, h_woff.data(), sizeof(int) * h_woff.size(), hipMemcpyHostToDevice);
     return e;
   }
 
@@ -566,6 +661,11 @@ struct Engine {
 
   int retile(int si, Tiling t) {
     sp[si].t = t;
+    return replan();
+  }
+
+  // Rebuilds and uploads the task lists after a change of tiling or grid mode.
+  int replan() {
     build_tasks();
     if ((int)(h_tasks.size() + h_wtasks.size()) > total_mb) return fail(ICU_E_STATE, "task list exceeds its allocation");
     // kernels of an earlier call may still read the task list
@@ -834,15 +934,16 @@ int icu_me_autotune(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref, int r
       }
     }
   }
-  // one MB per wave, where the stream's window takes it
-  if (rc == ICU_OK && e.any_wave_stream()) {
-    e.grid_mode = 3;
+  // one MB per wave, where the stream's window takes it: one launch per
+  // window class, or the queue kernel
+  for (int mode = 3; mode <= 4 && rc == ICU_OK && e.any_wave_stream(); mode++) {
+    e.grid_mode = mode;
     float ms = 0;
     if ((rc = e.retile(0, s.t)) == ICU_OK && (rc = time_estimates(h, d_cur, d_ref, reps, a, b, &ms)) == ICU_OK &&
         ms < best_ms) {
       best_ms = ms;
       best = s.t;
-      best_mode = 3;
+      best_mode = mode;
     }
   }
   hipEventDestroy(a);
@@ -1044,6 +1145,87 @@ int icu_me_batch_capture(icu_me_batch* b, const uint8_t* const* curs, const uint
 int icu_me_batch_graph_launch(icu_me_batch* b, void* stream) {
   if (!b || !b->eng.gexec) return fail(ICU_E_STATE, "no captured graph");
   HIP_TRY(hipGraphLaunch(b->eng.gexec, pick(stream, b->eng)));
+  return ICU_OK;
+}
+
+int icu_me_batch_autotune(icu_me_batch* b, const uint8_t* const* curs, const uint8_t* const* refs, int16_t* d_mv,
+                          int32_t* d_dist, int reps, float* best_ms) {
+  if (!b || !curs || !refs || !d_mv || !d_dist) return fail(ICU_E_ARG, "null argument");
+  Engine& e = b->eng;
+  if (reps < 1) reps = 1;
+  struct Plan {
+    int mode, waves;
+  };
+  std::vector<Plan> plans;
+  for (int m = 0; m < 3; m++) plans.push_back(Plan{m, e.queue_waves});
+  if (e.any_wave_stream()) {
+    plans.push_back(Plan{3, e.queue_waves});
+    plans.push_back(Plan{5, e.queue_waves});
+    for (int w : {6, 8}) plans.push_back(Plan{4, w});
+  }
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  HIP_TRY(hipEventCreate(&ev[0]));
+  if (hipEventCreate(&ev[1]) != hipSuccess) {
+    hipEventDestroy(ev[0]);
+    return fail(ICU_E_DEVICE, "hipEventCreate failed");
+  }
+  Plan best{e.grid_mode, e.queue_waves};
+  float best_t = 1e30f;
+  int rc = ICU_OK;
+  // each plan is timed as the batch runs: captured in a graph and replayed
+  auto run = [&]() { return icu_me_batch_graph_launch(b, e.own); };
+  for (const Plan& p : plans) {
+    e.grid_mode = p.mode;
+    e.queue_waves = p.waves;
+    if ((rc = e.replan()) != ICU_OK || (rc = icu_me_batch_capture(b, curs, refs, d_mv, d_dist)) != ICU_OK) break;
+    for (int w = 0; w < 2 && rc == ICU_OK; w++) rc = run();  // warm-up
+    // the least of three timed runs of `reps` estimates each
+    float t_min = 1e30f;
+    for (int trial = 0; trial < 3 && rc == ICU_OK; trial++) {
+      if (hipEventRecord(ev[0], e.own) != hipSuccess) rc = fail(ICU_E_DEVICE, "hipEventRecord failed");
+      for (int r = 0; r < reps && rc == ICU_OK; r++) rc = run();
+      float ms = 0;
+      if (rc == ICU_OK && (hipEventRecord(ev[1], e.own) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+                           hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess))
+        rc = fail(ICU_E_DEVICE, "event timing failed");
+      t_min = std::min(t_min, ms / reps);
+    }
+    if (rc != ICU_OK) break;
+    if (t_min < best_t) {
+      best_t = t_min;
+      best = p;
+    }
+  }
+  hipEventDestroy(ev[0]);
+  hipEventDestroy(ev[1]);
+  // the graph of the last plan tried is dropped: the caller captures again
+  if (hipStreamSynchronize(e.own) != hipSuccess && rc == ICU_OK) rc = fail(ICU_E_DEVICE, "tuning run failed");
+  if (e.gexec) hipGraphExecDestroy(e.gexec);
+  if (e.graph) hipGraphDestroy(e.graph);
+  e.gexec = nullptr;
+  e.graph = nullptr;
+  if (rc != ICU_OK) return rc;
+  e.grid_mode = best.mode;
+  e.queue_waves = best.waves;
+  if (best_ms) *best_ms = best_t;
+  return e.replan();
+}
+
+int icu_me_batch_set_plan(icu_me_batch* b, int grid_mode, int queue_waves) {
+  if (!b) return fail(ICU_E_ARG, "null batch");
+  if (grid_mode < 0 || grid_mode > 5) return fail(ICU_E_ARG, "grid_mode must be 0..5");
+  if (grid_mode >= 3 && !b->eng.any_wave_stream())
+    return fail(ICU_E_ARG, "grid modes 3..5 need a stream the one-MB-per-wave kernels take");
+  if (queue_waves < 1 || queue_waves > Engine::kMaxQueueWaves) return fail(ICU_E_ARG, "queue_waves must be 1..8");
+  b->eng.grid_mode = grid_mode;
+  b->eng.queue_waves = queue_waves;
+  return b->eng.replan();
+}
+
+int icu_me_batch_get_plan(icu_me_batch* b, int* grid_mode, int* queue_waves) {
+  if (!b) return fail(ICU_E_ARG, "null batch");
+  if (grid_mode) *grid_mode = b->eng.grid_mode;
+  if (queue_waves) *queue_waves = b->eng.queue_waves;
   return ICU_OK;
 }
 

@@ -79,11 +79,19 @@ void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int n
                    int metric, int vfix, bool table, int buf_dwords, IntResult* d_out,
                    uint32_t* d_table, const DecideDesc* d_decide, bool band, hipStream_t st);
 // One MB per wave (me_wave_kernel) for argmin SAD streams whose window
-// half-width L is L0 or L0 - 1, L0 = wave_class(L) in {8, 16, 32} (0: no
+// half-width L is L0 or L0 - 1, L0 = wave_class(L) in {8, 16, 32, 64} (0: no
 // class); decides each MB in place.  All tasks of one launch share L0.
 int wave_class(int L);
 void launch_wave_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int L0,
                         const DecideDesc* d_decide, bool band, hipStream_t st);
+// The same MBs, any mix of classes, in one persistent launch of `waves`
+// one-wave workgroups: wave b runs tasks [d_woff[b], d_woff[b + 1]) in order
+// (d_woff: waves + 1 offsets).
+void launch_wave_queue(const StreamDesc* d_streams, const SearchTask* d_tasks, const int* d_woff, int waves,
+                       const DecideDesc* d_decide, hipStream_t st);
+// Estimated work of the MB at pel (n, m) of a W x H picture searched by the
+// wave kernels with window half-width L, in candidates (the queue's LPT key).
+long wave_task_cost(int n, int m, int W, int H, int L);
 // Dwords of one staging buffer (window rows padded to 16 bytes + blocks).
 int search_stage_dwords(int B, int V, int Q, int G);
 int search_max_window_dwords();

@@ -669,13 +669,168 @@ struct WaveGeom {
   static_assert(QW * BB == 64 && BB * VV == 2 * L0 && VV % TR == 0, "lane tiling");
 };
 
+// The MB a wave searches: its position and window.
+struct WaveMb {
+  int n, m, L;
+  int xl, xr, yu, yd;
+};
+
+__device__ __forceinline__ WaveMb wave_mb(const StreamDesc& s, int mb) {
+  WaveMb w;
+  const int row = mb / s.mbw, col = mb - row * s.mbw;
+  w.n = 16 * col;
+  w.m = 16 * row;
+  w.L = s.L;
+  motion_window(w.n, w.m, s.W, s.H, w.L, &w.xl, &w.xr, &w.yu, &w.yd);
+  return w;
+}
+
+// One pass of the window class L0 over the 2 L0 x 2 L0 candidates whose
+// top-left is (dxb, dyb) (dxb - L0 a multiple of 16), folded into kbest:
+// the window rows m + dyb .. m + dyb + 2 L0 + 15 are staged from the
+// 16-byte boundary at or left of column n + dxb, then every lane runs its
+// tiles.  lcol / lrow: also the column dx = dxb + 2 L0 / the row
+// dy = dyb + 2 L0 (64 candidates each, lane l the l-th), and corner the
+// candidate where they meet.  Candidates outside the MB's window fold
+// nothing.  The caller has finished every LDS read of an earlier pass.
+template <int L0>
+__device__ __forceinline__ void wave_pass(const StreamDesc& s, const WaveMb& w, lds_u32* win, int lane, int dxb,
+                                          int dyb, bool lcol, bool lrow, bool corner, uint64_t& kbest) {
+  using G = WaveGeom<L0>;
+  // the block, loaded by every pass (nothing lives across passes): uniform
+  // addresses, so scalar loads straight into SGPRs (through the constant
+  // address space, where uniform loads become s_load)
+  uint32_t c[64];
+  {
+    typedef __attribute__((address_space(4))) const uint32_t const_u32;
+    const const_u32* cb = (const const_u32*)(s.cur + (long)w.m * s.cur_stride + w.n);
+    const int sdw = s.cur_stride >> 2;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) c[4 * i + k] = cb[i * sdw + k];
+  }
+  {
+    constexpr int kCpr = G::ROW_DW / 4, kChunks = G::ROWS * kCpr;
+    const uint8_t* wsrc = s.ref + (long)(w.m + dyb) * s.ref_stride + w.n + dxb - 4 * G::SHIFT;
+#pragma unroll
+    for (int base = 0; base < kChunks; base += 64) {
+      const int ch = base + lane;
+      const int r = ch / kCpr, c4 = ch - kCpr * r;
+      if (ch < kChunks)
+        __builtin_amdgcn_global_load_lds((const void*)(wsrc + (long)r * s.ref_stride + 16 * c4),
+                                         (__attribute__((address_space(3))) void*)(win + 4 * base), 16, 0, 0);
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // the block and the window have landed (one wave: no barrier)
+  dec::wave_sync();
+
+  const lds_u32* w0 = win + G::SHIFT;  // column dxb
+  const int q = lane % G::QW, b = lane / G::QW;
+  const int dx0 = dxb + 4 * q;
+  const int dy0 = dyb + b * G::VV;
+#pragma unroll
+  for (int h = 0; h < G::VV / G::TR; h++)
+    wave_tile<G::TR, 5, G::ROW_DW>(w0 + (b * G::VV + h * G::TR) * G::ROW_DW + q, c, dx0, dy0 + h * G::TR, w.xl,
+                                   w.xr, w.yu, w.yd, kbest);
+  const bool live = lane < 2 * L0;
+  const int l = live ? lane : 2 * L0 - 1;  // lanes past the line read in bounds, fold nothing
+  if (lcol) {
+    // candidate (dxb + 2 L0, dyb + l) from window rows l .. l + 15, bytes
+    // 2 L0 .. 2 L0 + 15
+    uint32_t acc = 0;
+    const lds_u32* rp = w0 + l * G::ROW_DW + L0 / 2;
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc = __builtin_amdgcn_sad_u8(rp[r * G::ROW_DW + k], c[4 * r + k], acc);
+    fold_key(kbest, live, acc, dxb + 2 * L0, dyb + l, w.xl, w.xr, w.yu, w.yd);
+  }
+  if (lrow) {
+    // candidate (dxb + l, dyb + 2 L0) from window rows 2 L0 .. 2 L0 + 15,
+    // bytes l .. l + 15
+    uint32_t acc = 0;
+    const lds_u32* rp = w0 + 2 * L0 * G::ROW_DW + (l >> 2);
+    const uint32_t sh = (uint32_t)(l & 3);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      uint32_t v[5];
+#pragma unroll
+      for (int j = 0; j < 5; j++) v[j] = rp[r * G::ROW_DW + j];
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(v[k + 1], v[k], sh), c[4 * r + k], acc);
+    }
+    fold_key(kbest, live, acc, dxb + l, dyb + 2 * L0, w.xl, w.xr, w.yu, w.yd);
+  }
+  if (corner) {
+    // lane l the block dword (l / 4, l % 4)
+    const uint32_t cw = *(const uint32_t*)(s.cur + (long)(w.m + (lane >> 2)) * s.cur_stride + w.n + 4 * (lane & 3));
+    const uint32_t rw = w0[(2 * L0 + (lane >> 2)) * G::ROW_DW + L0 / 2 + (lane & 3)];
+    const int tot = dec::wave_sum((int)__builtin_amdgcn_sad_u8(cw, rw, 0u));
+    fold_key(kbest, true, (uint32_t)tot, dxb + 2 * L0, dyb + 2 * L0, w.xl, w.xr, w.yu, w.yd);
+  }
+}
+
+// The integer search of one MB whose window class is L0 (8, 16, 32 or 64):
+// one pass, or for L0 = 64 four quadrant passes of the 32 class (candidate
+// columns from -64 and from 0, rows from -L and from 64 - L); quadrants
+// wholly outside the MB's window are skipped.  Returns the MB's least key.
+__device__ __forceinline__ uint64_t wave_search_mb(const StreamDesc& s, const WaveMb& w, lds_u32* win, int lane,
+                                                   int L0) {
+  uint64_t kbest = ~0ull;
+  const bool ex = w.L == L0;  // exhaustive: one more column and row
+  if (L0 == 8) {
+    wave_pass<8>(s, w, win, lane, -8, -w.L, ex, ex, ex, kbest);
+  } else if (L0 == 16) {
+    wave_pass<16>(s, w, win, lane, -16, -w.L, ex, ex, ex, kbest);
+  } else if (L0 == 32) {
+    wave_pass<32>(s, w, win, lane, -32, -w.L, ex, ex, ex, kbest);
+  } else {
+    const int np = 4;
+#pragma unroll 1
+    for (int p = 0; p < np; p++) {
+      const int qx = p & 1, qy = p >> 1;
+      const int dxb = np == 1 ? -32 : -64 + 64 * qx, dyb = -w.L + 64 * qy;
+      const bool lc = ex && (np == 1 || qx), lr = ex && (np == 1 || qy);
+      const int dxe = dxb + 63 + (lc ? 1 : 0), dye = dyb + 63 + (lr ? 1 : 0);
+      if (dxe < w.xl || dxb > w.xr || dye < w.yu || dyb > w.yd) continue;
+      dec::wave_sync();  // an earlier pass's LDS reads are done
+      int ln = lane;  // opaque per pass, so no per-lane address is held across passes
+      asm volatile("" : "+v"(ln));
+      __builtin_assume(ln >= 0 && ln < 64);
+      wave_pass<32>(s, w, win, ln, dxb, dyb, lc, lr, lc && lr, kbest);
+    }
+  }
+  return dec::wave_min_u64(kbest);
+}
+
+// The decision of a searched MB, in the same wave, with the window's LDS
+// reused (at a raised wave priority: it is the MB's serial tail).
+__device__ __forceinline__ void wave_decide_mb(const DecideDesc& d, uint32_t* lds, int mb, int lane,
+                                               uint64_t kbest) {
+  int mx, my;
+  key_vector(kbest, &mx, &my);
+  dec::wave_sync();  // every lane's window reads are done: the decision reuses the LDS
+  __builtin_amdgcn_s_setprio(2);
+  // inlined: with the block in SGPRs the tile and the decision fit 80 VGPRs
+  // together (6 waves per SIMD); called out of line the decision alone took 87
+  const IntResult r{mx, my, (int)key_cost(kbest), -1};
+  dec::decide_mb(d, *(dec::WaveLds*)lds, mb, 0, 0, lane, &r);
+}
+
+// LDS of any window class (the 64 class stages 32-class quadrants).
+constexpr int kWaveLdsBytes =
+    WaveGeom<32>::LDS_BYTES > WaveGeom<16>::LDS_BYTES
+        ? (WaveGeom<32>::LDS_BYTES > WaveGeom<8>::LDS_BYTES ? WaveGeom<32>::LDS_BYTES : WaveGeom<8>::LDS_BYTES)
+        : (WaveGeom<16>::LDS_BYTES > WaveGeom<8>::LDS_BYTES ? WaveGeom<16>::LDS_BYTES : WaveGeom<8>::LDS_BYTES);
+
 template <int L0>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
 void me_wave_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __restrict__ tasks, int ntasks,
                     const DecideDesc* __restrict__ decide, int band) {
-  using G = WaveGeom<L0>;
-  __shared__ __attribute__((aligned(16))) uint32_t win_raw[G::LDS_BYTES / 4];
-  lds_u32* const win = (lds_u32*)win_raw;
+  constexpr int kLds = L0 == 64 ? WaveGeom<32>::LDS_BYTES : WaveGeom<L0>::LDS_BYTES;
+  __shared__ __attribute__((aligned(16))) uint32_t win_raw[kLds / 4];
   const int lane = threadIdx.x;
   int ti = blockIdx.x;
   if (band) {
@@ -686,98 +841,42 @@ void me_wave_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __
   }
   const SearchTask t = tasks[ti];
   const StreamDesc s = streams[t.stream];
-  const int row = t.mb / s.mbw, col = t.mb - row * s.mbw;
-  const int n = 16 * col, m = 16 * row;
-  const int L = s.L;
-  // the block: uniform addresses, so scalar loads straight into SGPRs; cw
-  // is lane l's dword (l / 4, l % 4) of it
-  uint32_t c[64];
-  {
-    // through the constant address space: uniform loads there become s_load
-    typedef __attribute__((address_space(4))) const uint32_t const_u32;
-    const const_u32* cb = (const const_u32*)(s.cur + (long)m * s.cur_stride + n);
-    const int sdw = s.cur_stride >> 2;
-#pragma unroll
-    for (int i = 0; i < 16; i++)
-#pragma unroll
-      for (int k = 0; k < 4; k++) c[4 * i + k] = cb[i * sdw + k];
-  }
-  const uint32_t cw = *(const uint32_t*)(s.cur + (long)(m + (lane >> 2)) * s.cur_stride + n + 4 * (lane & 3));
-  // staging: rows m - L .. m - L + 2 L0 + 15, bytes from the 16-byte boundary
-  // at or left of n - L0 (n is a multiple of 16)
-  {
-    constexpr int kCpr = G::ROW_DW / 4, kChunks = G::ROWS * kCpr;
-    const uint8_t* wsrc = s.ref + (long)(m - L) * s.ref_stride + n - L0 - 4 * G::SHIFT;
-#pragma unroll
-    for (int base = 0; base < kChunks; base += 64) {
-      const int ch = base + lane;
-      const int r = ch / kCpr, c4 = ch - kCpr * r;
-      if (ch < kChunks)
-        __builtin_amdgcn_global_load_lds((const void*)(wsrc + (long)r * s.ref_stride + 16 * c4),
-                                         (__attribute__((address_space(3))) void*)(win + 4 * base), 16, 0, 0);
-    }
-  }
-  int xl, xr, yu, yd;
-  motion_window(n, m, s.W, s.H, L, &xl, &xr, &yu, &yd);
-  __builtin_amdgcn_s_waitcnt(0);  // the block and the window have landed (one wave: no barrier)
-  dec::wave_sync();
+  const uint64_t kbest = wave_search_mb(s, wave_mb(s, t.mb), (lds_u32*)win_raw, lane, L0);
+  wave_decide_mb(decide[t.stream], win_raw, t.mb, lane, kbest);
+}
 
-  const lds_u32* w0 = win + G::SHIFT;  // column -L0
-  const int q = lane % G::QW, b = lane / G::QW;
-  const int dx0 = -L0 + 4 * q;
-  const int dy0 = -L + b * G::VV;
-  uint64_t kbest = ~0ull;
-#pragma unroll
-  for (int h = 0; h < G::VV / G::TR; h++)
-    wave_tile<G::TR, 5, G::ROW_DW>(w0 + (b * G::VV + h * G::TR) * G::ROW_DW + q, c, dx0, dy0 + h * G::TR, xl, xr, yu,
-                                   yd, kbest);
-  if (L == L0) {
-    const bool live = lane < 2 * L0;
-    const int l = live ? lane : 2 * L0 - 1;  // lanes past the line read in bounds, fold nothing
-    // column +L0: candidate (L0, l - L0) from window rows l .. l + 15, bytes
-    // 2 L0 .. 2 L0 + 15
-    {
-      uint32_t acc = 0;
-      const lds_u32* rp = w0 + l * G::ROW_DW + L0 / 2;
-#pragma unroll
-      for (int r = 0; r < 16; r++)
-#pragma unroll
-        for (int k = 0; k < 4; k++) acc = __builtin_amdgcn_sad_u8(rp[r * G::ROW_DW + k], c[4 * r + k], acc);
-      fold_key(kbest, live, acc, L0, l - L0, xl, xr, yu, yd);
-    }
-    // row +L0: candidate (l - L0, L0) from window rows 2 L0 .. 2 L0 + 15,
-    // bytes l .. l + 15
-    {
-      uint32_t acc = 0;
-      const lds_u32* rp = w0 + 2 * L0 * G::ROW_DW + (l >> 2);
-      const uint32_t sh = (uint32_t)(l & 3);
-#pragma unroll
-      for (int r = 0; r < 16; r++) {
-        uint32_t w[5];
-#pragma unroll
-        for (int j = 0; j < 5; j++) w[j] = rp[r * G::ROW_DW + j];
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-          acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(w[k + 1], w[k], sh), c[4 * r + k], acc);
-      }
-      fold_key(kbest, live, acc, l - L0, L0, xl, xr, yu, yd);
-    }
-    // candidate (L0, L0): lane l the block dword (l / 4, l % 4)
-    {
-      const uint32_t rw = w0[(2 * L0 + (lane >> 2)) * G::ROW_DW + L0 / 2 + (lane & 3)];
-      const int tot = dec::wave_sum((int)__builtin_amdgcn_sad_u8(cw, rw, 0u));
-      fold_key(kbest, true, (uint32_t)tot, L0, L0, xl, xr, yu, yd);
-    }
+// Persistent form over a macroblock work queue: a grid of one-wave
+// workgroups, as many as stay resident, in which wave b runs the tasks
+// [woff[b], woff[b + 1]) of `tasks` (one MB each, any window class) one
+// after another.  The host builds the queue: the MBs are split over the 8
+// XCDs (workgroup b runs on XCD b mod 8) in contiguous runs per window
+// class, and each XCD's share is dealt to its waves costliest first, each MB
+// to the wave with the least work so far (LPT), by an estimate of the MB's
+// work; a wave's list is costliest first too, so waves of an XCD run
+// neighbouring MBs side by side and every wave ends with small ones.  No
+// atomics: a queue taken from a shared counter measured twice the time of
+// the one-launch-per-class plan on a single 1080p stream, its counter being
+// contended by every wave of an XCD.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
+void me_queue_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __restrict__ tasks,
+                     const int* __restrict__ woff, const DecideDesc* __restrict__ decide) {
+  __shared__ __attribute__((aligned(16))) uint32_t win_raw[kWaveLdsBytes / 4];
+  const int lane = threadIdx.x;
+  const int beg = woff[blockIdx.x], end = woff[blockIdx.x + 1];
+  for (int ti = beg; ti < end; ti++) {
+    __builtin_amdgcn_s_setprio(0);
+    // the lane index made opaque per task: otherwise every per-lane address
+    // of the task would be hoisted out of the loop and held (spilled) across it
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    __builtin_assume(ln >= 0 && ln < 64);
+    const SearchTask t = tasks[ti];
+    const StreamDesc s = streams[t.stream];
+    const int L0 = s.L <= 8 ? 8 : (s.L <= 16 ? 16 : (s.L <= 32 ? 32 : 64));
+    const uint64_t kbest = wave_search_mb(s, wave_mb(s, t.mb), (lds_u32*)win_raw, ln, L0);
+    wave_decide_mb(decide[t.stream], win_raw, t.mb, ln, kbest);
+    dec::wave_sync();  // the decision's LDS reads are done before the next staging
   }
-  kbest = dec::wave_min_u64(kbest);
-  int mx, my;
-  key_vector(kbest, &mx, &my);
-  dec::wave_sync();  // every lane's window reads are done: the decision reuses the LDS
-  __builtin_amdgcn_s_setprio(2);
-  // inlined: with the block in SGPRs the tile and the decision fit 80 VGPRs
-  // together (6 waves per SIMD); called out of line the decision alone took 87
-  const IntResult r{mx, my, (int)key_cost(kbest), -1};
-  dec::decide_mb(decide[t.stream], *(dec::WaveLds*)win_raw, t.mb, 0, 0, lane, &r);
 }
 
 void launch_wave_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int L0,
@@ -786,13 +885,42 @@ void launch_wave_search(const StreamDesc* d_streams, const SearchTask* d_tasks, 
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(ntasks), dim3(64), 0, st, d_streams, d_tasks, ntasks, d_decide, band ? 1 : 0);
   };
-  if (L0 == 32) go(me_wave_kernel<32>);
+  if (L0 == 64) go(me_wave_kernel<64>);
+  else if (L0 == 32) go(me_wave_kernel<32>);
   else if (L0 == 16) go(me_wave_kernel<16>);
   else go(me_wave_kernel<8>);
 }
 
+void launch_wave_queue(const StreamDesc* d_streams, const SearchTask* d_tasks, const int* d_woff, int waves,
+                       const DecideDesc* d_decide, hipStream_t st) {
+  if (waves <= 0) return;
+  hipLaunchKernelGGL(me_queue_kernel, dim3(waves), dim3(64), 0, st, d_streams, d_tasks, d_woff, d_decide);
+}
+
+long wave_task_cost(int n, int m, int W, int H, int L) {
+  // In candidates: per pass the 2 L0 x 2 L0 grid (and the exhaustive
+  // window's extra column and row) plus 780 for staging the window, plus 130
+  // for the decision.  Fitted to the one-launch-per-class times of a wave
+  // slot at 6 waves per SIMD: 13.0 / 21.8 / 56 / 217 us per MB at
+  // +-8 / 16 / 32 / 64, about 0.0109 us per candidate.
+  const int L0 = wave_class(L);
+  const long ex = L == L0 ? 1 : 0;
+  if (L0 != 64) return 4L * L0 * L0 + ex * (4L * L0 + 1) + 780 + 130;
+  int xl, xr, yu, yd;
+  motion_window(n, m, W, H, L, &xl, &xr, &yu, &yd);
+  long c = 130;
+  for (int p = 0; p < 4; p++) {
+    const int qx = p & 1, qy = p >> 1;
+    const int dxb = -64 + 64 * qx, dyb = -L + 64 * qy;
+    const int dxe = dxb + 63 + (ex && qx ? 1 : 0), dye = dyb + 63 + (ex && qy ? 1 : 0);
+    if (dxe < xl || dxb > xr || dye < yu || dyb > yd) continue;
+    c += 4096 + 64 * ex * (qx + qy) + 780;
+  }
+  return c;
+}
+
 int wave_class(int L) {
-  for (int L0 : {8, 16, 32})
+  for (int L0 : {8, 16, 32, 64})
     if (L == L0 || L == L0 - 1) return L0;
   return 0;
 }

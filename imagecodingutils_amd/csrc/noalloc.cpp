@@ -105,7 +105,7 @@ extern "C" int icu_me_estimate_device_noalloc(const icu_me_config* cfg, const ui
   const Tiling t = plan_tiling(mbw, L, cfg->metric, true, cfg->tile_rows, cfg->mbs_per_task);
   if (t.V == 0) return icu_fail(ICU_E_ARG, "tile_rows too tall for exact SAD argmin keys at this range");
   // one MB per wave where the window takes it (the handle path's default)
-  const int L0 = cfg->metric == ICU_METRIC_SAD && (cfg->grid_mode == 0 || cfg->grid_mode == 4) ? wave_class(L) : 0;
+  const int L0 = cfg->metric == ICU_METRIC_SAD && (cfg->grid_mode == 0 || cfg->grid_mode >= 4) ? wave_class(L) : 0;
   const int per = mbw / t.G, ntasks = L0 ? mbw * mbh : per * mbh + (mbw % t.G ? mbh : 0);
   // scratch layout: descriptors, tasks, padded reference plane
   const int ref_stride = (W + 2 * kPad + 255) & ~255;

@@ -48,7 +48,9 @@ typedef struct icu_me_config {
   int grid_mode;     /* 0: the planner's choice (one MB per wave where the window takes it, else one
                         task per workgroup) until tuned; else 1 + the mode icu_me_get_tiling
                         reports (1 one task per workgroup, 2 persistent, 3 persistent balanced,
-                        4 one MB per wave) */
+                        4 one MB per wave, a launch per window class, 5 one MB per wave in one
+                        persistent launch over per-XCD queues, 6 as 4 with the +-8 / +-16
+                        classes on a second stream, beside the others); a batch takes stream 0's */
 } icu_me_config;
 
 typedef struct icu_me icu_me;
@@ -201,12 +203,29 @@ int icu_me_batch_mb_base(icu_me_batch* b, int stream);
 int icu_me_batch_estimate_device(icu_me_batch* b, const uint8_t* const* d_curs,
                                  const uint8_t* const* d_refs, int16_t* d_mv, int32_t* d_dist,
                                  void* stream);
-/* Capture the whole batch (padding, one persistent search launch over the
- * MB work queue, decisions) into a HIP graph bound to these buffers, then
- * replay it with icu_me_batch_graph_launch. */
+/* Capture the whole batch (padding, the search launches with their in-kernel
+ * decisions) into a HIP graph bound to these buffers, then replay it with
+ * icu_me_batch_graph_launch.  With the queue plan (grid mode 4, the default
+ * for a batch whose streams the one-MB-per-wave kernels take) the search is
+ * one persistent launch over per-XCD macroblock queues. */
 int icu_me_batch_capture(icu_me_batch* b, const uint8_t* const* d_curs, const uint8_t* const* d_refs,
                          int16_t* d_mv, int32_t* d_dist);
 int icu_me_batch_graph_launch(icu_me_batch* b, void* stream);
+/* Times the batch's launch plans on these buffers (each captured in a graph,
+ * the least of three runs of `reps` replays) and keeps the fastest: grid
+ * modes 0-2 (workgroup tiles for every stream), 3 (one launch per window
+ * class), 5 (the same with the +-8 / +-16 classes on a second stream, beside
+ * the others) and 4 (the queue kernel at 6 or 8 waves per SIMD).  *best_ms
+ * (nullable): its time per replay.  It leaves no graph: capture again
+ * afterwards. */
+int icu_me_batch_autotune(icu_me_batch* b, const uint8_t* const* d_curs, const uint8_t* const* d_refs,
+                          int16_t* d_mv, int32_t* d_dist, int reps, float* best_ms);
+/* The batch's plan: grid mode (as icu_me_get_tiling reports it) and the queue
+ * kernel's waves per SIMD.  set_plan pins one (a plan tuned earlier, say):
+ * grid_mode 0..5 (3..5 need a stream the wave kernels take), queue_waves
+ * 1..8; a graph captured before must be captured again. */
+int icu_me_batch_get_plan(icu_me_batch* b, int* grid_mode, int* queue_waves);
+int icu_me_batch_set_plan(icu_me_batch* b, int grid_mode, int queue_waves);
 
 /* ------------------------------------------------- YUV motion compensation
  * Whole-frame YUV 4:2:0 compensation from a 16x16 quarter-pel vector list:

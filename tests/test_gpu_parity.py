@@ -256,11 +256,13 @@ def test_split_argmin_keys_tie_break_at_large_ranges(R):
 
 
 # One MB per wave (grid mode 3, config grid_mode 4): the default for SAD
-# argmin streams at +-7/8, +-15/16 and +-31/32.  Pictures small enough that
-# every window clamps, the reference and exhaustive windows, all sub-pel
-# depths and the final vector choice; the workgroup kernel (grid mode 0)
-# must agree too.
-@pytest.mark.parametrize("R", [8, 16, 32])
+# argmin streams at +-7/8, +-15/16, +-31/32 and +-63/64 (the last as four
+# quadrant passes).  Pictures small enough that every window clamps (some
+# quadrants of the 64 class lie wholly outside and are skipped), the
+# reference and exhaustive windows, all sub-pel depths and the final vector
+# choice; the queue kernel (config grid_mode 5) and the workgroup kernel
+# (grid mode 0) must agree too.
+@pytest.mark.parametrize("R", [8, 16, 32, 64])
 @pytest.mark.parametrize("size,seed", [((64, 48), 1), ((176, 144), 2), ((352, 288), 3), ((96, 160), 4)])
 @pytest.mark.parametrize("window,subpel,bestvec", [(1, 1, 0), (0, 2, 1), (1, 0, 0), (0, 1, 1)])
 def test_wave_kernel_matches_oracle(R, size, seed, window, subpel, bestvec):
@@ -268,10 +270,10 @@ def test_wave_kernel_matches_oracle(R, size, seed, window, subpel, bestvec):
     fp = synth.rect_motion_pair(W, H, 60 + seed + R, n_rects=8, max_mv=min(W // 3, H // 3, R))
     modes = dict(metric=1, window=window, filt=0, subpel=subpel, predictor=0, best_vector=bestvec)
     omv, odist, ointeg, _ = orc.me_estimate(fp.cur, fp.ref, R, **_oracle_modes(modes))
-    for gm in (4, 1, 0):
+    for gm in (4, 5, 1, 0):
         est = capi.MotionEstimator(_cfg(W, H, R, grid_mode=gm, **modes))
         if gm != 1:
-            assert est.tiling()[2] == 3
+            assert est.tiling()[2] == {4: 3, 5: 4, 0: 3}[gm]
         d_cur, d_ref = capi.DeviceBuffer.from_array(fp.cur), capi.DeviceBuffer.from_array(fp.ref)
         n = est.nmb
         d_mv, d_dist, d_int = capi.DeviceBuffer(4 * n), capi.DeviceBuffer(4 * n), capi.DeviceBuffer(12 * n)
@@ -282,7 +284,7 @@ def test_wave_kernel_matches_oracle(R, size, seed, window, subpel, bestvec):
         np.testing.assert_array_equal(d_dist.download(np.int32, n), odist, err_msg=f"grid {gm}")
 
 
-@pytest.mark.parametrize("R", [8, 16, 32])
+@pytest.mark.parametrize("R", [8, 16, 32, 64])
 @pytest.mark.parametrize("window", [0, 1])
 def test_wave_kernel_tie_break_on_flat_and_periodic_content(window, R):
     """Many equal costs across the wave's tiles, its leftover column and row
@@ -458,3 +460,54 @@ def test_batch_alone_equals_in_mix_and_graph_replays():
         np.testing.assert_array_equal(g_dist.download(np.int32, T), dist_b)
         batch.estimate_device(curs, refs, d_mv, d_dist, stream=s)
         s.sync()
+
+
+def test_batch_plans_agree_and_tuner_keeps_one():
+    """Every launch plan of a batch (workgroup tiles, one launch per window
+    class on one stream or two, the persistent queue kernel) gives the same vectors and
+    distortions, over streams of all four window classes under both window
+    rules; the queue kernel's counters reset between back-to-back launches;
+    the tuner keeps a valid plan and the graph captured after it agrees."""
+    streams = _mixed_streams(3)
+    modes = dict(metric=1, filt=0, subpel=1, predictor=0, best_vector=1)
+    cfgs = [_cfg(w, h, R, window=i % 2, **modes) for i, (w, h, R, _) in enumerate(streams)]
+    curs = [capi.DeviceBuffer.from_array(fp.cur) for (*_, fp) in streams]
+    refs = [capi.DeviceBuffer.from_array(fp.ref) for (*_, fp) in streams]
+    s = capi.Stream()
+    want = None
+    for gm in (0, 4, 5, 6, 3, 2):
+        for c in cfgs:
+            c.grid_mode = gm
+        batch = capi.MotionEstimatorBatch(cfgs)
+        assert batch.plan()[0] == {0: 5, 4: 3, 5: 4, 6: 5, 3: 2, 2: 1}[gm]
+        T = batch.total_mbs
+        d_mv, d_dist = capi.DeviceBuffer(4 * T), capi.DeviceBuffer(4 * T)
+        for rep in range(2):
+            d_mv.zero()
+            batch.estimate_device(curs, refs, d_mv, d_dist, stream=s)
+            s.sync()
+            got = (d_mv.download(np.int16, (T, 2)), d_dist.download(np.int32, T))
+            if want is None:
+                want = got
+                for i, (w, h, R, fp) in enumerate(streams):
+                    if w > 352:
+                        continue
+                    b0, n = batch.bases[i], (w // 16) * (h // 16)
+                    omv, odist, _, _ = orc.me_estimate(fp.cur, fp.ref, R, **_oracle_modes(dict(modes, window=i % 2)))
+                    np.testing.assert_array_equal(want[0][b0:b0 + n], omv, err_msg=f"stream {i}")
+                    np.testing.assert_array_equal(want[1][b0:b0 + n], odist, err_msg=f"stream {i}")
+            np.testing.assert_array_equal(got[0], want[0], err_msg=f"grid {gm} run {rep}")
+            np.testing.assert_array_equal(got[1], want[1], err_msg=f"grid {gm} run {rep}")
+        if gm == 0:
+            ms = batch.autotune(curs, refs, d_mv, d_dist, reps=2)
+            g, w = batch.plan()
+            assert ms > 0 and 0 <= g <= 5 and w in (6, 8)
+            g_mv, g_dist = capi.DeviceBuffer(4 * T), capi.DeviceBuffer(4 * T)
+            batch.capture(curs, refs, g_mv, g_dist)
+            for rep in range(2):
+                g_mv.zero()
+                batch.graph_launch(s)
+                s.sync()
+                np.testing.assert_array_equal(g_mv.download(np.int16, (T, 2)), want[0], err_msg=f"tuned replay {rep}")
+                np.testing.assert_array_equal(g_dist.download(np.int32, T), want[1])
+        batch.close()
