@@ -671,7 +671,8 @@ def main():
                                    "untuned_grid_mode": bt["default_plan"][0],
                                    "modes": "0-2 workgroup tiles, 3 a launch per window class, "
                                             "4 one persistent launch over per-XCD MB queues, 5 as 3 with "
-                                            "the +-8/+-16 classes on a second stream"}}
+                                            "the +-8/+-16 classes on a second stream, 6 the +-32/+-64 "
+                                            "classes as in 4 and the others as in 5"}}
         note("batched leg done")
     if args.core_only:
         print(json.dumps(out), flush=True)

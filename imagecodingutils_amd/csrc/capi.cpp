@@ -121,8 +121,8 @@ bool valid_cfg(const icu_me_config& c, std::string* why) {
     *why = "mbs_per_task must be 0..16";
     return false;
   }
-  if (c.grid_mode < 0 || c.grid_mode > 6) {
-    *why = "grid_mode must be 0..6";
+  if (c.grid_mode < 0 || c.grid_mode > 7) {
+    *why = "grid_mode must be 0..7";
     return false;
   }
   if (2 * c.range + 1 > 129) {
@@ -281,7 +281,8 @@ struct Engine {
       }
     }
     h_woff.clear();
-    if (grid_mode == 4) build_queue();
+    if (grid_mode == 4) build_queue(4);
+    if (grid_mode == 6) build_queue(2);
     for (size_t si = 0; si < sp.size(); si++) {
       const StreamPlan& s = sp[si];
       if (wave_stream(s)) continue;
@@ -308,12 +309,16 @@ struct Engine {
   // an XCD's MBs, costliest first, each to the XCD's wave with the least
   // estimated work so far (workgroup b runs on XCD b mod 8); then the tasks
   // wave by wave.
-  void build_queue() {
+  // Classes [0, kq) of h_wtasks (grouped by class) become the queue; the
+  // rest stay grouped by class after it.
+  void build_queue(int kq) {
     const int nw = queue_waves * 4 * cus;
     std::vector<std::vector<SearchTask>> lists(nw);
+    int nq_tasks = 0;
+    for (int k = 0; k < kq; k++) nq_tasks += wave_n[k];
     for (int x = 0; x < 8; x++) {
       std::vector<std::pair<long, SearchTask>> share;
-      for (int k = 0, off = 0; k < 4; off += wave_n[k], k++) {
+      for (int k = 0, off = 0; k < kq; off += wave_n[k], k++) {
         const int nq = wave_n[k] / 8, rq = wave_n[k] % 8;
         const int a = off + x * nq + std::min(x, rq), len = nq + (x < rq ? 1 : 0);
         for (int i = a; i < a + len; i++) {
@@ -347,6 +352,7 @@ struct Engine {
       out.insert(out.end(), lists[b].begin(), lists[b].end());
     }
     h_woff[nw] = (int)out.size();
+    out.insert(out.end(), h_wtasks.begin() + nq_tasks, h_wtasks.end());
     h_wtasks.swap(out);
   }
 
@@ -585,7 +591,23 @@ struct Engine {
         launch_search(d_streams, d_tasks, (int)h_tasks.size(), grid_mode >= 3 ? 0 : grid_mode, sp[0].cfg.metric,
                       std::max(vfix, 0), any_table, stage_dwords, d_ires, d_table, d_decide, sp.size() == 1, st);
       if (grid_mode == 4) {
-        launch_wave_queue(d_streams, d_tasks + h_tasks.size(), d_woff, queue_waves * 4 * cus, d_decide, st);
+        launch_wave_queue(d_streams, d_tasks + h_tasks.size(), d_woff, queue_waves * 4 * cus, false, d_decide, st);
+      } else if (grid_mode == 6) {
+        // classes 64 and 32 in one persistent launch on st, 16 and 8 one
+        // launch each on the side stream
+        const int nq = wave_n[0] + wave_n[1];
+        const bool split = nq > 0 && (wave_n[2] + wave_n[3]) > 0;
+        if (split) {
+          HIP_TRY(hipEventRecord(ds[set].fork, st));
+          HIP_TRY(hipStreamWaitEvent(side, ds[set].fork, 0));
+        }
+        if (nq) launch_wave_queue(d_streams, d_tasks + h_tasks.size(), d_woff, queue_waves * 4 * cus, true, d_decide, st);
+        for (int k = 2, off = (int)h_tasks.size() + nq; k < 4; off += wave_n[k], k++)
+          launch_wave_search(d_streams, d_tasks + off, wave_n[k], kWaveClasses[k], d_decide, true, split ? side : st);
+        if (split) {
+          HIP_TRY(hipEventRecord(ds[set].join, side));
+          HIP_TRY(hipStreamWaitEvent(st, ds[set].join, 0));
+        }
       } else {
         // mode 5: classes 64 and 32 on st, 16 and 8 on the side stream,
         // which starts after the padding and which st waits for at the end
@@ -1161,7 +1183,10 @@ int icu_me_batch_autotune(icu_me_batch* b, const uint8_t* const* curs, const uin
   if (e.any_wave_stream()) {
     plans.push_back(Plan{3, e.queue_waves});
     plans.push_back(Plan{5, e.queue_waves});
-    for (int w : {6, 8}) plans.push_back(Plan{4, w});
+    for (int w : {6, 8}) {
+      plans.push_back(Plan{4, w});
+      plans.push_back(Plan{6, w});
+    }
   }
   hipEvent_t ev[2] = {nullptr, nullptr};
   HIP_TRY(hipEventCreate(&ev[0]));
@@ -1213,9 +1238,9 @@ int icu_me_batch_autotune(icu_me_batch* b, const uint8_t* const* curs, const uin
 
 int icu_me_batch_set_plan(icu_me_batch* b, int grid_mode, int queue_waves) {
   if (!b) return fail(ICU_E_ARG, "null batch");
-  if (grid_mode < 0 || grid_mode > 5) return fail(ICU_E_ARG, "grid_mode must be 0..5");
+  if (grid_mode < 0 || grid_mode > 6) return fail(ICU_E_ARG, "grid_mode must be 0..6");
   if (grid_mode >= 3 && !b->eng.any_wave_stream())
-    return fail(ICU_E_ARG, "grid modes 3..5 need a stream the one-MB-per-wave kernels take");
+    return fail(ICU_E_ARG, "grid modes 3..6 need a stream the one-MB-per-wave kernels take");
   if (queue_waves < 1 || queue_waves > Engine::kMaxQueueWaves) return fail(ICU_E_ARG, "queue_waves must be 1..8");
   b->eng.grid_mode = grid_mode;
   b->eng.queue_waves = queue_waves;

@@ -86,9 +86,10 @@ void launch_wave_search(const StreamDesc* d_streams, const SearchTask* d_tasks, 
                         const DecideDesc* d_decide, bool band, hipStream_t st);
 // The same MBs, any mix of classes, in one persistent launch of `waves`
 // one-wave workgroups: wave b runs tasks [d_woff[b], d_woff[b + 1]) in order
-// (d_woff: waves + 1 offsets).
+// (d_woff: waves + 1 offsets).  big_only: every task is of class 32 or 64
+// (the kernel then carries only that search path).
 void launch_wave_queue(const StreamDesc* d_streams, const SearchTask* d_tasks, const int* d_woff, int waves,
-                       const DecideDesc* d_decide, hipStream_t st);
+                       bool big_only, const DecideDesc* d_decide, hipStream_t st);
 // Estimated work of the MB at pel (n, m) of a W x H picture searched by the
 // wave kernels with window half-width L, in candidates (the queue's LPT key).
 long wave_task_cost(int n, int m, int W, int H, int L);

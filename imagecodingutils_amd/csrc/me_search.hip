@@ -857,6 +857,9 @@ void me_wave_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __
 // atomics: a queue taken from a shared counter measured twice the time of
 // the one-launch-per-class plan on a single 1080p stream, its counter being
 // contended by every wave of an XCD.
+// BIG: the +-31/32 and +-63/64 classes only, so the kernel carries one
+// search path (the 32-class pass) beside the decision.
+template <bool BIG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
 void me_queue_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __restrict__ tasks,
                      const int* __restrict__ woff, const DecideDesc* __restrict__ decide) {
@@ -872,7 +875,7 @@ void me_queue_kernel(const StreamDesc* __restrict__ streams, const SearchTask* _
     __builtin_assume(ln >= 0 && ln < 64);
     const SearchTask t = tasks[ti];
     const StreamDesc s = streams[t.stream];
-    const int L0 = s.L <= 8 ? 8 : (s.L <= 16 ? 16 : (s.L <= 32 ? 32 : 64));
+    const int L0 = BIG ? (s.L <= 32 ? 32 : 64) : (s.L <= 8 ? 8 : (s.L <= 16 ? 16 : (s.L <= 32 ? 32 : 64)));
     const uint64_t kbest = wave_search_mb(s, wave_mb(s, t.mb), (lds_u32*)win_raw, ln, L0);
     wave_decide_mb(decide[t.stream], win_raw, t.mb, ln, kbest);
     dec::wave_sync();  // the decision's LDS reads are done before the next staging
@@ -892,9 +895,12 @@ void launch_wave_search(const StreamDesc* d_streams, const SearchTask* d_tasks, 
 }
 
 void launch_wave_queue(const StreamDesc* d_streams, const SearchTask* d_tasks, const int* d_woff, int waves,
-                       const DecideDesc* d_decide, hipStream_t st) {
+                       bool big_only, const DecideDesc* d_decide, hipStream_t st) {
   if (waves <= 0) return;
-  hipLaunchKernelGGL(me_queue_kernel, dim3(waves), dim3(64), 0, st, d_streams, d_tasks, d_woff, d_decide);
+  if (big_only)
+    hipLaunchKernelGGL(me_queue_kernel<true>, dim3(waves), dim3(64), 0, st, d_streams, d_tasks, d_woff, d_decide);
+  else
+    hipLaunchKernelGGL(me_queue_kernel<false>, dim3(waves), dim3(64), 0, st, d_streams, d_tasks, d_woff, d_decide);
 }
 
 long wave_task_cost(int n, int m, int W, int H, int L) {

@@ -50,7 +50,8 @@ typedef struct icu_me_config {
                         reports (1 one task per workgroup, 2 persistent, 3 persistent balanced,
                         4 one MB per wave, a launch per window class, 5 one MB per wave in one
                         persistent launch over per-XCD queues, 6 as 4 with the +-8 / +-16
-                        classes on a second stream, beside the others); a batch takes stream 0's */
+                        classes on a second stream, beside the others, 7 the +-31/32 and +-63/64
+                        classes as in 5 and the others as in 6); a batch takes stream 0's */
 } icu_me_config;
 
 typedef struct icu_me icu_me;
@@ -215,14 +216,15 @@ int icu_me_batch_graph_launch(icu_me_batch* b, void* stream);
  * the least of three runs of `reps` replays) and keeps the fastest: grid
  * modes 0-2 (workgroup tiles for every stream), 3 (one launch per window
  * class), 5 (the same with the +-8 / +-16 classes on a second stream, beside
- * the others) and 4 (the queue kernel at 6 or 8 waves per SIMD).  *best_ms
+ * the others), 4 (the queue kernel at 6 or 8 waves per SIMD) and 6 (the
+ * +-31/32 and +-63/64 classes in the queue kernel, the others as in 5).  *best_ms
  * (nullable): its time per replay.  It leaves no graph: capture again
  * afterwards. */
 int icu_me_batch_autotune(icu_me_batch* b, const uint8_t* const* d_curs, const uint8_t* const* d_refs,
                           int16_t* d_mv, int32_t* d_dist, int reps, float* best_ms);
 /* The batch's plan: grid mode (as icu_me_get_tiling reports it) and the queue
  * kernel's waves per SIMD.  set_plan pins one (a plan tuned earlier, say):
- * grid_mode 0..5 (3..5 need a stream the wave kernels take), queue_waves
+ * grid_mode 0..6 (3..6 need a stream the wave kernels take), queue_waves
  * 1..8; a graph captured before must be captured again. */
 int icu_me_batch_get_plan(icu_me_batch* b, int* grid_mode, int* queue_waves);
 int icu_me_batch_set_plan(icu_me_batch* b, int grid_mode, int queue_waves);

@@ -26,7 +26,7 @@ def test_binding_covers_header():
 
 @pytest.mark.parametrize("bad", [
     dict(width=30), dict(height=0), dict(R=0), dict(R=65), dict(metric=2), dict(subpel=3),
-    dict(predictor=2), dict(tile_rows=5), dict(mbs_per_task=17), dict(grid_mode=7), dict(grid_mode=-1)])
+    dict(predictor=2), dict(tile_rows=5), dict(mbs_per_task=17), dict(grid_mode=8), dict(grid_mode=-1)])
 def test_create_rejects_bad_config(bad):
     kw = dict(width=64, height=48, R=8)
     kw.update(bad)
