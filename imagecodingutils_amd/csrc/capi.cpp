@@ -170,9 +170,11 @@ struct Engine {
   std::vector<DecideItem> h_items;
   // Descriptor slots.  kEager slots serve eager calls: a call uses the slot
   // already bound to its buffers, or refills the least recently used one
-  // after waiting for that slot's last launch (an event, not a device-wide
-  // synchronisation) and uploads it on the call's stream from a pinned
-  // mirror.  Slot kGraph belongs to a captured graph, so eager calls never
+  // after a device synchronisation (its launches, on whatever streams, and
+  // its last upload are then done) and uploads it on the call's stream from
+  // a pinned mirror.  A call on a bound slot records nothing: an event
+  // recorded after every estimate held the next launch on the stream back by
+  // 4-5 us.  Slot kGraph belongs to a captured graph, so eager calls never
   // redirect a graph's replays.
   static constexpr int kEager = 3, kGraph = kEager;
   struct DescSet {
@@ -186,9 +188,8 @@ struct Engine {
     int32_t* dist = nullptr;
     int32_t* integ = nullptr;
     bool valid = false;
-    bool used = false;        // `done` records this slot's last launch
+    bool used = false;        // launched or uploaded since the last device synchronisation
     hipEvent_t fork = nullptr, join = nullptr;  // grid mode 5: the side stream's start and end
-    hipEvent_t done = nullptr;
     unsigned long long stamp = 0;
   } ds[kEager + 1];
   unsigned long long use_clock = 0;
@@ -230,11 +231,12 @@ struct Engine {
     for (auto& s : sp)
       if (s.refpad_base) hipFree(s.refpad_base);
     sp.clear();
+    bool any_used = false;
+    for (auto& d : ds) any_used |= d.used;
+    if (any_used) hipDeviceSynchronize();
     for (auto& d : ds) {
-      if (d.used) hipEventSynchronize(d.done);
       hipFree(d.d_block);
       hipHostFree(d.h_block);
-      if (d.done) hipEventDestroy(d.done);
       if (d.fork) hipEventDestroy(d.fork);
       if (d.join) hipEventDestroy(d.join);
       d = DescSet{};
@@ -411,9 +413,10 @@ struct Engine {
       HIP_TRY(hipMalloc(&d.d_block, desc_bytes));
       dev_bytes += desc_bytes;
       HIP_TRY(hipHostMalloc(&d.h_block, desc_bytes, hipHostMallocDefault));
-      HIP_TRY(hipEventCreateWithFlags(&d.done, hipEventDisableTiming));
-      HIP_TRY(hipEventCreateWithFlags(&d.fork, hipEventDisableTiming));
-      HIP_TRY(hipEventCreateWithFlags(&d.join, hipEventDisableTiming));
+      // fork / join order two device streams: a device-scope release is
+      // enough (a system-scope one writes the caches back)
+      HIP_TRY(hipEventCreateWithFlags(&d.fork, hipEventDisableTiming | hipEventReleaseToDevice));
+      HIP_TRY(hipEventCreateWithFlags(&d.join, hipEventDisableTiming | hipEventReleaseToDevice));
       d.d_streams = (StreamDesc*)d.d_block;
       d.d_decide = (DecideDesc*)((char*)d.d_block + o_dec);
       d.d_pad = (PadDesc*)((char*)d.d_block + o_pad);
@@ -471,12 +474,14 @@ struct Engine {
       if (ds[k].stamp < ds[pick_slot].stamp) pick_slot = k;
     }
     DescSet& D = ds[pick_slot];
-    if (D.used) HIP_TRY(hipEventSynchronize(D.done));  // its launches and its last upload are done
+    if (D.used) {
+      HIP_TRY(hipDeviceSynchronize());  // its launches and its last upload are done
+      for (auto& x : ds) x.used = false;
+    }
     int rc = fill(D, curs, refs, mv, dist, integ);
     if (rc) return rc;
+    // the mirror stays in use until the copy ran
     HIP_TRY(hipMemcpyAsync(D.d_block, D.h_block, desc_bytes, hipMemcpyHostToDevice, st));
-    // the mirror stays in use until the copy ran (the launches re-record it)
-    HIP_TRY(hipEventRecord(D.done, st));
     D.used = true;
     D.stamp = ++use_clock;
     *slot = pick_slot;
@@ -639,10 +644,7 @@ struct Engine {
     }
     if (ev) HIP_TRY(hipEventRecord(ev[3], st));
     HIP_TRY(hipGetLastError());
-    if (set != kGraph) {
-      HIP_TRY(hipEventRecord(ds[set].done, st));
-      ds[set].used = true;
-    }
+    if (set != kGraph) ds[set].used = true;
     return ICU_OK;
   }
 
@@ -1333,7 +1335,9 @@ int icu_stream_sync(void* s) {
 
 void* icu_event_create(void) {
   hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) {
+  // timing events: a device-scope release when recorded, so recording one
+  // does not force the caches' write-back to host scope between launches
+  if (hipEventCreateWithFlags(&e, hipEventReleaseToDevice) != hipSuccess) {
     g_err = "hipEventCreate failed";
     return nullptr;
   }

@@ -520,6 +520,8 @@ int icu_memset(void* d, int value, size_t bytes, void* stream);
 void* icu_stream_create(void);
 int icu_stream_destroy(void* stream);
 int icu_stream_sync(void* stream);
+/* Timing events (device-scope release: after icu_event_record the host
+ * reads device results through icu_memcpy_dtoh, not through the event). */
 void* icu_event_create(void);
 int icu_event_destroy(void* ev);
 int icu_event_record(void* ev, void* stream);
