@@ -817,7 +817,7 @@ int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref,
   int rc = e.bind_eager(curs, refs, d_mv, d_dist, nullptr, st, &slot);
   if (rc) return rc;
   hipEvent_t ev[4];
-  for (auto& x : ev) HIP_TRY(hipEventCreate(&x));
+  for (auto& x : ev) HIP_TRY(hipEventCreateWithFlags(&x, hipEventReleaseToDevice));
   rc = e.enqueue(slot, st, ev, search_reps);
   if (!rc) {
     HIP_TRY(hipEventSynchronize(ev[3]));
