@@ -84,16 +84,26 @@ def run_plain(capi, synth, steps, warmup, tune=False, settle_ms=100.0):
     for _ in range(warmup):
         step()
     s.sync()
-    ev = [capi.Event() for _ in range(steps + 1)]
+    # the timed steps: one event pair around all of them, nothing between
+    a, b = capi.Event(), capi.Event()
     t0 = time.perf_counter()
+    a.record(s)
+    for k in range(steps):
+        step()
+    b.record(s)
+    s.sync()
+    wall = time.perf_counter() - t0
+    total_ms = a.elapsed_ms(b)
+    # per-step latencies (p50 / p99) in a second pass, an event after every
+    # step (recording one holds the next step back by ~2.5 us, so this pass
+    # does not give the throughput)
+    ev = [capi.Event() for _ in range(steps + 1)]
     ev[0].record(s)
     for k in range(steps):
         step()
         ev[k + 1].record(s)
     s.sync()
-    wall = time.perf_counter() - t0
     per = [ev[k].elapsed_ms(ev[k + 1]) for k in range(steps)]
-    total_ms = ev[0].elapsed_ms(ev[steps])
     # per-stage device times of the estimate on the same stream; the search
     # stage is the mean of 20 back-to-back launches between one event pair
     stages = np.array([est.estimate_timed(d_cur, d_ref, d_mv, d_dist, stream=s, search_reps=20)
