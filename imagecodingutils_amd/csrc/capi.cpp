@@ -166,6 +166,7 @@ struct Engine {
   std::vector<int> h_woff;
   int* d_woff = nullptr;             // kMaxQueueWaves x 4 x cus + 1 ints
   bool wave_on = false;              // the wave kernels run the streams they take
+  bool lean = false;                 // every wave stream is of the lean modes (wave_lean)
   int queue_waves = 6;               // grid of the queue kernel: waves per SIMD x 4 x CUs
   std::vector<DecideItem> h_items;
   // Descriptor slots.  kEager slots serve eager calls: a call uses the slot
@@ -273,6 +274,9 @@ struct Engine {
     h_tasks.clear();
     h_wtasks.clear();
     wave_on = grid_mode >= 3;
+    lean = true;
+    for (const auto& s : sp)
+      if (wave_stream(s)) lean &= wave_lean(s.cfg.subpel, s.cfg.best_vector);
     for (int k = 0; k < 4; k++) {
       wave_n[k] = 0;
       for (size_t si = 0; si < sp.size(); si++) {
@@ -596,7 +600,8 @@ struct Engine {
         launch_search(d_streams, d_tasks, (int)h_tasks.size(), grid_mode >= 3 ? 0 : grid_mode, sp[0].cfg.metric,
                       std::max(vfix, 0), any_table, stage_dwords, d_ires, d_table, d_decide, sp.size() == 1, st);
       if (grid_mode == 4) {
-        launch_wave_queue(d_streams, d_tasks + h_tasks.size(), d_woff, queue_waves * 4 * cus, false, d_decide, st);
+        launch_wave_queue(d_streams, d_tasks + h_tasks.size(), d_woff, queue_waves * 4 * cus, false, lean, d_decide,
+                          st);
       } else if (grid_mode == 6) {
         // classes 64 and 32 in one persistent launch on st, 16 and 8 one
         // launch each on the side stream
@@ -606,9 +611,12 @@ struct Engine {
           HIP_TRY(hipEventRecord(ds[set].fork, st));
           HIP_TRY(hipStreamWaitEvent(side, ds[set].fork, 0));
         }
-        if (nq) launch_wave_queue(d_streams, d_tasks + h_tasks.size(), d_woff, queue_waves * 4 * cus, true, d_decide, st);
+        if (nq)
+          launch_wave_queue(d_streams, d_tasks + h_tasks.size(), d_woff, queue_waves * 4 * cus, true, lean, d_decide,
+                            st);
         for (int k = 2, off = (int)h_tasks.size() + nq; k < 4; off += wave_n[k], k++)
-          launch_wave_search(d_streams, d_tasks + off, wave_n[k], kWaveClasses[k], d_decide, true, split ? side : st);
+          launch_wave_search(d_streams, d_tasks + off, wave_n[k], kWaveClasses[k], d_decide, true, lean,
+                             split ? side : st);
         if (split) {
           HIP_TRY(hipEventRecord(ds[set].join, side));
           HIP_TRY(hipStreamWaitEvent(st, ds[set].join, 0));
@@ -622,7 +630,7 @@ struct Engine {
           HIP_TRY(hipStreamWaitEvent(side, ds[set].fork, 0));
         }
         for (int k = 0, off = (int)h_tasks.size(); k < 4; off += wave_n[k], k++)
-          launch_wave_search(d_streams, d_tasks + off, wave_n[k], kWaveClasses[k], d_decide, true,
+          launch_wave_search(d_streams, d_tasks + off, wave_n[k], kWaveClasses[k], d_decide, true, lean,
                              split && k >= 2 ? side : st);
         if (split) {
           HIP_TRY(hipEventRecord(ds[set].join, side));

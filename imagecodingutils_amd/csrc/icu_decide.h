@@ -635,6 +635,44 @@ struct PredFree {
                         // icu_me_tie_count reads it there)
 };
 
+// decide_mb for the lean modes: SAD, no cost table, integer or half-pel
+// refinement, no final vector choice (the zero predictor's GetBestVector is
+// off).  The same outputs as decide_mb there, from far less code: the wave
+// kernels that take only such streams carry this instead, which keeps them
+// inside the instruction cache.
+__device__ __forceinline__ void decide_mb_lean(const DecideDesc& d, int mb, int lane, const IntResult& r) {
+  const int col = mb % d.mbw, row = mb / d.mbw;
+  const int n = col * 16, m = row * 16;
+  const int slot = d.mb_base + mb;
+  int minD = r.cost, mvx = r.mx * 4, mvy = r.my * 4;
+  if (d.subpel == 1) {
+    const uint32_t cw = *(const uint32_t*)(d.cur + (long)(m + (lane >> 2)) * d.cur_stride + n + 4 * (lane & 3));
+    int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    half_pel_direct<kMetricSAD>(d, n + r.mx, m + r.my, cw, lane, acc);
+    wave_sum8_small(acc);
+    int hxh = 0, hyh = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (acc[k] < minD) {
+        minD = acc[k];
+        hxh = kSubX[k];
+        hyh = kSubY[k];
+      }
+    mvx += 2 * hxh;
+    mvy += 2 * hyh;
+  }
+  if (lane == 0) {
+    const uint32_t w = (uint32_t)(uint16_t)(int16_t)mvx | ((uint32_t)(uint16_t)(int16_t)mvy << 16);
+    __hip_atomic_store((uint32_t*)(d.mv_out + 2 * slot), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    d.dist_out[slot] = minD;
+    if (d.int_out) {
+      d.int_out[3 * slot] = r.mx;
+      d.int_out[3 * slot + 1] = r.my;
+      d.int_out[3 * slot + 2] = r.cost;
+    }
+  }
+}
+
 // staged_tab (nullable): the MB's cost table already copied to LDS.
 __device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, int predX, int predY,
                                      int lane, const IntResult* known = nullptr,

@@ -81,15 +81,20 @@ void launch_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int n
 // One MB per wave (me_wave_kernel) for argmin SAD streams whose window
 // half-width L is L0 or L0 - 1, L0 = wave_class(L) in {8, 16, 32, 64} (0: no
 // class); decides each MB in place.  All tasks of one launch share L0.
+// lean: every stream of the launch is of the lean modes (wave_lean), and
+// the kernel carries only their decision.
 int wave_class(int L);
 void launch_wave_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int L0,
-                        const DecideDesc* d_decide, bool band, hipStream_t st);
+                        const DecideDesc* d_decide, bool band, bool lean, hipStream_t st);
+// A wave-class stream whose decision is integer or half pel with no final
+// vector choice (GetBestVector off).
+inline bool wave_lean(int subpel, int best_vector) { return subpel <= 1 && !best_vector; }
 // The same MBs, any mix of classes, in one persistent launch of `waves`
 // one-wave workgroups: wave b runs tasks [d_woff[b], d_woff[b + 1]) in order
 // (d_woff: waves + 1 offsets).  big_only: every task is of class 32 or 64
 // (the kernel then carries only that search path).
 void launch_wave_queue(const StreamDesc* d_streams, const SearchTask* d_tasks, const int* d_woff, int waves,
-                       bool big_only, const DecideDesc* d_decide, hipStream_t st);
+                       bool big_only, bool lean, const DecideDesc* d_decide, hipStream_t st);
 // Estimated work of the MB at pel (n, m) of a W x H picture searched by the
 // wave kernels with window half-width L, in candidates (the queue's LPT key).
 long wave_task_cost(int n, int m, int W, int H, int L);

@@ -806,7 +806,10 @@ __device__ __forceinline__ uint64_t wave_search_mb(const StreamDesc& s, const Wa
 }
 
 // The decision of a searched MB, in the same wave, with the window's LDS
-// reused (at a raised wave priority: it is the MB's serial tail).
+// reused (at a raised wave priority: it is the MB's serial tail).  LEAN: the
+// launch's streams are all of the lean modes (SAD, integer or half pel, no
+// final vector choice).
+template <bool LEAN>
 __device__ __forceinline__ void wave_decide_mb(const DecideDesc& d, uint32_t* lds, int mb, int lane,
                                                uint64_t kbest) {
   int mx, my;
@@ -816,7 +819,8 @@ __device__ __forceinline__ void wave_decide_mb(const DecideDesc& d, uint32_t* ld
   // inlined: with the block in SGPRs the tile and the decision fit 80 VGPRs
   // together (6 waves per SIMD); called out of line the decision alone took 87
   const IntResult r{mx, my, (int)key_cost(kbest), -1};
-  dec::decide_mb(d, *(dec::WaveLds*)lds, mb, 0, 0, lane, &r);
+  if (LEAN) dec::decide_mb_lean(d, mb, lane, r);
+  else dec::decide_mb(d, *(dec::WaveLds*)lds, mb, 0, 0, lane, &r);
 }
 
 // LDS of any window class (the 64 class stages 32-class quadrants).
@@ -825,7 +829,7 @@ constexpr int kWaveLdsBytes =
         ? (WaveGeom<32>::LDS_BYTES > WaveGeom<8>::LDS_BYTES ? WaveGeom<32>::LDS_BYTES : WaveGeom<8>::LDS_BYTES)
         : (WaveGeom<16>::LDS_BYTES > WaveGeom<8>::LDS_BYTES ? WaveGeom<16>::LDS_BYTES : WaveGeom<8>::LDS_BYTES);
 
-template <int L0>
+template <int L0, bool LEAN>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
 void me_wave_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __restrict__ tasks, int ntasks,
                     const DecideDesc* __restrict__ decide, int band) {
@@ -842,7 +846,7 @@ void me_wave_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __
   const SearchTask t = tasks[ti];
   const StreamDesc s = streams[t.stream];
   const uint64_t kbest = wave_search_mb(s, wave_mb(s, t.mb), (lds_u32*)win_raw, lane, L0);
-  wave_decide_mb(decide[t.stream], win_raw, t.mb, lane, kbest);
+  wave_decide_mb<LEAN>(decide[t.stream], win_raw, t.mb, lane, kbest);
 }
 
 // Persistent form over a macroblock work queue: a grid of one-wave
@@ -859,7 +863,7 @@ void me_wave_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __
 // contended by every wave of an XCD.
 // BIG: the +-31/32 and +-63/64 classes only, so the kernel carries one
 // search path (the 32-class pass) beside the decision.
-template <bool BIG>
+template <bool BIG, bool LEAN>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
 void me_queue_kernel(const StreamDesc* __restrict__ streams, const SearchTask* __restrict__ tasks,
                      const int* __restrict__ woff, const DecideDesc* __restrict__ decide) {
@@ -877,30 +881,43 @@ void me_queue_kernel(const StreamDesc* __restrict__ streams, const SearchTask* _
     const StreamDesc s = streams[t.stream];
     const int L0 = BIG ? (s.L <= 32 ? 32 : 64) : (s.L <= 8 ? 8 : (s.L <= 16 ? 16 : (s.L <= 32 ? 32 : 64)));
     const uint64_t kbest = wave_search_mb(s, wave_mb(s, t.mb), (lds_u32*)win_raw, ln, L0);
-    wave_decide_mb(decide[t.stream], win_raw, t.mb, ln, kbest);
+    wave_decide_mb<LEAN>(decide[t.stream], win_raw, t.mb, ln, kbest);
     dec::wave_sync();  // the decision's LDS reads are done before the next staging
   }
 }
 
 void launch_wave_search(const StreamDesc* d_streams, const SearchTask* d_tasks, int ntasks, int L0,
-                        const DecideDesc* d_decide, bool band, hipStream_t st) {
+                        const DecideDesc* d_decide, bool band, bool lean, hipStream_t st) {
   if (ntasks <= 0) return;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(ntasks), dim3(64), 0, st, d_streams, d_tasks, ntasks, d_decide, band ? 1 : 0);
   };
-  if (L0 == 64) go(me_wave_kernel<64>);
-  else if (L0 == 32) go(me_wave_kernel<32>);
-  else if (L0 == 16) go(me_wave_kernel<16>);
-  else go(me_wave_kernel<8>);
+  if (lean) {
+    if (L0 == 64) go(me_wave_kernel<64, true>);
+    else if (L0 == 32) go(me_wave_kernel<32, true>);
+    else if (L0 == 16) go(me_wave_kernel<16, true>);
+    else go(me_wave_kernel<8, true>);
+  } else {
+    if (L0 == 64) go(me_wave_kernel<64, false>);
+    else if (L0 == 32) go(me_wave_kernel<32, false>);
+    else if (L0 == 16) go(me_wave_kernel<16, false>);
+    else go(me_wave_kernel<8, false>);
+  }
 }
 
 void launch_wave_queue(const StreamDesc* d_streams, const SearchTask* d_tasks, const int* d_woff, int waves,
-                       bool big_only, const DecideDesc* d_decide, hipStream_t st) {
+                       bool big_only, bool lean, const DecideDesc* d_decide, hipStream_t st) {
   if (waves <= 0) return;
-  if (big_only)
-    hipLaunchKernelGGL(me_queue_kernel<true>, dim3(waves), dim3(64), 0, st, d_streams, d_tasks, d_woff, d_decide);
-  else
-    hipLaunchKernelGGL(me_queue_kernel<false>, dim3(waves), dim3(64), 0, st, d_streams, d_tasks, d_woff, d_decide);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(waves), dim3(64), 0, st, d_streams, d_tasks, d_woff, d_decide);
+  };
+  if (big_only) {
+    if (lean) go(me_queue_kernel<true, true>);
+    else go(me_queue_kernel<true, false>);
+  } else {
+    if (lean) go(me_queue_kernel<false, true>);
+    else go(me_queue_kernel<false, false>);
+  }
 }
 
 long wave_task_cost(int n, int m, int W, int H, int L) {

@@ -175,7 +175,7 @@ extern "C" int icu_me_estimate_device_noalloc(const icu_me_config* cfg, const ui
                      p_task, ntasks, mbw, L0 ? 0 : t.G);
   launch_pad_planes(p_pd, 1, pad_plane_workgroups(W, H), st);
   if (L0) {
-    launch_wave_search(p_sd, p_task, ntasks, L0, p_dd, true, st);
+    launch_wave_search(p_sd, p_task, ntasks, L0, p_dd, true, wave_lean(cfg->subpel, cfg->best_vector), st);
   } else {
     const int stage = (search_stage_dwords(t.B, t.V, t.Q, t.G) + 63) & ~63;
     launch_search(p_sd, p_task, ntasks, cfg->grid_mode ? cfg->grid_mode - 1 : 0, cfg->metric, t.V, false, stage,
