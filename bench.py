@@ -28,13 +28,13 @@ W, H, R = 1920, 1088, 32
 QPS = (22, 28, 34)
 
 # HBM traffic of one search launch at this workload, from two rocprofv3
-# --pmc runs of `bench.py --no-cpu --no-batched --no-4k --serve-seconds 0
-# --steps 20 --warmup 5 --copy-probe 256` (KiB per dispatch of
-# me_wave_kernel<32>), each counter scaled by what it reads
-# for the 256 MiB device copy / fill of --copy-probe: FETCH_SIZE reads half
-# the copy's 262144 KiB (x2.0), WRITE_SIZE reads it exactly (x1.0).  Measured
-# counters, not computed here: refresh them when the kernel changes.
-SEARCH_FETCH_KIB, FETCH_SCALE = 2940.3, 2.0
+# --pmc runs of `bench.py --core-only --no-cpu --no-batched --steps 20
+# --warmup 5 --copy-probe 256` (KiB per dispatch of me_wave_kernel<32, true>),
+# each counter scaled by what it reads for the 256 MiB device copy / fill of
+# --copy-probe: FETCH_SIZE reads half the copy's 262144 KiB (x2.0),
+# WRITE_SIZE reads it exactly (x1.0).  Measured counters, not computed here:
+# refresh them when the kernel changes.
+SEARCH_FETCH_KIB, FETCH_SCALE = 3925.3, 2.0
 SEARCH_WRITE_KIB, WRITE_SCALE = 287.0, 1.0
 MODES = dict(metric=1, window=1, filt=0, subpel=1, predictor=0, best_vector=0)
 
@@ -664,7 +664,8 @@ def main():
                      "traffic": round((SEARCH_FETCH_KIB * FETCH_SCALE + SEARCH_WRITE_KIB * WRITE_SCALE) * 1024),
                      # padded reference + current plane read once, 8 B of results per MB
                      "traffic_algorithmic": (W + 160) * (H + 160) + W * H + W * H // 256 * 8,
-                     "kernel": "me_wave_kernel<32> (one MB per wave: integer search + in-kernel decision)",
+                     "kernel": "me_wave_kernel<32, lean> (one MB per wave: integer search + in-kernel half-pel "
+                               "decision)",
                      "sad_ops_per_launch": ops, "search_ms": round(search_ms, 5)},
         "device": info,
     }
