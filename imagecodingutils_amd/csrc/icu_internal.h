@@ -126,6 +126,7 @@ struct VbsParams {
   int W, H, mbw, nmb;
   int R, N, nrefs, lambda;
   int dx0, Q, shift, V, Gr, rows, row_dw;  // tile plan (vbs_plan)
+  int wave;                            // 8 or 16: one MB per wave (vbs_wave_kernel<wave>); 0: workgroups
   uint8_t* mode;                       // per MB
   int16_t* mv8;                        // 2 per 8x8 block, quarter pel
   uint8_t* ref8;                       // per 8x8 block
@@ -133,7 +134,10 @@ struct VbsParams {
 };
 // Fills the tile plan fields of p for range R and nrefs references.
 void vbs_plan(int R, int nrefs, VbsParams* p);
+// The one-MB-per-wave class of range R: 8 or 16 for R = 7, 8, 15, 16, else 0.
+int vbs_wave_class(int R);
 int vbs_lds_bytes(const VbsParams& p);
+int vbs_tile_rows(const VbsParams& p);
 void launch_vbs_search(const VbsParams& p, hipStream_t st);
 // Pads up to kVbsMaxRefs planes in one launch, descriptors passed by value.
 struct PadSet {

@@ -253,7 +253,202 @@ void vbs_search_kernel(const VbsParams p) {
   }
 }
 
+// ------------------------------------------------------- one MB per wave
+// For ranges R = L0 or L0 - 1 with L0 = 8 or 16 (the window's half-width is
+// R, clamped as GetMotionRange clamps it) one wave searches one MB over all
+// references in turn.  The MB's block lives in SGPRs (every lane uses the
+// same 64 dwords; qsad takes its block operand from an SGPR); each
+// reference's window is staged with global_load_lds into the wave's own LDS
+// and waited for with s_waitcnt, with no barrier.  Lanes tile the candidates
+// from (-L0, -R) as QW column quads x BB row bands of VV rows, enough to
+// cover the 2R + 1 columns and rows (lanes past QW x BB idle): 9 x 7 x 5 at
+// R = 16, 8 x 8 x 4 at 15, 5 x 9 x 2 at 8, 4 x 16 x 1 at 7; candidates
+// outside the MB's window get the key offset 2^31 and never win.  Per
+// (reference row, block row) a lane issues 4 qsads into the accumulators of
+// the quadrant that block row and column half belong to, and every finished
+// candidate row folds into the lane's nine partition keys exactly as the
+// workgroup kernel's; the wave's minima then decide the mode.
+template <int L0, int QW, int BB, int VV>
+struct VbsWaveGeom {
+  static constexpr int SHIFT = ((16 - L0 % 16) % 16) / 4;  // dwords from the 16-byte boundary to column -L0
+  static constexpr int ROW_DW = ((QW + 4 + SHIFT) + 3) / 4 * 4;
+  static constexpr int ROWS = BB * VV + 15;
+  static constexpr int WIN_DW = ROWS * ROW_DW;
+  static_assert(QW * BB <= 64, "lane tiling");
+};
+
+template <int L0, int QW, int BB, int VV>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
+void vbs_wave_kernel(const VbsParams p) {
+  using G = VbsWaveGeom<L0, QW, BB, VV>;
+  __shared__ __attribute__((aligned(16))) uint32_t win_raw[G::WIN_DW];
+  lds_u32* const win = (lds_u32*)win_raw;
+  const int lane = threadIdx.x;
+  int mb;
+  {
+    const int x = (int)blockIdx.x & 7, k = (int)blockIdx.x >> 3;
+    const int q = p.nmb >> 3, r = p.nmb & 7;
+    mb = x * q + min(x, r) + k;
+  }
+  const int mcol = mb % p.mbw, mrow = mb / p.mbw;
+  const int n = 16 * mcol, m = 16 * mrow;
+  const int R = p.R, N = p.N, NN = N * N;
+  int xl, xr, yu, yd;
+  motion_window(n, m, p.W, p.H, R, &xl, &xr, &yu, &yd);
+  uint32_t best[9];
+#pragma unroll
+  for (int i = 0; i < 9; i++) best[i] = ~0u;
+
+#pragma unroll 1
+  for (int r = 0; r < p.nrefs; r++) {
+    // Nothing but the keys lives across references: the block is loaded
+    // again (uniform: scalar loads into SGPRs) and the lane index is made
+    // opaque, so no per-lane value is hoisted out of the loop and spilled.
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    __builtin_assume(ln >= 0 && ln < 64);
+    uint32_t c[64];
+    {
+      typedef __attribute__((address_space(4))) const uint32_t const_u32;
+      const const_u32* cb = (const const_u32*)(p.cur + (long)m * p.W + n);
+      const int sdw = p.W >> 2;
+#pragma unroll
+      for (int i = 0; i < 16; i++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) c[4 * i + k] = cb[i * sdw + k];
+    }
+    const int q = ln % QW, b = ln / QW;
+    const bool active = b < BB;
+    const int dx0 = -L0 + 4 * q, dy0 = -R + (active ? b : 0) * VV;
+    const uint32_t k15 = 1u << 15;
+    uint32_t offc[4];
+    bool colok[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int dx = dx0 + j;
+      colok[j] = active && dx >= xl && dx <= xr;
+      offc[j] = ((uint32_t)(p.lambda * (dx < 0 ? -dx : dx)) << 15) + (uint32_t)(dx + R);
+    }
+    const uint8_t* rb = r == 0 ? p.ref[0] : (r == 1 ? p.ref[1] : (r == 2 ? p.ref[2] : p.ref[3]));
+    // stage rows m - R .. m - R + ROWS - 1 from the 16-byte boundary at or
+    // left of column n - L0
+    {
+      constexpr int kCpr = G::ROW_DW / 4, kChunks = G::ROWS * kCpr;
+      const uint8_t* wsrc = rb + (long)(m - R) * p.ref_stride + n - L0 - 4 * G::SHIFT;
+#pragma unroll
+      for (int base = 0; base < kChunks; base += 64) {
+        const int ch = base + ln;
+        const int row = ch / kCpr, c4 = ch - kCpr * row;
+        if (ch < kChunks)
+          __builtin_amdgcn_global_load_lds((const void*)(wsrc + (long)row * p.ref_stride + 16 * c4),
+                                           (__attribute__((address_space(3))) void*)(win + 4 * base), 16, 0, 0);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t rowbase = (uint32_t)(r * NN + (dy0 + R) * N);
+    uint64_t acc[4][VV];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+      for (int v = 0; v < VV; v++) acc[k][v] = 0;
+    // idle lanes walk band 0's rows (in bounds) and fold nothing
+    const lds_u32* rp = win + G::SHIFT + (active ? b : 0) * VV * G::ROW_DW + q;
+    auto span = [&](int k) { return *(const lds_u64_a4*)(rp + k); };
+    uint64_t n0 = span(0), n1 = span(1), n2 = span(2), n3 = span(3);
+#pragma clang loop unroll(full)
+    for (int rho = 0; rho < VV + 15; rho++) {
+      uint64_t p0 = n0, p1 = n1, p2 = n2, p3 = n3;
+      asm volatile("" : "+v"(p0), "+v"(p1), "+v"(p2), "+v"(p3), "+v"(rp) : : "memory");
+      if (rho + 1 < VV + 15) {
+        rp += G::ROW_DW;
+        n0 = span(0); n1 = span(1); n2 = span(2); n3 = span(3);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const uint64_t pk = k == 0 ? p0 : (k == 1 ? p1 : (k == 2 ? p2 : p3));
+#pragma unroll
+        for (int v = 0; v < VV; v++) {
+          const int br = rho - v;
+          if (br >= 0 && br < 16) {
+            const int quad = (br >= 8 ? 2 : 0) + (k >> 1);
+            acc[quad][v] = __builtin_amdgcn_qsad_pk_u16_u8(pk, c[4 * br + k], acc[quad][v]);
+          }
+        }
+      }
+      if (rho >= 15) {
+        const int v = rho - 15;
+        const int dy = dy0 + v;
+        const bool row_ok = dy >= yu && dy <= yd;
+        const uint32_t offr = (__umul24((uint32_t)p.lambda, (uint32_t)(dy < 0 ? -dy : dy)) << 15) + rowbase +
+                              (uint32_t)(v * N);
+        uint32_t off[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) off[j] = (row_ok && colok[j]) ? offr + offc[j] : 0x80000000u;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const uint32_t s0 = (uint32_t)(acc[0][v] >> (32 * h)), s1 = (uint32_t)(acc[1][v] >> (32 * h));
+          const uint32_t s2 = (uint32_t)(acc[2][v] >> (32 * h)), s3 = (uint32_t)(acc[3][v] >> (32 * h));
+          const uint32_t top = add_u16x2(s0, s1), bot = add_u16x2(s2, s3);
+          const uint32_t part[9] = {add_u16x2(top, bot), top, bot, add_u16x2(s0, s2), add_u16x2(s1, s3),
+                                    s0, s1, s2, s3};
+#pragma unroll
+          for (int i = 0; i < 9; i++)
+            best[i] = min3_u32(best[i], key_lo16(part[i], k15, off[2 * h]), key_hi16(part[i], k15, off[2 * h + 1]));
+        }
+      }
+    }
+    // every lane's window reads are done before the next reference lands
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  // the wave's minimum of each partition's key
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    uint32_t k = best[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) k = min(k, (uint32_t)__shfl_xor((int)k, o, 64));
+    best[i] = k;
+  }
+  if (lane == 0) {
+    uint32_t J[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) J[i] = best[i] >> 15;
+    const uint32_t mj[4] = {J[0], J[1] + J[2], J[3] + J[4], J[5] + J[6] + J[7] + J[8]};
+    int mode = 0;
+#pragma unroll
+    for (int k = 1; k < 4; k++)
+      if (mj[k] < mj[mode]) mode = k;
+    p.mode[mb] = (uint8_t)mode;
+    p.cost[mb] = (int32_t)mj[mode];
+    const int bw = p.W >> 3;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t idx = best[part_of_quadrant(mode, k)] & 0x7FFFu;
+      const int rr = (int)idx / NN, rem = (int)idx - rr * NN;
+      const int dy = rem / N - R, dx = rem - (rem / N) * N - R;
+      const long b8 = (long)(2 * mrow + (k >> 1)) * bw + 2 * mcol + (k & 1);
+      p.mv8[2 * b8] = (int16_t)(4 * dx);
+      p.mv8[2 * b8 + 1] = (int16_t)(4 * dy);
+      p.ref8[b8] = (uint8_t)rr;
+    }
+  }
+}
+
+// The lane tilings of the four instantiations (see above).
+#define ICU_VBS_WAVE_R16 16, 9, 7, 5
+#define ICU_VBS_WAVE_R15 16, 8, 8, 4
+#define ICU_VBS_WAVE_R8 8, 5, 9, 2
+#define ICU_VBS_WAVE_R7 8, 4, 16, 1
+
 }  // namespace
+
+int vbs_wave_class(int R) {
+  for (int L0 : {8, 16})
+    if (R == L0 || R == L0 - 1) return L0;
+  return 0;
+}
 
 void vbs_plan(int R, int nrefs, VbsParams* p) {
   const int N = 2 * R + 1;
@@ -280,14 +475,39 @@ void vbs_plan(int R, int nrefs, VbsParams* p) {
   p->Gr = (N + bestV - 1) / bestV;
   p->rows = p->Gr * bestV + 15;
   p->row_dw = ((p->shift + p->Q + 4) + 3) & ~3;
+  p->wave = vbs_wave_class(R);
   // the kernel divides with udiv_small: at R <= 32 and 4 references the
   // chunk and task counts stay far inside u < 2^17, d <= 2048 (at most
   // 4 x 87 x 6 chunks, 4 x 17 x 17 tasks)
 }
 
-int vbs_lds_bytes(const VbsParams& p) { return p.nrefs * p.rows * p.row_dw * 4; }
+int vbs_lds_bytes(const VbsParams& p) {
+  if (p.wave) {
+    switch (p.R) {
+      case 16: return VbsWaveGeom<ICU_VBS_WAVE_R16>::WIN_DW * 4;
+      case 15: return VbsWaveGeom<ICU_VBS_WAVE_R15>::WIN_DW * 4;
+      case 8: return VbsWaveGeom<ICU_VBS_WAVE_R8>::WIN_DW * 4;
+      default: return VbsWaveGeom<ICU_VBS_WAVE_R7>::WIN_DW * 4;
+    }
+  }
+  return p.nrefs * p.rows * p.row_dw * 4;
+}
+int vbs_tile_rows(const VbsParams& p) {
+  if (!p.wave) return p.V;
+  return p.R == 16 ? 5 : (p.R == 15 ? 4 : (p.R == 8 ? 2 : 1));
+}
 
 void launch_vbs_search(const VbsParams& p, hipStream_t st) {
+  if (p.wave) {
+    const dim3 g((unsigned)p.nmb), b(64);
+    switch (p.R) {
+      case 16: hipLaunchKernelGGL((vbs_wave_kernel<ICU_VBS_WAVE_R16>), g, b, 0, st, p); break;
+      case 15: hipLaunchKernelGGL((vbs_wave_kernel<ICU_VBS_WAVE_R15>), g, b, 0, st, p); break;
+      case 8: hipLaunchKernelGGL((vbs_wave_kernel<ICU_VBS_WAVE_R8>), g, b, 0, st, p); break;
+      default: hipLaunchKernelGGL((vbs_wave_kernel<ICU_VBS_WAVE_R7>), g, b, 0, st, p); break;
+    }
+    return;
+  }
   const size_t shmem = (size_t)vbs_lds_bytes(p);
   const dim3 grid((unsigned)p.nmb), block(kThreads);
   switch (p.V) {

@@ -143,7 +143,7 @@ int icu_vbs_code_device(icu_vbs* h, const uint8_t* d_cur, const int16_t* d_mv8, 
 
 int icu_vbs_get_plan(icu_vbs* h, int* tile_rows, int* lds_bytes) {
   if (!h) return icu_fail(ICU_E_ARG, "null handle");
-  if (tile_rows) *tile_rows = h->plan.V;
+  if (tile_rows) *tile_rows = vbs_tile_rows(h->plan);
   if (lds_bytes) *lds_bytes = vbs_lds_bytes(h->plan);
   return ICU_OK;
 }

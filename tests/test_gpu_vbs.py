@@ -45,6 +45,12 @@ CASES = [  # (W, H, R, nrefs, lambda, seed)
     (96, 64, 5, 4, 1, 6),
     (48, 32, 32, 2, 7, 7),
     (176, 144, 27, 4, 1000, 8),
+    # one MB per wave: R = 16, 15, 8, 7 (exhaustive tilings 9x7x5, 8x8x4,
+    # 5x9x2, 4x16x1 with idle lanes), pictures that clamp every window
+    (176, 144, 15, 3, 2, 9),
+    (96, 64, 7, 2, 5, 10),
+    (64, 48, 16, 4, 0, 11),
+    (48, 32, 8, 3, 9, 12),
 ]
 
 
