@@ -442,6 +442,107 @@ __device__ __forceinline__ void half_pel_direct(const DecideDesc& d, int x0, int
   }
 }
 
+// The SAD form of half_pel_direct in packed 16-bit arithmetic: the same
+// samples and the same distortions, with two columns in one register.
+// Pairs (p[k][i], p[k][i + 1]) are cut straight out of the loaded dwords by
+// v_perm (bytes sh + i and sh + i + 1 of dwords i / 4 and i / 4 + 1, each
+// zero-extended to 16 bits); the vertical sums (-2550 .. 10710) and the
+// G/b/h samples stay in 16-bit halves, the j samples (up to 450k before
+// rounding) come from three v_dot2_i32_i16 each, and every distortion
+// is a v_sad_u16 against a pair of block pels.
+typedef short icu_s16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ icu_s16x2 pk_s16(uint32_t x) { return __builtin_bit_cast(icu_s16x2, x); }
+__device__ __forceinline__ uint32_t pk_u32(icu_s16x2 x) { return __builtin_bit_cast(uint32_t, x); }
+
+__device__ __forceinline__ icu_s16x2 pk_tap6(icu_s16x2 a, icu_s16x2 b, icu_s16x2 c, icu_s16x2 d, icu_s16x2 e,
+                                             icu_s16x2 f) {
+  return (a + f) + (c + d) * (icu_s16x2){20, 20} + (b + e) * (icu_s16x2){-5, -5};
+}
+
+// clip255((x + 16) >> 5) in each half
+__device__ __forceinline__ icu_s16x2 pk_round5(icu_s16x2 x) {
+  const icu_s16x2 r = (x + (icu_s16x2){16, 16}) >> (icu_s16x2){5, 5};
+  return __builtin_elementwise_min(__builtin_elementwise_max(r, (icu_s16x2){0, 0}), (icu_s16x2){255, 255});
+}
+
+// clip255((TAP6(v[i .. i + 5]) + 512) >> 10) from the pairs (v[i], v[i + 1]),
+// (v[i + 2], v[i + 3]), (v[i + 4], v[i + 5])
+__device__ __forceinline__ uint32_t pk_j(icu_s16x2 q0, icu_s16x2 q2, icu_s16x2 q4) {
+  int t = __builtin_amdgcn_sdot2(q4, (icu_s16x2){-5, 1}, 512, false);
+  t = __builtin_amdgcn_sdot2(q2, (icu_s16x2){20, 20}, t, false);
+  t = __builtin_amdgcn_sdot2(q0, (icu_s16x2){1, -5}, t, false);
+  return (uint32_t)clip255(t >> 10);
+}
+
+__device__ __forceinline__ void half_pel_sad_pk(const DecideDesc& d, int x0, int y0, uint32_t cw, int lane,
+                                                int (&acc)[8]) {
+  const int r = lane >> 2, c0 = 4 * (lane & 3);
+  const int xs = x0 - 3 + c0, sh = xs & 3;  // 0..3 also for negative xs
+  const uint32_t* base = (const uint32_t*)(d.ref + (long)(y0 - 3 + r) * d.ref_stride + (xs - sh));
+  const int sdw = d.ref_stride >> 2;
+  uint32_t w[7][4];
+#pragma unroll
+  for (int k = 0; k < 7; k++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[k][q] = base[k * sdw + q];
+  // selector of the pair at i with i % 4 == t: bytes sh + t, sh + t + 1 of
+  // (w[i / 4], w[i / 4 + 1]), each followed by a zero byte
+  uint32_t sel[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) sel[t] = 0x0c010c00u + (uint32_t)(sh + t) * 0x00010001u;
+  // row pairs (p[k][2e], p[k][2e + 1]) and their vertical sums
+  icu_s16x2 P[7][5];
+#pragma unroll
+  for (int k = 0; k < 7; k++)
+#pragma unroll
+    for (int e = 0; e < 5; e++) P[k][e] = pk_s16(__builtin_amdgcn_perm(w[k][e / 2 + 1], w[k][e / 2], sel[(2 * e) & 3]));
+  // VP[a][i] = (v[a][i], v[a][i + 1]), i = 0..8
+  icu_s16x2 VP[2][9];
+#pragma unroll
+  for (int a = 0; a < 2; a++) {
+#pragma unroll
+    for (int e = 0; e < 5; e++)
+      VP[a][2 * e] = pk_tap6(P[a][e], P[a + 1][e], P[a + 2][e], P[a + 3][e], P[a + 4][e], P[a + 5][e]);
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+      VP[a][2 * e + 1] = pk_s16(__builtin_amdgcn_alignbyte(pk_u32(VP[a][2 * e + 2]), pk_u32(VP[a][2 * e]), 2));
+  }
+  // row y0 + r: every pair (p[3][i], p[3][i + 1]), i = 0..9, and the b
+  // samples in pairs (B[0], B[1]), (B[2], B[3]), (B[4], -)
+  icu_s16x2 R3[10];
+#pragma unroll
+  for (int i = 0; i < 10; i++) R3[i] = pk_s16(__builtin_amdgcn_perm(w[3][i / 4 + 1], w[3][i / 4], sel[i & 3]));
+  icu_s16x2 PB[3];
+#pragma unroll
+  for (int e = 0; e < 3; e++)
+    PB[e] = pk_round5(pk_tap6(R3[2 * e], R3[2 * e + 1], R3[2 * e + 2], R3[2 * e + 3], R3[2 * e + 4], R3[2 * e + 5]));
+  const uint32_t b12 = __builtin_amdgcn_alignbyte(pk_u32(PB[1]), pk_u32(PB[0]), 2);
+  const uint32_t b34 = __builtin_amdgcn_alignbyte(pk_u32(PB[2]), pk_u32(PB[1]), 2);
+  // the block pels (c0, c1), (c2, c3) of this lane
+  const uint32_t C0 = __builtin_amdgcn_perm(0u, cw, 0x0c010c00u), C1 = __builtin_amdgcn_perm(0u, cw, 0x0c030c02u);
+#pragma unroll
+  for (int a = 0; a < 2; a++) {
+    uint32_t J[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) J[i] = pk_j(VP[a][i], VP[a][i + 2], VP[a][i + 4]);
+    const uint32_t h01 = pk_u32(pk_round5(VP[a][3])), h23 = pk_u32(pk_round5(VP[a][5]));
+    int& aj0 = acc[a ? 5 : 0];  // (-1, -1 + 2a)
+    int& ah = acc[a ? 6 : 1];   // ( 0, -1 + 2a)
+    int& aj1 = acc[a ? 7 : 2];  // ( 1, -1 + 2a)
+    aj0 = (int)__builtin_amdgcn_sad_u16(J[0] | (J[1] << 16), C0, (uint32_t)aj0);
+    aj0 = (int)__builtin_amdgcn_sad_u16(J[2] | (J[3] << 16), C1, (uint32_t)aj0);
+    ah = (int)__builtin_amdgcn_sad_u16(h01, C0, (uint32_t)ah);
+    ah = (int)__builtin_amdgcn_sad_u16(h23, C1, (uint32_t)ah);
+    aj1 = (int)__builtin_amdgcn_sad_u16(J[1] | (J[2] << 16), C0, (uint32_t)aj1);
+    aj1 = (int)__builtin_amdgcn_sad_u16(J[3] | (J[4] << 16), C1, (uint32_t)aj1);
+  }
+  acc[3] = (int)__builtin_amdgcn_sad_u16(pk_u32(PB[0]), C0, (uint32_t)acc[3]);  // (-1, 0)
+  acc[3] = (int)__builtin_amdgcn_sad_u16(pk_u32(PB[1]), C1, (uint32_t)acc[3]);
+  acc[4] = (int)__builtin_amdgcn_sad_u16(b12, C0, (uint32_t)acc[4]);  // (1, 0)
+  acc[4] = (int)__builtin_amdgcn_sad_u16(b34, C1, (uint32_t)acc[4]);
+}
+
 // Half-pel then (subpel >= 2) quarter-pel refinement around full-pel
 // (mx, my) of the MB at (n, m), strictly better distortions only
 // (Full.cpp:452-517; UMHS.cpp:795-862 is the same stage).  cw: this lane's
@@ -457,7 +558,7 @@ __device__ inline void refine_subpel(const DecideDesc& d, WaveLds& s, int n, int
     // half pel only: the eight candidates filtered in registers
     int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (d.metric == kMetricSAD) {
-      half_pel_direct<kMetricSAD>(d, n + mx, m + my, cw, lane, acc);
+      half_pel_sad_pk(d, n + mx, m + my, cw, lane, acc);
       wave_sum8_small(acc);
     } else {
       half_pel_direct<kMetricSSD>(d, n + mx, m + my, cw, lane, acc);
@@ -648,7 +749,7 @@ __device__ __forceinline__ void decide_mb_lean(const DecideDesc& d, int mb, int 
   if (d.subpel == 1) {
     const uint32_t cw = *(const uint32_t*)(d.cur + (long)(m + (lane >> 2)) * d.cur_stride + n + 4 * (lane & 3));
     int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    half_pel_direct<kMetricSAD>(d, n + r.mx, m + r.my, cw, lane, acc);
+    half_pel_sad_pk(d, n + r.mx, m + r.my, cw, lane, acc);
     wave_sum8_small(acc);
     int hxh = 0, hyh = 0;
 #pragma unroll

@@ -747,21 +747,33 @@ __device__ __forceinline__ void wave_pass(const StreamDesc& s, const WaveMb& w, 
     fold_key(kbest, live, acc, dxb + 2 * L0, dyb + l, w.xl, w.xr, w.yu, w.yd);
   }
   if (lrow) {
-    // candidate (dxb + l, dyb + 2 L0) from window rows 2 L0 .. 2 L0 + 15,
-    // bytes l .. l + 15
-    uint32_t acc = 0;
-    const lds_u32* rp = w0 + 2 * L0 * G::ROW_DW + (l >> 2);
-    const uint32_t sh = (uint32_t)(l & 3);
+    // candidate (dxb + l, dyb + 2 L0), l = 4 g + t, from window rows
+    // 2 L0 .. 2 L0 + 15: lane l sums block rows 4 t .. 4 t + 3 (loaded by
+    // the lane itself) against candidates 4 g .. 4 g + 3, one
+    // v_qsad_pk_u16_u8 per block dword on dword-aligned reference bytes;
+    // the quad's four partial sums (each below 2^16 per candidate, so two
+    // 16-bit sums share a dword) are added by DPP and lane l keeps its own.
+    const int g = (live ? lane : 2 * L0 - 1) >> 2, t = lane & 3;
+    const lds_u32* rp = w0 + (2 * L0 + 4 * t) * G::ROW_DW + g;
+    const uint32_t* cb = (const uint32_t*)(s.cur + (long)(w.m + 4 * t) * s.cur_stride + w.n);
+    const int sdw = s.cur_stride >> 2;
+    uint64_t acc = 0;
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
+    for (int r = 0; r < 4; r++) {
       uint32_t v[5];
 #pragma unroll
       for (int j = 0; j < 5; j++) v[j] = rp[r * G::ROW_DW + j];
 #pragma unroll
       for (int k = 0; k < 4; k++)
-        acc = __builtin_amdgcn_sad_u8(__builtin_amdgcn_alignbyte(v[k + 1], v[k], sh), c[4 * r + k], acc);
+        acc = __builtin_amdgcn_qsad_pk_u16_u8(((uint64_t)v[k + 1] << 32) | v[k], cb[r * sdw + k], acc);
     }
-    fold_key(kbest, live, acc, dxb + l, dyb + 2 * L0, w.xl, w.xr, w.yu, w.yd);
+    uint32_t lo = (uint32_t)acc, hi = (uint32_t)(acc >> 32);
+    lo += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0xB1, 0xf, 0xf, false);  // quad_perm [1,0,3,2]
+    hi += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0xB1, 0xf, 0xf, false);
+    lo += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, 0x4E, 0xf, 0xf, false);  // quad_perm [2,3,0,1]
+    hi += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, 0x4E, 0xf, 0xf, false);
+    const uint32_t cost = ((t & 2 ? hi : lo) >> (16 * (t & 1))) & 0xFFFFu;
+    fold_key(kbest, live, cost, dxb + l, dyb + 2 * L0, w.xl, w.xr, w.yu, w.yd);
   }
   if (corner) {
     // lane l the block dword (l / 4, l % 4)
