@@ -133,23 +133,37 @@ __global__ __launch_bounds__(256) void pad_set_kernel(const PadSet s) {
 typedef uint64_t __attribute__((aligned(4))) u64_a4;
 typedef __attribute__((address_space(3))) u64_a4 lds_u64_a4;
 
-template <int V, typename Done>
+// RDW: the row pitch in dwords when known at compile time (0: row_dw).  With
+// it, P rows at a time are read at constant offsets from one base (the
+// offsets of ds_read2_b32 reach 255 dwords), so the base moves once per P
+// rows instead of once per row.
+template <int V, int RDW = 0, typename Done>
 __device__ __forceinline__ void sad_tile(const lds_u32* wp, int row_dw,
                                          const uint32_t (&c)[64], Done&& done) {
+  constexpr int P = RDW ? (251 / RDW) : 1;
+  const int rdw = RDW ? RDW : row_dw;
   uint64_t acc[V];
 #pragma unroll
   for (int v = 0; v < V; v++) acc[v] = 0;
   const lds_u32* rp = wp;
-  auto span = [&](int k) { return *(const lds_u64_a4*)(rp + k); };
-  uint64_t n0 = span(0), n1 = span(1), n2 = span(2), n3 = span(3);
+  // opaque, so no constant part of wp is folded into the offsets
+  if (P > 1) asm volatile("" : "+v"(rp));
+  auto span = [&](int rho, int k) { return *(const lds_u64_a4*)(rp + (rho % P) * rdw + k); };
+  uint64_t n0 = span(0, 0), n1 = span(0, 1), n2 = span(0, 2), n3 = span(0, 3);
 #pragma unroll
   for (int rho = 0; rho < V + 15; rho++) {
     uint64_t p0 = n0, p1 = n1, p2 = n2, p3 = n3;
-    // also makes rp opaque, so the next address is one add, not recomputed
-    asm volatile("" : "+v"(p0), "+v"(p1), "+v"(p2), "+v"(p3), "+v"(rp) : : "memory");
+    // with P == 1 also makes rp opaque, so the next address is one add, not
+    // recomputed
+    if (P == 1) asm volatile("" : "+v"(p0), "+v"(p1), "+v"(p2), "+v"(p3), "+v"(rp) : : "memory");
+    else asm volatile("" : "+v"(p0), "+v"(p1), "+v"(p2), "+v"(p3) : : "memory");
     if (rho + 1 < V + 15) {
-      rp += row_dw;
-      n0 = span(0); n1 = span(1); n2 = span(2); n3 = span(3);
+      if ((rho + 1) % P == 0) {
+        rp += P * rdw;
+        if (P > 1) asm volatile("" : "+v"(rp));
+      }
+      const int nr = rho + 1;
+      n0 = span(nr, 0); n1 = span(nr, 1); n2 = span(nr, 2); n3 = span(nr, 3);
     }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -606,13 +620,17 @@ void me_search_kernel(const StreamDesc* __restrict__ streams, const SearchTask* 
 // lane key (cost << 16) | ((dist2 - least dist2 of the tile) << IB) |
 // (2^IB - 1 - raster index), folded into the 64-bit key.  The codes fit 16
 // bits for tiles of up to 8 rows at every range up to 64 (IB = 5).
-template <int R, int IB, int ROW_DW>
+// MASKED: some candidate of the pass may lie outside the MB's window (the
+// caller knows it per wave); without it the per-row masks are left out.
+template <int R, int IB, int ROW_DW, bool MASKED>
 __device__ __forceinline__ void wave_tile(const lds_u32* wp, const uint32_t (&c)[64], int dx0, int dy0, int xl,
                                           int xr, int yu, int yd, uint64_t& kbest) {
-  const uint32_t cm_lo = (dx0 >= xl && dx0 <= xr ? 0u : 0xFFFFu) |
-                         (dx0 + 1 >= xl && dx0 + 1 <= xr ? 0u : 0xFFFF0000u);
-  const uint32_t cm_hi = (dx0 + 2 >= xl && dx0 + 2 <= xr ? 0u : 0xFFFFu) |
-                         (dx0 + 3 >= xl && dx0 + 3 <= xr ? 0u : 0xFFFF0000u);
+  const uint32_t cm_lo = !MASKED ? 0u
+                                 : (dx0 >= xl && dx0 <= xr ? 0u : 0xFFFFu) |
+                                       (dx0 + 1 >= xl && dx0 + 1 <= xr ? 0u : 0xFFFF0000u);
+  const uint32_t cm_hi = !MASKED ? 0u
+                                 : (dx0 + 2 >= xl && dx0 + 2 <= xr ? 0u : 0xFFFFu) |
+                                       (dx0 + 3 >= xl && dx0 + 3 <= xr ? 0u : 0xFFFF0000u);
   const int ax = dx0 > 0 ? dx0 : (dx0 + 3 < 0 ? -(dx0 + 3) : 0);
   const int ay = dy0 > 0 ? dy0 : (dy0 + R - 1 < 0 ? -(dy0 + R - 1) : 0);
   uint32_t Ck[4];
@@ -624,9 +642,9 @@ __device__ __forceinline__ void wave_tile(const lds_u32* wp, const uint32_t (&c)
   const int vlo = max(yu - dy0, 0), vhi = min(yd - dy0, R - 1);
   const uint32_t inv_rows = vlo > vhi ? 0xFFFFFFFFu : ~(((2u << vhi) - 1u) & ~((1u << vlo) - 1u));
   uint32_t lbest = 0xFFFFFFFFu;
-  sad_tile<R>(wp, ROW_DW, c, [&](int v, uint32_t lo, uint32_t hi) {
-    const uint32_t rmask = (uint32_t)__builtin_amdgcn_sbfe((int)inv_rows, v, 1);
-    const uint32_t l2 = lo | cm_lo | rmask, h2 = hi | cm_hi | rmask;
+  sad_tile<R, ROW_DW>(wp, ROW_DW, c, [&](int v, uint32_t lo, uint32_t hi) {
+    const uint32_t rmask = MASKED ? (uint32_t)__builtin_amdgcn_sbfe((int)inv_rows, v, 1) : 0u;
+    const uint32_t l2 = MASKED ? lo | cm_lo | rmask : lo, h2 = MASKED ? hi | cm_hi | rmask : hi;
     const uint32_t k0 = (l2 << 16) | Ck[0];
     const uint32_t k1 = (l2 & 0xFFFF0000u) | Ck[1];
     const uint32_t k2 = (h2 << 16) | Ck[2];
@@ -711,14 +729,20 @@ __device__ __forceinline__ void wave_pass(const StreamDesc& s, const WaveMb& w, 
       for (int k = 0; k < 4; k++) c[4 * i + k] = cb[i * sdw + k];
   }
   {
+    // chunk base + lane of the window: row r, 16-byte column c4, at a 32-bit
+    // offset from the uniform row origin (a scalar base and one VGPR offset
+    // per load)
     constexpr int kCpr = G::ROW_DW / 4, kChunks = G::ROWS * kCpr;
     const uint8_t* wsrc = s.ref + (long)(w.m + dyb) * s.ref_stride + w.n + dxb - 4 * G::SHIFT;
+    __builtin_assume(lane >= 0 && lane < 64);
 #pragma unroll
     for (int base = 0; base < kChunks; base += 64) {
-      const int ch = base + lane;
-      const int r = ch / kCpr, c4 = ch - kCpr * r;
-      if (ch < kChunks)
-        __builtin_amdgcn_global_load_lds((const void*)(wsrc + (long)r * s.ref_stride + 16 * c4),
+      // r * stride + 16 (ch - kCpr r), strides below 2^23
+      const uint32_t ch = (uint32_t)(base + lane);
+      const uint32_t r = ch / kCpr;
+      const uint32_t off = (uint32_t)__mul24((int)r, s.ref_stride - 16 * kCpr) + (ch << 4);
+      if (base + 64 <= kChunks || ch < (uint32_t)kChunks)
+        __builtin_amdgcn_global_load_lds((const void*)(wsrc + off),
                                          (__attribute__((address_space(3))) void*)(win + 4 * base), 16, 0, 0);
     }
   }
@@ -729,10 +753,20 @@ __device__ __forceinline__ void wave_pass(const StreamDesc& s, const WaveMb& w, 
   const int q = lane % G::QW, b = lane / G::QW;
   const int dx0 = dxb + 4 * q;
   const int dy0 = dyb + b * G::VV;
+  // every candidate of the pass inside the window (the interior MBs): the
+  // tiles need no masks
+  const bool inside = dxb >= w.xl && dxb + 2 * L0 - 1 <= w.xr && dyb >= w.yu && dyb + 2 * L0 - 1 <= w.yd;
+  if (inside) {
 #pragma unroll
-  for (int h = 0; h < G::VV / G::TR; h++)
-    wave_tile<G::TR, 5, G::ROW_DW>(w0 + (b * G::VV + h * G::TR) * G::ROW_DW + q, c, dx0, dy0 + h * G::TR, w.xl,
-                                   w.xr, w.yu, w.yd, kbest);
+    for (int h = 0; h < G::VV / G::TR; h++)
+      wave_tile<G::TR, 5, G::ROW_DW, false>(w0 + (b * G::VV + h * G::TR) * G::ROW_DW + q, c, dx0, dy0 + h * G::TR,
+                                            w.xl, w.xr, w.yu, w.yd, kbest);
+  } else {
+#pragma unroll
+    for (int h = 0; h < G::VV / G::TR; h++)
+      wave_tile<G::TR, 5, G::ROW_DW, true>(w0 + (b * G::VV + h * G::TR) * G::ROW_DW + q, c, dx0, dy0 + h * G::TR,
+                                           w.xl, w.xr, w.yu, w.yd, kbest);
+  }
   const bool live = lane < 2 * L0;
   const int l = live ? lane : 2 * L0 - 1;  // lanes past the line read in bounds, fold nothing
   if (lcol) {
