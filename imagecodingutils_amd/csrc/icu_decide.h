@@ -132,6 +132,28 @@ __device__ __forceinline__ void wave_sum8_small(int (&a)[8]) {
     a[2 * i + 1] = (int)(t >> 16);
   }
 }
+// The least 32-bit value over the wave, in every lane as a uniform value:
+// row_ror 1, 2, 4, 8 leave each lane its row's minimum, row_bcast:15 and
+// row_bcast:31 carry rows 0 and 2 into 1 and 3 and lane 31 into rows 2 and
+// 3, so lane 63 holds the wave's (lanes outside a row mask read the
+// identity, ~0, so each step is one v_min_u32 with DPP).
+__device__ __forceinline__ uint32_t wave_min_u32_uniform(uint32_t v) {
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x121, 0xf, 0xf, false));  // row_ror:1
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x122, 0xf, 0xf, false));  // row_ror:2
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x124, 0xf, 0xf, false));  // row_ror:4
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x128, 0xf, 0xf, false));  // row_ror:8
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+// The least 64-bit key over the wave as a uniform value: the least high
+// word, then the least low word among the lanes that hold it.  Every caller
+// runs it with all 64 lanes active.
+__device__ __forceinline__ unsigned long long wave_min_u64_uniform(unsigned long long v) {
+  const uint32_t hi = wave_min_u32_uniform((uint32_t)(v >> 32));
+  const uint32_t lo = wave_min_u32_uniform((uint32_t)(v >> 32) == hi ? (uint32_t)v : 0xFFFFFFFFu);
+  return ((unsigned long long)hi << 32) | lo;
+}
 __device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -478,14 +500,21 @@ __device__ __forceinline__ uint32_t pk_j(icu_s16x2 q0, icu_s16x2 q2, icu_s16x2 q
 __device__ __forceinline__ void half_pel_sad_pk(const DecideDesc& d, int x0, int y0, uint32_t cw, int lane,
                                                 int (&acc)[8]) {
   const int r = lane >> 2, c0 = 4 * (lane & 3);
-  const int xs = x0 - 3 + c0, sh = xs & 3;  // 0..3 also for negative xs
-  const uint32_t* base = (const uint32_t*)(d.ref + (long)(y0 - 3 + r) * d.ref_stride + (xs - sh));
-  const int sdw = d.ref_stride >> 2;
+  // the byte offset of column x0 - 3 + c0 in its dword, the same in every
+  // lane (c0 is a multiple of 4; 0..3 also for negative columns); rows are
+  // read at 32-bit offsets from the wave's origin
+  const int sh = (x0 - 3) & 3;
+  typedef __attribute__((address_space(1))) const uint8_t g_u8;
+  typedef __attribute__((address_space(1))) const uint32_t g_u32;
+  g_u8* org = (g_u8*)d.ref + (long)(y0 - 3) * d.ref_stride + (x0 - 3 - sh);
+  const uint32_t off = (uint32_t)(r * d.ref_stride + c0);
   uint32_t w[7][4];
 #pragma unroll
-  for (int k = 0; k < 7; k++)
+  for (int k = 0; k < 7; k++) {
+    g_u32* row = (g_u32*)(org + (off + (uint32_t)(k * d.ref_stride)));
 #pragma unroll
-    for (int q = 0; q < 4; q++) w[k][q] = base[k * sdw + q];
+    for (int q = 0; q < 4; q++) w[k][q] = row[q];
+  }
   // selector of the pair at i with i % 4 == t: bytes sh + t, sh + t + 1 of
   // (w[i / 4], w[i / 4 + 1]), each followed by a zero byte
   uint32_t sel[4];
@@ -747,7 +776,9 @@ __device__ __forceinline__ void decide_mb_lean(const DecideDesc& d, int mb, int 
   const int slot = d.mb_base + mb;
   int minD = r.cost, mvx = r.mx * 4, mvy = r.my * 4;
   if (d.subpel == 1) {
-    const uint32_t cw = *(const uint32_t*)(d.cur + (long)(m + (lane >> 2)) * d.cur_stride + n + 4 * (lane & 3));
+    typedef __attribute__((address_space(1))) const uint8_t g_u8;
+    const uint32_t cw = *(__attribute__((address_space(1))) const uint32_t*)(
+        (g_u8*)d.cur + (long)m * d.cur_stride + n + (uint32_t)((lane >> 2) * d.cur_stride + 4 * (lane & 3)));
     int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     half_pel_sad_pk(d, n + r.mx, m + r.my, cw, lane, acc);
     wave_sum8_small(acc);

@@ -848,7 +848,7 @@ __device__ __forceinline__ uint64_t wave_search_mb(const StreamDesc& s, const Wa
       wave_pass<32>(s, w, win, ln, dxb, dyb, lc, lr, lc && lr, kbest);
     }
   }
-  return dec::wave_min_u64(kbest);
+  return dec::wave_min_u64_uniform(kbest);
 }
 
 // The decision of a searched MB, in the same wave, with the window's LDS
