@@ -81,6 +81,20 @@ __host__ __device__ inline void motion_window(int n, int m, int W, int H, int L,
   *yd = (m + L < H) ? L : H - m;
 }
 
+// The least 32-bit value over the wave, in every lane as a uniform value:
+// row_ror 1, 2, 4, 8 leave each lane its row's minimum, row_bcast:15 and
+// row_bcast:31 carry rows 0 and 2 into 1 and 3 and lane 31 into rows 2 and
+// 3, so lane 63 holds the wave's (lanes outside a row mask read the
+// identity, ~0, so each step is one v_min_u32 with DPP).
+__device__ __forceinline__ uint32_t wave_min_u32_uniform(uint32_t v) {
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x121, 0xf, 0xf, false));  // row_ror:1
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x122, 0xf, 0xf, false));  // row_ror:2
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x124, 0xf, 0xf, false));  // row_ror:4
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x128, 0xf, 0xf, false));  // row_ror:8
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
 // First candidate column covered by the lane tiles: -L rounded down to a
 // multiple of 4 so every lane's 4-candidate quad starts on a dword.
 __host__ __device__ inline int tile_dx0(int L) { return -4 * ((L + 3) / 4); }

@@ -333,14 +333,17 @@ void vbs_wave_kernel(const VbsParams p) {
     // stage rows m - R .. m - R + ROWS - 1 from the 16-byte boundary at or
     // left of column n - L0
     {
+      // chunk base + ln: row ch / kCpr, 16-byte column ch % kCpr, at the
+      // 32-bit offset row * stride + 16 (ch - kCpr row) from the uniform
+      // origin (strides below 2^23)
       constexpr int kCpr = G::ROW_DW / 4, kChunks = G::ROWS * kCpr;
       const uint8_t* wsrc = rb + (long)(m - R) * p.ref_stride + n - L0 - 4 * G::SHIFT;
 #pragma unroll
       for (int base = 0; base < kChunks; base += 64) {
-        const int ch = base + ln;
-        const int row = ch / kCpr, c4 = ch - kCpr * row;
-        if (ch < kChunks)
-          __builtin_amdgcn_global_load_lds((const void*)(wsrc + (long)row * p.ref_stride + 16 * c4),
+        const uint32_t ch = (uint32_t)(base + ln);
+        const uint32_t off = (uint32_t)__mul24((int)(ch / kCpr), p.ref_stride - 16 * kCpr) + (ch << 4);
+        if (base + 64 <= kChunks || ch < (uint32_t)kChunks)
+          __builtin_amdgcn_global_load_lds((const void*)(wsrc + off),
                                            (__attribute__((address_space(3))) void*)(win + 4 * base), 16, 0, 0);
       }
     }
@@ -354,16 +357,24 @@ void vbs_wave_kernel(const VbsParams p) {
 #pragma unroll
       for (int v = 0; v < VV; v++) acc[k][v] = 0;
     // idle lanes walk band 0's rows (in bounds) and fold nothing
+    // P rows at a time are read at constant offsets from one base (the
+    // offsets of ds_read2_b32 reach 255 dwords)
+    constexpr int P = 251 / G::ROW_DW;
     const lds_u32* rp = win + G::SHIFT + (active ? b : 0) * VV * G::ROW_DW + q;
-    auto span = [&](int k) { return *(const lds_u64_a4*)(rp + k); };
-    uint64_t n0 = span(0), n1 = span(1), n2 = span(2), n3 = span(3);
+    asm volatile("" : "+v"(rp));
+    auto span = [&](int rho, int k) { return *(const lds_u64_a4*)(rp + (rho % P) * G::ROW_DW + k); };
+    uint64_t n0 = span(0, 0), n1 = span(0, 1), n2 = span(0, 2), n3 = span(0, 3);
 #pragma clang loop unroll(full)
     for (int rho = 0; rho < VV + 15; rho++) {
       uint64_t p0 = n0, p1 = n1, p2 = n2, p3 = n3;
-      asm volatile("" : "+v"(p0), "+v"(p1), "+v"(p2), "+v"(p3), "+v"(rp) : : "memory");
+      asm volatile("" : "+v"(p0), "+v"(p1), "+v"(p2), "+v"(p3) : : "memory");
       if (rho + 1 < VV + 15) {
-        rp += G::ROW_DW;
-        n0 = span(0); n1 = span(1); n2 = span(2); n3 = span(3);
+        if ((rho + 1) % P == 0) {
+          rp += P * G::ROW_DW;
+          asm volatile("" : "+v"(rp));
+        }
+        const int nr = rho + 1;
+        n0 = span(nr, 0); n1 = span(nr, 1); n2 = span(nr, 2); n3 = span(nr, 3);
       }
 #pragma unroll
       for (int k = 0; k < 4; k++) {
@@ -403,14 +414,9 @@ void vbs_wave_kernel(const VbsParams p) {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
-  // the wave's minimum of each partition's key
+  // the wave's minimum of each partition's key (DPP, uniform)
 #pragma unroll
-  for (int i = 0; i < 9; i++) {
-    uint32_t k = best[i];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) k = min(k, (uint32_t)__shfl_xor((int)k, o, 64));
-    best[i] = k;
-  }
+  for (int i = 0; i < 9; i++) best[i] = wave_min_u32_uniform(best[i]);
   if (lane == 0) {
     uint32_t J[9];
 #pragma unroll
