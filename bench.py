@@ -658,7 +658,7 @@ def main():
                    "mbs_per_step": plain["nmb"], "tiling": list(plain["tiling"])},
         "p50_ms": round(pctl(plain["per_ms"], 50), 4),
         "p99_ms": round(pctl(plain["per_ms"], 99), 4),
-        "stage_ms": {"pad": plain["stages_ms"][0], "search": search_ms, "decide_subpel": plain["stages_ms"][2]},
+        "stage_ms": {"pad": plain["stages_ms"][0], "search": search_ms, "decision_kernels": plain["stages_ms"][2]},
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": round(peak, 2),
                      "unit": "T SAD-op/s", "frac": round(achieved / peak, 4),
                      "traffic": round((SEARCH_FETCH_KIB * FETCH_SCALE + SEARCH_WRITE_KIB * WRITE_SCALE) * 1024),

@@ -582,6 +582,7 @@ struct Engine {
   // search_reps > 1 (timing only): the search launch is repeated; it reads
   // only the padded references and the current planes, so every repeat
   // writes the same results.
+  int decide_launches = 0;  // decision kernels of the last enqueue
   int enqueue(int set, hipStream_t st, hipEvent_t* ev = nullptr, int search_reps = 1) {
     StreamDesc* d_streams = ds[set].d_streams;
     DecideDesc* d_decide = ds[set].d_decide;
@@ -640,9 +641,14 @@ struct Engine {
     }
     if (ev) HIP_TRY(hipEventRecord(ev[2], st));
     // without cost tables the search kernel decides every MB itself
-    if (any_table && !h_items.empty()) launch_decide_parallel(d_decide, d_items, (int)h_items.size(), st);
+    decide_launches = 0;
+    if (any_table && !h_items.empty()) {
+      launch_decide_parallel(d_decide, d_items, (int)h_items.size(), st);
+      decide_launches++;
+    }
     for (size_t i = 0; i < sp.size(); i++) {
       if (!sp[i].cfg.predictor) continue;
+      decide_launches += 2;
       // progress counters and the row ticket; the error flag after them
       // stays set until check_wave_errors reads it
       uint8_t* pf = d_pf + (size_t)predfree_bytes() * sp[i].mb_base;
@@ -831,6 +837,8 @@ int icu_me_estimate_timed(icu_me* h, const uint8_t* d_cur, const uint8_t* d_ref,
     HIP_TRY(hipEventSynchronize(ev[3]));
     for (int i = 0; i < 3; i++) HIP_TRY(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
     stage_ms[1] /= (float)search_reps;
+    // no decision kernels: the last pair brackets nothing but its own gap
+    if (e.decide_launches == 0) stage_ms[2] = 0.f;
   }
   for (auto& x : ev) hipEventDestroy(x);
   return rc;

@@ -103,7 +103,8 @@ int icu_me_estimate_device_noalloc(const icu_me_config* cfg, const uint8_t* d_cu
 /* icu_me_estimate_device with HIP events between its stages, recorded on
  * the stream the kernels run on; waits for completion and writes the
  * duration of each stage in ms: [0] reference padding, [1] integer search
- * (with the decision when no cost table is kept), [2] decision kernels.
+ * (with the decision when no cost table is kept), [2] decision kernels (0
+ * when the search kernel decides every MB itself).
  * search_reps (1..1000) back-to-back search launches share one event pair
  * and [1] is their mean, which takes the events' own dispatch gap out of a
  * single launch's time (the search rewrites the same results each time).
