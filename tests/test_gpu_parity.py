@@ -321,7 +321,7 @@ def test_tuned_estimator_matches_oracle():
     d_cur, d_ref = capi.DeviceBuffer.from_array(fp.cur), capi.DeviceBuffer.from_array(fp.ref)
     est.autotune(d_cur, d_ref, reps=1)
     v, g, p = est.tiling()
-    assert v in (8, 11, 13, 16) and 1 <= g <= 16 and p in (0, 1, 2, 3)
+    assert v in (8, 11, 13, 16) and 1 <= g <= 16 and p in (0, 1, 2, 3, 4)
     mv, dist, _ = est.estimate(fp.cur.astype(np.int16), fp.ref.astype(np.int16))
     omv, odist, _, _ = orc.me_estimate(fp.cur, fp.ref, R, **_oracle_modes(modes))
     np.testing.assert_array_equal(mv, omv)
