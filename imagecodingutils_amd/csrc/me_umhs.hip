@@ -139,6 +139,50 @@ __device__ __forceinline__ void try8(const DecideDesc& d, const WinRef& wr, int 
   }
 }
 
+// A stage's candidates one per lane (lane k the stage's k-th, ok: inside
+// the window): each lane sums its candidate's SSD over the whole block from
+// the staged window, as sum c^2 + sum r^2 - 2 sum c r with v_dot4_u32_u8
+// (the block rows read from s.cur4, the same address in every lane), then
+// the reference's loop body (UMHS.cpp:598-609) is replayed in order by
+// ballot: the first candidate after the last accepted one that is no worse
+// in distortion and strictly cheaper in cost replaces the best, until none
+// is left.  One pass for a stage of up to 64 candidates, where try8 takes
+// one per 8.  csq: the block's sum of squares.  Lanes without a candidate
+// read the zero vector's block (always staged) and take no part.
+__device__ __forceinline__ void try_lanes(const WinRef& wr, const WaveLds& s, int n, int m, int mx, int my, int ox,
+                                          int oy, bool ok, uint32_t csq, int pX0, int pY0, int lane, Best& b) {
+  const int x = ok ? n + mx + ox : n, y = ok ? m + my + oy : m;
+  const int xs = x - wr.w.x0, sh = xs & 3;
+  const uint8_t* rowp = wr.lds + (y - wr.w.y0) * wr.w.cols + (xs - sh);
+  uint32_t cr = 0, rr = 0;
+#pragma unroll 4
+  for (int r = 0; r < 16; r++) {
+    const uint32_t* q = (const uint32_t*)(rowp + r * wr.w.cols);
+    uint32_t v[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) v[j] = q[j];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint32_t a = __builtin_amdgcn_alignbyte(v[k + 1], v[k], sh);
+      cr = __builtin_amdgcn_udot4(s.cur4[4 * r + k], a, cr, false);
+      rr = __builtin_amdgcn_udot4(a, a, rr, false);
+    }
+  }
+  const int acc = (int)(csq + rr - 2u * cr);
+  const int cst = ok ? umhs_cost(acc, mx + ox - pX0, my + oy - pY0) : 0x7FFFFFFF;
+  int start = 0;
+  for (;;) {
+    const uint64_t take = __ballot(ok && lane >= start && acc <= b.minD && cst < b.minC);
+    if (take == 0) break;
+    const int k = __builtin_ctzll(take);
+    b.minD = __builtin_amdgcn_readlane(acc, k);
+    b.minC = __builtin_amdgcn_readlane(cst, k);
+    b.rmx = __builtin_amdgcn_readlane(ox, k);
+    b.rmy = __builtin_amdgcn_readlane(oy, k);
+    start = k + 1;
+  }
+}
+
 struct Window {
   int xl, xr, yu, yd;
   __device__ __forceinline__ bool in(int x, int y) const { return y >= yu && y <= yd && x >= xl && x <= xr; }
@@ -228,6 +272,8 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
 
     // ---- starting points (UMHS.cpp:493-565)
     s.cur4[lane] = cw;
+    const uint32_t csq = (uint32_t)wave_sum((int)__builtin_amdgcn_udot4(cw, cw, 0u, false));
+    wave_sync();  // s.cur4 is read across lanes by try_lanes
     const int predXQ = predX % 4, predYQ = predY % 4;
     int predX0 = predX / 4, predY0 = predY / 4;
     const int oX0 = predX0, oY0 = predY0;
@@ -281,17 +327,12 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
       b.rmx = b.rmy = 0;
       int prior = b.minD;
       const int vlim = u.mrng / 2;
-      for (int g = 0; g < 4; g++) {  // w = 1..15 odd, two w per group of 8
-        int ox[8], oy[8];
-        bool ok[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-          const int w = 2 * (2 * g + (k >> 2)) + 1;
-          ox[k] = w * kCross[k & 3][0];
-          oy[k] = w * kCross[k & 3][1];
-          ok[k] = oy[k] < vlim && oy[k] > -vlim && win.in(mx + ox[k], my + oy[k]);
-        }
-        try8(d, wr, n, m, mx, my, ox, oy, ok, 8, predX0, predY0, cw, lane, b);
+      {
+        // 32 candidates: w = 1..15 odd, the four directions of each w in turn
+        const int k = lane & 31, w = 2 * (k >> 2) + 1;
+        const int ox = w * kCross[k & 3][0], oy = w * kCross[k & 3][1];
+        const bool ok = lane < 32 && oy < vlim && oy > -vlim && win.in(mx + ox, my + oy);
+        try_lanes(wr, s, n, m, mx, my, ox, oy, ok, csq, predX0, predY0, lane, b);
       }
       mx += b.rmx;
       my += b.rmy;
@@ -304,18 +345,12 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
         // centre only through the hexagon stage
         const int sx = b.rmx, sy = b.rmy;
         b.rmx = b.rmy = 0;
-        for (int g = 0; g < 24; g += 8) {
-          int ox[8], oy[8];
-          bool ok[8];
-#pragma unroll
-          for (int k = 0; k < 8; k++) {
-            ox[k] = kSq5[g + k][0];
-            oy[k] = kSq5[g + k][1];
-            const bool done = (sx || sy) && ((sy == 0 && (ox[k] == 2 || ox[k] == -2)) ||
-                                             (sx == 0 && (oy[k] == 2 || oy[k] == -2)));
-            ok[k] = !done && win.in(mx + ox[k], my + oy[k]);
-          }
-          try8(d, wr, n, m, mx, my, ox, oy, ok, 8, predX0, predY0, cw, lane, b);
+        {
+          const int k = lane < 24 ? lane : 0;
+          const int ox = kSq5[k][0], oy = kSq5[k][1];
+          const bool done = (sx || sy) && ((sy == 0 && (ox == 2 || ox == -2)) || (sx == 0 && (oy == 2 || oy == -2)));
+          const bool ok = lane < 24 && !done && win.in(mx + ox, my + oy);
+          try_lanes(wr, s, n, m, mx, my, ox, oy, ok, csq, predX0, predY0, lane, b);
         }
         if (b.minD < 1000) {
           stage = 2;
@@ -324,7 +359,10 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
           // improved enough
           prior = b.minD;
           for (int w = 1; w < 4; w++) {
-            try_table(d, wr, kHex16, 16, w, n, m, mx, my, win, predX0, predY0, cw, lane, b);
+            const int k = lane & 15;
+            const int ox = w * kHex16[k][0], oy = w * kHex16[k][1];
+            const bool ok = lane < 16 && win.in(mx + ox, my + oy);
+            try_lanes(wr, s, n, m, mx, my, ox, oy, ok, csq, predX0, predY0, lane, b);
             if ((b.rmx || b.rmy) && (b.minD < 2000 || b.minD < prior / 5)) break;
           }
           mx += b.rmx;
