@@ -172,6 +172,51 @@ __device__ __forceinline__ void try8(Ctx& c, int mx, int my, const int (&ox)[8],
   }
 }
 
+// A stage's candidates one per lane (lane k the stage's k-th, offset (ox,
+// oy) from (mx, my)): each lane sums its candidate's squared errors along
+// the 80 path points in the ten groups of 8 the early exit looks at (the
+// block's pels at the points from the lanes that hold them, the reference's
+// from the staged window), then the reference's loop body is replayed in
+// order by ballot: the first candidate after the last accepted one whose
+// path distortion against the current minimum is no worse and whose cost
+// is strictly lower replaces the best, until none is left.  One pass for up
+// to 64 candidates, where try8 takes one per 8.  Lanes without a candidate
+// (live false, or outside the window) read the zero vector's block and
+// take no part.
+__device__ __forceinline__ void try_lanes(Ctx& c, int mx, int my, int ox, int oy, bool live) {
+  const int x = mx + ox, y = my + oy;
+  const bool ok = live && y >= c.yu && y <= c.yd && x >= c.xl && x <= c.xr;
+  const int bx = ok ? c.n + x : c.n, by = ok ? c.m + y : c.m;  // window candidates are always staged
+  const uint8_t* base = c.win + (by - c.w.y0) * c.w.cols + (bx - c.w.x0);
+  Path80 p;
+#pragma unroll
+  for (int g = 0; g < 10; g++) {
+    int a = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int t = 8 * g + i;
+      const int pt = kPath80[t];
+      const int cv = __builtin_amdgcn_readlane(t < 64 ? c.cv1 : c.cv2, t & 63);
+      a += sq(cv, base[(pt >> 4) * c.w.cols + (pt & 15)]);
+    }
+    p.g[g] = a;
+  }
+  const int q = 269 * (int)sqrt_int_rn((x - c.pX0) * (x - c.pX0) + (y - c.pY0) * (y - c.pY0));
+  int start = 0;
+  for (;;) {
+    const int v = p.lt(c.minD);
+    const int cost = (v + q) >> 8;
+    const uint64_t take = __ballot(ok && c.lane >= start && v <= c.minD && cost < c.minC);
+    if (take == 0) break;
+    const int k = __builtin_ctzll(take);
+    c.minD = __builtin_amdgcn_readlane(v, k);
+    c.minC = __builtin_amdgcn_readlane(cost, k);
+    c.rmx = __builtin_amdgcn_readlane(ox, k);
+    c.rmy = __builtin_amdgcn_readlane(oy, k);
+    start = k + 1;
+  }
+}
+
 __device__ __forceinline__ void try_table(Ctx& c, const int8_t (*tab)[2], int cnt, int w, int mx, int my) {
   for (int g = 0; g < cnt; g += 8) {
     int ox[8], oy[8];
@@ -392,7 +437,11 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
       else stage = 0;
       if (stage == 0) {
         c.rmx = c.rmy = 0;
-        for (int w = 1; w <= 4; w++) try_table(c, kHex16, 16, w, mx, my);
+        {
+          // the four scales' 16 points each, scale by scale: 64 candidates
+          const int k = lane & 15, w = (lane >> 4) + 1;
+          try_lanes(c, mx, my, w * kHex16[k][0], w * kHex16[k][1], true);
+        }
         mx += c.rmx;
         my += c.rmy;
       }
