@@ -382,17 +382,14 @@ __device__ inline void build_half_grid(WaveLds& s, int lane) {
   wave_sync();
 }
 
-// The eight half-pel candidates around full-pel (x0, y0) without the LDS
-// lattice: lane l owns the 4 pixels of block row l / 4, columns 4 (l % 4)
-// .. +3 (its dword cw of the input block).  Their samples need reference
-// rows y0 + r - 3 .. y0 + r + 3 and columns x0 + c0 - 3 .. x0 + c0 + 6, read
-// as 7 x 4 aligned dwords from the padded plane (L2-resident after the
-// search); every G/b/h/j sample is then filtered in registers with the same
-// arithmetic as build_half_grid (vertical 6-tap sums first, j from them).
-// acc[k] receives this lane's distortion of candidate k (kSubX/kSubY order).
-template <int METRIC>
-__device__ __forceinline__ void half_pel_direct(const DecideDesc& d, int x0, int y0, uint32_t cw,
-                                                int lane, int (&acc)[8]) {
+// The sub-pel stages without the LDS lattice: lane l owns the 4 pixels of
+// block row l / 4, columns 4 (l % 4) .. +3 (its dword cw of the input
+// block), whose samples need reference rows y0 + r - 3 .. y0 + r + 3 and
+// columns x0 + c0 - 3 .. x0 + c0 + 6.
+// This lane's reference pels p[k][i] = (x0 + c0 - 3 + i, y0 + r - 3 + k),
+// k = 0..6, i = 0..9 (r = lane / 4, c0 = 4 (lane % 4)), read as 7 x 4
+// aligned dwords from the padded plane (L2-resident after the search).
+__device__ __forceinline__ void load_pels(const DecideDesc& d, int x0, int y0, int lane, int (&p)[7][10]) {
   const int r = lane >> 2, c0 = 4 * (lane & 3);
   const int xs = x0 - 3 + c0, sh = xs & 3;  // 0..3 also for negative xs
   const uint32_t* base = (const uint32_t*)(d.ref + (long)(y0 - 3 + r) * d.ref_stride + (xs - sh));
@@ -403,8 +400,6 @@ __device__ __forceinline__ void half_pel_direct(const DecideDesc& d, int x0, int
   for (int k = 0; k < 7; k++)
 #pragma unroll
     for (int q = 0; q < 4; q++) w[k][q] = base[k * sdw + q];
-  // p[k][i]: reference (x0 + c0 - 3 + i, y0 + r - 3 + k), i = 0..9
-  int p[7][10];
 #pragma unroll
   for (int k = 0; k < 7; k++) {
     const uint32_t a0 = __builtin_amdgcn_alignbyte(w[k][1], w[k][0], sh);
@@ -416,6 +411,15 @@ __device__ __forceinline__ void half_pel_direct(const DecideDesc& d, int x0, int
       p[k][i] = (int)((a >> (8 * (i & 3))) & 0xFFu);
     }
   }
+}
+
+// The eight half-pel candidates' distortions over this lane's 4 pels from
+// its reference pels (load_pels): every G/b/h/j sample is filtered in
+// registers with the same arithmetic as build_half_grid (vertical 6-tap
+// sums first, j from them).  acc[k] receives candidate k (kSubX/kSubY
+// order).
+template <int METRIC>
+__device__ __forceinline__ void half_from_pels(const int (&p)[7][10], uint32_t cw, int (&acc)[8]) {
   // raw vertical sums between rows y0 + r - 1, y0 + r (a = 0) and y0 + r,
   // y0 + r + 1 (a = 1)
   int v[2][10];
@@ -447,6 +451,87 @@ __device__ __forceinline__ void half_pel_direct(const DecideDesc& d, int x0, int
     acc[5] = dist_add<METRIC>(acc[5], cv, J[1][j]);      // (-1, 1)
     acc[6] = dist_add<METRIC>(acc[6], cv, Hh[1][j]);     // ( 0, 1)
     acc[7] = dist_add<METRIC>(acc[7], cv, J[1][j + 1]);  // ( 1, 1)
+  }
+}
+
+// The eight half-pel candidates around full-pel (x0, y0) without the LDS
+// lattice: lane l owns the 4 pixels of block row l / 4, columns 4 (l % 4)
+// .. +3 (its dword cw of the input block).
+template <int METRIC>
+__device__ __forceinline__ void half_pel_direct(const DecideDesc& d, int x0, int y0, uint32_t cw,
+                                                int lane, int (&acc)[8]) {
+  int p[7][10];
+  load_pels(d, x0, y0, lane, p);
+  half_from_pels<METRIC>(p, cw, acc);
+}
+
+// floor(v / 2) of a compile-time value
+constexpr int floor_half(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
+
+// The half-lattice sample at (Y, X) half pels from this lane's pel jj
+// (Y, X in -2..2, known at compile time): G, b, h or j as build_half_grid
+// stores them at Hg(2 + 2 r + Y, 2 + 2 c + X), from the lane's reference
+// pels and the raw vertical sums V[t + 1] between rows r + t, r + t + 1
+// (t = -1, 0).
+template <int Y, int X>
+__device__ __forceinline__ int lattice_at(const int (&p)[7][10], const int (&V)[2][10], int jj) {
+  constexpr int t = floor_half(Y), sx = floor_half(X);
+  constexpr bool oy = (Y & 1) != 0, ox = (X & 1) != 0;
+  if constexpr (!oy && !ox) {
+    return p[3 + t][3 + jj + sx];
+  } else if constexpr (!oy) {
+    const int* q = p[3 + t] + jj + sx + 1;
+    return clip255((ICU_TAP6(q[0], q[1], q[2], q[3], q[4], q[5]) + 16) >> 5);
+  } else if constexpr (!ox) {
+    return clip255((V[t + 1][3 + jj + sx] + 16) >> 5);
+  } else {
+    const int* q = V[t + 1] + jj + sx + 1;
+    return clip255((ICU_TAP6(q[0], q[1], q[2], q[3], q[4], q[5]) + 512) >> 10);
+  }
+}
+
+// The eight quarter-pel candidates around the half-pel winner (HX, HY)
+// half pels from full pel (x0, y0): the estimator's pairwise averages of
+// lattice samples around the winner (LoadQuartPelWindow, as refine_subpel
+// reads them from the LDS lattice), here from this lane's reference pels.
+template <int HY, int HX>
+__device__ __forceinline__ void quarter_from_pels(const int (&p)[7][10], uint32_t cw, int metric, int (&acc)[8]) {
+  int V[2][10];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int i = 0; i < 10; i++)
+      V[a][i] = ICU_TAP6(p[a][i], p[a + 1][i], p[a + 2][i], p[a + 3][i], p[a + 4][i], p[a + 5][i]);
+#pragma unroll
+  for (int jj = 0; jj < 4; jj++) {
+    const int cv = (int)((cw >> (8 * jj)) & 0xFFu);
+    const int uu = lattice_at<HY - 1, HX - 1>(p, V, jj), u0 = lattice_at<HY - 1, HX>(p, V, jj);
+    const int l0 = lattice_at<HY, HX - 1>(p, V, jj), c0 = lattice_at<HY, HX>(p, V, jj);
+    const int r0 = lattice_at<HY, HX + 1>(p, V, jj), d0 = lattice_at<HY + 1, HX>(p, V, jj);
+    acc[0] += pix_dist(cv, (uu + c0 + 1) >> 1, metric);  // (-1,-1)
+    acc[1] += pix_dist(cv, (u0 + c0 + 1) >> 1, metric);  // ( 0,-1)
+    acc[2] += pix_dist(cv, (u0 + r0 + 1) >> 1, metric);  // ( 1,-1)
+    acc[3] += pix_dist(cv, (l0 + c0 + 1) >> 1, metric);  // (-1, 0)
+    acc[4] += pix_dist(cv, (r0 + c0 + 1) >> 1, metric);  // ( 1, 0)
+    acc[5] += pix_dist(cv, (l0 + d0 + 1) >> 1, metric);  // (-1, 1)
+    acc[6] += pix_dist(cv, (d0 + c0 + 1) >> 1, metric);  // ( 0, 1)
+    acc[7] += pix_dist(cv, (d0 + r0 + 1) >> 1, metric);  // ( 1, 1)
+  }
+}
+
+// Dispatch on the (wave-uniform) half-pel winner.
+__device__ __forceinline__ void quarter_direct(const int (&p)[7][10], uint32_t cw, int metric, int hyh, int hxh,
+                                               int (&acc)[8]) {
+  switch ((hyh + 1) * 3 + (hxh + 1)) {
+    case 0: quarter_from_pels<-1, -1>(p, cw, metric, acc); break;
+    case 1: quarter_from_pels<-1, 0>(p, cw, metric, acc); break;
+    case 2: quarter_from_pels<-1, 1>(p, cw, metric, acc); break;
+    case 3: quarter_from_pels<0, -1>(p, cw, metric, acc); break;
+    case 4: quarter_from_pels<0, 0>(p, cw, metric, acc); break;
+    case 5: quarter_from_pels<0, 1>(p, cw, metric, acc); break;
+    case 6: quarter_from_pels<1, -1>(p, cw, metric, acc); break;
+    case 7: quarter_from_pels<1, 0>(p, cw, metric, acc); break;
+    default: quarter_from_pels<1, 1>(p, cw, metric, acc); break;
   }
 }
 
@@ -563,10 +648,12 @@ __device__ __forceinline__ void half_pel_sad_pk(const DecideDesc& d, int x0, int
 // (Full.cpp:452-517; UMHS.cpp:795-862 is the same stage).  cw: this lane's
 // dword of the input block (row lane / 4, columns 4 (lane % 4)), also
 // staged in s.cur4.  Updates minD; returns the vector in quarter pels.
-// patch_ready: s.P already holds the 22 x 22 patch around (n + mx, m + my).
+// Quarter pel: LATTICE builds the half-pel lattice in LDS (s.P, s.hv, s.Hg;
+// for kernels whose VGPRs are spoken for), else every sample is filtered in
+// registers from the padded plane (no LDS, fewer dependent steps).
+template <bool LATTICE = false>
 __device__ inline void refine_subpel(const DecideDesc& d, WaveLds& s, int n, int m, int mx, int my, uint32_t cw,
-                                     int lane, int& minD, int& mvx, int& mvy, bool patch_ready = false) {
-  const uint8_t* cur8 = (const uint8_t*)s.cur4;
+                                     int lane, int& minD, int& mvx, int& mvy) {
   mvx = mx * 4;
   mvy = my * 4;
   if (d.subpel == 1) {
@@ -589,8 +676,10 @@ __device__ inline void refine_subpel(const DecideDesc& d, WaveLds& s, int n, int
       }
     mvx += 2 * hxh;
     mvy += 2 * hyh;
-  } else if (d.subpel >= 2) {
-    if (!patch_ready) load_patch(d, n + mx, m + my, s.P, lane);
+  } else if (d.subpel >= 2 && LATTICE) {
+    // the 35 x 35 half-pel lattice built in LDS from the reference patch
+    const uint8_t* cur8 = (const uint8_t*)s.cur4;
+    load_patch(d, n + mx, m + my, s.P, lane);
     wave_sync();
     build_half_grid(s, lane);
     // half-pel candidates, all eight summed at once, then taken in the
@@ -637,6 +726,48 @@ __device__ inline void refine_subpel(const DecideDesc& d, WaveLds& s, int n, int
         acc[7] += pix_dist(cv, (d0 + r0 + 1) >> 1, d.metric);  // ( 1, 1)
       }
       wave_sum8(acc);
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (acc[k] < minD) {
+          minD = acc[k];
+          qmx = hx + kSubX[k];
+          qmy = hy + kSubY[k];
+        }
+    }
+    mvx += qmx;
+    mvy += qmy;
+    wave_sync();
+  } else if (d.subpel >= 2) {
+    // both stages from this lane's reference pels, filtered in registers:
+    // the eight half-pel candidates (strictly better only), then the eight
+    // quarter-pel averages around the half-pel winner
+    int p[7][10];
+    load_pels(d, n + mx, m + my, lane, p);
+    int hxh = 0, hyh = 0;
+    {
+      int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (d.metric == kMetricSAD) {
+        half_from_pels<kMetricSAD>(p, cw, acc);
+        wave_sum8_small(acc);
+      } else {
+        half_from_pels<kMetricSSD>(p, cw, acc);
+        wave_sum8(acc);
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (acc[k] < minD) {
+          minD = acc[k];
+          hxh = kSubX[k];
+          hyh = kSubY[k];
+        }
+    }
+    int qmx = 2 * hxh, qmy = 2 * hyh;
+    {
+      const int hx = 2 * hxh, hy = 2 * hyh;
+      int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      quarter_direct(p, cw, d.metric, hyh, hxh, acc);
+      if (d.metric == kMetricSAD) wave_sum8_small(acc);
+      else wave_sum8(acc);
 #pragma unroll
       for (int k = 0; k < 8; k++)
         if (acc[k] < minD) {
@@ -792,6 +923,7 @@ __device__ __forceinline__ void decide_mb_lean(const DecideDesc& d, int mb, int 
 }
 
 // staged_tab (nullable): the MB's cost table already copied to LDS.
+template <bool LATTICE = false>
 __device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, int predX, int predY,
                                      int lane, const IntResult* known = nullptr,
                                      const uint32_t* staged_tab = nullptr) {
@@ -849,7 +981,7 @@ __device__ inline Decision decide_mb(const DecideDesc& d, WaveLds& s, int mb, in
   out.iminD = minD;
 
   int mvx, mvy;
-  refine_subpel(d, s, n, m, mx, my, cw, lane, minD, mvx, mvy);
+  refine_subpel<LATTICE>(d, s, n, m, mx, my, cw, lane, minD, mvx, mvy);
   return finish_mb(d, s, mb, predX, predY, lane, mvx, mvy, minD, zeroD, out);
 }
 

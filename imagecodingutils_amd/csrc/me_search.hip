@@ -454,7 +454,7 @@ __device__ __forceinline__ void issue_stage(const StreamDesc& s, const SearchTas
 __device__ __attribute__((noinline)) void decide_in_search(const DecideDesc* d, dec::WaveLds* s, int mb,
                                                            int lane, int mx, int my, int cost) {
   const IntResult r{mx, my, cost, -1};
-  dec::decide_mb(*d, *s, mb, 0, 0, lane, &r);
+  dec::decide_mb<true>(*d, *s, mb, 0, 0, lane, &r);
 }
 
 // VFIX: tile height fixed at compile time, or 0 to take each stream's own
@@ -866,7 +866,7 @@ __device__ __forceinline__ void wave_decide_mb(const DecideDesc& d, uint32_t* ld
   // together (6 waves per SIMD); called out of line the decision alone took 87
   const IntResult r{mx, my, (int)key_cost(kbest), -1};
   if (LEAN) dec::decide_mb_lean(d, mb, lane, r);
-  else dec::decide_mb(d, *(dec::WaveLds*)lds, mb, 0, 0, lane, &r);
+  else dec::decide_mb<true>(d, *(dec::WaveLds*)lds, mb, 0, 0, lane, &r);
 }
 
 // LDS of any window class (the 64 class stages 32-class quadrants).

@@ -390,12 +390,7 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
 
     // ---- sub-pel stages and the predictor at quarter pel (UMHS.cpp:795-903)
     int minD = b.minD, mvx = mx * 4, mvy = my * 4;
-    if (d.subpel >= 2) {
-      fw::patch(d, wr.lds, wr.w, wr.w.has(n + mx, m + my, 16, 16, 3), n + mx, m + my, s.P, lane);
-      refine_subpel(d, s, n, m, mx, my, cw, lane, minD, mvx, mvy, true);
-    } else if (d.subpel == 1) {
-      refine_subpel(d, s, n, m, mx, my, cw, lane, minD, mvx, mvy);
-    }
+    if (d.subpel >= 1) refine_subpel(d, s, n, m, mx, my, cw, lane, minD, mvx, mvy);
     const int rX = predX0 * 4 + predXQ, rY = predY0 * 4 + predYQ;
     if (predXQ || predYQ) predVecD = qdist(d, s, wr, n + predX0, m + predY0, predXQ, predYQ, lane);
     else if (trunc) predVecD = block_ssd(d, wr, n + predX0, m + predY0, cw, lane);
