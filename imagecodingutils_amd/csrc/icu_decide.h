@@ -519,6 +519,63 @@ __device__ __forceinline__ void quarter_from_pels(const int (&p)[7][10], uint32_
   }
 }
 
+// Reference pels p[k][i] = (x - 3 + i, y - 3 + k) around one pel (x, y)
+// of this lane, k = 0..6, i = 0..9, and the raw vertical sums V[a][i]
+// between rows y - 1 + a and y + a: what lattice_at needs for jj = 0.
+__device__ __forceinline__ void load_pels_at(const DecideDesc& d, int x, int y, int (&p)[7][10], int (&V)[2][10]) {
+  const int xs = x - 3, sh = xs & 3;
+  const uint32_t* base = (const uint32_t*)(d.ref + (long)(y - 3) * d.ref_stride + (xs - sh));
+  const int sdw = d.ref_stride >> 2;
+  uint32_t w[7][4];
+#pragma unroll
+  for (int k = 0; k < 7; k++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[k][q] = base[k * sdw + q];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    const uint32_t a0 = __builtin_amdgcn_alignbyte(w[k][1], w[k][0], sh);
+    const uint32_t a1 = __builtin_amdgcn_alignbyte(w[k][2], w[k][1], sh);
+    const uint32_t a2 = __builtin_amdgcn_alignbyte(w[k][3], w[k][2], sh);
+#pragma unroll
+    for (int i = 0; i < 10; i++) {
+      const uint32_t a = i < 4 ? a0 : (i < 8 ? a1 : a2);
+      p[k][i] = (int)((a >> (8 * (i & 3))) & 0xFFu);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int i = 0; i < 10; i++)
+      V[a][i] = ICU_TAP6(p[a][i], p[a + 1][i], p[a + 2][i], p[a + 3][i], p[a + 4][i], p[a + 5][i]);
+}
+
+// The six lattice samples the quarter-pel averages take around the half-pel
+// winner (HX, HY) for pel jj: (HY - 1, HX - 1), (HY - 1, HX), (HY, HX - 1),
+// (HY, HX), (HY, HX + 1), (HY + 1, HX).
+template <int HY, int HX>
+__device__ __forceinline__ void winner_samples(const int (&p)[7][10], const int (&V)[2][10], int jj, int (&q)[6]) {
+  q[0] = lattice_at<HY - 1, HX - 1>(p, V, jj);
+  q[1] = lattice_at<HY - 1, HX>(p, V, jj);
+  q[2] = lattice_at<HY, HX - 1>(p, V, jj);
+  q[3] = lattice_at<HY, HX>(p, V, jj);
+  q[4] = lattice_at<HY, HX + 1>(p, V, jj);
+  q[5] = lattice_at<HY + 1, HX>(p, V, jj);
+}
+__device__ __forceinline__ void winner_samples_at(const int (&p)[7][10], const int (&V)[2][10], int jj, int hyh,
+                                                  int hxh, int (&q)[6]) {
+  switch ((hyh + 1) * 3 + (hxh + 1)) {
+    case 0: winner_samples<-1, -1>(p, V, jj, q); break;
+    case 1: winner_samples<-1, 0>(p, V, jj, q); break;
+    case 2: winner_samples<-1, 1>(p, V, jj, q); break;
+    case 3: winner_samples<0, -1>(p, V, jj, q); break;
+    case 4: winner_samples<0, 0>(p, V, jj, q); break;
+    case 5: winner_samples<0, 1>(p, V, jj, q); break;
+    case 6: winner_samples<1, -1>(p, V, jj, q); break;
+    case 7: winner_samples<1, 0>(p, V, jj, q); break;
+    default: winner_samples<1, 1>(p, V, jj, q); break;
+  }
+}
+
 // Dispatch on the (wave-uniform) half-pel winner.
 __device__ __forceinline__ void quarter_direct(const int (&p)[7][10], uint32_t cw, int metric, int hyh, int hxh,
                                                int (&acc)[8]) {

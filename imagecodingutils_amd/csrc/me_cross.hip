@@ -160,18 +160,18 @@ __device__ void cross_mb(const CrossDesc& c, WaveLds& s, const uint8_t* win, con
 
   int mvx = mx * 4, mvy = my * 4;
   if (d.subpel >= 1) {
-    // the half-pel lattice around (mx, my), then the 8 half candidates and
-    // (quarter mode) the 8 quarter averages around the winner, sampled at
-    // this lane's path point
-    fw::patch(d, win, w, w.has(n + mx, m + my, 16, 16, 3), n + mx, m + my, s.P, lane);
-    wave_sync();
-    build_half_grid(s, lane);
-    const int16_t* h = s.Hg + (2 + 2 * py) * kHalf + 2 + 2 * px;
+    // the 8 half candidates and (quarter mode) the 8 quarter averages around
+    // the winner, sampled at this lane's path point: its lattice samples
+    // filtered in registers from the reference pels around it (the same
+    // G / b / h / j as build_half_grid's lattice)
+    int p[7][10], V[2][10];
+    load_pels_at(d, n + mx + px, m + my + py, p, V);
     int hxh = 0, hyh = 0;
     {
-      int e[8];
-#pragma unroll
-      for (int k = 0; k < 8; k++) e[k] = sq(cv, h[kSubY[k] * kHalf + kSubX[k]]);
+      int e[8] = {sq(cv, lattice_at<-1, -1>(p, V, 0)), sq(cv, lattice_at<-1, 0>(p, V, 0)),
+                  sq(cv, lattice_at<-1, 1>(p, V, 0)),  sq(cv, lattice_at<0, -1>(p, V, 0)),
+                  sq(cv, lattice_at<0, 1>(p, V, 0)),   sq(cv, lattice_at<1, -1>(p, V, 0)),
+                  sq(cv, lattice_at<1, 0>(p, V, 0)),   sq(cv, lattice_at<1, 1>(p, V, 0))};
       Batches b[8];
       batch_sums(e, b);
 #pragma unroll
@@ -186,8 +186,9 @@ __device__ void cross_mb(const CrossDesc& c, WaveLds& s, const uint8_t* win, con
     }
     int qmx = 2 * hxh, qmy = 2 * hyh;
     if (d.subpel >= 2) {
-      const int16_t* q = h + hyh * kHalf + hxh;
-      const int uu = q[-kHalf - 1], u0 = q[-kHalf], l0 = q[-1], c0 = q[0], r0 = q[1], d0 = q[kHalf];
+      int q[6];
+      winner_samples_at(p, V, 0, hyh, hxh, q);
+      const int uu = q[0], u0 = q[1], l0 = q[2], c0 = q[3], r0 = q[4], d0 = q[5];
       int e[8] = {sq(cv, (uu + c0 + 1) >> 1), sq(cv, (u0 + c0 + 1) >> 1), sq(cv, (u0 + r0 + 1) >> 1),
                   sq(cv, (l0 + c0 + 1) >> 1), sq(cv, (r0 + c0 + 1) >> 1), sq(cv, (l0 + d0 + 1) >> 1),
                   sq(cv, (d0 + c0 + 1) >> 1), sq(cv, (d0 + r0 + 1) >> 1)};
@@ -206,7 +207,6 @@ __device__ void cross_mb(const CrossDesc& c, WaveLds& s, const uint8_t* win, con
     }
     mvx += qmx;
     mvy += qmy;
-    wave_sync();
   }
 
   // the predictor, truncated to one MB beyond the picture, at quarter pel
