@@ -327,7 +327,7 @@ void vbs_wave_kernel(const VbsParams p) {
     for (int j = 0; j < 4; j++) {
       const int dx = dx0 + j;
       colok[j] = active && dx >= xl && dx <= xr;
-      offc[j] = ((uint32_t)(p.lambda * (dx < 0 ? -dx : dx)) << 15) + (uint32_t)(dx + R);
+      offc[j] = (__umul24((uint32_t)p.lambda, (uint32_t)(dx < 0 ? -dx : dx)) << 15) + (uint32_t)(dx + R);
     }
     const uint8_t* rb = r == 0 ? p.ref[0] : (r == 1 ? p.ref[1] : (r == 2 ? p.ref[2] : p.ref[3]));
     // stage rows m - R .. m - R + ROWS - 1 from the 16-byte boundary at or
@@ -350,7 +350,7 @@ void vbs_wave_kernel(const VbsParams p) {
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    const uint32_t rowbase = (uint32_t)(r * NN + (dy0 + R) * N);
+    const uint32_t rowbase = __umul24((uint32_t)r, (uint32_t)NN) + __umul24((uint32_t)(dy0 + R), (uint32_t)N);
     uint64_t acc[4][VV];
 #pragma unroll
     for (int k = 0; k < 4; k++)
@@ -393,7 +393,7 @@ void vbs_wave_kernel(const VbsParams p) {
         const int dy = dy0 + v;
         const bool row_ok = dy >= yu && dy <= yd;
         const uint32_t offr = (__umul24((uint32_t)p.lambda, (uint32_t)(dy < 0 ? -dy : dy)) << 15) + rowbase +
-                              (uint32_t)(v * N);
+                              __umul24((uint32_t)v, (uint32_t)N);
         uint32_t off[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) off[j] = (row_ok && colok[j]) ? offr + offc[j] : 0x80000000u;
