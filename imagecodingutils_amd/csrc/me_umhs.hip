@@ -85,28 +85,6 @@ __device__ __forceinline__ int block_ssd(const DecideDesc& d, const WinRef& wr, 
   return wave_sum(ssd4(d, wr, x, y, cw, lane));
 }
 
-// Distortion against the quarter-pel read at full pel (x0, y0) + (fx, fy)
-// (fractions -3..3; a negative one reads from the pel to the left / above),
-// its patch from the staged window when it lies inside.
-__device__ __forceinline__ int qdist(const DecideDesc& d, WaveLds& s, const WinRef& wr, int x0, int y0, int fx, int fy,
-                                     int lane) {
-  const uint8_t* cur8 = (const uint8_t*)s.cur4;
-  int fox = 0, foy = 0;
-  if (fx < 0) { fox = -1; fx += 4; }
-  if (fy < 0) { foy = -1; fy += 4; }
-  const int x = x0 + fox, y = y0 + foy;
-  fw::patch(d, wr.lds, wr.w, wr.w.has(x, y, 16, 16, 3), x, y, s.P, lane);
-  wave_sync();
-  int acc = 0;
-  for (int p = lane; p < 256; p += 64) {
-    const int r = p >> 4, c = p & 15;
-    acc += pix_dist(cur8[p], qpel_p(s.P, 3 + c, 3 + r, fx, fy), d.metric);
-  }
-  const int v = wave_sum(acc);
-  wave_sync();
-  return v;
-}
-
 // The running minimum of the search (minDiff, minCost, offset (rmx, rmy)
 // from the stage centre), wave-uniform.
 struct Best {
@@ -392,7 +370,7 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
     int minD = b.minD, mvx = mx * 4, mvy = my * 4;
     if (d.subpel >= 1) refine_subpel(d, s, n, m, mx, my, cw, lane, minD, mvx, mvy);
     const int rX = predX0 * 4 + predXQ, rY = predY0 * 4 + predYQ;
-    if (predXQ || predYQ) predVecD = qdist(d, s, wr, n + predX0, m + predY0, predXQ, predYQ, lane);
+    if (predXQ || predYQ) predVecD = quarter_dist(d, s, n + predX0, m + predY0, predXQ, predYQ, lane);
     else if (trunc) predVecD = block_ssd(d, wr, n + predX0, m + predY0, cw, lane);
     if (umhs_cost(predVecD, rX - predX, rY - predY) <= umhs_cost(minD, mvx - predX, mvy - predY)) {
       minD = predVecD;
