@@ -222,10 +222,10 @@ __device__ void cross_mb(const CrossDesc& c, WaveLds& s, const uint8_t* win, con
       int fox = 0, fx = predXQ, foy = 0, fy = predYQ;
       if (fx < 0) { fox = -1; fx += 4; }
       if (fy < 0) { foy = -1; fy += 4; }
-      const int qx = n + predX0 + fox, qy = m + predY0 + foy;
-      fw::patch(d, win, w, w.has(qx, qy, 16, 16, 3), qx, qy, s.P, lane);
-      wave_sync();
-      e[0] = sq(cv, qpel_p(s.P, 3 + px, 3 + py, fx, fy));
+      // this lane's path pel's quarter sample, filtered in registers
+      int q[4];
+      qpel_row4(d.ref, d.ref_stride, n + predX0 + fox + px, m + predY0 + foy + py, fx, fy, q);
+      e[0] = sq(cv, q[0]);
     } else {
       e[0] = sq(cv, at(predX0, predY0));
     }

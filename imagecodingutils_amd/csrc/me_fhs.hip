@@ -379,18 +379,8 @@ __global__ __launch_bounds__(64) void fhs_kernel(FhsDesc f) {
     c.pY0 = predY0;
     int predVecD;
     if (pXQ || pYQ) {
-      // QuarterRead at the predictor, its patch from the window when staged
-      int fx = pXQ, fy = pYQ, fox = 0, foy = 0;
-      if (fx < 0) { fox = -1; fx += 4; }
-      if (fy < 0) { foy = -1; fy += 4; }
-      const int px = n + predX0 + fox, py = m + predY0 + foy;
-      fw::patch(d, s_win, w, w.has(px, py, 16, 16, 3), px, py, s.w.P, lane);
-      wave_sync();
-      const uint8_t* cur8 = (const uint8_t*)s.w.cur4;
-      int a = 0;
-      for (int p = lane; p < 256; p += 64) a += sq(cur8[p], qpel_p(s.w.P, 3 + (p & 15), 3 + (p >> 4), fx, fy));
-      predVecD = wave_sum(a);
-      wave_sync();
+      // QuarterRead at the predictor, filtered in registers (SSD)
+      predVecD = quarter_dist(d, s.w, n + predX0, m + predY0, pXQ, pYQ, lane);
     } else {
       const int px = n + predX0, py = m + predY0;
       const uint32_t rw = fw::row4(d, s_win, w, w.has(px, py, 16, 16, 0), px, py, lane);
