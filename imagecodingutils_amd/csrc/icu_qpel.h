@@ -80,8 +80,10 @@ __device__ __forceinline__ double sqrt_int_rn(int m) {
 
 // Cost of GetBestVector (include/CodecUtils/MotionEstimatorH264ImplFull.h:
 // 140-168): dist / 256 + 1.05 * sqrt(mag), each operation rounded on its own.
+// dist / 256 of an integer is exact in a double, so it is the product with
+// 2^-8 (one multiply instead of a correctly rounded division).
 __device__ __forceinline__ double best_vector_cost(int dist, int mag) {
-  const double a = __ddiv_rn((double)dist, 256.0);
+  const double a = __dmul_rn((double)dist, 0.00390625);
   const double b = __dmul_rn(1.05, sqrt_int_rn(mag));
   return __dadd_rn(a, b);
 }

@@ -82,9 +82,10 @@ __device__ __forceinline__ void batch_sums(int (&e)[K], Batches (&out)[K]) {
 
 __device__ __forceinline__ int sq(int a, int b) { return (a - b) * (a - b); }
 
-// GetBestVector's cost with lambda 0.9 and norm 64, correctly rounded.
+// GetBestVector's cost with lambda 0.9 and norm 64, correctly rounded
+// (dist / 64 of an integer is exact: the product with 2^-6).
 __device__ __forceinline__ double cross_cost(int dist, int mag) {
-  return __dadd_rn(__ddiv_rn((double)dist, 64.0), __dmul_rn(0.9, sqrt_int_rn(mag)));
+  return __dadd_rn(__dmul_rn((double)dist, 0.015625), __dmul_rn(0.9, sqrt_int_rn(mag)));
 }
 
 // The MB's search window, staged by stage_window before cross_mb.
