@@ -22,8 +22,9 @@
 //   IContextAwareRunLevelCodec / CAVLCH264Impl, BitStreamWriterMSB / BitStreamReaderMSB
 //       include/CodecUtils/IContextAwareRunLevelCodec.h:49-103, CAVLCH264Impl.h:59-150,
 //       BitStreamBaseMSB.h:62-87
-// They live in namespace icu_codecutils; INTEGRATION.md shows the adapter a
-// codec adds to bind them to its own CodecUtils interface types instead.
+// They live in namespace icu_codecutils; a codec that must keep its own
+// CodecUtils interface types binds them through a thin adapter class that
+// derives from its interface and forwards each call to these.
 //
 // Header only; link with libicu_hip.so.
 #pragma once
