@@ -389,17 +389,7 @@ __device__ inline void build_half_grid(WaveLds& s, int lane) {
 // This lane's reference pels p[k][i] = (x0 + c0 - 3 + i, y0 + r - 3 + k),
 // k = 0..6, i = 0..9 (r = lane / 4, c0 = 4 (lane % 4)), read as 7 x 4
 // aligned dwords from the padded plane (L2-resident after the search).
-__device__ __forceinline__ void load_pels(const DecideDesc& d, int x0, int y0, int lane, int (&p)[7][10]) {
-  const int r = lane >> 2, c0 = 4 * (lane & 3);
-  const int xs = x0 - 3 + c0, sh = xs & 3;  // 0..3 also for negative xs
-  const uint32_t* base = (const uint32_t*)(d.ref + (long)(y0 - 3 + r) * d.ref_stride + (xs - sh));
-  const int sdw = d.ref_stride >> 2;
-  // 10 bytes from byte sh of a row span 4 dwords when sh == 3
-  uint32_t w[7][4];
-#pragma unroll
-  for (int k = 0; k < 7; k++)
-#pragma unroll
-    for (int q = 0; q < 4; q++) w[k][q] = base[k * sdw + q];
+__device__ __forceinline__ void unpack_pels(const uint32_t (&w)[7][4], int sh, int (&p)[7][10]) {
 #pragma unroll
   for (int k = 0; k < 7; k++) {
     const uint32_t a0 = __builtin_amdgcn_alignbyte(w[k][1], w[k][0], sh);
@@ -412,7 +402,19 @@ __device__ __forceinline__ void load_pels(const DecideDesc& d, int x0, int y0, i
     }
   }
 }
-
+__device__ __forceinline__ void load_pels(const DecideDesc& d, int x0, int y0, int lane, int (&p)[7][10]) {
+  const int r = lane >> 2, c0 = 4 * (lane & 3);
+  const int xs = x0 - 3 + c0, sh = xs & 3;  // 0..3 also for negative xs
+  const uint32_t* base = (const uint32_t*)(d.ref + (long)(y0 - 3 + r) * d.ref_stride + (xs - sh));
+  const int sdw = d.ref_stride >> 2;
+  // 10 bytes from byte sh of a row span 4 dwords when sh == 3
+  uint32_t w[7][4];
+#pragma unroll
+  for (int k = 0; k < 7; k++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) w[k][q] = base[k * sdw + q];
+  unpack_pels(w, sh, p);
+}
 // The eight half-pel candidates' distortions over this lane's 4 pels from
 // its reference pels (load_pels): every G/b/h/j sample is filtered in
 // registers with the same arithmetic as build_half_grid (vertical 6-tap
@@ -700,6 +702,50 @@ __device__ __forceinline__ void half_pel_sad_pk(const DecideDesc& d, int x0, int
   acc[4] = (int)__builtin_amdgcn_sad_u16(b34, C1, (uint32_t)acc[4]);
 }
 
+// The half- and quarter-pel stages of refine_subpel (subpel >= 2) from
+// this lane's reference pels around the full-pel winner (load_pels): the
+// eight half-pel candidates (strictly better only),
+// then the eight quarter-pel averages around the half-pel winner.  mvx /
+// mvy hold the full-pel vector in quarter pels on entry.
+__device__ __forceinline__ void subpel_from_pels(const DecideDesc& d, const int (&p)[7][10], uint32_t cw, int& minD,
+                                                 int& mvx, int& mvy) {
+  int hxh = 0, hyh = 0;
+  {
+    int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (d.metric == kMetricSAD) {
+      half_from_pels<kMetricSAD>(p, cw, acc);
+      wave_sum8_small(acc);
+    } else {
+      half_from_pels<kMetricSSD>(p, cw, acc);
+      wave_sum8(acc);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (acc[k] < minD) {
+        minD = acc[k];
+        hxh = kSubX[k];
+        hyh = kSubY[k];
+      }
+  }
+  int qmx = 2 * hxh, qmy = 2 * hyh;
+  {
+    const int hx = 2 * hxh, hy = 2 * hyh;
+    int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    quarter_direct(p, cw, d.metric, hyh, hxh, acc);
+    if (d.metric == kMetricSAD) wave_sum8_small(acc);
+    else wave_sum8(acc);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (acc[k] < minD) {
+        minD = acc[k];
+        qmx = hx + kSubX[k];
+        qmy = hy + kSubY[k];
+      }
+  }
+  mvx += qmx;
+  mvy += qmy;
+}
+
 // Half-pel then (subpel >= 2) quarter-pel refinement around full-pel
 // (mx, my) of the MB at (n, m), strictly better distortions only
 // (Full.cpp:452-517; UMHS.cpp:795-862 is the same stage).  cw: this lane's
@@ -795,46 +841,10 @@ __device__ inline void refine_subpel(const DecideDesc& d, WaveLds& s, int n, int
     mvy += qmy;
     wave_sync();
   } else if (d.subpel >= 2) {
-    // both stages from this lane's reference pels, filtered in registers:
-    // the eight half-pel candidates (strictly better only), then the eight
-    // quarter-pel averages around the half-pel winner
+    // both stages from this lane's reference pels, filtered in registers
     int p[7][10];
     load_pels(d, n + mx, m + my, lane, p);
-    int hxh = 0, hyh = 0;
-    {
-      int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      if (d.metric == kMetricSAD) {
-        half_from_pels<kMetricSAD>(p, cw, acc);
-        wave_sum8_small(acc);
-      } else {
-        half_from_pels<kMetricSSD>(p, cw, acc);
-        wave_sum8(acc);
-      }
-#pragma unroll
-      for (int k = 0; k < 8; k++)
-        if (acc[k] < minD) {
-          minD = acc[k];
-          hxh = kSubX[k];
-          hyh = kSubY[k];
-        }
-    }
-    int qmx = 2 * hxh, qmy = 2 * hyh;
-    {
-      const int hx = 2 * hxh, hy = 2 * hyh;
-      int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      quarter_direct(p, cw, d.metric, hyh, hxh, acc);
-      if (d.metric == kMetricSAD) wave_sum8_small(acc);
-      else wave_sum8(acc);
-#pragma unroll
-      for (int k = 0; k < 8; k++)
-        if (acc[k] < minD) {
-          minD = acc[k];
-          qmx = hx + kSubX[k];
-          qmy = hy + kSubY[k];
-        }
-    }
-    mvx += qmx;
-    mvy += qmy;
+    subpel_from_pels(d, p, cw, minD, mvx, mvy);
     wave_sync();
   }
 
