@@ -328,6 +328,29 @@ def test_tuned_estimator_matches_oracle():
     np.testing.assert_array_equal(dist, odist)
 
 
+def test_timed_estimate_stages_and_results():
+    """icu_me_estimate_timed: the same vectors as the plain device estimate;
+    the decision stage is 0 when the search kernel decides every MB (no cost
+    table) and positive when decision kernels run (filter on)."""
+    W, H, R = 352, 288, 16
+    fp = synth.rect_motion_pair(W, H, 77, n_rects=8, max_mv=14)
+    d_cur, d_ref = capi.DeviceBuffer.from_array(fp.cur), capi.DeviceBuffer.from_array(fp.ref)
+    n = (W // 16) * (H // 16)
+    for filt, decide_runs in ((0, False), (1, True)):
+        est = capi.MotionEstimator(_cfg(W, H, R, metric=1, window=1, filt=filt, subpel=1, predictor=0,
+                                        best_vector=0))
+        d_mv, d_dist = capi.DeviceBuffer(4 * n), capi.DeviceBuffer(4 * n)
+        e_mv, e_dist = capi.DeviceBuffer(4 * n), capi.DeviceBuffer(4 * n)
+        s = capi.Stream()
+        pad_ms, search_ms, decide_ms = est.estimate_timed(d_cur, d_ref, d_mv, d_dist, stream=s, search_reps=3)
+        est.estimate_device(d_cur, d_ref, e_mv, e_dist, stream=s)
+        s.sync()
+        assert pad_ms > 0 and search_ms > 0
+        assert (decide_ms > 0) == decide_runs
+        np.testing.assert_array_equal(d_mv.download(np.int16, 2 * n), e_mv.download(np.int16, 2 * n))
+        np.testing.assert_array_equal(d_dist.download(np.int32, n), e_dist.download(np.int32, n))
+
+
 def test_out_of_range_input_is_rejected():
     est = capi.MotionEstimator(_cfg(64, 48, 8))
     cur = np.full((48, 64), 300, np.int16)
