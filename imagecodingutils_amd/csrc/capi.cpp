@@ -200,7 +200,7 @@ struct Engine {
   DecideItem* d_items = nullptr;
   IntResult* d_ires = nullptr;
   uint32_t* d_table = nullptr;
-  int* d_wave = nullptr;  // per stream: mbh progress counters + 2 (ticket, error)
+  int* d_wave = nullptr;  // per stream: 2 words (row ticket, error flag)
   uint8_t* d_pf = nullptr;  // predictor-free passes of the median-predictor streams' MBs
   std::vector<int> wave_off;
   std::vector<int> pad_base;  // first padding workgroup of each stream
@@ -399,7 +399,7 @@ struct Engine {
       dev_bytes += bytes;
       s.refpad = s.refpad_base + (size_t)kPad * s.ref_stride + kPad;
       wave_off.push_back(wave_total);
-      wave_total += s.mbh + 2;
+      wave_total += 2;
       pad_base.push_back(pad_total);
       pad_total += pad_plane_workgroups(s.cfg.width, s.cfg.height);
     }
@@ -563,8 +563,7 @@ struct Engine {
       e.mv_out = mv;
       e.dist_out = dist;
       e.int_out = integ;
-      e.progress = d_wave + wave_off[i];
-      e.ticket = d_wave + wave_off[i] + s.mbh;
+      e.ticket = d_wave + wave_off[i];
     }
     for (size_t i = 0; i < sp.size(); i++) {
       D.cur[i] = curs[i];
@@ -649,11 +648,11 @@ struct Engine {
     for (size_t i = 0; i < sp.size(); i++) {
       if (!sp[i].cfg.predictor) continue;
       decide_launches += 2;
-      // progress counters and the row ticket; the error flag after them
-      // stays set until check_wave_errors reads it
+      // the predictor-free pass resets the row ticket and marks every
+      // vector pending; the error flag stays set until check_wave_errors
+      // reads it
       uint8_t* pf = d_pf + (size_t)predfree_bytes() * sp[i].mb_base;
       launch_decide_predfree(d_decide + i, pf, sp[i].nmb, st);
-      launch_fill_i32(d_wave + wave_off[i], 0, sp[i].mbh + 1, st);
       launch_decide_wavefront(d_decide + i, sp[i].table ? 2 * sp[i].L + 1 : 0, sp[i].mbh, pf, st);
     }
     if (ev) HIP_TRY(hipEventRecord(ev[3], st));
@@ -684,7 +683,7 @@ struct Engine {
     bool timed_out = false;
     for (size_t i = 0; i < sp.size(); i++) {
       if (!sp[i].cfg.predictor) continue;
-      int* flag = d_wave + wave_off[i] + sp[i].mbh + 1;
+      int* flag = d_wave + wave_off[i] + 1;
       int v = 0;
       HIP_TRY(hipMemcpyAsync(&v, flag, sizeof(v), hipMemcpyDeviceToHost, own));
       HIP_TRY(hipStreamSynchronize(own));
@@ -740,7 +739,7 @@ struct icu_me_batch {
 int* icu::me_wave_flag(icu_me* h) {
   const Engine& e = h->eng;
   if (!e.sp[0].cfg.predictor) return nullptr;
-  return e.d_wave + e.wave_off[0] + e.sp[0].mbh + 1;
+  return e.d_wave + e.wave_off[0] + 1;
 }
 
 int icu::me_check(icu_me* h) { return h->eng.check_wave_errors(); }

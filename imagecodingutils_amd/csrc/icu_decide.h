@@ -20,8 +20,8 @@
 // With the median predictor (H264MotionVectorPredictorImpl1 over
 // MacroBlockH264::GetMbMotionMedianPred, MacroBlockH264.cpp:757-816) an MB
 // depends on its left, above-left, above and above-right neighbours, so MB
-// rows run as a wavefront: each row is owned by one wave that waits on the
-// row above through a progress counter.
+// rows run as a wavefront: each row is owned by one wave that polls the row
+// above's vector words until they are final.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

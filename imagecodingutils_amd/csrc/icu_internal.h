@@ -52,7 +52,6 @@ struct DecideDesc {
   int16_t* mv_out;             // 2 per MB slot
   int32_t* dist_out;           // 1 per MB slot
   int32_t* int_out;            // 3 per MB slot (integer stage), may be null
-  int* progress;               // mbh counters (median wavefront)
   int* ticket;                 // [0] row ticket, [1] error flag
 };
 

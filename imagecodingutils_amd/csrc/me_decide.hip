@@ -18,8 +18,8 @@
 // With the median predictor (H264MotionVectorPredictorImpl1 over
 // MacroBlockH264::GetMbMotionMedianPred, MacroBlockH264.cpp:757-816) an MB
 // depends on its left, above-left, above and above-right neighbours, so MB
-// rows run as a wavefront: each row is owned by one wave that waits on the
-// row above through a progress counter.
+// rows run as a wavefront: each row is owned by one wave that polls the row
+// above's vector words until they are final.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -32,6 +32,9 @@ namespace {
 using namespace dec;
 
 constexpr int kSpinLimit = 1 << 26;
+// A vector word not yet decided: (-32768, -32768) quarter pel, which no MB
+// can get (every vector stays within 4 (range + 1) of zero).
+constexpr uint32_t kVecPending = 0x80008000u;
 
 __global__ __launch_bounds__(64) void decide_parallel_kernel(const DecideDesc* __restrict__ descs,
                                                              const DecideItem* __restrict__ items,
@@ -52,14 +55,25 @@ __global__ __launch_bounds__(64) void decide_predfree_kernel(const DecideDesc* _
                                                              PredFree* __restrict__ pf, int nmb) {
   __shared__ WaveLds s;
   const int lane = threadIdx.x;
+  const DecideDesc& d = *desc;
+  // the wavefront that follows starts from row ticket 0 and polls each
+  // neighbour's vector word until it no longer holds kVecPending
+  if (blockIdx.x == 0 && lane == 0) d.ticket[0] = 0;
   for (int mb = blockIdx.x; mb < nmb; mb += gridDim.x) {
-    predfree_mb(*desc, s, mb, lane, pf + mb);
+    if (lane == 0) *(uint32_t*)(d.mv_out + 2 * (d.mb_base + mb)) = kVecPending;
+    predfree_mb(d, s, mb, lane, pf + mb);
     __syncthreads();
   }
 }
 
 // One wave per MB row; rows are handed out by ticket in start order, so the
 // row a wave waits on always belongs to a wave that is already running.
+// The hand-off is the vector word itself: the predictor-free pass sets every
+// MB's word to kVecPending, an MB polls its above and above-right (or
+// above-left) words until both are final, and its own single vector store
+// publishes it.  No progress counter, and no drain of the other stores
+// before publishing: one memory round trip per MB on the critical path
+// instead of three.
 // pf: the MBs' predictor-free passes; an MB whose scan met no tie only adds
 // its predictor's distortion and the final choice, one that did is decided
 // in full.  stage_tab: such an MB's cost table (table_dim^2 dwords) is
@@ -72,7 +86,6 @@ __global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* 
   extern __shared__ uint32_t s_tab[];
   const DecideDesc& d = *desc;
   const int lane = threadIdx.x;
-  int* progress = d.progress;
   int* ticket = d.ticket;
   if (lane == 0) row_s = atomicAdd(&ticket[0], 1);
   __syncthreads();
@@ -93,15 +106,22 @@ __global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* 
           __builtin_amdgcn_global_load_lds((const void*)(src + base + lane),
                                            (__attribute__((address_space(3))) void*)(s_tab + base), 4, 0, 0);
     }
+    // neighbour vectors (MacroBlockH264.cpp:757-816)
+    const int mb = row * d.mbw + col;
+    // each (x, y) pair is one aligned 32-bit word; read it coherently
+    const uint32_t* mvw = (const uint32_t*)(d.mv_out + 2 * d.mb_base);
+    int Bx = 0, By = 0, Cx = 0, Cy = 0;
     if (row > 0) {
-      // above-right (or above for the last column) must be final
-      const int need = min(col + 2, d.mbw);
+      // lane 0: above; lane 1: above-right, or above-left in the last column
+      const bool needC = col + 1 < d.mbw || col > 0;
+      const int k = lane == 0 ? mb - d.mbw : (col + 1 < d.mbw ? mb - d.mbw + 1 : mb - d.mbw - 1);
+      uint32_t w = 0;
       // Bounded: a wait that runs out sets the error flag (ticket[1]), which
       // stays set until the host reads it; every other wait then gives up at
       // once instead of spinning its own full limit.
-      if (lane == 0) {
+      if (lane == 0 || (lane == 1 && needC)) {
         int spins = 0;
-        while (poll_progress(&progress[row - 1]) < need) {
+        while ((w = __hip_atomic_load(&mvw[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == kVecPending) {
           if (++spins > kSpinLimit) {
             __hip_atomic_store(&ticket[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
@@ -110,23 +130,16 @@ __global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* 
           __builtin_amdgcn_s_sleep(1);
         }
       }
-      __syncthreads();
+      const uint32_t wB = __builtin_amdgcn_readlane(w, 0), wC = __builtin_amdgcn_readlane(w, 1);
+      Bx = (int)(int16_t)(wB & 0xFFFF);
+      By = (int)(int16_t)(wB >> 16);
+      if (needC) {
+        Cx = (int)(int16_t)(wC & 0xFFFF);
+        Cy = (int)(int16_t)(wC >> 16);
+      }
+    } else {
+      Bx = Ax; By = Ay; Cx = Ax; Cy = Ay;
     }
-    // neighbour vectors (MacroBlockH264.cpp:757-816)
-    const int mb = row * d.mbw + col;
-    // each (x, y) pair is one aligned 32-bit word; read it coherently
-    const uint32_t* mvw = (const uint32_t*)(d.mv_out + 2 * d.mb_base);
-    auto ld = [&](int k, int* x, int* y) {
-      const uint32_t w = __hip_atomic_load(&mvw[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      *x = (int)(int16_t)(w & 0xFFFF);
-      *y = (int)(int16_t)(w >> 16);
-    };
-    int Bx = 0, By = 0, Cx = 0, Cy = 0;
-    const bool hasB = row > 0, hasC = row > 0 && col + 1 < d.mbw;
-    if (hasB) ld(mb - d.mbw, &Bx, &By);
-    if (hasC) ld(mb - d.mbw + 1, &Cx, &Cy);
-    else if (col > 0 && row > 0) ld(mb - d.mbw - 1, &Cx, &Cy);
-    if (!hasB && !hasC) { Bx = Ax; By = Ay; Cx = Ax; Cy = Ay; }
     const int px = median3(Ax, Bx, Cx), py = median3(Ay, By, Cy);
     if (stage) {
       __builtin_amdgcn_s_waitcnt(0);  // the table has landed
@@ -136,9 +149,6 @@ __global__ __launch_bounds__(64) void decide_wavefront_kernel(const DecideDesc* 
     Ax = dec.mvx;
     Ay = dec.mvy;
     __syncthreads();
-    if (lane == 0) {
-      publish_progress(&progress[row], col + 1);
-    }
   }
 }
 
