@@ -1162,6 +1162,10 @@ __device__ inline Decision decide_mb_after(const DecideDesc& d, WaveLds& s, int 
 // before its progress store, and the consumer only a relaxed poll.  An
 // acquire / release at agent scope would add an L2 invalidate to every poll
 // and an L2 write-back to every MB, for data that no plain load reads.
+// A vector word not yet decided, the median wavefronts' hand-off value:
+// (-32768, -32768) quarter pel, which no MB can get (every vector stays
+// within 4 (range + 1) of zero).
+constexpr uint32_t kVecPending = 0x80008000u;
 __device__ __forceinline__ int poll_progress(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

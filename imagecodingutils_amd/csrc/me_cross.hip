@@ -47,7 +47,6 @@ struct CrossDesc {
   int median;       // 1: H.264 median predictor (wavefront), 0: zero
   int16_t* mv_out;
   int32_t* dist_out;
-  int* progress;    // mbh counters (median)
   int* flags;       // [0] row ticket, [1] timed out
 };
 
@@ -97,8 +96,9 @@ __device__ __forceinline__ fw::Win stage_window(const CrossDesc& c, uint8_t* win
   return w;
 }
 
-__device__ void cross_mb(const CrossDesc& c, WaveLds& s, const uint8_t* win, const fw::Win& w, int mb, int predX,
-                         int predY, int lane) {
+// Returns the chosen vector as its output word (x low, y high), in every lane.
+__device__ uint32_t cross_mb(const CrossDesc& c, WaveLds& s, const uint8_t* win, const fw::Win& w, int mb, int predX,
+                             int predY, int lane) {
   const DecideDesc& d = c.d;
   const int col = mb % d.mbw, row = mb / d.mbw;
   const int n = col * 16, m = row * 16;
@@ -249,11 +249,12 @@ __device__ void cross_mb(const CrossDesc& c, WaveLds& s, const uint8_t* win, con
     outx = outy = 0;
     outd = zeroD;
   }
+  const uint32_t wv = (uint32_t)(uint16_t)(int16_t)outx | ((uint32_t)(uint16_t)(int16_t)outy << 16);
   if (lane == 0) {
-    const uint32_t wv = (uint32_t)(uint16_t)(int16_t)outx | ((uint32_t)(uint16_t)(int16_t)outy << 16);
     __hip_atomic_store((uint32_t*)(c.mv_out + 2 * mb), wv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     c.dist_out[mb] = outd;
   }
+  return wv;
 }
 
 // Zero predictor: MBs are independent.
@@ -271,7 +272,11 @@ __global__ __launch_bounds__(64) void cross_parallel_kernel(CrossDesc c) {
 }
 
 // Median predictor: one wave per MB row, rows by ticket in start order,
-// each waiting for the above-right MB of the row above.
+// each waiting for the above-right MB of the row above.  The hand-off is the
+// vector word itself: the host sets every word to kVecPending first, an MB
+// polls its above and above-right (or above-left) words until both are
+// final, and its own single vector store publishes it (one memory round
+// trip per MB on the critical path; a progress counter took three).
 __global__ __launch_bounds__(64) void cross_wavefront_kernel(CrossDesc c) {
   __shared__ WaveLds s;
   __shared__ __align__(16) uint8_t s_win[fw::kBytes];
@@ -283,14 +288,20 @@ __global__ __launch_bounds__(64) void cross_wavefront_kernel(CrossDesc c) {
   const int row = row_s;
   if (row >= d.mbh) return;
   const uint32_t* mvw = (const uint32_t*)c.mv_out;
+  int Ax = 0, Ay = 0;  // the left neighbour: this wave's previous MB
   for (int col = 0; col < d.mbw; col++) {
     // the MB's window into LDS while it waits for its neighbours
     const fw::Win w = stage_window(c, s_win, row * d.mbw + col, lane);
+    const int mb = row * d.mbw + col;
+    int Bx = 0, By = 0, Cx = 0, Cy = 0;
     if (row > 0) {
-      const int need = min(col + 2, d.mbw);
-      if (lane == 0) {
+      // lane 0: above; lane 1: above-right, or above-left in the last column
+      const bool needC = col + 1 < d.mbw || col > 0;
+      const int k = lane == 0 ? mb - d.mbw : (col + 1 < d.mbw ? mb - d.mbw + 1 : mb - d.mbw - 1);
+      uint32_t wn = 0;
+      if (lane == 0 || (lane == 1 && needC)) {
         int spins = 0;
-        while (poll_progress(&c.progress[row - 1]) < need) {
+        while ((wn = __hip_atomic_load(&mvw[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == kVecPending) {
           if (++spins > kSpin) {
             atomicExch(&c.flags[1], 1);
             break;
@@ -298,29 +309,22 @@ __global__ __launch_bounds__(64) void cross_wavefront_kernel(CrossDesc c) {
           __builtin_amdgcn_s_sleep(1);
         }
       }
-      __syncthreads();
-    }
-    const int mb = row * d.mbw + col;
-    auto ld = [&](int k, int* x, int* y) {
-      const uint32_t w = __hip_atomic_load(&mvw[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      *x = (int)(int16_t)(w & 0xFFFF);
-      *y = (int)(int16_t)(w >> 16);
-    };
-    int Ax = 0, Ay = 0, Bx = 0, By = 0, Cx = 0, Cy = 0;
-    const bool hasB = row > 0, hasC = row > 0 && col + 1 < d.mbw;
-    if (col > 0) ld(mb - 1, &Ax, &Ay);
-    if (hasB) ld(mb - d.mbw, &Bx, &By);
-    if (hasC) ld(mb - d.mbw + 1, &Cx, &Cy);
-    else if (col > 0 && row > 0) ld(mb - d.mbw - 1, &Cx, &Cy);
-    if (!hasB && !hasC) {
+      const uint32_t wB = __builtin_amdgcn_readlane(wn, 0), wC = __builtin_amdgcn_readlane(wn, 1);
+      Bx = (int)(int16_t)(wB & 0xFFFF);
+      By = (int)(int16_t)(wB >> 16);
+      if (needC) {
+        Cx = (int)(int16_t)(wC & 0xFFFF);
+        Cy = (int)(int16_t)(wC >> 16);
+      }
+    } else {
       Bx = Cx = Ax;
       By = Cy = Ay;
     }
-    cross_mb(c, s, s_win, w, mb, median3(Ax, Bx, Cx), median3(Ay, By, Cy), lane);
+    const uint32_t wv =
+        __builtin_amdgcn_readfirstlane(cross_mb(c, s, s_win, w, mb, median3(Ax, Bx, Cx), median3(Ay, By, Cy), lane));
+    Ax = (int)(int16_t)(wv & 0xFFFF);
+    Ay = (int)(int16_t)(wv >> 16);
     __syncthreads();
-    if (lane == 0) {
-      publish_progress(&c.progress[row], col + 1);
-    }
   }
 }
 
@@ -331,7 +335,7 @@ struct icu_cross {
   uint8_t* pad_base = nullptr;
   uint8_t* pad = nullptr;
   int stride = 0, pad_wgs = 0;
-  int* d_wave = nullptr;  // mbh progress counters, then ticket and timeout flag
+  int* d_wave = nullptr;  // row ticket and timeout flag
   uint8_t *d_cur = nullptr, *d_ref = nullptr;
   int16_t* d_mv = nullptr;
   int32_t* d_dist = nullptr;
@@ -380,7 +384,7 @@ int icu_cross_create(int width, int height, int motion_range, int mode, int pred
   h->pad_wgs = pad_plane_workgroups(width, height);
   const size_t n = (size_t)h->mbw * h->mbh;
   if (hipMalloc(&h->pad_base, (size_t)h->stride * (height + 2 * kPad)) != hipSuccess ||
-      hipMalloc(&h->d_wave, sizeof(int) * (h->mbh + 2)) != hipSuccess ||
+      hipMalloc(&h->d_wave, sizeof(int) * 2) != hipSuccess ||
       hipMalloc(&h->d_cur, (size_t)width * height) != hipSuccess ||
       hipMalloc(&h->d_ref, (size_t)width * height) != hipSuccess || hipMalloc(&h->d_mv, 4 * n) != hipSuccess ||
       hipMalloc(&h->d_dist, 4 * n) != hipSuccess || hipStreamCreate(&h->own) != hipSuccess) {
@@ -425,10 +429,10 @@ int icu_cross_estimate_device(icu_cross* h, const uint8_t* d_cur, const uint8_t*
   c.median = h->median;
   c.mv_out = d_mv;
   c.dist_out = d_dist;
-  c.progress = h->d_wave;
-  c.flags = h->d_wave + h->mbh;
+  c.flags = h->d_wave;
   if (h->median) {
-    launch_fill_i32(h->d_wave, 0, h->mbh + 2, st);
+    launch_fill_i32((int*)d_mv, (int)dec::kVecPending, h->mbw * h->mbh, st);
+    launch_fill_i32(h->d_wave, 0, 2, st);
     hipLaunchKernelGGL(cross_wavefront_kernel, dim3(h->mbh), dim3(64), 0, st, c);
   } else {
     const int nmb = h->mbw * h->mbh;
@@ -442,7 +446,7 @@ int icu_cross_check(icu_cross* h) {
   if (!h) return icu_fail(ICU_E_ARG, "null argument");
   int flags[2] = {0, 0};
   CR_TRY(hipDeviceSynchronize());
-  if (h->median) CR_TRY(hipMemcpy(flags, h->d_wave + h->mbh, sizeof(flags), hipMemcpyDeviceToHost));
+  if (h->median) CR_TRY(hipMemcpy(flags, h->d_wave, sizeof(flags), hipMemcpyDeviceToHost));
   return flags[1] ? icu_fail(ICU_E_STATE, "Cross wavefront timed out") : ICU_OK;
 }
 
@@ -468,7 +472,7 @@ int icu_cross_estimate(icu_cross* h, const int16_t* cur, const int16_t* ref, con
   // threads overlap on the device (a null-stream copy would wait for them all)
   int flags[2] = {0, 0};
   if (h->median)
-    CR_TRY(hipMemcpyAsync(flags, h->d_wave + h->mbh, sizeof(flags), hipMemcpyDeviceToHost, h->own));
+    CR_TRY(hipMemcpyAsync(flags, h->d_wave, sizeof(flags), hipMemcpyDeviceToHost, h->own));
   CR_TRY(hipStreamSynchronize(h->own));
   if (flags[1]) return icu_fail(ICU_E_STATE, "Cross wavefront timed out");
   if (dist) std::copy(dd.begin(), dd.end(), dist);

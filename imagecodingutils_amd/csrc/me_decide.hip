@@ -32,9 +32,6 @@ namespace {
 using namespace dec;
 
 constexpr int kSpinLimit = 1 << 26;
-// A vector word not yet decided: (-32768, -32768) quarter pel, which no MB
-// can get (every vector stays within 4 (range + 1) of zero).
-constexpr uint32_t kVecPending = 0x80008000u;
 
 __global__ __launch_bounds__(64) void decide_parallel_kernel(const DecideDesc* __restrict__ descs,
                                                              const DecideItem* __restrict__ items,

@@ -6,7 +6,8 @@
 // and above-right MBs' vectors with their distortions
 // (MacroBlockH264::GetMbMotionMedianPred, MacroBlockH264.cpp:818-915), so MB
 // rows run as a wavefront: one wave per MB row, rows handed out by ticket in
-// start order, each waiting on the row above through a progress counter.
+// start order, each polling the row above's hand-off words (vector and
+// distortion in one 64-bit word, kVecPending until the MB is final).
 // Within an MB, every pattern stage evaluates all of its candidates at once
 // (lane l holds the input block's row l / 4, columns 4 (l % 4) .. +3, and
 // sums its 4 pixels of every candidate; the wave then reduces), and the
@@ -53,8 +54,8 @@ struct UmhsDesc {
   const uint8_t* prev_intra;   // and intra flags
   int16_t* mv_out;
   int32_t* dist_out;
-  int* progress;               // mbh counters
   int* flags;                  // [0] row ticket, [1] timed out
+  uint64_t* hand;              // per MB: vector word (low), distortion (high)
 };
 
 // MEH264IUMHS_COST: distortion / 256 + 1.05 x distance, truncated.
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
   __syncthreads();
   const int row = row_s;
   if (row >= d.mbh) return;
-  const uint32_t* mvw = (const uint32_t*)u.mv_out;
+  int Ax = 0, Ay = 0, Ad = 0;  // the left neighbour: this wave's previous MB
   for (int col = 0; col < d.mbw; col++) {
     const int n = col * 16, m = row * 16;
     // the MB's search window into LDS and its block into registers, while
@@ -206,36 +207,38 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
     WinRef wr{s_win, fw::shape(n, m, d.W, d.H, u.mrng - 1)};
     fw::issue(d, wr.w, s_win, lane);
     const uint32_t cw = *(const uint32_t*)(d.cur + (long)(m + (lane >> 2)) * d.cur_stride + n + 4 * (lane & 3));
-    if (row > 0) {
-      const int need = min(col + 2, d.mbw);
-      if (lane == 0) {
-        int spins = 0;
-        while (poll_progress(&u.progress[row - 1]) < need) {
-          if (++spins > kSpin) {
-            atomicExch(&u.flags[1], 1);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
+    const int mb = row * d.mbw + col;
+    // lane 0: above; lane 1: above-right, or above-left in the last column;
+    // one 64-bit word each, polled until its vector is no longer pending
+    uint64_t hw = 0;
+    const bool needC = row > 0 && (col + 1 < d.mbw || col > 0);
+    if (row > 0 && (lane == 0 || (lane == 1 && needC))) {
+      const int k = lane == 0 ? mb - d.mbw : (col + 1 < d.mbw ? mb - d.mbw + 1 : mb - d.mbw - 1);
+      int spins = 0;
+      while ((uint32_t)(hw = __hip_atomic_load(&u.hand[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ==
+             kVecPending) {
+        if (++spins > kSpin) {
+          atomicExch(&u.flags[1], 1);
+          break;
         }
+        __builtin_amdgcn_s_sleep(1);
       }
-      __syncthreads();
     }
     fw::wait();
-    const int mb = row * d.mbw + col;
     // ---- predictor with distortion (MacroBlockH264.cpp:818-915)
-    auto ld = [&](int k, int* x, int* y, int* dd) {
-      const uint32_t w = __hip_atomic_load(&mvw[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      *x = (int)(int16_t)(w & 0xFFFF);
-      *y = (int)(int16_t)(w >> 16);
-      *dd = __hip_atomic_load(&u.dist_out[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    };
-    int Ax = 0, Ay = 0, Ad = 0, Bx = 0, By = 0, Bd = 0, Cx = 0, Cy = 0, Cd = 0;
-    const bool hasB = row > 0, hasC = row > 0 && col + 1 < d.mbw;
-    if (col > 0) ld(mb - 1, &Ax, &Ay, &Ad);
-    if (hasB) ld(mb - d.mbw, &Bx, &By, &Bd);
-    if (hasC) ld(mb - d.mbw + 1, &Cx, &Cy, &Cd);
-    else if (col > 0 && row > 0) ld(mb - d.mbw - 1, &Cx, &Cy, &Cd);
-    if (!hasB && !hasC) {
+    int Bx = 0, By = 0, Bd = 0, Cx = 0, Cy = 0, Cd = 0;
+    if (row > 0) {
+      const uint32_t bl = __builtin_amdgcn_readlane((uint32_t)hw, 0), bh = __builtin_amdgcn_readlane((uint32_t)(hw >> 32), 0);
+      const uint32_t cl = __builtin_amdgcn_readlane((uint32_t)hw, 1), ch = __builtin_amdgcn_readlane((uint32_t)(hw >> 32), 1);
+      Bx = (int)(int16_t)(bl & 0xFFFF);
+      By = (int)(int16_t)(bl >> 16);
+      Bd = (int)bh;
+      if (needC) {
+        Cx = (int)(int16_t)(cl & 0xFFFF);
+        Cy = (int)(int16_t)(cl >> 16);
+        Cd = (int)ch;
+      }
+    } else {
       Bx = Cx = Ax;
       By = Cy = Ay;
       Bd = Cd = Ad;
@@ -379,10 +382,14 @@ __global__ __launch_bounds__(64) void umhs_kernel(UmhsDesc u) {
     }
     if (lane == 0) {
       const uint32_t w = (uint32_t)(uint16_t)(int16_t)mvx | ((uint32_t)(uint16_t)(int16_t)mvy << 16);
-      __hip_atomic_store((uint32_t*)(u.mv_out + 2 * mb), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&u.dist_out[mb], minD, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      publish_progress(&u.progress[row], col + 1);
+      *(uint32_t*)(u.mv_out + 2 * mb) = w;
+      u.dist_out[mb] = minD;
+      __hip_atomic_store(&u.hand[mb], (uint64_t)w | ((uint64_t)(uint32_t)minD << 32), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     }
+    Ax = __builtin_amdgcn_readfirstlane(mvx);
+    Ay = __builtin_amdgcn_readfirstlane(mvy);
+    Ad = __builtin_amdgcn_readfirstlane(minD);
     __syncthreads();
   }
 }
@@ -394,7 +401,7 @@ struct icu_umhs {
   uint8_t* pad_base = nullptr;
   uint8_t* pad = nullptr;
   int stride = 0, pad_wgs = 0;
-  int* d_wave = nullptr;  // mbh progress counters, then ticket and timeout flag
+  int* d_wave = nullptr;  // ticket and timeout flag, then one 64-bit hand-off word per MB
   // staging for the host entry point
   uint8_t *d_cur = nullptr, *d_ref = nullptr, *d_prev_intra = nullptr;
   int16_t *d_prev = nullptr, *d_mv = nullptr;
@@ -444,7 +451,7 @@ int icu_umhs_create(int width, int height, int motion_range, int mode, icu_umhs*
   h->pad_wgs = pad_plane_workgroups(width, height);
   const size_t n = (size_t)h->mbw * h->mbh;
   if (hipMalloc(&h->pad_base, (size_t)h->stride * (height + 2 * kPad)) != hipSuccess ||
-      hipMalloc(&h->d_wave, sizeof(int) * (h->mbh + 2)) != hipSuccess ||
+      hipMalloc(&h->d_wave, sizeof(int) * (2 + 2 * n)) != hipSuccess ||
       hipMalloc(&h->d_cur, (size_t)width * height) != hipSuccess ||
       hipMalloc(&h->d_ref, (size_t)width * height) != hipSuccess ||
       hipMalloc(&h->d_prev_intra, n) != hipSuccess || hipMalloc(&h->d_prev, 4 * n) != hipSuccess ||
@@ -475,7 +482,8 @@ int icu_umhs_estimate_device(icu_umhs* h, const uint8_t* d_cur, const uint8_t* d
   ps.n = 1;
   ps.d[0] = PadDesc{d_ref, h->pad, h->W, h->H, h->W, h->stride, 0};
   launch_pad_set(ps, h->pad_wgs, st);
-  launch_fill_i32(h->d_wave, 0, h->mbh + 2, st);
+  launch_fill_i32(h->d_wave, 0, 2, st);
+  launch_fill_i32(h->d_wave + 2, (int)dec::kVecPending, 2 * h->mbw * h->mbh, st);
   UmhsDesc u{};
   DecideDesc& d = u.d;
   d.cur = d_cur;
@@ -495,8 +503,8 @@ int icu_umhs_estimate_device(icu_umhs* h, const uint8_t* d_cur, const uint8_t* d
   u.prev_intra = d_prev_intra;
   u.mv_out = d_mv;
   u.dist_out = d_dist;
-  u.progress = h->d_wave;
-  u.flags = h->d_wave + h->mbh;
+  u.flags = h->d_wave;
+  u.hand = (uint64_t*)(h->d_wave + 2);
   hipLaunchKernelGGL(umhs_kernel, dim3(h->mbh), dim3(64), 0, st, u);
   UM_TRY(hipGetLastError());
   return ICU_OK;
@@ -506,7 +514,7 @@ int icu_umhs_check(icu_umhs* h) {
   if (!h) return icu_fail(ICU_E_ARG, "null argument");
   int flags[2] = {0, 0};
   UM_TRY(hipDeviceSynchronize());
-  UM_TRY(hipMemcpy(flags, h->d_wave + h->mbh, sizeof(flags), hipMemcpyDeviceToHost));
+  UM_TRY(hipMemcpy(flags, h->d_wave, sizeof(flags), hipMemcpyDeviceToHost));
   return flags[1] ? icu_fail(ICU_E_STATE, "UMHS wavefront timed out") : ICU_OK;
 }
 
@@ -538,7 +546,7 @@ int icu_umhs_estimate(icu_umhs* h, const int16_t* cur, const int16_t* ref, const
   // everything on the handle's own stream: objects driven from different
   // threads overlap on the device (a null-stream copy would wait for them all)
   int flags[2] = {0, 0};
-  UM_TRY(hipMemcpyAsync(flags, h->d_wave + h->mbh, sizeof(flags), hipMemcpyDeviceToHost, h->own));
+  UM_TRY(hipMemcpyAsync(flags, h->d_wave, sizeof(flags), hipMemcpyDeviceToHost, h->own));
   UM_TRY(hipStreamSynchronize(h->own));
   if (flags[1]) return icu_fail(ICU_E_STATE, "UMHS wavefront timed out");
   if (dist) std::copy(dd.begin(), dd.end(), dist);
