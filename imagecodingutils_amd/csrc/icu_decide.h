@@ -1158,8 +1158,11 @@ __device__ inline Decision decide_mb_after(const DecideDesc& d, WaveLds& s, int 
 // reads of another's results in the same launch (vectors, distortions, the
 // FHS fill records, the progress counters) goes through relaxed agent-scope
 // atomics, which the hardware keeps coherent across the XCDs by themselves.
-// So publishing needs only the producer's own stores drained (s_waitcnt)
-// before its progress store, and the consumer only a relaxed poll.  An
+// So a progress counter (FHS) needs only the producer's own stores drained
+// (s_waitcnt) before its store, and the consumer only a relaxed poll; a
+// hand-off word that carries the result itself (the decision, UMHS and
+// Cross wavefronts) needs no drain at all: the consumer polls that word
+// until it is no longer kVecPending.  An
 // acquire / release at agent scope would add an L2 invalidate to every poll
 // and an L2 write-back to every MB, for data that no plain load reads.
 // A vector word not yet decided, the median wavefronts' hand-off value:
